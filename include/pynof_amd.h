@@ -1,0 +1,114 @@
+/* pynof_amd — MI355X-native density-fitted JK build: the C ABI.
+ *
+ * The entry points replace the RI branch of PyNOF's JK builds (paths relative
+ * to the PyNOF source tree):
+ *   computeJKj(C, I, b_mnl, p)        src/pynof/integrals.py:172-191  -> J^n, K^n
+ *     JKj_RI        integrals.py:208-218 (numpy), JKj_RI_GPU :234-245 (cupy),
+ *     JKj_RI_jit    integrals.py:282-301 (numba)
+ *   computeJKH_MO(C, H, I, b_mnl, p)  src/pynof/integrals.py:307-326  -> J_MO, K_MO, H_core
+ *     JKH_MO_RI     integrals.py:348-362, JKH_MO_RI_GPU :383-399, JKH_MO_RI_jit :457-484
+ *   the device upload of b          src/pynof/integrals.py:51-53  (cp.array(b_mnl))
+ *
+ * Conventions: plain pointers and sizes; arrays are C-order FP64. A pointer may
+ * be host or device memory (HIP unified addressing decides) unless stated.
+ * Every call returns PNJK_OK (0) or an error code; pnjk_last_error() gives the
+ * calling thread's message. Calls on one engine must not overlap.
+ */
+#ifndef PYNOF_AMD_H
+#define PYNOF_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PNJK_OK 0
+#define PNJK_EINVAL 1   /* bad argument: sizes, null pointers, non-finite or asymmetric b */
+#define PNJK_EHIP 2     /* a HIP runtime call failed */
+#define PNJK_ENOMEM 3   /* device allocation failed */
+#define PNJK_ESTATE 4   /* engine not ready (no b uploaded) */
+#define PNJK_EUNSUPPORTED 5
+
+/* Arithmetic of a build. */
+#define PNJK_FP64 0          /* every contraction in FP64 (MFMA f64) */
+#define PNJK_SPLIT 1         /* bulk contractions as BF16x3 split products, FP32 accumulation */
+#define PNJK_SPLIT_REFINE 2  /* SPLIT for J^n/K^n; MO quantities refined to FP64 accuracy */
+
+/* Output selection bits for pnjk_jk_build. */
+#define PNJK_OUT_J 1u
+#define PNJK_OUT_K 2u
+#define PNJK_OUT_JMO 4u
+#define PNJK_OUT_KMO 8u
+#define PNJK_OUT_HCORE 16u
+#define PNJK_OUT_ALL 31u
+
+typedef struct pnjk_engine pnjk_engine;
+
+/* Last error message of the calling thread ("" if none). */
+const char* pnjk_last_error(void);
+
+/* Library version string. */
+const char* pnjk_version(void);
+
+/* Creates an engine bound to HIP device `device`. Replaces the implicit cupy
+ * device of integrals.py:8-11/:51-53. */
+int pnjk_create(int device, pnjk_engine** out);
+int pnjk_destroy(pnjk_engine* e);
+
+/* Uploads b_mnl (nbf, nbf, naux), C-order, host or device memory, and keeps it
+ * resident in the engine's packed layout (lower triangle mu >= nu, aux stride
+ * padded). Replaces cp.array(b_mnl) at integrals.py:51-53. Rejects non-finite
+ * values, and b whose max |b[mu,nu,P] - b[nu,mu,P]| exceeds `sym_tol` (b is
+ * symmetric in mu,nu by construction, integrals.py:35-49). Writes the resident
+ * footprint in bytes to *footprint (may be NULL) and the measured asymmetry to
+ * *asym (may be NULL). */
+int pnjk_upload_b(pnjk_engine* e, const double* b_mnl, int64_t nbf, int64_t naux, double sym_tol,
+                  uint64_t* footprint, double* asym);
+
+/* Aux stride of the packed layout the engine uses for (nbf, naux). */
+int64_t pnjk_packed_ld(int64_t nbf, int64_t naux);
+
+/* Adopts an already packed b: device rows t = mu*(mu+1)/2 + nu (mu >= nu), each
+ * `ld` doubles with b[mu,nu,0:naux] first (ld = pnjk_packed_ld(nbf, naux), zero
+ * tail). The data is copied. */
+int pnjk_upload_b_packed(pnjk_engine* e, const double* btri_dev, int64_t nbf, int64_t naux, int64_t ld,
+                         uint64_t* footprint);
+
+/* One fused JK build for orbitals C (element (mu, q) at C[mu*ldc + q], mu < nbf,
+ * q < nbf5; only the first nbf5 columns are read, integrals.py:211,:353) and,
+ * when PNJK_OUT_HCORE is set, H (element (mu, nu) at H[mu*ldh + nu]).
+ * Outputs (C-order, FP64), each written when its bit is set in `outputs`:
+ *   J, K     (nbf5, nbf, nbf)   J^n, K^n           integrals.py:214, :216
+ *   J_MO     (nbf5, nbf5)                          integrals.py:356
+ *   K_MO     (nbf5, nbf5)                          integrals.py:358
+ *   H_core   (nbf5)                                integrals.py:360
+ * Output pointers may be host or device memory. `stream` is a hipStream_t (NULL:
+ * the engine's own stream). The call returns after the outputs are complete
+ * when any output is host memory; otherwise it is asynchronous on `stream`. */
+int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, const double* H, int64_t ldh,
+                  int precision, uint32_t outputs, double* J, double* K, double* J_MO, double* K_MO,
+                  double* H_core, void* stream);
+
+/* Per-stage timing with HIP events recorded on the build stream (no
+ * counterpart in the reference, which has wall-clock prints only,
+ * energy.py:29, 223-224). When on, every build records one event per stage;
+ * pnjk_stage_times writes the last profiled build's stage durations in
+ * milliseconds to ms[0 .. pnjk_stage_count()-1], waiting for that build. */
+int pnjk_set_profiling(pnjk_engine* e, int on);
+int pnjk_stage_count(void);
+const char* pnjk_stage_name(int i);
+int pnjk_stage_times(pnjk_engine* e, double* ms);
+
+/* Device bytes the engine holds now (b plus workspace). */
+uint64_t pnjk_footprint(const pnjk_engine* e);
+
+/* Device bytes a build of this size needs beyond b (workspace), for admission
+ * control. */
+uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYNOF_AMD_H */

@@ -1,0 +1,172 @@
+"""FP64 CPU oracle for the density-fitted JK build.
+
+TEST INFRASTRUCTURE ONLY. Nothing in the product imports this module: it is
+used by ``tests/``, by ``__graft_entry__.smoke()`` as the checker and by
+``bench.py``'s CPU-baseline leg. It is a restatement, in numpy, of the
+reference's RI contractions, written from the mathematics; every function
+cites the reference lines it follows (paths relative to the reference root).
+
+Pinning (see tests/test_oracle.py): every function below is checked against
+(1) explicit index loops on tiny inputs, (2) the independent 4-index route
+I = b b^T through the reference's ``JKj_Full``/``JKH_MO_Full`` formulas,
+(3) closed forms at C = identity and naux = 1, (4) invariants (symmetry,
+diag K_MO = diag J_MO, J_MO PSD, contraction of J^n/K^n with D_p reproducing
+J_MO/K_MO, orbital invariance of sum_q J^q and sum_q K^q), and (5) the
+committed golden fixtures of the water-sized workload. The reference ships
+no tests or stored outputs, so no reference-held vector exists to pin against.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def jkj_ri(C: np.ndarray, b_mnl: np.ndarray, nbf5: int):
+    """J^n, K^n with the reference's call sequence (integrals.py:208-218).
+
+    b_qnl[q,nu,P] = sum_mu C[mu,q] b[mu,nu,P]          (tensordot, :211)
+    d[q,P]        = sum_nu C[nu,q] b_qnl[q,nu,P]        (einsum, :212)
+    J[q,mu,nu]    = sum_P d[q,P] b[mu,nu,P]             (tensordot, :214)
+    K[q,mu,nu]    = sum_P b_qnl[q,mu,P] b_qnl[q,nu,P]   (einsum, :216)
+    """
+    Cq = C[:, 0:nbf5]
+    b_qnl = np.tensordot(Cq, b_mnl, axes=([0], [0]))
+    b_qql = np.einsum("nq,qnl->ql", Cq, b_qnl, optimize=True)
+    J = np.tensordot(b_qql, b_mnl, axes=([1], [2]))
+    K = np.einsum("qml,qnl->qmn", b_qnl, b_qnl, optimize=True)
+    return J, K
+
+
+def jkh_mo_ri(C: np.ndarray, H: np.ndarray, b_mnl: np.ndarray, nbf5: int):
+    """J_MO, K_MO, H_core with the reference's call sequence (integrals.py:348-362).
+
+    D[i,mu,nu]   = C[mu,i] C[nu,i]                           (:351)
+    b_pql[p,q,P] = sum_nu C[nu,q] sum_mu C[mu,p] b[mu,nu,P]  (:353-354)
+    J_MO[p,q]    = sum_P b_ppP b_qqP                         (:356)
+    K_MO[p,q]    = sum_P b_pqP^2                             (:358)
+    H_core[i]    = sum_{mu nu} D[i,mu,nu] H[mu,nu]           (:360)
+    """
+    Cq = C[:, 0:nbf5]
+    D = np.einsum("mi,ni->imn", Cq, Cq, optimize=True)
+    b_pnl = np.tensordot(Cq, b_mnl, axes=([0], [0]))
+    b_pql = np.einsum("nq,pnl->pql", Cq, b_pnl, optimize=True)
+    J_MO = np.einsum("ppl,qql->pq", b_pql, b_pql, optimize=True)
+    K_MO = np.einsum("pql,pql->pq", b_pql, b_pql, optimize=True)
+    H_core = np.tensordot(D, H, axes=([1, 2], [0, 1]))
+    return J_MO, K_MO, H_core
+
+
+def jk_build(C, H, b_mnl, nbf5):
+    """One fused build: the outputs of one computeJKj plus one computeJKH_MO call,
+    with the half-transform done once and H_core as diag(C^T H C) (no D tensor).
+    Same mathematics as ``jkj_ri`` + ``jkh_mo_ri``; used where the reference's
+    duplicate work would only cost time."""
+    Cq = np.ascontiguousarray(C[:, 0:nbf5])
+    nbf, _, naux = b_mnl.shape
+    X = (Cq.T @ b_mnl.reshape(nbf, nbf * naux)).reshape(nbf5, nbf, naux)
+    Y = np.einsum("nq,pnl->pql", Cq, X, optimize=True)
+    d = np.ascontiguousarray(np.einsum("ppl->pl", Y))
+    J = (d @ b_mnl.reshape(nbf * nbf, naux).T).reshape(nbf5, nbf, nbf)
+    K = np.matmul(X, X.transpose(0, 2, 1))
+    J_MO = d @ d.T
+    K_MO = np.einsum("pql,pql->pq", Y, Y, optimize=True)
+    H_core = np.einsum("mi,mi->i", Cq, H @ Cq)
+    return J, K, J_MO, K_MO, H_core
+
+
+def jkj_full(C: np.ndarray, I: np.ndarray, nbf5: int):
+    """4-index route (integrals.py:197-205): J = D:I over (2,3), K = D:I over (1,3)."""
+    Cq = C[:, 0:nbf5]
+    D = np.einsum("mi,ni->imn", Cq, Cq, optimize=True)
+    J = np.tensordot(D, I, axes=([1, 2], [2, 3]))
+    K = np.tensordot(D, I, axes=([1, 2], [1, 3]))
+    return J, K
+
+
+def jkh_mo_full(C: np.ndarray, H: np.ndarray, I: np.ndarray, nbf5: int):
+    """4-index route for the MO quantities (integrals.py:332-345)."""
+    Cq = C[:, 0:nbf5]
+    D = np.einsum("mi,ni->imn", Cq, Cq, optimize=True)
+    J = np.tensordot(D, I, axes=([1, 2], [2, 3]))
+    J_MO = np.tensordot(J, D, axes=((1, 2), (1, 2)))
+    K = np.tensordot(D, I, axes=([1, 2], [1, 3]))
+    K_MO = np.tensordot(K, D, axes=([1, 2], [1, 2]))
+    H_core = np.tensordot(D, H, axes=([1, 2], [0, 1]))
+    return J_MO, K_MO, H_core
+
+
+def four_index(b_mnl: np.ndarray) -> np.ndarray:
+    """I[mu,nu,lam,sig] = sum_P b[mu,nu,P] b[lam,sig,P] (the RI factorisation, integrals.py:30-49)."""
+    nbf, _, naux = b_mnl.shape
+    B = b_mnl.reshape(nbf * nbf, naux)
+    return (B @ B.T).reshape(nbf, nbf, nbf, nbf)
+
+
+def jkj_loops(C, b_mnl, nbf5):
+    """Explicit loops with the accumulation order of JKj_RI_jit (integrals.py:282-301).
+    Pure Python: tiny inputs only."""
+    nbf, _, naux = b_mnl.shape
+    J = np.zeros((nbf5, nbf, nbf))
+    K = np.zeros((nbf5, nbf, nbf))
+    for q in range(nbf5):
+        bq = np.zeros((nbf, naux))
+        dq = np.zeros(naux)
+        for l in range(naux):
+            for n in range(nbf):
+                s = 0.0
+                for m in range(nbf):
+                    s += C[m, q] * b_mnl[m, n, l]
+                bq[n, l] = s
+                dq[l] += C[n, q] * bq[n, l]
+            for n in range(nbf):
+                for m in range(nbf):
+                    K[q, m, n] += bq[m, l] * bq[n, l]
+                    J[q, m, n] += dq[l] * b_mnl[m, n, l]
+    return J, K
+
+
+def jkh_mo_loops(C, H, b_mnl, nbf5):
+    """Explicit loops following JKH_MO_RI_jit (integrals.py:457-484), full q range."""
+    nbf, _, naux = b_mnl.shape
+    bpq = np.zeros((nbf5, nbf5, naux))
+    for p in range(nbf5):
+        for n in range(nbf):
+            for l in range(naux):
+                s = 0.0
+                for m in range(nbf):
+                    s += C[m, p] * b_mnl[m, n, l]
+                for q in range(nbf5):
+                    bpq[p, q, l] += C[n, q] * s
+    H_core = np.zeros(nbf5)
+    J_MO = np.zeros((nbf5, nbf5))
+    K_MO = np.zeros((nbf5, nbf5))
+    for p in range(nbf5):
+        for m in range(nbf):
+            for n in range(nbf):
+                H_core[p] += C[m, p] * H[m, n] * C[n, p]
+        for q in range(nbf5):
+            for l in range(naux):
+                J_MO[p, q] += bpq[p, p, l] * bpq[q, q, l]
+                K_MO[p, q] += bpq[p, q, l] * bpq[p, q, l]
+    return J_MO, K_MO, H_core
+
+
+def energy(n, J_MO, K_MO, H_core):
+    """E_1 + E_2 with E_1 = 2 sum n H_core and E_2 = sum_pq n_p n_q (2J_pq - K_pq)."""
+    e1 = 2.0 * float(np.dot(n, H_core))
+    e2 = float(n @ (2.0 * J_MO - K_MO) @ n)
+    return e1 + e2
+
+
+def calce_closed_shell(n, cj12, ck12, J_MO, K_MO, H_core, nbeta, nalpha, nbf5):
+    """Closed-shell (MSpin = 0) energy of pnof.py:905-922, for given C^J, C^K."""
+    cj = np.array(cj12, dtype=np.float64, copy=True)
+    ck = np.array(ck12, dtype=np.float64, copy=True)
+    E = 2.0 * np.einsum("i,i", n, H_core)
+    dj = np.diagonal(J_MO)
+    E += np.einsum("i,i", n[:nbeta], dj[:nbeta])
+    E += np.einsum("i,i", n[nalpha:nbf5], dj[nalpha:nbf5])
+    np.fill_diagonal(cj, 0.0)
+    E += np.einsum("ij,ji->", cj, J_MO)
+    np.fill_diagonal(ck, 0.0)
+    E -= np.einsum("ij,ji->", ck, K_MO)
+    return float(E)

@@ -1,0 +1,12 @@
+"""pynof_amd: the density-fitted JK build of PyNOF on AMD MI355X (gfx950).
+
+Entry points:
+  computeJKj, computeJKH_MO   drop-in replacements of PyNOF's RI JK builds
+  JKEngine                    device-resident b and fused builds
+"""
+from ._lib import PnjkError  # noqa: F401
+from .engine import JKEngine  # noqa: F401
+from .integrals import computeJKj, computeJKH_MO  # noqa: F401
+from .parameters import param  # noqa: F401
+
+__all__ = ["JKEngine", "computeJKj", "computeJKH_MO", "param", "PnjkError"]

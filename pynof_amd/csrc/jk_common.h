@@ -1,0 +1,50 @@
+// Shared definitions for the MI355X (gfx950) RI JK build kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+namespace pnjk {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// Records a message as the calling thread's last error; returns `code`.
+int set_error(int code, const std::string& msg);
+
+#define PNJK_HIP_CHECK(expr)                                                        \
+  do {                                                                              \
+    hipError_t _e = (expr);                                                         \
+    if (_e != hipSuccess)                                                           \
+      return ::pnjk::set_error(2, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+// Workgroup -> XCD placement on MI355X: consecutive workgroup ids are dealt
+// round-robin over the 8 XCDs. This maps a workgroup id to a logical tile id so
+// that each XCD receives one contiguous run of logical tiles, keeping tiles that
+// share an operand in the same L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+  constexpr int kXcd = 8;
+  const int per = nblocks / kXcd, rem = nblocks % kXcd;
+  const int x = bid % kXcd, i = bid / kXcd;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
+// Row index of the packed lower triangle: t = mu*(mu+1)/2 + nu for mu >= nu.
+__device__ __forceinline__ int64_t tri_index(int a, int b) {
+  const int64_t hi = a > b ? a : b, lo = a > b ? b : a;
+  return hi * (hi + 1) / 2 + lo;
+}
+
+// Inverse of tri_index for t < 2^40: returns mu (row), nu = t - mu*(mu+1)/2.
+__device__ __forceinline__ int tri_row(int64_t t) {
+  int mu = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((int64_t)(mu + 1) * (mu + 2) / 2 <= t) ++mu;
+  while ((int64_t)mu * (mu + 1) / 2 > t) --mu;
+  return mu;
+}
+
+}  // namespace pnjk

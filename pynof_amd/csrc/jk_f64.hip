@@ -1,0 +1,627 @@
+// FP64 kernels of the RI JK build for gfx950 (MI355X).
+//
+// Every dense contraction of the build is one instantiation of a single
+// LDS-staged GEMM core on v_mfma_f64_16x16x4f64. A workgroup of 4 waves owns a
+// BM x BN output tile (2 x 2 waves, each BM/2 x BN/2 as 16x16 MFMA blocks);
+// operand tiles are staged through double-buffered LDS in k-major layout with
+// a register prefetch of the next k-slice. Each contraction is a "problem"
+// struct that supplies the tile schedule, the two operand sources and the
+// epilogue; the contractions and their reference lines are:
+//
+//   HT   X[q][nu,P]  = sum_mu C[mu,q] b[mu,nu,P]          integrals.py:211, :353
+//   QT   Y_p[q][P]   = sum_nu C[nu,q] X[p][nu][P]         integrals.py:354
+//        -> d[p][P] = Y_p[p][P] (integrals.py:212), K_MO row sums (integrals.py:358)
+//   SYRK K[q] = X[q] X[q]^T                               integrals.py:216
+//        J_MO = d d^T                                     integrals.py:356
+//   JN   J[q][mu,nu] = sum_P d[q][P] b[mu,nu,P]           integrals.py:214
+//   HC   H_core[i] = sum C[mu,i] H[mu,nu] C[nu,i]         integrals.py:360
+//
+// b is stored once, as the packed lower triangle btri[tri(mu,nu)][P] (b is
+// symmetric in mu,nu, integrals.py:35-49), halving its HBM footprint; the
+// symmetric outputs are computed on one triangle of tiles and mirrored.
+#include "jk_common.h"
+#include "jk_kernels.h"
+
+namespace pnjk {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kPad = 2;  // LDS row padding (doubles); keeps 16-byte alignment
+
+enum Kind { KM = 0, MK = 1 };
+
+// Global -> register -> LDS loader for one operand tile of ROWS x BK.
+// LDS layout is k-major: lds[k * (ROWS + kPad) + r].
+template <int KIND, int ROWS, int BK>
+struct Loader;
+
+// Source element (k, r) at src.at(k, r); r is contiguous in memory (pairs).
+template <int ROWS, int BK>
+struct Loader<KM, ROWS, BK> {
+  static constexpr int kChunks = (ROWS / 2) * BK;
+  static constexpr int kPer = (kChunks + kThreads - 1) / kThreads;
+  v2d v[kPer];
+  template <class S>
+  __device__ __forceinline__ void load(const S& s, int r0, int k0) {
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int c = threadIdx.x + i * kThreads;
+      if (kChunks % kThreads == 0 || c < kChunks) {
+        const int k = c / (ROWS / 2), rp = (c % (ROWS / 2)) * 2;
+        v[i] = *reinterpret_cast<const v2d*>(s.at(k0 + k, r0 + rp));
+      }
+    }
+  }
+  __device__ __forceinline__ void store(double* lds) const {
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int c = threadIdx.x + i * kThreads;
+      if (kChunks % kThreads == 0 || c < kChunks) {
+        const int k = c / (ROWS / 2), rp = (c % (ROWS / 2)) * 2;
+        *reinterpret_cast<v2d*>(lds + k * (ROWS + kPad) + rp) = v[i];
+      }
+    }
+  }
+};
+
+// Source element (r, k) at src.row(r)[k]; k is contiguous in memory.
+template <int ROWS, int BK>
+struct Loader<MK, ROWS, BK> {
+  static constexpr int kChunks = ROWS * (BK / 2);
+  static constexpr int kPer = (kChunks + kThreads - 1) / kThreads;
+  v2d v[kPer];
+  template <class S>
+  __device__ __forceinline__ void load(const S& s, int r0, int k0) {
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int c = threadIdx.x + i * kThreads;
+      if (kChunks % kThreads == 0 || c < kChunks) {
+        const int r = c % ROWS, kp = (c / ROWS) * 2;
+        v[i] = *reinterpret_cast<const v2d*>(s.row(r0 + r) + k0 + kp);
+      }
+    }
+  }
+  __device__ __forceinline__ void store(double* lds) const {
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int c = threadIdx.x + i * kThreads;
+      if (kChunks % kThreads == 0 || c < kChunks) {
+        const int r = c % ROWS, kp = (c / ROWS) * 2;
+        lds[kp * (ROWS + kPad) + r] = v[i].x;
+        lds[(kp + 1) * (ROWS + kPad) + r] = v[i].y;
+      }
+    }
+  }
+};
+
+struct Tile {
+  int batch, mt, nt, m0, n0;
+};
+
+// The GEMM core. Accumulator element (fm, fn, r) of a lane is output element
+//   m = m0 + wm + 16*fm + 4*(lane/16) + r,  n = n0 + wn + 16*fn + lane%16
+// (the v_mfma_f64_16x16x4f64 result layout).
+template <int FM, int FN, int BK, class Prob>
+__global__ void __launch_bounds__(kThreads) k_gemm_f64(const Prob prob) {
+  constexpr int BM = 32 * FM, BN = 32 * FN;
+  __shared__ __align__(16) double sA[2][BK * (BM + kPad)];
+  __shared__ __align__(16) double sB[2][BK * (BN + kPad)];
+
+  Tile t;
+  if (!prob.tile(xcd_remap(blockIdx.x, gridDim.x), BM, BN, t)) return;
+  const auto as = prob.a_src(t);
+  const auto bs = prob.b_src(t);
+  const int nk = prob.ksteps(BK);
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
+
+  Loader<Prob::kA, BM, BK> la;
+  Loader<Prob::kB, BN, BK> lb;
+  la.load(as, t.m0, 0);
+  lb.load(bs, t.n0, 0);
+  la.store(sA[0]);
+  lb.store(sB[0]);
+  __syncthreads();
+
+  v4d acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) {
+      la.load(as, t.m0, (kt + 1) * BK);
+      lb.load(bs, t.n0, (kt + 1) * BK);
+    }
+    const double* a_l = sA[cur] + wm + (lane & 15);
+    const double* b_l = sB[cur] + wn + (lane & 15);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      double a[FM], b[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) a[i] = a_l[kr * (BM + kPad) + 16 * i];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) b[j] = b_l[kr * (BN + kPad) + 16 * j];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) {
+      la.store(sA[cur ^ 1]);
+      lb.store(sB[cur ^ 1]);
+    }
+    __syncthreads();
+  }
+  prob.template epilogue<FM, FN>(t, acc, wm, wn, lane, &sA[0][0]);
+}
+
+// ---------------------------------------------------------------- sources
+struct SrcKMDense {  // element (k, r) at base[k*ld + r]
+  const double* base;
+  int64_t ld;
+  __device__ __forceinline__ const double* at(int k, int r) const { return base + (int64_t)k * ld + r; }
+};
+
+struct SrcMKDense {  // element (r, k) at base[min(r, rmax-1)*ld + k]
+  const double* base;
+  int64_t ld;
+  int rmax;
+  __device__ __forceinline__ const double* row(int r) const {
+    return base + (int64_t)(r < rmax ? r : rmax - 1) * ld;
+  }
+};
+
+struct SrcKMRows {  // element (k, r) at base[min(k, kmax-1)*ld + r]
+  const double* base;
+  int64_t ld;
+  int kmax;
+  __device__ __forceinline__ const double* at(int k, int r) const {
+    return base + (int64_t)(k < kmax ? k : kmax - 1) * ld + r;
+  }
+};
+
+// b as a K x N operand of the half-transform: element (mu, n), n = nu*naux_p + P,
+// at btri[tri(mu, nu)][P]. Rows past nbf are clamped (their C weights are zero).
+struct SrcKMBtri {
+  const double* btri;
+  int nbf, naux_p;
+  __device__ __forceinline__ const double* at(int k, int n) const {
+    int nu = n / naux_p;
+    const int p = n - nu * naux_p;
+    nu = nu < nbf ? nu : nbf - 1;
+    const int mu = k < nbf ? k : nbf - 1;
+    return btri + tri_index(mu, nu) * naux_p + p;
+  }
+};
+
+// ---------------------------------------------------------------- problems
+template <int FM, int FN>
+__device__ __forceinline__ void elem_coords(int lane, int wm, int wn, int i, int j, int r, int& dm, int& dn) {
+  dm = wm + 16 * i + 4 * (lane >> 4) + r;
+  dn = wn + 16 * j + (lane & 15);
+}
+
+struct ProbHT {
+  static constexpr int kA = KM, kB = KM;
+  const double* Cp;
+  const double* btri;
+  double* X;
+  int nbf, naux_p, nbf5, ldcp, nbf_p, mtiles;
+  int64_t N;
+  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.batch = 0;
+    t.mt = id % mtiles;
+    t.nt = id / mtiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ int ksteps(int BK) const { return nbf_p / BK; }
+  __device__ SrcKMDense a_src(const Tile&) const { return {Cp, ldcp}; }
+  __device__ SrcKMBtri b_src(const Tile&) const { return {btri, nbf, naux_p}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4d (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int q = t.m0 + dm;
+          const int64_t n = (int64_t)t.n0 + dn;
+          if (q < nbf5 && n < N) X[(int64_t)q * N + n] = acc[i][j][r];
+        }
+  }
+};
+
+struct ProbQT {
+  static constexpr int kA = KM, kB = KM;
+  const double* Cp;
+  const double* X;
+  double* kpart;
+  double* d;
+  int nbf, naux_p, nbf5, ldcp, nbf_p, mtiles, ntiles;
+  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.mt = id % mtiles;
+    const int rest = id / mtiles;
+    t.nt = rest % ntiles;
+    t.batch = rest / ntiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return t.m0 <= t.batch;  // only q <= p is needed (Y_p[q] = Y_q[p])
+  }
+  __device__ int ksteps(int BK) const { return nbf_p / BK; }
+  __device__ SrcKMDense a_src(const Tile&) const { return {Cp, ldcp}; }
+  __device__ SrcKMRows b_src(const Tile& t) const {
+    return {X + (int64_t)t.batch * nbf * naux_p, naux_p, nbf};
+  }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4d (&acc)[FM][FN], int wm, int wn, int lane, double* lds) const {
+    const int p = t.batch;
+    // d[p][P] = Y_p[p][P]
+    if (p >= t.m0 && p < t.m0 + 32 * FM) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            int dm, dn;
+            elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+            if (t.m0 + dm == p) d[(int64_t)p * naux_p + t.n0 + dn] = acc[i][j][r];
+          }
+    }
+    // Row sums of squares over this P-tile, reduced over the 16 column lanes,
+    // then over the two column waves through LDS (fixed order: deterministic).
+    double s[FM][4];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < FN; ++j) v += acc[i][j][r] * acc[i][j][r];
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
+        s[i][r] = v;
+      }
+    const int wave = threadIdx.x >> 6;
+    if ((wave & 1) && (lane & 15) == 0) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) lds[wm + 16 * i + 4 * (lane >> 4) + r] = s[i][r];
+    }
+    __syncthreads();
+    if (!(wave & 1) && (lane & 15) == 0) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int dm = wm + 16 * i + 4 * (lane >> 4) + r;
+          const int q = t.m0 + dm;
+          if (q <= p && q < nbf5)
+            kpart[((int64_t)p * nbf5 + q) * ntiles + t.nt] = s[i][r] + lds[dm];
+        }
+    }
+  }
+};
+
+struct ProbSYRK {
+  static constexpr int kA = MK, kB = MK;
+  const double* A;
+  double* O;
+  int64_t lda, sa, ldo, so;
+  int M, K, tiles_side, tiles_per_batch;
+  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.batch = id / tiles_per_batch;
+    const int rem = id - t.batch * tiles_per_batch;
+    int mt = (int)((sqrt(8.0 * rem + 1.0) - 1.0) * 0.5);
+    while ((mt + 1) * (mt + 2) / 2 <= rem) ++mt;
+    while (mt * (mt + 1) / 2 > rem) --mt;
+    t.mt = mt;
+    t.nt = rem - mt * (mt + 1) / 2;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ int ksteps(int BK) const { return K / BK; }
+  __device__ SrcMKDense a_src(const Tile& t) const { return {A + (int64_t)t.batch * sa, lda, M}; }
+  __device__ SrcMKDense b_src(const Tile& t) const { return {A + (int64_t)t.batch * sa, lda, M}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4d (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+    double* o = O + (int64_t)t.batch * so;
+    const bool mirror = t.mt != t.nt;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int m = t.m0 + dm, n = t.n0 + dn;
+          if (m < M && n < M) {
+            o[(int64_t)m * ldo + n] = acc[i][j][r];
+            if (mirror) o[(int64_t)n * ldo + m] = acc[i][j][r];
+          }
+        }
+  }
+};
+
+struct ProbJN {
+  static constexpr int kA = MK, kB = MK;
+  const double* d;
+  const double* btri;
+  double* J;
+  int nbf, naux_p, nbf5, mtiles;
+  int64_t ntri;
+  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.batch = 0;
+    t.mt = id % mtiles;
+    t.nt = id / mtiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ int ksteps(int BK) const { return naux_p / BK; }
+  __device__ SrcMKDense a_src(const Tile&) const { return {d, naux_p, nbf5}; }
+  __device__ SrcMKDense b_src(const Tile&) const { return {btri, naux_p, (int)(ntri < (1 << 30) ? ntri : (1 << 30))}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4d (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+    const int64_t nn = (int64_t)nbf * nbf;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      int dm, dn;
+      elem_coords<FM, FN>(lane, wm, wn, 0, j, 0, dm, dn);
+      const int64_t tt = (int64_t)t.n0 + dn;
+      if (tt >= ntri) continue;
+      const int mu = tri_row(tt);
+      const int nu = (int)(tt - (int64_t)mu * (mu + 1) / 2);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int q = t.m0 + dm;
+          if (q < nbf5) {
+            double* jq = J + (int64_t)q * nn;
+            jq[(int64_t)mu * nbf + nu] = acc[i][j][r];
+            if (mu != nu) jq[(int64_t)nu * nbf + mu] = acc[i][j][r];
+          }
+        }
+    }
+  }
+};
+
+struct ProbGemmT {  // T[mu][i] = sum_nu Hp[mu][nu] Cp[nu][i]
+  static constexpr int kA = MK, kB = KM;
+  const double* Hp;
+  const double* Cp;
+  double* T;
+  int ldh, ldcp, nbf, nbf_p, mtiles;
+  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.batch = 0;
+    t.mt = id % mtiles;
+    t.nt = id / mtiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ int ksteps(int BK) const { return nbf_p / BK; }
+  __device__ SrcMKDense a_src(const Tile&) const { return {Hp, ldh, nbf}; }
+  __device__ SrcKMDense b_src(const Tile&) const { return {Cp, ldcp}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4d (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int m = t.m0 + dm, n = t.n0 + dn;
+          if (m < nbf && n < ldcp) T[(int64_t)m * ldcp + n] = acc[i][j][r];
+        }
+  }
+};
+
+// ---------------------------------------------------------------- small kernels
+__global__ void k_kmo_reduce(int nbf5, int nt, const double* __restrict__ kpart, double* __restrict__ kmo) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)nbf5 * nbf5) return;
+  const int p = (int)(idx / nbf5), q = (int)(idx % nbf5);
+  if (q > p) return;
+  const double* src = kpart + ((int64_t)p * nbf5 + q) * nt;
+  double s = 0.0;
+  for (int t = 0; t < nt; ++t) s += src[t];
+  kmo[(int64_t)p * nbf5 + q] = s;
+  kmo[(int64_t)q * nbf5 + p] = s;
+}
+
+__global__ void k_hcore(int nbf, int nbf5, int ldcp, const double* __restrict__ Cp, const double* __restrict__ T,
+                        double* __restrict__ hcore) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbf5) return;
+  double s = 0.0;
+  for (int mu = 0; mu < nbf; ++mu) s += Cp[(int64_t)mu * ldcp + i] * T[(int64_t)mu * ldcp + i];
+  hcore[i] = s;
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+// Packs rows mu in [mu0, mu1) of a full b (bc = b[mu0:mu1, :, :]) into btri.
+// One workgroup per (mu, nu) pair, looping over P. Pairs nu > mu are only
+// checked against their mirror: inside the chunk when nu < mu1, otherwise
+// against the packed row tri(nu, mu), which a chunk of higher mu has already
+// written (chunks are packed in descending mu order on one stream).
+__global__ void k_pack_b(int nbf, int naux, int naux_p, int mu0, int mu1, const double* __restrict__ bc,
+                         double* __restrict__ btri, double* asym_max, unsigned long long* nonfinite) {
+  const int64_t row = blockIdx.x;  // (mu - mu0) * nbf + nu
+  const int mu = mu0 + (int)(row / nbf), nu = (int)(row % nbf);
+  const double* src = bc + row * naux;
+  double amax = 0.0;
+  unsigned long long bad = 0;
+  if (nu > mu) {
+    const double* mir = nu < mu1 ? bc + ((int64_t)(nu - mu0) * nbf + mu) * naux : btri + tri_index(nu, mu) * naux_p;
+    for (int p = threadIdx.x; p < naux; p += blockDim.x) {
+      const double v = src[p];
+      if (!isfinite(v)) ++bad;
+      const double a = fabs(v - mir[p]);
+      amax = a > amax ? a : amax;
+    }
+  } else {
+    double* dst = btri + tri_index(mu, nu) * naux_p;
+    for (int p = threadIdx.x; p < naux_p; p += blockDim.x) {
+      double v = 0.0;
+      if (p < naux) {
+        v = src[p];
+        if (!isfinite(v)) ++bad;
+      }
+      dst[p] = v;
+    }
+  }
+  if (bad) atomicAdd(nonfinite, bad);
+  if (amax > 0.0) atomic_max_nonneg(asym_max, amax);
+}
+
+template <class Prob>
+hipError_t launch_prob(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  const dim3 grid(nblocks), block(kThreads);
+#define PNJK_CASE(FMV, FNV, BKV)                                                  \
+  if (tc.fm == FMV && tc.fn == FNV) {                                             \
+    hipLaunchKernelGGL((k_gemm_f64<FMV, FNV, BKV, Prob>), grid, block, 0, s, prob); \
+    return hipGetLastError();                                                     \
+  }
+  PNJK_CASE(2, 2, 16)
+  PNJK_CASE(4, 4, 8)
+  PNJK_CASE(5, 4, 8)
+  PNJK_CASE(4, 2, 16)
+  PNJK_CASE(2, 4, 16)
+#undef PNJK_CASE
+  return hipErrorInvalidValue;
+}
+
+inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+}  // namespace
+
+hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* btri, double* X,
+                         hipStream_t s) {
+  ProbHT pr;
+  pr.Cp = Cp;
+  pr.btri = btri;
+  pr.X = X;
+  pr.nbf = g.nbf;
+  pr.naux_p = g.naux_p;
+  pr.nbf5 = g.nbf5;
+  pr.ldcp = g.ldcp;
+  pr.nbf_p = g.nbf_p;
+  pr.N = (int64_t)g.nbf * g.naux_p;
+  const int BM = 32 * tc.fm, BN = 32 * tc.fn;
+  pr.mtiles = cdiv(g.nbf5, BM);
+  return launch_prob(pr, tc, pr.mtiles * cdiv(pr.N, BN), s);
+}
+
+hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* X, double* kpart,
+                         double* d, int* ntiles, hipStream_t s) {
+  const int BM = 32 * tc.fm, BN = 32 * tc.fn;
+  if (g.naux_p % BN != 0) return hipErrorInvalidValue;
+  ProbQT pr;
+  pr.Cp = Cp;
+  pr.X = X;
+  pr.kpart = kpart;
+  pr.d = d;
+  pr.nbf = g.nbf;
+  pr.naux_p = g.naux_p;
+  pr.nbf5 = g.nbf5;
+  pr.ldcp = g.ldcp;
+  pr.nbf_p = g.nbf_p;
+  pr.mtiles = cdiv(g.nbf5, BM);
+  pr.ntiles = g.naux_p / BN;
+  *ntiles = pr.ntiles;
+  return launch_prob(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s);
+}
+
+hipError_t launch_kmo_reduce(int nbf5, int nt, const double* kpart, double* kmo, hipStream_t s) {
+  const int64_t n = (int64_t)nbf5 * nbf5;
+  hipLaunchKernelGGL(k_kmo_reduce, dim3(cdiv(n, 256)), dim3(256), 0, s, nbf5, nt, kpart, kmo);
+  return hipGetLastError();
+}
+
+hipError_t launch_syrk_f64(int batch, int M, int K, const double* A, int64_t lda, int64_t sa, double* O,
+                           int64_t ldo, int64_t so, TileF64 tc, hipStream_t s) {
+  if (tc.fm != tc.fn) return hipErrorInvalidValue;
+  const int BM = 32 * tc.fm;
+  const int BK = tc.fm == 2 ? 16 : 8;
+  if (K % BK != 0) return hipErrorInvalidValue;
+  ProbSYRK pr;
+  pr.A = A;
+  pr.O = O;
+  pr.lda = lda;
+  pr.sa = sa;
+  pr.ldo = ldo;
+  pr.so = so;
+  pr.M = M;
+  pr.K = K;
+  pr.tiles_side = cdiv(M, BM);
+  pr.tiles_per_batch = pr.tiles_side * (pr.tiles_side + 1) / 2;
+  return launch_prob(pr, tc, pr.tiles_per_batch * batch, s);
+}
+
+hipError_t launch_jn_f64(const BuildGeom& g, TileF64 tc, const double* d, const double* btri, double* J,
+                         hipStream_t s) {
+  const int BM = 32 * tc.fm, BN = 32 * tc.fn;
+  ProbJN pr;
+  pr.d = d;
+  pr.btri = btri;
+  pr.J = J;
+  pr.nbf = g.nbf;
+  pr.naux_p = g.naux_p;
+  pr.nbf5 = g.nbf5;
+  pr.ntri = g.ntri;
+  pr.mtiles = cdiv(g.nbf5, BM);
+  return launch_prob(pr, tc, pr.mtiles * cdiv(g.ntri, BN), s);
+}
+
+hipError_t launch_hcore_f64(const BuildGeom& g, const double* Hp, int ldh, const double* Cp, double* T,
+                            double* hcore, hipStream_t s) {
+  const TileF64 tc{2, 2};
+  ProbGemmT pr;
+  pr.Hp = Hp;
+  pr.Cp = Cp;
+  pr.T = T;
+  pr.ldh = ldh;
+  pr.ldcp = g.ldcp;
+  pr.nbf = g.nbf;
+  pr.nbf_p = g.nbf_p;
+  pr.mtiles = cdiv(g.nbf, 64);
+  hipError_t e = launch_prob(pr, tc, pr.mtiles * cdiv(g.ldcp, 64), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_hcore, dim3(cdiv(g.nbf5, 256)), dim3(256), 0, s, g.nbf, g.nbf5, g.ldcp, Cp, T, hcore);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_b(int nbf, int naux, int naux_p, int mu0, int mu1, const double* b_chunk, double* btri,
+                         double* asym_max, unsigned long long* nonfinite, hipStream_t s) {
+  const int64_t rows = (int64_t)(mu1 - mu0) * nbf;
+  if (rows <= 0) return hipSuccess;
+  if (rows > 2147483647LL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_pack_b, dim3((unsigned)rows), dim3(256), 0, s, nbf, naux, naux_p, mu0, mu1, b_chunk, btri,
+                     asym_max, nonfinite);
+  return hipGetLastError();
+}
+
+}  // namespace pnjk

@@ -1,0 +1,61 @@
+// Host-side launch interface of the JK build kernels (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace pnjk {
+
+// Tile shape of one FP64 MFMA GEMM launch: BM = 32*fm rows, BN = 32*fn columns.
+struct TileF64 {
+  int fm;
+  int fn;
+};
+
+// Geometry of one build. Row strides are in elements.
+struct BuildGeom {
+  int nbf;        // basis functions
+  int naux;       // auxiliary functions
+  int naux_p;     // padded aux stride (multiple of 64), zero-filled tail
+  int nbf5;       // active orbitals
+  int64_t ntri;   // nbf*(nbf+1)/2 packed pair rows
+  int ldcp;       // row stride of the padded C block Cp[nbf_p][ldcp]; the
+                  // allocation carries slack rows so whole tiles stay in bounds
+  int nbf_p;      // rows of Cp / columns of Hp (multiple of 32), zero-filled tail
+};
+
+// X[q][nu*naux_p + P] = sum_mu Cp[mu][q] b_tri[tri(mu,nu)][P]   (half-transform)
+hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* btri, double* X,
+                         hipStream_t s);
+
+// Y_p[q][P] = sum_nu Cp[nu][q] X[p][nu][P] for q <= p; writes row sums of Y_p^2
+// per P-tile into kpart[(p*nbf5+q)*nt + tile] and d[p][P] = Y_p[p][P].
+// Returns the number of P-tiles in *ntiles.
+hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* X, double* kpart,
+                         double* d, int* ntiles, hipStream_t s);
+
+// K_MO[p][q] = K_MO[q][p] = sum_t kpart[(p*nbf5+q)*nt + t]  (q <= p)
+hipError_t launch_kmo_reduce(int nbf5, int nt, const double* kpart, double* kmo, hipStream_t s);
+
+// O[b][i][j] = sum_k A[b][i][k] A[b][j][k] for i, j < M; full symmetric output.
+hipError_t launch_syrk_f64(int batch, int M, int K, const double* A, int64_t lda, int64_t sa, double* O,
+                           int64_t ldo, int64_t so, TileF64 tc, hipStream_t s);
+
+// J[q][mu][nu] = sum_P d[q][P] b_tri[tri(mu,nu)][P], full symmetric (nbf5, nbf, nbf) output.
+hipError_t launch_jn_f64(const BuildGeom& g, TileF64 tc, const double* d, const double* btri, double* J,
+                         hipStream_t s);
+
+// T[mu][i] = sum_nu Hp[mu][nu] Cp[nu][i] (mu < nbf, i < ldcp), then
+// hcore[i] = sum_mu Cp[mu][i] T[mu][i] for i < nbf5.
+hipError_t launch_hcore_f64(const BuildGeom& g, const double* Hp, int ldh, const double* Cp, double* T,
+                            double* hcore, hipStream_t s);
+
+// Packs rows mu in [mu0, mu1) of a full (nbf, nbf, naux) b (b_chunk holds
+// b[mu0:mu1, :, :]) into btri rows t = tri(mu, nu), nu <= mu, stride naux_p with
+// a zero tail. Chunks must be packed in descending mu order on one stream.
+// Folds max |b[mu,nu,P] - b[nu,mu,P]| into *asym_max and counts non-finite
+// values into *nonfinite (both device scalars).
+hipError_t launch_pack_b(int nbf, int naux, int naux_p, int mu0, int mu1, const double* b_chunk, double* btri,
+                         double* asym_max, unsigned long long* nonfinite, hipStream_t s);
+
+}  // namespace pnjk

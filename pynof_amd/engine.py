@@ -1,0 +1,161 @@
+"""Host-side engine: one device-resident b tensor and the builds against it.
+
+``JKEngine`` wraps one ``pnjk_engine`` of the C ABI. b is uploaded once (the
+reference's ``cp.array(b_mnl)``, integrals.py:51-53) and stays on the device in
+the packed lower-triangle layout; each ``build`` call runs the fused JK build
+(half-transform once, integrals.py:211 and :353) for one set of orbitals C.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _ptr(x) -> int:
+    if _is_torch(x):
+        return x.data_ptr()
+    return x.ctypes.data
+
+
+def _as_input(x, name: str):
+    """C-contiguous FP64 view of a host (numpy) or device (torch) array."""
+    if _is_torch(x):
+        if x.dtype != _torch().float64:
+            raise TypeError(f"{name} must be float64, got {x.dtype}")
+        return x.contiguous()
+    a = np.asarray(x)
+    if a.dtype != np.float64:
+        raise TypeError(f"{name} must be float64, got {a.dtype}")
+    return np.ascontiguousarray(a)
+
+
+OUTPUT_BITS = {"J": _lib.OUT_J, "K": _lib.OUT_K, "J_MO": _lib.OUT_JMO, "K_MO": _lib.OUT_KMO,
+               "H_core": _lib.OUT_HCORE}
+
+
+class JKEngine:
+    """A device-resident b tensor and the JK builds that read it."""
+
+    def __init__(self, device: int = 0):
+        L = _lib.lib()
+        h = ctypes.c_void_p()
+        _lib.check(L.pnjk_create(int(device), ctypes.byref(h)))
+        self._h = h
+        self.device = int(device)
+        self.nbf = 0
+        self.naux = 0
+        self.asym = None
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().pnjk_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def footprint(self) -> int:
+        return int(_lib.lib().pnjk_footprint(self._h))
+
+    @staticmethod
+    def packed_ld(nbf: int, naux: int) -> int:
+        return int(_lib.lib().pnjk_packed_ld(int(nbf), int(naux)))
+
+    def upload_b(self, b_mnl, sym_tol: float = 1e-10) -> int:
+        """Uploads b (nbf, nbf, naux) from host or device memory; returns the footprint."""
+        b = _as_input(b_mnl, "b_mnl")
+        if b.ndim != 3 or b.shape[0] != b.shape[1]:
+            raise ValueError(f"b_mnl must have shape (nbf, nbf, naux); got {tuple(b.shape)}")
+        nbf, _, naux = (int(s) for s in b.shape)
+        fp = ctypes.c_uint64()
+        asym = ctypes.c_double()
+        if _is_torch(b):
+            _torch().cuda.current_stream(b.device).synchronize()
+        _lib.check(_lib.lib().pnjk_upload_b(self._h, _ptr(b), nbf, naux, float(sym_tol), ctypes.byref(fp),
+                                            ctypes.byref(asym)))
+        self.nbf, self.naux, self.asym = nbf, naux, asym.value
+        return int(fp.value)
+
+    def upload_b_packed(self, btri, nbf: int, naux: int) -> int:
+        """Adopts (copies) a device tensor already in the packed layout."""
+        ld = self.packed_ld(nbf, naux)
+        if tuple(btri.shape) != (nbf * (nbf + 1) // 2, ld):
+            raise ValueError(f"packed b must have shape {(nbf * (nbf + 1) // 2, ld)}; got {tuple(btri.shape)}")
+        _torch().cuda.current_stream(btri.device).synchronize()
+        fp = ctypes.c_uint64()
+        _lib.check(_lib.lib().pnjk_upload_b_packed(self._h, btri.data_ptr(), nbf, naux, ld, ctypes.byref(fp)))
+        self.nbf, self.naux = nbf, naux
+        return int(fp.value)
+
+    def build(self, C, nbf5: int, H=None, precision: str = "fp64",
+              outputs=("J", "K", "J_MO", "K_MO", "H_core"), out=None, stream=None):
+        """One fused JK build.
+
+        C: (nbf, >= nbf5) orbitals, host numpy or device torch, FP64.
+        H: (nbf, nbf) core Hamiltonian, needed for H_core.
+        out: optional dict name -> preallocated array (numpy or device torch) to
+        write into; missing outputs are allocated as numpy (synchronous call) or,
+        when C is a device tensor, as device tensors (asynchronous on `stream`).
+        Returns a dict name -> array.
+        """
+        if self.nbf == 0:
+            raise _lib.PnjkError(_lib.ESTATE, "no b uploaded")
+        prec = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        nbf5 = int(nbf5)
+        Ci = _as_input(C, "C")
+        if Ci.ndim != 2 or Ci.shape[0] != self.nbf or Ci.shape[1] < nbf5:
+            raise ValueError(f"C must have shape ({self.nbf}, >= {nbf5}); got {tuple(Ci.shape)}")
+        want = list(outputs)
+        mask = 0
+        for name in want:
+            mask |= OUTPUT_BITS[name]
+        Hi = None
+        if "H_core" in want:
+            if H is None:
+                raise ValueError("H_core requested but H is None")
+            Hi = _as_input(H, "H")
+            if tuple(Hi.shape) != (self.nbf, self.nbf):
+                raise ValueError(f"H must have shape ({self.nbf}, {self.nbf}); got {tuple(Hi.shape)}")
+        on_device = _is_torch(Ci) and Ci.is_cuda
+        shapes = {"J": (nbf5, self.nbf, self.nbf), "K": (nbf5, self.nbf, self.nbf), "J_MO": (nbf5, nbf5),
+                  "K_MO": (nbf5, nbf5), "H_core": (nbf5,)}
+        res = {}
+        for name in want:
+            if out is not None and name in out:
+                a = out[name]
+                if tuple(a.shape) != shapes[name]:
+                    raise ValueError(f"out[{name}] must have shape {shapes[name]}")
+                if _is_torch(a):
+                    if a.dtype != _torch().float64 or not a.is_contiguous():
+                        raise ValueError(f"out[{name}] must be contiguous float64")
+                elif a.dtype != np.float64 or not a.flags.c_contiguous:
+                    raise ValueError(f"out[{name}] must be C-contiguous float64")
+                res[name] = a
+            elif on_device:
+                res[name] = _torch().empty(shapes[name], dtype=_torch().float64, device=Ci.device)
+            else:
+                res[name] = np.empty(shapes[name], dtype=np.float64)
+        if stream is None and on_device:
+            stream = _torch().cuda.current_stream(Ci.device).cuda_stream
+        ptrs = [(_ptr(res[n]) if n in res else None) for n in ("J", "K", "J_MO", "K_MO", "H_core")]
+        _lib.check(_lib.lib().pnjk_jk_build(self._h, _ptr(Ci), Ci.shape[1], nbf5,
+                                            _ptr(Hi) if Hi is not None else None, self.nbf if Hi is not None else 0,
+                                            prec, mask, *ptrs, stream))
+        return res

@@ -1,0 +1,72 @@
+"""Drop-in replacements for PyNOF's RI JK builds.
+
+``computeJKj(C, I, b_mnl, p)`` and ``computeJKH_MO(C, H, I, b_mnl, p)`` keep the
+reference's signatures (src/pynof/integrals.py:172 and :307), argument meaning
+and return values: host FP64 arrays J, K of shape (nbf5, nbf, nbf), and J_MO,
+K_MO (nbf5, nbf5) with H_core (nbf5,). ``I`` is ignored (it is ``[]`` in RI mode,
+integrals.py:28-33). Both run on the MI355X through the HIP library; there is no
+CPU path. ``p`` must provide ``nbf5``; ``p.RI`` must be true. ``p.precision``
+(optional, default "fp64") selects "fp64", "split" or "split_refine".
+
+b_mnl is uploaded to the device on first use and kept there for as long as the
+array object lives (the reference uploads it once, integrals.py:51-53).
+"""
+from __future__ import annotations
+
+import threading
+import weakref
+
+from .engine import JKEngine
+
+_cache_lock = threading.Lock()
+_engines: dict[int, tuple] = {}
+
+
+def _engine_for(b_mnl, p) -> JKEngine:
+    key = id(b_mnl)
+    with _cache_lock:
+        hit = _engines.get(key)
+        if hit is not None:
+            ref, shape, eng = hit
+            if ref() is b_mnl and shape == tuple(b_mnl.shape):
+                return eng
+        eng = JKEngine(int(getattr(p, "device", 0)))
+        eng.upload_b(b_mnl, sym_tol=float(getattr(p, "sym_tol", 1e-10)))
+        try:
+            ref = weakref.ref(b_mnl, lambda _r, k=key: _engines.pop(k, None))
+        except TypeError:
+            ref = lambda b=b_mnl: b  # noqa: E731 - objects without weakref support stay cached
+        _engines[key] = (ref, tuple(b_mnl.shape), eng)
+        return eng
+
+
+def _check_p(p):
+    if not getattr(p, "RI", False):
+        raise NotImplementedError("pynof_amd implements the density-fitted (p.RI = True) JK build only")
+    if not hasattr(p, "nbf5"):
+        raise AttributeError("p must provide nbf5")
+
+
+def computeJKj(C, I, b_mnl, p):
+    """J^n, K^n for the first p.nbf5 orbitals (integrals.py:172-191, RI branch)."""
+    _check_p(p)
+    eng = _engine_for(b_mnl, p)
+    r = eng.build(C, p.nbf5, precision=getattr(p, "precision", "fp64"), outputs=("J", "K"))
+    return r["J"], r["K"]
+
+
+def computeJKH_MO(C, H, I, b_mnl, p):
+    """J_MO, K_MO, H_core for the first p.nbf5 orbitals (integrals.py:307-326, RI branch)."""
+    _check_p(p)
+    eng = _engine_for(b_mnl, p)
+    r = eng.build(C, p.nbf5, H=H, precision=getattr(p, "precision", "fp64"),
+                  outputs=("J_MO", "K_MO", "H_core"))
+    return r["J_MO"], r["K_MO"], r["H_core"]
+
+
+def release_cached_engines() -> None:
+    """Drops every cached device copy of b."""
+    with _cache_lock:
+        for _, _, eng in _engines.values():
+            eng.close()
+        _engines.clear()

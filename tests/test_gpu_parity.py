@@ -1,0 +1,188 @@
+"""FP64 parity of the HIP JK build against the FP64 oracle (GPU only).
+
+Bar: normwise relative error max|gpu - oracle| / max|oracle| <= 1e-10 for every
+output of the FP64 form (the stated FP64 parity tolerance), energy within
+1e-10 relative.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import jk_oracle as O
+from pynof_amd import JKEngine, PnjkError, computeJKH_MO, computeJKj, param, synth
+from pynof_amd import _lib
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+HERE = os.path.dirname(os.path.abspath(__file__))
+NAMES = ("J", "K", "J_MO", "K_MO", "H_core")
+
+
+def rel(a, r):
+    return float(np.max(np.abs(np.asarray(a) - r)) / max(1e-300, np.max(np.abs(r))))
+
+
+@pytest.fixture(scope="module")
+def eng(gpu_required):
+    e = JKEngine(0)
+    yield e
+    e.close()
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def test_water_golden(eng):
+    z = np.load(os.path.join(HERE, "golden", "water_fp64.npz"))
+    meta = json.load(open(os.path.join(HERE, "golden", "water_fp64.json")))
+    _say("case: water golden nbf=24 naux=116 nbf5=24")
+    eng.upload_b(z["b"])
+    r = eng.build(z["C"], meta["nbf5"], H=z["H"])
+    for k in NAMES:
+        assert rel(r[k], z[k]) <= TOL, (k, rel(r[k], z[k]))
+    e = O.energy(z["n"], r["J_MO"], r["K_MO"], r["H_core"])
+    assert abs(e - meta["E"]) <= TOL * abs(meta["E"])
+
+
+SHAPES = [(1, 1, 1), (2, 3, 1), (5, 7, 3), (17, 33, 9), (40, 144, 40), (65, 130, 64), (129, 200, 100),
+          (130, 70, 130), (161, 96, 161), (97, 520, 33)]
+
+
+@pytest.mark.parametrize("nbf,naux,nbf5", SHAPES)
+def test_shapes_vs_oracle(eng, nbf, naux, nbf5):
+    _say(f"case: shape nbf={nbf} naux={naux} nbf5={nbf5}")
+    b = synth.make_b(nbf, naux, 100 + nbf, 0.05)
+    C = synth.make_C(nbf, 200 + nbf)
+    H = synth.make_H(nbf, 300 + nbf)
+    eng.upload_b(b)
+    r = eng.build(C, nbf5, H=H)
+    ref = O.jk_build(C, H, b, nbf5)
+    for k, a in zip(NAMES, ref):
+        assert rel(r[k], a) <= TOL, (k, rel(r[k], a))
+
+
+def test_mid_workload(eng):
+    w = synth.workload(1)
+    _say(f"case: {w.name}")
+    b, C, H, n = synth.make_inputs(w)
+    eng.upload_b(b)
+    r = eng.build(C, w.nbf5, H=H)
+    ref = O.jk_build(C, H, b, w.nbf5)
+    for k, a in zip(NAMES, ref):
+        assert rel(r[k], a) <= TOL, (k, rel(r[k], a))
+    e_ref = O.energy(n, *ref[2:])
+    e = O.energy(n, r["J_MO"], r["K_MO"], r["H_core"])
+    assert abs(e - e_ref) <= TOL * abs(e_ref)
+
+
+def test_device_inputs_and_outputs(eng):
+    import torch
+
+    _say("case: device inputs/outputs nbf=70 naux=150 nbf5=50")
+    nbf, naux, nbf5 = 70, 150, 50
+    b = synth.make_b(nbf, naux, 7, 0.05)
+    C = synth.make_C(nbf, 8)
+    H = synth.make_H(nbf, 9)
+    eng.upload_b(b)
+    host = eng.build(C, nbf5, H=H)
+    e2 = JKEngine(0)
+    e2.upload_b(torch.from_numpy(b).cuda())
+    dev = e2.build(torch.from_numpy(C).cuda(), nbf5, H=torch.from_numpy(H).cuda())
+    torch.cuda.synchronize()
+    for k in NAMES:
+        assert dev[k].is_cuda
+        assert np.array_equal(dev[k].cpu().numpy(), host[k]), k
+    e2.close()
+
+
+def test_fortran_order_C(eng):
+    _say("case: F-order C nbf=30 naux=64 nbf5=20")
+    nbf, naux, nbf5 = 30, 64, 20
+    b = synth.make_b(nbf, naux, 17, 0.05)
+    C = np.asfortranarray(synth.make_C(nbf, 18))
+    eng.upload_b(b)
+    r = eng.build(C, nbf5, outputs=("J", "K"))
+    J, K = O.jkj_ri(C, b, nbf5)
+    assert rel(r["J"], J) <= TOL and rel(r["K"], K) <= TOL
+
+
+def test_dropin_matches_reference_sequence(gpu_required):
+    _say("case: drop-in computeJKj / computeJKH_MO water")
+    w = synth.workload(0)
+    b, C, H, n = synth.make_inputs(w)
+    p = param(w.nbf, w.nbf5, nbfaux=w.naux, ndoc=w.ndoc)
+    J, K = computeJKj(C, [], b, p)
+    JM, KM, Hc = computeJKH_MO(C, H, [], b, p)
+    Jr, Kr = O.jkj_ri(C, b, w.nbf5)
+    JMr, KMr, Hr = O.jkh_mo_ri(C, H, b, w.nbf5)
+    for a, r in ((J, Jr), (K, Kr), (JM, JMr), (KM, KMr), (Hc, Hr)):
+        assert isinstance(a, np.ndarray) and a.dtype == np.float64 and a.shape == r.shape
+        assert rel(a, r) <= TOL
+    # the cached device copy of b is reused for a rotated C
+    C2 = C @ synth.rotation(w.nbf, 6000)
+    J2, _ = computeJKj(C2, [], b, p)
+    assert rel(J2, O.jkj_ri(C2, b, w.nbf5)[0]) <= TOL
+
+
+def test_errors(eng):
+    _say("case: error paths")
+    fresh = JKEngine(0)
+    with pytest.raises(PnjkError) as ei:
+        fresh.build(np.eye(3), 2)
+    assert ei.value.code == _lib.ESTATE
+    b = synth.make_b(6, 5, 1, 0.05)
+    fresh.upload_b(b)
+    with pytest.raises(ValueError):
+        fresh.build(np.eye(6), 7)
+    with pytest.raises(PnjkError) as ei:
+        fresh.build(np.eye(6), 3, precision=7)
+    assert ei.value.code in (_lib.EINVAL, _lib.EUNSUPPORTED)
+    bad = b.copy()
+    bad[1, 2, 3] = np.nan
+    with pytest.raises(PnjkError) as ei:
+        fresh.upload_b(bad)
+    assert ei.value.code == _lib.EINVAL and "non-finite" in str(ei.value)
+    asym = b.copy()
+    asym[4, 1, 0] += 1e-3
+    with pytest.raises(PnjkError) as ei:
+        fresh.upload_b(asym)
+    assert "symmetric" in str(ei.value)
+    fresh.upload_b(asym, sym_tol=1e-2)  # accepted when the caller allows it
+    fresh.close()
+
+
+def test_large_workload_properties(gpu_required):
+    """nbf=1000, naux=3600, nbf5=300 at full size: size-independent identities."""
+    import torch
+
+    w = synth.workload(2)
+    _say(f"case: {w.name} properties")
+    ld = JKEngine.packed_ld(w.nbf, w.naux)
+    btri = synth.device_b_packed(w.nbf, w.naux, w.seed_b, w.lam, ld)
+    e = JKEngine(0)
+    e.upload_b_packed(btri, w.nbf, w.naux)
+    del btri
+    C = torch.from_numpy(synth.make_C(w.nbf, w.seed_c)).cuda()
+    H = torch.from_numpy(synth.make_H(w.nbf, w.seed_h)).cuda()
+    r = e.build(C, w.nbf5, H=H)
+    torch.cuda.synchronize()
+    JM, KM = r["J_MO"], r["K_MO"]
+    # J_MO = d d^T and K_MO = row sums of Y^2 come from different kernels; diag must agree
+    assert torch.max(torch.abs(torch.diag(JM) - torch.diag(KM))) <= 1e-12 * torch.max(torch.abs(JM))
+    assert torch.equal(JM, JM.T) and torch.equal(KM, KM.T)
+    # J^q, K^q contracted with D_p = C_p C_p^T reproduce J_MO[q, p], K_MO[q, p] (integrals.py:337-341)
+    Cq = C[:, : w.nbf5]
+    for q in (0, 1, 150, 299):
+        jq = Cq.T @ r["J"][q] @ Cq
+        kq = Cq.T @ r["K"][q] @ Cq
+        assert torch.max(torch.abs(torch.diag(jq) - JM[q])) <= 1e-10 * torch.max(torch.abs(JM))
+        assert torch.max(torch.abs(torch.diag(kq) - KM[q])) <= 1e-10 * torch.max(torch.abs(KM))
+        assert torch.equal(r["J"][q], r["J"][q].T) and torch.equal(r["K"][q], r["K"][q].T)
+    hc = torch.einsum("mi,mi->i", Cq, H @ Cq)
+    assert torch.max(torch.abs(hc - r["H_core"])) <= 1e-12 * torch.max(torch.abs(hc))
+    e.close()
+    del r
+    torch.cuda.empty_cache()

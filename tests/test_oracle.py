@@ -1,0 +1,161 @@
+"""Pins the FP64 oracle (oracle/jk_oracle.py) without the reference importable.
+
+Each check is independent of the oracle's own formula: explicit index loops,
+the 4-index route I = b b^T through the reference's JKj_Full/JKH_MO_Full
+formulas (integrals.py:197-205, :332-345), closed forms at permutation C and
+naux = 1, C-independent sums, and invariants. Planted mistakes in a copy of the
+oracle must fail these checks.
+"""
+import numpy as np
+import pytest
+
+from oracle import jk_oracle as O
+from pynof_amd import synth
+
+
+def tiny(nbf=5, naux=7, nbf5=3, seed=11, lam=0.3):
+    b = synth.make_b(nbf, naux, seed, lam)
+    C = synth.make_C(nbf, seed + 1)
+    H = synth.make_H(nbf, seed + 2)
+    return b, C, H, nbf5
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(1e-300, np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("nbf,naux,nbf5", [(5, 7, 3), (4, 3, 4), (6, 8, 1), (3, 1, 2)])
+def test_loops_match(nbf, naux, nbf5):
+    b, C, H, _ = tiny(nbf, naux, nbf5)
+    J, K = O.jkj_ri(C, b, nbf5)
+    Jl, Kl = O.jkj_loops(C, b, nbf5)
+    assert rel(J, Jl) < 1e-13 and rel(K, Kl) < 1e-13
+    JM, KM, Hc = O.jkh_mo_ri(C, H, b, nbf5)
+    JMl, KMl, Hcl = O.jkh_mo_loops(C, H, b, nbf5)
+    assert rel(JM, JMl) < 1e-13 and rel(KM, KMl) < 1e-13 and rel(Hc, Hcl) < 1e-13
+    Jf, Kf, JMf, KMf, Hf = O.jk_build(C, H, b, nbf5)
+    for a, r in ((Jf, Jl), (Kf, Kl), (JMf, JMl), (KMf, KMl), (Hf, Hcl)):
+        assert rel(a, r) < 1e-13
+
+
+@pytest.mark.parametrize("nbf,naux,nbf5", [(6, 9, 4), (8, 5, 8)])
+def test_four_index_route(nbf, naux, nbf5):
+    b, C, H, _ = tiny(nbf, naux, nbf5, seed=21)
+    I4 = O.four_index(b)
+    J, K = O.jkj_ri(C, b, nbf5)
+    Jf, Kf = O.jkj_full(C, I4, nbf5)
+    assert rel(J, Jf) < 1e-13 and rel(K, Kf) < 1e-13
+    JM, KM, Hc = O.jkh_mo_ri(C, H, b, nbf5)
+    JMf, KMf, Hf = O.jkh_mo_full(C, H, I4, nbf5)
+    assert rel(JM, JMf) < 1e-13 and rel(KM, KMf) < 1e-13 and rel(Hc, Hf) < 1e-13
+
+
+def test_permutation_closed_form():
+    nbf, naux = 6, 5
+    b, _, H, _ = tiny(nbf, naux, nbf, seed=31)
+    perm = np.roll(np.arange(nbf), 2)  # column q of C is the unit vector e_perm[q]
+    C = np.zeros((nbf, nbf))
+    C[perm, np.arange(nbf)] = 1.0
+    J, K, JM, KM, Hc = O.jk_build(C, H, b, nbf)
+    for q in range(nbf):
+        a = perm[q]
+        assert np.allclose(J[q], np.einsum("l,mnl->mn", b[a, a], b), rtol=0, atol=1e-14)
+        assert np.allclose(K[q], b[a] @ b[a].T, rtol=0, atol=1e-14)
+        assert abs(Hc[q] - H[a, a]) < 1e-15
+        for s in range(nbf):
+            c = perm[s]
+            assert abs(KM[q, s] - np.sum(b[a, c] ** 2)) < 1e-14
+            assert abs(JM[q, s] - np.dot(b[a, a], b[c, c])) < 1e-14
+
+
+def test_rank_one_naux():
+    nbf, nbf5 = 5, 4
+    b, C, H, _ = tiny(nbf, 1, nbf5, seed=41)
+    v = b[:, :, 0]
+    J, K, JM, KM, Hc = O.jk_build(C, H, b, nbf5)
+    for q in range(nbf5):
+        c = C[:, q]
+        assert np.allclose(J[q], (c @ v @ c) * v, rtol=1e-13, atol=1e-15)
+        w = v @ c
+        assert np.allclose(K[q], np.outer(w, w), rtol=1e-13, atol=1e-15)
+        for s in range(nbf5):
+            e = C[:, s]
+            assert abs(JM[q, s] - (c @ v @ c) * (e @ v @ e)) < 1e-14
+            assert abs(KM[q, s] - (c @ v @ e) ** 2) < 1e-14
+
+
+def test_orbital_invariant_sums():
+    nbf, naux = 7, 6
+    b, C, H, _ = tiny(nbf, naux, nbf, seed=51)
+    J, K, *_ = O.jk_build(C, H, b, nbf)
+    sumK = sum(b[l] @ b[l].T for l in range(nbf))
+    tr = np.einsum("mml->l", b)
+    sumJ = np.einsum("l,mnl->mn", tr, b)
+    assert rel(J.sum(0), sumJ) < 1e-12
+    assert rel(K.sum(0), sumK) < 1e-12
+
+
+def test_invariants_water():
+    w = synth.workload(0)
+    b, C, H, n = synth.make_inputs(w)
+    J, K, JM, KM, Hc = O.jk_build(C, H, b, w.nbf5)
+    assert np.allclose(J, J.transpose(0, 2, 1), rtol=0, atol=1e-13)
+    assert np.allclose(K, K.transpose(0, 2, 1), rtol=0, atol=1e-13)
+    assert np.allclose(JM, JM.T, atol=1e-13) and np.allclose(KM, KM.T, atol=1e-13)
+    assert np.allclose(np.diag(JM), np.diag(KM), rtol=1e-13)
+    assert np.min(np.linalg.eigvalsh(JM)) > -1e-10 * np.max(np.abs(JM))
+    Cq = C[:, : w.nbf5]
+    JMc = np.einsum("qmn,mp,np->qp", J, Cq, Cq)
+    KMc = np.einsum("qmn,mp,np->qp", K, Cq, Cq)
+    assert rel(JMc, JM) < 1e-12 and rel(KMc, KM) < 1e-12
+
+
+def test_energy_forms():
+    w = synth.workload(0)
+    b, C, H, n = synth.make_inputs(w)
+    assert abs(n.sum() - w.ndoc) < 1e-12 and np.all(n > 0) and np.all(n <= 1)
+    _, _, JM, KM, Hc = O.jk_build(C, H, b, w.nbf5)
+    e = O.energy(n, JM, KM, Hc)
+    brute = 2 * sum(n[i] * Hc[i] for i in range(len(n)))
+    brute += sum(n[p] * n[q] * (2 * JM[p, q] - KM[p, q]) for p in range(len(n)) for q in range(len(n)))
+    assert abs(e - brute) < 1e-11 * abs(brute)
+    cj, ck = 2 * np.outer(n, n), np.outer(n, n)
+    ec = O.calce_closed_shell(n, cj, ck, JM, KM, Hc, w.ndoc, w.ndoc, w.nbf5)
+    assert abs(ec - (e + np.sum((n - n * n) * np.diag(JM)))) < 1e-11 * abs(e)
+
+
+# ---------------------------------------------------------------- planted mistakes
+def _mutant_transposed_ht(C, b, nbf5):
+    Cq = C[:, :nbf5]
+    X = np.tensordot(C.T[:, :nbf5], b, axes=([0], [0]))  # half-transform with a transposed C
+    d = np.einsum("nq,qnl->ql", Cq, X)
+    J = np.tensordot(d, b, axes=([1], [2]))
+    K = np.einsum("qml,qnl->qmn", X, X)
+    return J, K
+
+
+def _mutant_kmo_offdiag(C, H, b, nbf5):
+    JM, KM, Hc = O.jkh_mo_ri(C, H, b, nbf5)
+    Cq = C[:, :nbf5]
+    bp = np.einsum("nq,pnl->pql", Cq, np.tensordot(Cq, b, axes=([0], [0])))
+    JM = np.einsum("pql,qpl->pq", bp, bp)  # exchange-like instead of diagonal products
+    return JM, KM, Hc
+
+
+def _mutant_dropped_sqrt2(b):
+    return b * np.sqrt(2.0)
+
+
+def test_planted_mistakes_fail():
+    b, C, H, nbf5 = tiny(5, 6, 4, seed=61)
+    Jl, Kl = O.jkj_loops(C, b, nbf5)
+    Jm, Km = _mutant_transposed_ht(C, b, nbf5)
+    assert rel(Jm, Jl) > 1e-3 or rel(Km, Kl) > 1e-3
+    JMl, KMl, _ = O.jkh_mo_loops(C, H, b, nbf5)
+    JMm, _, _ = _mutant_kmo_offdiag(C, H, b, nbf5)
+    assert rel(JMm, JMl) > 1e-3
+    # a wrongly scaled b breaks the 4-index identity against the unscaled I
+    I4 = O.four_index(b)
+    Jw, _ = O.jkj_ri(C, _mutant_dropped_sqrt2(b), nbf5)
+    Jf, _ = O.jkj_full(C, I4, nbf5)
+    assert rel(Jw, Jf) > 0.5
