@@ -99,7 +99,7 @@ struct Tile {
 };
 
 // The GEMM core. Accumulator element (fm, fn, r) of a lane is output element
-//   m = m0 + wm + 16*fm + 4*(lane/16) + r,  n = n0 + wn + 16*fn + lane%16
+//   m = m0 + wm + 16*fm + lane/16 + 4*r,  n = n0 + wn + 16*fn + lane%16
 // (the v_mfma_f64_16x16x4f64 result layout).
 template <int FM, int FN, int BK, class Prob>
 __global__ void __launch_bounds__(kThreads) k_gemm_f64(const Prob prob) {
@@ -203,7 +203,7 @@ struct SrcKMBtri {
 // ---------------------------------------------------------------- problems
 template <int FM, int FN>
 __device__ __forceinline__ void elem_coords(int lane, int wm, int wn, int i, int j, int r, int& dm, int& dn) {
-  dm = wm + 16 * i + 4 * (lane >> 4) + r;
+  dm = wm + 16 * i + (lane >> 4) + 4 * r;
   dn = wn + 16 * j + (lane & 15);
 }
 
@@ -298,7 +298,7 @@ struct ProbQT {
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) lds[wm + 16 * i + 4 * (lane >> 4) + r] = s[i][r];
+        for (int r = 0; r < 4; ++r) lds[wm + 16 * i + (lane >> 4) + 4 * r] = s[i][r];
     }
     __syncthreads();
     if (!(wave & 1) && (lane & 15) == 0) {
@@ -306,7 +306,7 @@ struct ProbQT {
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int dm = wm + 16 * i + 4 * (lane >> 4) + r;
+          const int dm = wm + 16 * i + (lane >> 4) + 4 * r;
           const int q = t.m0 + dm;
           if (q <= p && q < nbf5)
             kpart[((int64_t)p * nbf5 + q) * ntiles + t.nt] = s[i][r] + lds[dm];

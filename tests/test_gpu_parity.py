@@ -138,7 +138,7 @@ def test_errors(eng):
     with pytest.raises(ValueError):
         fresh.build(np.eye(6), 7)
     with pytest.raises(PnjkError) as ei:
-        fresh.build(np.eye(6), 3, precision=7)
+        fresh.build(np.eye(6), 3, precision=7, outputs=("J", "K"))
     assert ei.value.code in (_lib.EINVAL, _lib.EUNSUPPORTED)
     bad = b.copy()
     bad[1, 2, 3] = np.nan
