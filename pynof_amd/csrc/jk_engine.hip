@@ -109,7 +109,17 @@ struct pnjk_engine {
   int nbf = 0, naux = 0, naux_p = 0;
   int64_t ntri = 0;
   DevBuf btri;
+  DevBuf bt16h, bt16l, bht16h, bht16l;  // split planes of b, built on the first split build
   DevBuf Cp, Hp, X, d, kpart, T, outs, stage, scal;
+  DevBuf ct16h, ct16l, x16h, x16l, d16h, d16l;  // split workspace
+  bool split_ready = false;
+  DevBuf* all[22];
+  int nall = 0;
+  pnjk_engine() {
+    for (DevBuf* b : {&btri, &bt16h, &bt16l, &bht16h, &bht16l, &Cp, &Hp, &X, &d, &kpart, &T, &outs, &stage, &scal,
+                      &ct16h, &ct16l, &x16h, &x16l, &d16h, &d16l})
+      all[nall++] = b;
+  }
 };
 
 extern "C" {
@@ -141,8 +151,7 @@ int pnjk_destroy(pnjk_engine* e) {
   if (!e) return PNJK_OK;
   hipSetDevice(e->device);
   hipStreamSynchronize(e->own);
-  for (DevBuf* b : {&e->btri, &e->Cp, &e->Hp, &e->X, &e->d, &e->kpart, &e->T, &e->outs, &e->stage, &e->scal})
-    b->release();
+  for (int i = 0; i < e->nall; ++i) e->all[i]->release();
   for (hipEvent_t& ev : e->ev)
     if (ev) hipEventDestroy(ev);
   hipStreamDestroy(e->own);
@@ -158,13 +167,11 @@ int64_t pnjk_packed_ld(int64_t nbf, int64_t naux) {
 uint64_t pnjk_footprint(const pnjk_engine* e) {
   if (!e) return 0;
   uint64_t s = 0;
-  for (const DevBuf* b : {&e->btri, &e->Cp, &e->Hp, &e->X, &e->d, &e->kpart, &e->T, &e->outs, &e->stage, &e->scal})
-    s += b->bytes;
+  for (int i = 0; i < e->nall; ++i) s += e->all[i]->bytes;
   return s;
 }
 
 uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int precision) {
-  (void)precision;
   const int64_t naux_p = pnjk_packed_ld(nbf, naux);
   const int64_t nbf_p = roundup(nbf, 32), ldcp = roundup(nbf5, 32);
   uint64_t s = 0;
@@ -174,10 +181,35 @@ uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int preci
   s += (uint64_t)(roundup(nbf5, 160) * naux_p) * 8;          // d
   s += (uint64_t)(nbf5 * nbf5 * (naux_p / 64)) * 8;          // kpart
   s += (uint64_t)(nbf * ldcp) * 8;                           // T
+  if (precision != PNJK_FP64) {
+    const int64_t ntri = nbf * (nbf + 1) / 2;
+    s += (uint64_t)(ntri * naux_p) * 4;                      // btri16 hi/lo
+    s += (uint64_t)(nbf * naux_p * nbf_p) * 4;               // bht16 hi/lo
+    s += (uint64_t)(nbf5 * nbf_p) * 4;                       // Ct16 hi/lo
+    s += (uint64_t)(nbf5 * nbf * naux_p) * 4;                // X16 hi/lo
+    s += (uint64_t)(nbf5 * naux_p) * 4;                      // d16 hi/lo
+  }
   return s;
 }
 
+// Builds the bf16 planes of b the split kernels read (once per uploaded b).
+static int ensure_split_planes(pnjk_engine* e, hipStream_t s) {
+  if (e->split_ready) return PNJK_OK;
+  const int64_t ntri = e->ntri, naux_p = e->naux_p, nbf = e->nbf, ldk = roundup(nbf, 32);
+  int rc;
+  const size_t tri_bytes = (size_t)(ntri * naux_p) * 2, ht_bytes = (size_t)(nbf * naux_p * ldk) * 2;
+  if ((rc = e->bt16h.ensure(tri_bytes)) || (rc = e->bt16l.ensure(tri_bytes))) return rc;
+  if ((rc = e->bht16h.ensure(ht_bytes)) || (rc = e->bht16l.ensure(ht_bytes))) return rc;
+  PNJK_HIP_CHECK(launch_split_btri(e->btri.as<double>(), ntri * naux_p, e->bt16h.as<uint16_t>(),
+                                   e->bt16l.as<uint16_t>(), s));
+  PNJK_HIP_CHECK(launch_build_bht(e->btri.as<double>(), (int)nbf, (int)naux_p, (int)ldk, e->bht16h.as<uint16_t>(),
+                                  e->bht16l.as<uint16_t>(), s));
+  e->split_ready = true;
+  return PNJK_OK;
+}
+
 static int upload_common(pnjk_engine* e, int64_t nbf, int64_t naux) {
+  e->split_ready = false;
   if (nbf < 1 || naux < 1) return set_error(PNJK_EINVAL, "upload_b: nbf and naux must be >= 1");
   if (nbf > 46000) return set_error(PNJK_EINVAL, "upload_b: nbf too large");
   e->nbf = (int)nbf;
@@ -272,8 +304,8 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
   if ((outputs & PNJK_OUT_HCORE) && (!H_core || !H))
     return set_error(PNJK_EINVAL, "pnjk_jk_build: H_core requested but H or H_core is NULL");
   if ((outputs & PNJK_OUT_HCORE) && ldh < e->nbf) return set_error(PNJK_EINVAL, "pnjk_jk_build: ldh < nbf");
-  if (precision != PNJK_FP64)
-    return set_error(PNJK_EUNSUPPORTED, "pnjk_jk_build: precision " + std::to_string(precision) + " not available");
+  if (precision != PNJK_FP64 && precision != PNJK_SPLIT && precision != PNJK_SPLIT_REFINE)
+    return set_error(PNJK_EUNSUPPORTED, "pnjk_jk_build: unknown precision " + std::to_string(precision));
   PNJK_HIP_CHECK(hipSetDevice(e->device));
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : e->own;
   auto mark = [&](int st) {
@@ -293,7 +325,7 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
   int rc;
   const int64_t cp_elems = (int64_t)(g.nbf_p + 8) * g.ldcp + 1024;
   if ((rc = e->Cp.ensure((size_t)cp_elems * 8))) return rc;
-  if ((rc = e->X.ensure((size_t)(n5 * nbf * g.naux_p) * 8))) return rc;
+  if (precision != PNJK_SPLIT && (rc = e->X.ensure((size_t)(n5 * nbf * g.naux_p) * 8))) return rc;
   const int64_t d_rows = roundup(n5, 160);
   if ((rc = e->d.ensure((size_t)(d_rows * g.naux_p) * 8))) return rc;
   const int qt_bn = 64;
@@ -345,24 +377,65 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
   }
 
   const TileF64 ht = pick_ht_tile(g.nbf5);
-  mark(ST_HT);
-  PNJK_HIP_CHECK(launch_ht_f64(g, ht, Cp, e->btri.as<double>(), e->X.as<double>(), s));
-  int nt = 0;
-  mark(ST_QT);
   const TileF64 qt = g.nbf5 > 64 ? TileF64{4, 2} : TileF64{2, 2};
-  PNJK_HIP_CHECK(launch_qt_f64(g, qt, Cp, e->X.as<double>(), e->kpart.as<double>(), e->d.as<double>(), &nt, s));
+  const TileF64 kt = g.nbf >= 256 ? TileF64{4, 4} : TileF64{2, 2};
+  int nt = 0;
+  SplitGeom sg;
+  sg.nbf = g.nbf;
+  sg.naux_p = g.naux_p;
+  sg.nbf5 = g.nbf5;
+  sg.ldk = g.nbf_p;
+  if (precision != PNJK_FP64) {
+    if ((rc = ensure_split_planes(e, s))) return rc;
+    const size_t ct_bytes = (size_t)(n5 * sg.ldk) * 2, x_bytes = (size_t)(n5 * nbf * g.naux_p) * 2,
+                 d_bytes = (size_t)(n5 * g.naux_p) * 2;
+    if ((rc = e->ct16h.ensure(ct_bytes)) || (rc = e->ct16l.ensure(ct_bytes))) return rc;
+    if ((rc = e->x16h.ensure(x_bytes)) || (rc = e->x16l.ensure(x_bytes))) return rc;
+    if ((rc = e->d16h.ensure(d_bytes)) || (rc = e->d16l.ensure(d_bytes))) return rc;
+  }
+  uint16_t *ct_h = e->ct16h.as<uint16_t>(), *ct_l = e->ct16l.as<uint16_t>();
+  uint16_t *x_h = e->x16h.as<uint16_t>(), *x_l = e->x16l.as<uint16_t>();
+  uint16_t *d_h = e->d16h.as<uint16_t>(), *d_l = e->d16l.as<uint16_t>();
+  mark(ST_HT);
+  if (precision == PNJK_SPLIT) {
+    PNJK_HIP_CHECK(launch_split_ct(Cp, g.ldcp, g.nbf, g.nbf5, g.nbf5, sg.ldk, ct_h, ct_l, s));
+    PNJK_HIP_CHECK(launch_ht_x3(sg, ht, ct_h, ct_l, e->bht16h.as<uint16_t>(), e->bht16l.as<uint16_t>(), x_h, x_l,
+                                s));
+  } else {
+    PNJK_HIP_CHECK(launch_ht_f64(g, ht, Cp, e->btri.as<double>(), e->X.as<double>(), s));
+  }
+  mark(ST_QT);
+  if (precision == PNJK_SPLIT) {
+    PNJK_HIP_CHECK(launch_qt_x3(sg, qt, ct_h, ct_l, x_h, x_l, e->kpart.as<double>(), e->d.as<double>(), d_h, d_l,
+                                &nt, s));
+  } else {
+    PNJK_HIP_CHECK(
+        launch_qt_f64(g, qt, Cp, e->X.as<double>(), e->kpart.as<double>(), e->d.as<double>(), &nt, s));
+  }
   mark(ST_KMO);
   PNJK_HIP_CHECK(launch_kmo_reduce(g.nbf5, nt, e->kpart.as<double>(), dKMO, s));
   mark(ST_JMO);
   PNJK_HIP_CHECK(launch_syrk_f64(1, g.nbf5, g.naux_p, e->d.as<double>(), g.naux_p, 0, dJMO, g.nbf5, 0,
                                  TileF64{2, 2}, s));
+  if (precision == PNJK_SPLIT_REFINE && (want_j || want_k)) {
+    // J^n/K^n in split form from the exact X and d.
+    if (want_k) PNJK_HIP_CHECK(launch_split_btri(e->X.as<double>(), n5 * nbf * g.naux_p, x_h, x_l, s));
+    if (want_j) PNJK_HIP_CHECK(launch_split_btri(e->d.as<double>(), n5 * g.naux_p, d_h, d_l, s));
+  }
   mark(ST_JN);
-  if (want_j) PNJK_HIP_CHECK(launch_jn_f64(g, ht, e->d.as<double>(), e->btri.as<double>(), dJ, s));
+  if (want_j) {
+    if (precision == PNJK_FP64)
+      PNJK_HIP_CHECK(launch_jn_f64(g, ht, e->d.as<double>(), e->btri.as<double>(), dJ, s));
+    else
+      PNJK_HIP_CHECK(launch_jn_x3(sg, ht, d_h, d_l, e->bt16h.as<uint16_t>(), e->bt16l.as<uint16_t>(), dJ, s));
+  }
   mark(ST_K);
   if (want_k) {
-    const TileF64 kt = g.nbf >= 256 ? TileF64{4, 4} : TileF64{2, 2};
-    PNJK_HIP_CHECK(launch_syrk_f64(g.nbf5, g.nbf, g.naux_p, e->X.as<double>(), g.naux_p, nbf * g.naux_p, dK, nbf,
-                                   nbf * nbf, kt, s));
+    if (precision == PNJK_FP64)
+      PNJK_HIP_CHECK(launch_syrk_f64(g.nbf5, g.nbf, g.naux_p, e->X.as<double>(), g.naux_p, nbf * g.naux_p, dK, nbf,
+                                     nbf * nbf, kt, s));
+    else
+      PNJK_HIP_CHECK(launch_kn_x3(sg, kt, x_h, x_l, dK, s));
   }
   mark(ST_HC);
   if (want_h) {

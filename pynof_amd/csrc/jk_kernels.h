@@ -58,4 +58,35 @@ hipError_t launch_hcore_f64(const BuildGeom& g, const double* Hp, int ldh, const
 hipError_t launch_pack_b(int nbf, int naux, int naux_p, int mu0, int mu1, const double* b_chunk, double* btri,
                          double* asym_max, unsigned long long* nonfinite, hipStream_t s);
 
+// ---------------------------------------------------------------- split (BF16x3)
+struct SplitGeom {
+  int nbf;     // basis functions
+  int naux_p;  // padded aux stride (multiple of 64)
+  int nbf5;    // active orbitals
+  int ldk;     // contraction stride over mu/nu of Ct16 and bht16 (nbf rounded up to 32, zero tail)
+};
+
+// hi/lo bf16 planes of n packed FP64 values.
+hipError_t launch_split_btri(const double* btri, int64_t n, uint16_t* hi, uint16_t* lo, hipStream_t s);
+// bht[nu][P][mu] planes (mu contiguous, stride ldk) from btri.
+hipError_t launch_build_bht(const double* btri, int nbf, int naux_p, int ldk, uint16_t* hi, uint16_t* lo,
+                            hipStream_t s);
+// Ct[q][mu] planes (rows q < rows, stride ldk) from Cp[mu][q].
+hipError_t launch_split_ct(const double* Cp, int ldcp, int nbf, int nbf5, int rows, int ldk, uint16_t* hi,
+                           uint16_t* lo, hipStream_t s);
+// X16[q][nu][P] = split(sum_mu C[mu][q] b[mu][nu][P]).
+hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
+                        const uint16_t* bht_hi, const uint16_t* bht_lo, uint16_t* x_hi, uint16_t* x_lo,
+                        hipStream_t s);
+// Quarter transform from X16: K_MO row-sum partials, d (FP64 and split planes).
+hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
+                        const uint16_t* x_hi, const uint16_t* x_lo, double* kpart, double* d, uint16_t* d_hi,
+                        uint16_t* d_lo, int* ntiles, hipStream_t s);
+// J^n from split d and btri planes (FP64 output, mirrored).
+hipError_t launch_jn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* d_hi, const uint16_t* d_lo,
+                        const uint16_t* bt_hi, const uint16_t* bt_lo, double* J, hipStream_t s);
+// K^n from split X16 (FP64 output, mirrored).
+hipError_t launch_kn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* x_hi, const uint16_t* x_lo, double* K,
+                        hipStream_t s);
+
 }  // namespace pnjk

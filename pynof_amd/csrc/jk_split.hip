@@ -1,0 +1,587 @@
+// Split-precision (BF16x3) kernels of the RI JK build for gfx950.
+//
+// Each FP64 operand x is stored as two bf16 planes, x_hi = bf16(x) and
+// x_lo = bf16(x - x_hi); a product a*b is formed as a_hi*b_hi + a_hi*b_lo +
+// a_lo*b_hi on v_mfma_f32_16x16x32_bf16 with FP32 accumulation. The GEMM core
+// stages both planes of both operands through double-buffered LDS in a
+// row-major, k-contiguous layout (one 16-byte fragment per lane per MFMA), so
+// every operand needs its contraction index contiguous. Layouts chosen for that:
+//   Ct16[q][mu]        C^T, per build                 (A of HT, QT)
+//   bht16[nu][P][mu]   b with mu contiguous, at upload (B of HT)
+//   btri16[t][P]       packed lower triangle, at upload (B of JN)
+//   X16[q][nu][P]      half-transformed b, per build  (A/B of K^n; B of QT via
+//                      a transposing LDS loader)
+// Contractions (reference lines as in the FP64 kernels): HT integrals.py:211,
+// QT :354 (-> d :212, K_MO :358), JN :214, K^n :216.
+#include "jk_common.h"
+#include "jk_kernels.h"
+
+namespace pnjk {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kThreads = 256;
+constexpr int kBK = 32;
+constexpr int kLDK = kBK + 8;  // LDS row stride (elements): conflict-free 16-byte fragment reads
+
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ void split2(double x, uint16_t& hi, uint16_t& lo) {
+  hi = bf16_rne((float)x);
+  lo = bf16_rne((float)(x - (double)bf16_to_f(hi)));
+}
+
+// Output row of accumulator register r of `lane` inside a 16x16 f32 MFMA block.
+__device__ __forceinline__ int out_row(int lane, int r) { return 4 * (lane >> 4) + r; }
+
+// A bf16 operand: two planes with identical layout.
+struct Planes {
+  const uint16_t* hi;
+  const uint16_t* lo;
+};
+
+enum LdKind { RK = 0, KR = 1 };
+
+// Loader of a ROWS x 32 tile (both planes) into LDS rows of kLDK elements.
+template <int KIND, int ROWS>
+struct Ld;
+
+// Global element (r, k) at off(r) + k, k contiguous: 16-byte vector copies.
+template <int ROWS>
+struct Ld<RK, ROWS> {
+  static constexpr int kChunks = ROWS * (kBK / 8);
+  static constexpr int kPer = (kChunks + kThreads - 1) / kThreads;
+  u16x8 h[kPer], l[kPer];
+  template <class S>
+  __device__ __forceinline__ void load(const S& s, int r0, int k0) {
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int c = threadIdx.x + i * kThreads;
+      if (kChunks % kThreads == 0 || c < kChunks) {
+        const int r = c >> 2, kc = (c & 3) * 8;
+        const int64_t o = s.off(r0 + r) + k0 + kc;
+        h[i] = *reinterpret_cast<const u16x8*>(s.p.hi + o);
+        l[i] = *reinterpret_cast<const u16x8*>(s.p.lo + o);
+      }
+    }
+  }
+  __device__ __forceinline__ void store(uint16_t* sh, uint16_t* sl) const {
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int c = threadIdx.x + i * kThreads;
+      if (kChunks % kThreads == 0 || c < kChunks) {
+        const int r = c >> 2, kc = (c & 3) * 8;
+        *reinterpret_cast<u16x8*>(sh + r * kLDK + kc) = h[i];
+        *reinterpret_cast<u16x8*>(sl + r * kLDK + kc) = l[i];
+      }
+    }
+  }
+};
+
+// Global element (k, r) at off(k) + r, r contiguous: 16-byte loads along r,
+// transposed into LDS with 16-bit stores.
+template <int ROWS>
+struct Ld<KR, ROWS> {
+  static constexpr int kChunks = kBK * (ROWS / 8);
+  static constexpr int kPer = (kChunks + kThreads - 1) / kThreads;
+  u16x8 h[kPer], l[kPer];
+  template <class S>
+  __device__ __forceinline__ void load(const S& s, int r0, int k0) {
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int c = threadIdx.x + i * kThreads;
+      if (kChunks % kThreads == 0 || c < kChunks) {
+        const int k = c % kBK, rc = (c / kBK) * 8;
+        const int64_t o = s.off(k0 + k) + r0 + rc;
+        h[i] = *reinterpret_cast<const u16x8*>(s.p.hi + o);
+        l[i] = *reinterpret_cast<const u16x8*>(s.p.lo + o);
+      }
+    }
+  }
+  __device__ __forceinline__ void store(uint16_t* sh, uint16_t* sl) const {
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int c = threadIdx.x + i * kThreads;
+      if (kChunks % kThreads == 0 || c < kChunks) {
+        const int k = c % kBK, rc = (c / kBK) * 8;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          sh[(rc + e) * kLDK + k] = h[i][e];
+          sl[(rc + e) * kLDK + k] = l[i][e];
+        }
+      }
+    }
+  }
+};
+
+struct Tile {
+  int batch, mt, nt, m0, n0;
+};
+
+template <int FM, int FN, class Prob>
+__global__ void __launch_bounds__(kThreads) k_gemm_x3(const Prob prob) {
+  constexpr int BM = 32 * FM, BN = 32 * FN;
+  __shared__ __align__(16) uint16_t sA[2][2][BM * kLDK];
+  __shared__ __align__(16) uint16_t sB[2][2][BN * kLDK];
+
+  Tile t;
+  if (!prob.tile(xcd_remap(blockIdx.x, gridDim.x), BM, BN, t)) return;
+  const auto as = prob.a_src(t);
+  const auto bs = prob.b_src(t);
+  const int nk = prob.ksteps();
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
+
+  Ld<Prob::kA, BM> la;
+  Ld<Prob::kB, BN> lb;
+  la.load(as, t.m0, 0);
+  lb.load(bs, t.n0, 0);
+  la.store(sA[0][0], sA[0][1]);
+  lb.store(sB[0][0], sB[0][1]);
+  __syncthreads();
+
+  v4f acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  const int frow = lane & 15, fk = 8 * (lane >> 4);
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) {
+      la.load(as, t.m0, (kt + 1) * kBK);
+      lb.load(bs, t.n0, (kt + 1) * kBK);
+    }
+    bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const int o = (wm + 16 * i + frow) * kLDK + fk;
+      ah[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sA[cur][0][o]));
+      al[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sA[cur][1][o]));
+    }
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int o = (wn + 16 * j + frow) * kLDK + fk;
+      bh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sB[cur][0][o]));
+      bl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sB[cur][1][o]));
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      }
+    if (more) {
+      la.store(sA[cur ^ 1][0], sA[cur ^ 1][1]);
+      lb.store(sB[cur ^ 1][0], sB[cur ^ 1][1]);
+    }
+    __syncthreads();
+  }
+  prob.template epilogue<FM, FN>(t, acc, wm, wn, lane, reinterpret_cast<double*>(&sA[0][0][0]));
+}
+
+// ---------------------------------------------------------------- sources
+struct SrcRows {  // element (r, k) at r_clamped*ld + k
+  Planes p;
+  int64_t base, ld;
+  int rmax;
+  __device__ __forceinline__ int64_t off(int r) const { return base + (int64_t)(r < rmax ? r : rmax - 1) * ld; }
+};
+
+template <int FM, int FN>
+__device__ __forceinline__ void coords(int lane, int wm, int wn, int i, int j, int r, int& dm, int& dn) {
+  dm = wm + 16 * i + out_row(lane, r);
+  dn = wn + 16 * j + (lane & 15);
+}
+
+// X16[q][n] = split(sum_mu Ct[q][mu] bht[n][mu]),  n = nu*naux_p + P
+struct ProbHT {
+  static constexpr int kA = RK, kB = RK;
+  Planes ct, bht;
+  uint16_t* xhi;
+  uint16_t* xlo;
+  int nbf5, ldk, mtiles;
+  int64_t N;
+  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.batch = 0;
+    t.mt = id % mtiles;
+    t.nt = id / mtiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ int ksteps() const { return ldk / kBK; }
+  __device__ SrcRows a_src(const Tile&) const { return {ct, 0, ldk, nbf5}; }
+  __device__ SrcRows b_src(const Tile&) const { return {bht, 0, ldk, (int)N}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int q = t.m0 + dm;
+          const int64_t n = (int64_t)t.n0 + dn;
+          if (q < nbf5 && n < N) {
+            const float v = acc[i][j][r];
+            const uint16_t h = bf16_rne(v);
+            xhi[(int64_t)q * N + n] = h;
+            xlo[(int64_t)q * N + n] = bf16_rne(v - bf16_to_f(h));
+          }
+        }
+  }
+};
+
+// Batch p: Y_p[q][P] = sum_nu Ct[q][nu] X16[p][nu][P] for q <= p; writes d[p][P]
+// (FP64 and split planes) and per-P-tile row sums of Y_p^2 (FP64) into kpart.
+struct SrcXT {  // element (k = nu, r = P) at p*nbf*naux_p + min(nu, nbf-1)*naux_p + P
+  Planes p;
+  int64_t base, ld;
+  int kmax;
+  __device__ __forceinline__ int64_t off(int k) const { return base + (int64_t)(k < kmax ? k : kmax - 1) * ld; }
+};
+
+struct ProbQT {
+  static constexpr int kA = RK, kB = KR;
+  Planes ct, x;
+  double* kpart;
+  double* d;
+  uint16_t* dhi;
+  uint16_t* dlo;
+  int nbf, naux_p, nbf5, ldk, mtiles, ntiles;
+  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.mt = id % mtiles;
+    const int rest = id / mtiles;
+    t.nt = rest % ntiles;
+    t.batch = rest / ntiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return t.m0 <= t.batch;
+  }
+  __device__ int ksteps() const { return ldk / kBK; }
+  __device__ SrcRows a_src(const Tile&) const { return {ct, 0, ldk, nbf5}; }
+  __device__ SrcXT b_src(const Tile& t) const { return {x, (int64_t)t.batch * nbf * naux_p, naux_p, nbf}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double* lds) const {
+    const int p = t.batch;
+    if (p >= t.m0 && p < t.m0 + 32 * FM) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            int dm, dn;
+            coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+            if (t.m0 + dm == p) {
+              const int64_t o = (int64_t)p * naux_p + t.n0 + dn;
+              const float v = acc[i][j][r];
+              d[o] = v;
+              const uint16_t h = bf16_rne(v);
+              dhi[o] = h;
+              dlo[o] = bf16_rne(v - bf16_to_f(h));
+            }
+          }
+    }
+    double s[FM][4];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < FN; ++j) v += (double)acc[i][j][r] * (double)acc[i][j][r];
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
+        s[i][r] = v;
+      }
+    const int wave = threadIdx.x >> 6;
+    if ((wave & 1) && (lane & 15) == 0) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) lds[wm + 16 * i + out_row(lane, r)] = s[i][r];
+    }
+    __syncthreads();
+    if (!(wave & 1) && (lane & 15) == 0) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int dm = wm + 16 * i + out_row(lane, r);
+          const int q = t.m0 + dm;
+          if (q <= p && q < nbf5) kpart[((int64_t)p * nbf5 + q) * ntiles + t.nt] = s[i][r] + lds[dm];
+        }
+    }
+  }
+};
+
+// J[q][mu][nu] = sum_P d16[q][P] btri16[t][P], full symmetric FP64 output
+struct ProbJN {
+  static constexpr int kA = RK, kB = RK;
+  Planes dd, bt;
+  double* J;
+  int nbf, naux_p, nbf5, mtiles;
+  int64_t ntri;
+  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.batch = 0;
+    t.mt = id % mtiles;
+    t.nt = id / mtiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ int ksteps() const { return naux_p / kBK; }
+  __device__ SrcRows a_src(const Tile&) const { return {dd, 0, naux_p, nbf5}; }
+  __device__ SrcRows b_src(const Tile&) const { return {bt, 0, naux_p, (int)ntri}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+    const int64_t nn = (int64_t)nbf * nbf;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      int dm, dn;
+      coords<FM, FN>(lane, wm, wn, 0, j, 0, dm, dn);
+      const int64_t tt = (int64_t)t.n0 + dn;
+      if (tt >= ntri) continue;
+      const int mu = tri_row(tt);
+      const int nu = (int)(tt - (int64_t)mu * (mu + 1) / 2);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int q = t.m0 + dm;
+          if (q < nbf5) {
+            double* jq = J + (int64_t)q * nn;
+            const double v = acc[i][j][r];
+            jq[(int64_t)mu * nbf + nu] = v;
+            if (mu != nu) jq[(int64_t)nu * nbf + mu] = v;
+          }
+        }
+    }
+  }
+};
+
+// K[b][i][j] = sum_P X16[b][i][P] X16[b][j][P], lower tile triangle, mirrored FP64 output
+struct ProbSYRK {
+  static constexpr int kA = RK, kB = RK;
+  Planes x;
+  double* O;
+  int64_t lda, sa, ldo, so;
+  int M, K, tiles_per_batch;
+  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.batch = id / tiles_per_batch;
+    const int rem = id - t.batch * tiles_per_batch;
+    int mt = (int)((sqrt(8.0 * rem + 1.0) - 1.0) * 0.5);
+    while ((mt + 1) * (mt + 2) / 2 <= rem) ++mt;
+    while (mt * (mt + 1) / 2 > rem) --mt;
+    t.mt = mt;
+    t.nt = rem - mt * (mt + 1) / 2;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ int ksteps() const { return K / kBK; }
+  __device__ SrcRows a_src(const Tile& t) const { return {x, (int64_t)t.batch * sa, lda, M}; }
+  __device__ SrcRows b_src(const Tile& t) const { return {x, (int64_t)t.batch * sa, lda, M}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+    double* o = O + (int64_t)t.batch * so;
+    const bool mirror = t.mt != t.nt;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int m = t.m0 + dm, n = t.n0 + dn;
+          if (m < M && n < M) {
+            const double v = acc[i][j][r];
+            o[(int64_t)m * ldo + n] = v;
+            if (mirror) o[(int64_t)n * ldo + m] = v;
+          }
+        }
+  }
+};
+
+// ---------------------------------------------------------------- conversions
+__global__ void k_split_packed(const double* __restrict__ src, int64_t n, uint16_t* __restrict__ hi,
+                               uint16_t* __restrict__ lo) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint16_t h, l;
+  split2(src[i], h, l);
+  hi[i] = h;
+  lo[i] = l;
+}
+
+// bht[nu][P][mu] (mu < ldk, zero for mu >= nbf) from btri[tri(mu,nu)][P]:
+// a 32 (mu) x 64 (P) tile through LDS per workgroup; grid (P tiles, mu tiles, nu).
+__global__ void k_build_bht(const double* __restrict__ btri, int nbf, int naux_p, int ldk, uint16_t* __restrict__ hi,
+                            uint16_t* __restrict__ lo) {
+  __shared__ double tileb[32][65];
+  const int nu = blockIdx.z, mu0 = blockIdx.y * 32, p0 = blockIdx.x * 64;
+  for (int e = threadIdx.x; e < 32 * 64; e += blockDim.x) {
+    const int m = e / 64, pp = e % 64;
+    const int mu = mu0 + m, p = p0 + pp;
+    double v = 0.0;
+    if (mu < nbf && p < naux_p) v = btri[tri_index(mu, nu) * naux_p + p];
+    tileb[m][pp] = v;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 32 * 64; e += blockDim.x) {
+    const int pp = e / 32, m = e % 32;
+    const int mu = mu0 + m, p = p0 + pp;
+    if (p < naux_p && mu < ldk) {
+      uint16_t h, l;
+      split2(tileb[m][pp], h, l);
+      const int64_t o = ((int64_t)nu * naux_p + p) * ldk + mu;
+      hi[o] = h;
+      lo[o] = l;
+    }
+  }
+}
+
+// Ct[q][mu] (q < rows, mu < ldk) from Cp[mu][q] (zero outside nbf x nbf5).
+__global__ void k_split_ct(const double* __restrict__ Cp, int ldcp, int nbf, int nbf5, int rows, int ldk,
+                           uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)rows * ldk) return;
+  const int q = (int)(i / ldk), mu = (int)(i % ldk);
+  const double v = (q < nbf5 && mu < nbf) ? Cp[(int64_t)mu * ldcp + q] : 0.0;
+  uint16_t h, l;
+  split2(v, h, l);
+  hi[i] = h;
+  lo[i] = l;
+}
+
+template <class Prob>
+hipError_t launch_x3(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  const dim3 grid(nblocks), block(kThreads);
+#define PNJK_CASE(FMV, FNV)                                                   \
+  if (tc.fm == FMV && tc.fn == FNV) {                                         \
+    hipLaunchKernelGGL((k_gemm_x3<FMV, FNV, Prob>), grid, block, 0, s, prob); \
+    return hipGetLastError();                                                 \
+  }
+  PNJK_CASE(2, 2)
+  PNJK_CASE(4, 4)
+  PNJK_CASE(5, 4)
+  PNJK_CASE(4, 2)
+#undef PNJK_CASE
+  return hipErrorInvalidValue;
+}
+
+inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+}  // namespace
+
+hipError_t launch_split_btri(const double* btri, int64_t n, uint16_t* hi, uint16_t* lo, hipStream_t s) {
+  hipLaunchKernelGGL(k_split_packed, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, btri, n, hi, lo);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_bht(const double* btri, int nbf, int naux_p, int ldk, uint16_t* hi, uint16_t* lo,
+                            hipStream_t s) {
+  const dim3 grid(cdiv(naux_p, 64), cdiv(ldk, 32), nbf);
+  hipLaunchKernelGGL(k_build_bht, grid, dim3(256), 0, s, btri, nbf, naux_p, ldk, hi, lo);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_ct(const double* Cp, int ldcp, int nbf, int nbf5, int rows, int ldk, uint16_t* hi,
+                           uint16_t* lo, hipStream_t s) {
+  const int64_t n = (int64_t)rows * ldk;
+  hipLaunchKernelGGL(k_split_ct, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, Cp, ldcp, nbf, nbf5, rows, ldk, hi,
+                     lo);
+  return hipGetLastError();
+}
+
+hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
+                        const uint16_t* bht_hi, const uint16_t* bht_lo, uint16_t* x_hi, uint16_t* x_lo,
+                        hipStream_t s) {
+  ProbHT pr;
+  pr.ct = {ct_hi, ct_lo};
+  pr.bht = {bht_hi, bht_lo};
+  pr.xhi = x_hi;
+  pr.xlo = x_lo;
+  pr.nbf5 = g.nbf5;
+  pr.ldk = g.ldk;
+  pr.N = (int64_t)g.nbf * g.naux_p;
+  const int BM = 32 * tc.fm, BN = 32 * tc.fn;
+  pr.mtiles = cdiv(g.nbf5, BM);
+  return launch_x3(pr, tc, pr.mtiles * cdiv(pr.N, BN), s);
+}
+
+hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
+                        const uint16_t* x_hi, const uint16_t* x_lo, double* kpart, double* d, uint16_t* d_hi,
+                        uint16_t* d_lo, int* ntiles, hipStream_t s) {
+  const int BM = 32 * tc.fm, BN = 32 * tc.fn;
+  if (g.naux_p % BN != 0) return hipErrorInvalidValue;
+  ProbQT pr;
+  pr.ct = {ct_hi, ct_lo};
+  pr.x = {x_hi, x_lo};
+  pr.kpart = kpart;
+  pr.d = d;
+  pr.dhi = d_hi;
+  pr.dlo = d_lo;
+  pr.nbf = g.nbf;
+  pr.naux_p = g.naux_p;
+  pr.nbf5 = g.nbf5;
+  pr.ldk = g.ldk;
+  pr.mtiles = cdiv(g.nbf5, BM);
+  pr.ntiles = g.naux_p / BN;
+  *ntiles = pr.ntiles;
+  return launch_x3(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s);
+}
+
+hipError_t launch_jn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* d_hi, const uint16_t* d_lo,
+                        const uint16_t* bt_hi, const uint16_t* bt_lo, double* J, hipStream_t s) {
+  const int BM = 32 * tc.fm, BN = 32 * tc.fn;
+  ProbJN pr;
+  pr.dd = {d_hi, d_lo};
+  pr.bt = {bt_hi, bt_lo};
+  pr.J = J;
+  pr.nbf = g.nbf;
+  pr.naux_p = g.naux_p;
+  pr.nbf5 = g.nbf5;
+  pr.ntri = (int64_t)g.nbf * (g.nbf + 1) / 2;
+  pr.mtiles = cdiv(g.nbf5, BM);
+  return launch_x3(pr, tc, pr.mtiles * cdiv(pr.ntri, BN), s);
+}
+
+hipError_t launch_kn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* x_hi, const uint16_t* x_lo, double* K,
+                        hipStream_t s) {
+  if (tc.fm != tc.fn) return hipErrorInvalidValue;
+  const int BM = 32 * tc.fm;
+  ProbSYRK pr;
+  pr.x = {x_hi, x_lo};
+  pr.O = K;
+  pr.lda = g.naux_p;
+  pr.sa = (int64_t)g.nbf * g.naux_p;
+  pr.ldo = g.nbf;
+  pr.so = (int64_t)g.nbf * g.nbf;
+  pr.M = g.nbf;
+  pr.K = g.naux_p;
+  const int side = cdiv(g.nbf, BM);
+  pr.tiles_per_batch = side * (side + 1) / 2;
+  return launch_x3(pr, tc, pr.tiles_per_batch * g.nbf5, s);
+}
+
+}  // namespace pnjk

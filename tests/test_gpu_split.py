@@ -1,0 +1,83 @@
+"""Split-precision forms against the FP64 oracle (GPU only).
+
+split:        every contraction in BF16x3 (hi*hi + hi*lo + lo*hi, FP32 accumulation).
+              Bar: normwise relative error <= 3e-5 for J, K, J_MO, K_MO (a BF16x3
+              product carries ~2^-16 relative error); H_core is FP64 (1e-10).
+split_refine: J_MO, K_MO, H_core through the FP64 path (bar 1e-10, energy 1e-10
+              relative); J, K in BF16x3 from the exact half-transform (3e-5).
+"""
+import numpy as np
+import pytest
+
+from oracle import jk_oracle as O
+from pynof_amd import JKEngine, synth
+
+pytestmark = pytest.mark.gpu
+SPLIT_TOL = 3e-5
+FP64_TOL = 1e-10
+NAMES = ("J", "K", "J_MO", "K_MO", "H_core")
+
+
+def rel(a, r):
+    return float(np.max(np.abs(np.asarray(a) - r)) / max(1e-300, np.max(np.abs(r))))
+
+
+@pytest.fixture(scope="module")
+def eng(gpu_required):
+    e = JKEngine(0)
+    yield e
+    e.close()
+
+
+SHAPES = [(1, 1, 1), (5, 7, 3), (24, 116, 24), (40, 144, 40), (65, 130, 64), (130, 70, 130), (161, 96, 161),
+          (97, 520, 33)]
+
+
+@pytest.mark.parametrize("nbf,naux,nbf5", SHAPES)
+def test_split_shapes(eng, nbf, naux, nbf5):
+    print(f"case: split shape nbf={nbf} naux={naux} nbf5={nbf5}", flush=True)
+    b = synth.make_b(nbf, naux, 500 + nbf, 0.05)
+    C = synth.make_C(nbf, 600 + nbf)
+    H = synth.make_H(nbf, 700 + nbf)
+    eng.upload_b(b)
+    ref = dict(zip(NAMES, O.jk_build(C, H, b, nbf5)))
+    s = eng.build(C, nbf5, H=H, precision="split")
+    for k in ("J", "K", "J_MO", "K_MO"):
+        assert rel(s[k], ref[k]) <= SPLIT_TOL, (k, rel(s[k], ref[k]))
+    assert rel(s["H_core"], ref["H_core"]) <= FP64_TOL
+    r = eng.build(C, nbf5, H=H, precision="split_refine")
+    for k in ("J_MO", "K_MO", "H_core"):
+        assert rel(r[k], ref[k]) <= FP64_TOL, (k, rel(r[k], ref[k]))
+    for k in ("J", "K"):
+        assert rel(r[k], ref[k]) <= SPLIT_TOL, (k, rel(r[k], ref[k]))
+
+
+def test_split_energies_mid(eng):
+    w = synth.workload(1)
+    print(f"case: split energies {w.name}", flush=True)
+    b, C, H, n = synth.make_inputs(w)
+    eng.upload_b(b)
+    ref = O.jk_build(C, H, b, w.nbf5)
+    e_ref = O.energy(n, *ref[2:])
+    out = {}
+    for prec in ("fp64", "split", "split_refine"):
+        r = eng.build(C, w.nbf5, H=H, precision=prec)
+        e = O.energy(n, r["J_MO"], r["K_MO"], r["H_core"])
+        out[prec] = (e - e_ref, {k: rel(r[k], a) for k, a in zip(NAMES, ref)})
+        print(prec, "dE", e - e_ref, out[prec][1], flush=True)
+    assert abs(out["fp64"][0]) <= FP64_TOL * abs(e_ref)
+    assert abs(out["split_refine"][0]) <= FP64_TOL * abs(e_ref)
+    assert abs(out["split"][0]) <= 1e-6 * abs(e_ref)  # BF16x3 energy: ~1e-7 relative expected
+    for k in ("J", "K", "J_MO", "K_MO"):
+        assert out["split"][1][k] <= SPLIT_TOL
+
+
+def test_split_is_deterministic(eng):
+    print("case: split determinism nbf=90 naux=200 nbf5=60", flush=True)
+    b = synth.make_b(90, 200, 9, 0.05)
+    C = synth.make_C(90, 10)
+    eng.upload_b(b)
+    a = eng.build(C, 60, precision="split", outputs=("J", "K", "J_MO", "K_MO"))
+    c = eng.build(C, 60, precision="split", outputs=("J", "K", "J_MO", "K_MO"))
+    for k in a:
+        assert np.array_equal(a[k], c[k]), k
