@@ -4,12 +4,21 @@
 
 One step is one fused JK build (J^n, K^n, J_MO, K_MO, H_core for one set of
 orbitals C) against a device-resident b, with C already on the device.
-Default workload: nbf=1000, naux=3600, nbf5=300, b decay 0.02 (the size the
-headline builds/s metric is quoted on). Prints one JSON line.
+Default workload: nbf=1000, naux=3600, nbf5=300, b decay 0.02, the size the
+headline builds/s metric is quoted on.
+
+The timed region runs K builds as orbital-optimisation requests of 20 builds:
+C is rotated by a seeded exp(kappa), ||kappa||_F = 1e-2, between builds; the
+first 19 builds of a request use the split form and the last one, whose energy
+the request reports, uses split_refine (J_MO, K_MO, H_core through the FP64
+path). `value` is builds/s of that stream. The same run also times every form
+alone (fp64, split, split_refine) and states each form's errors against the
+FP64 form at this size. Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import platform
@@ -22,7 +31,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "JK builds/s at nbf=1000,naux=3600 split vs FP64 plain @1e-9 Eh; p50/p99 latency"
-PEAKS = {"fp64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
+PEAK_FP64 = 78.6      # TFLOP/s, MI355X dense FP64 MFMA
+PEAK_BF16 = 2500.0    # TFLOP/s, MI355X dense BF16 MFMA
+REQUEST = 20          # builds per orbital-optimisation request
+OUTS = ("J", "K", "J_MO", "K_MO", "H_core")
 
 
 def cpu_model() -> str:
@@ -64,18 +76,23 @@ def cpu_baseline(w, sample_naux: int):
     }
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="large", choices=["water", "mid", "large"])
+    ap.add_argument("--form-steps", type=int, default=5, help="builds timed per single form")
     ap.add_argument("--cpu-sample-naux", type=int, default=72)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every box here has one MI355X, so --gpus must be 1 "
-              "(a multi-GPU build would shard orbitals q across GPUs; see the design notes)", file=sys.stderr)
+              "(a multi-GPU build would shard the orbitals q over GPUs and all-gather d)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -84,78 +101,125 @@ def main():
 
     k = {"water": 0, "mid": 1, "large": 2}[args.workload]
     w = synth.workload(k)
+    nbf, naux, nbf5 = w.nbf, w.naux, w.nbf5
     dev = torch.device("cuda", 0)
-    print(f"[bench] workload {w.name}: generating b on the device", file=sys.stderr, flush=True)
+    log(f"workload {w.name}: b on the device")
     eng = JKEngine(0)
     if k == 2:
-        btri = synth.device_b_packed(w.nbf, w.naux, w.seed_b, w.lam, JKEngine.packed_ld(w.nbf, w.naux))
-        eng.upload_b_packed(btri, w.nbf, w.naux)
+        btri = synth.device_b_packed(nbf, naux, w.seed_b, w.lam, JKEngine.packed_ld(nbf, naux))
+        eng.upload_b_packed(btri, nbf, naux)
         del btri
         torch.cuda.empty_cache()
-        data = "synthetic: b ~ N(0,1) symmetrised, x 0.3 exp(-0.02|mu-nu|), drawn on the device (torch Philox)"
+        data = "synthetic: b ~ N(0,1) symmetrised x 0.3 exp(-0.02|mu-nu|) drawn on the device; seeded C, H, n"
     else:
         b, _, _, _ = synth.make_inputs(w)
         eng.upload_b(b)
         data = "synthetic: seeded numpy b/C/H/n generators"
-    C0 = synth.make_C(w.nbf, w.seed_c)
-    Cs = [C0]
-    for j in range(2):
-        Cs.append(Cs[-1] @ synth.rotation(w.nbf, 6000 + j))
+    n_occ = synth.make_n(nbf5, w.ndoc, w.seed_n)
+    # C sequence of one request: C_{i+1} = C_i exp(kappa_i)
+    Cs = [synth.make_C(nbf, w.seed_c)]
+    for j in range(REQUEST - 1):
+        Cs.append(Cs[-1] @ synth.rotation(nbf, 6000 + j))
     Cd = [torch.from_numpy(np.ascontiguousarray(c)).to(dev) for c in Cs]
-    Hd = torch.from_numpy(synth.make_H(w.nbf, w.seed_h)).to(dev)
-    nbf, naux, nbf5 = w.nbf, w.naux, w.nbf5
-    out = {
-        "J": torch.empty((nbf5, nbf, nbf), dtype=torch.float64, device=dev),
-        "K": torch.empty((nbf5, nbf, nbf), dtype=torch.float64, device=dev),
-        "J_MO": torch.empty((nbf5, nbf5), dtype=torch.float64, device=dev),
-        "K_MO": torch.empty((nbf5, nbf5), dtype=torch.float64, device=dev),
-        "H_core": torch.empty((nbf5,), dtype=torch.float64, device=dev),
-    }
+    Hd = torch.from_numpy(synth.make_H(nbf, w.seed_h)).to(dev)
+
+    def alloc():
+        return {"J": torch.empty((nbf5, nbf, nbf), dtype=torch.float64, device=dev),
+                "K": torch.empty((nbf5, nbf, nbf), dtype=torch.float64, device=dev),
+                "J_MO": torch.empty((nbf5, nbf5), dtype=torch.float64, device=dev),
+                "K_MO": torch.empty((nbf5, nbf5), dtype=torch.float64, device=dev),
+                "H_core": torch.empty((nbf5,), dtype=torch.float64, device=dev)}
+
+    out = alloc()
     stream = torch.cuda.current_stream(dev).cuda_stream
     L = _lib.lib()
     nst = L.pnjk_stage_count()
     names = [L.pnjk_stage_name(i).decode() for i in range(nst)]
-    n_out = ("J", "K", "J_MO", "K_MO", "H_core")
-
-    def step(i):
-        eng.build(Cd[i % len(Cd)], nbf5, H=Hd, precision="fp64", outputs=n_out, out=out, stream=stream)
-
-    print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
-    for i in range(args.warmup):
-        step(i)
-    torch.cuda.synchronize()
-    _lib.check(L.pnjk_set_profiling(eng._h, 1))
-    import ctypes
-
     st = (ctypes.c_double * nst)()
-    stage_ms = np.zeros((args.steps, nst))
-    lat = []
-    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        ts = time.perf_counter()
-        step(i)
-        _lib.check(L.pnjk_stage_times(eng._h, st))  # waits for this build's last event
-        lat.append(time.perf_counter() - ts)
-        stage_ms[i] = np.frombuffer(st, dtype=np.float64)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    _lib.check(L.pnjk_set_profiling(eng._h, 0))
-    ms_per_step = 1e3 * elapsed / args.steps
 
+    def build(i, form, o=None):
+        eng.build(Cd[i % REQUEST], nbf5, H=Hd, precision=form, outputs=OUTS, out=o or out, stream=stream)
+
+    def timed(forms, steps):
+        """Runs `steps` builds, form forms(i); returns (seconds, latencies, stage ms per build)."""
+        _lib.check(L.pnjk_set_profiling(eng._h, 1))
+        lat, stages = [], []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            ts = time.perf_counter()
+            build(i, forms(i))
+            _lib.check(L.pnjk_stage_times(eng._h, st))  # waits for this build's last event
+            lat.append(time.perf_counter() - ts)
+            stages.append((forms(i), np.frombuffer(st, dtype=np.float64).copy()))
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        _lib.check(L.pnjk_set_profiling(eng._h, 0))
+        return el, lat, stages
+
+    def energy(o):
+        e1, e2 = synth.energy_terms(n_occ, o["J_MO"].cpu().numpy(), o["K_MO"].cpu().numpy(),
+                                    o["H_core"].cpu().numpy())
+        return e1 + e2
+
+    # ---- every form alone, and its errors against the FP64 form
     flops = workcount.stage_flops(nbf, naux, nbf5)
-    avg = stage_ms.mean(axis=0)
-    stages = {names[i]: round(float(avg[i]), 4) for i in range(nst)}
-    dom = max((n for n in names if n in flops), key=lambda n: stages[n])
-    dom_ms = stages[dom]
-    achieved = flops[dom] / (dom_ms * 1e-3) / 1e12
-    peak = PEAKS["fp64_mfma_tflops"]
-    roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": None,
-                "per_stage_tflops": {n: round(flops[n] / (stages[n] * 1e-3) / 1e12, 3)
-                                     for n in flops if stages.get(n, 0) > 0}}
     total_flops = workcount.build_flops(nbf, naux, nbf5)
+    ref = alloc()
+    forms = {}
+    for form in ("fp64", "split", "split_refine"):
+        log(f"form {form}: warmup {args.warmup}, {args.form_steps} builds")
+        for i in range(args.warmup):
+            build(i, form)
+        el, lat, stg = timed(lambda i, f=form: f, args.form_steps)
+        avg = np.mean([s for _, s in stg], axis=0)
+        res = {"builds_per_s": round(args.form_steps / el, 3), "ms_per_build": round(1e3 * el / args.form_steps, 3),
+               "stage_ms": {names[i]: round(float(avg[i]), 4) for i in range(nst)}}
+        o = ref if form == "fp64" else out
+        build(REQUEST - 1, form, o)
+        torch.cuda.synchronize()
+        if form == "fp64":
+            e_ref = energy(ref)
+            res["energy_Eh"] = e_ref
+        else:
+            e = energy(out)
+            res["energy_err_Eh"] = e - e_ref
+            res["rel_err_vs_fp64"] = {
+                nm: float(torch.max(torch.abs(out[nm] - ref[nm])) / torch.max(torch.abs(ref[nm]))) for nm in OUTS}
+        forms[form] = res
+    del ref
+    torch.cuda.empty_cache()
+
+    # ---- headline: requests of 20 builds, split x19 + split_refine x1
+    def seq_form(i):
+        return "split_refine" if (i + 1) % REQUEST == 0 else "split"
+
+    log(f"headline stream: warmup {args.warmup}, {args.steps} builds")
+    for i in range(args.warmup):
+        build(i, "split")
+    el, lat, stg = timed(seq_form, args.steps)
+    ms_per_step = 1e3 * el / args.steps
+    # final energy of the request (the split_refine build) against the FP64 energy of the same C
+    build(REQUEST - 1, "split_refine")
+    torch.cuda.synchronize()
+    e_final = energy(out)
+
+    # roofline of the dominant kernel of the stream's split builds (events over the timed region)
+    split_stages = np.mean([s for f, s in stg if f == "split"] or [stg[0][1]], axis=0)
+    sms = {names[i]: float(split_stages[i]) for i in range(nst)}
+    dom = max((n for n in names if n in flops), key=lambda n: sms[n])
+    achieved = flops[dom] / (sms[dom] * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_BF16,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16, 4), "traffic": None,
+                "note": "BF16x3: each useful multiply-add issues 3 bf16 MFMA products; "
+                        f"issued rate {3 * achieved:.1f} TFLOP/s = {3 * achieved / PEAK_BF16:.3f} of peak",
+                "per_stage_tflops": {n: round(flops[n] / (sms[n] * 1e-3) / 1e12, 3) for n in flops if sms[n] > 0}}
+    fp64_stages = forms["fp64"]["stage_ms"]
+    fd = max((n for n in flops), key=lambda n: fp64_stages[n])
+    roofline_fp64 = {"bound": "mfma", "kernel": fd, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                     "achieved": round(flops[fd] / (fp64_stages[fd] * 1e-3) / 1e12, 3)}
+    roofline_fp64["frac"] = round(roofline_fp64["achieved"] / PEAK_FP64, 4)
+
     line = {
         "metric": METRIC,
         "value": round(1e3 / ms_per_step, 4),
@@ -167,21 +231,33 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16x3 (fp32 accumulate) + f64 refinement",
         "data": data,
-        "config": {"workload": w.name, "nbf": nbf, "naux": naux, "nbf5": nbf5, "form": "fp64",
-                   "outputs": list(n_out), "device_resident": True},
+        "config": {"workload": w.name, "nbf": nbf, "naux": naux, "nbf5": nbf5, "ndoc": w.ndoc,
+                   "form": f"orbital-optimisation requests of {REQUEST} builds: split x{REQUEST - 1} + "
+                           "split_refine x1 (final energy)",
+                   "outputs": list(OUTS), "device_resident": True},
+        "speedup_vs_fp64": round((1e3 / ms_per_step) / forms["fp64"]["builds_per_s"], 3),
+        "final_energy_err_Eh": e_final - forms["fp64"]["energy_Eh"],
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
-        "build_tflops": round(total_flops / (ms_per_step * 1e-3) / 1e12, 3),
-        "stage_ms": stages,
+        "forms": forms,
+        "fp64_build_tflops": round(total_flops * forms["fp64"]["builds_per_s"] / 1e12, 3),
         "roofline": roofline,
+        "roofline_fp64": roofline_fp64,
     }
     eng.close()
     del out, Cd, Hd
     torch.cuda.empty_cache()
+    # A plain 1 GiB device copy: the reference kernel for scaling FETCH_SIZE/WRITE_SIZE
+    # counters of profiled runs (known bytes moved).
+    src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
     if not args.no_cpu_baseline:
-        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(w, min(args.cpu_sample_naux, w.naux))
     else:
         line["cpu_baseline"] = None
