@@ -100,6 +100,14 @@ int pnjk_stage_count(void);
 const char* pnjk_stage_name(int i);
 int pnjk_stage_times(pnjk_engine* e, double* ms);
 
+/* Tile shape override for the kernels of one arithmetic (0: FP64 MFMA, 1: BF16x3)
+ * and one stage (index as in pnjk_stage_name: 1 half_transform, 2
+ * quarter_transform, 5 coulomb_jn, 6 exchange_kn), used by the autotuner;
+ * BM = 32*fm rows, BN = 32*fn columns per workgroup. fm = fn = 0 restores the
+ * built-in choice. No counterpart in the reference. A tile shape changes no
+ * result bit: each output's summation order is fixed by the MFMA sequence. */
+int pnjk_set_tile(pnjk_engine* e, int arith, int stage, int fm, int fn);
+
 /* Device bytes the engine holds now (b plus workspace). */
 uint64_t pnjk_footprint(const pnjk_engine* e);
 

@@ -38,6 +38,7 @@ _SIGS = {
                                      ctypes.c_void_p, ctypes.c_void_p]),
     "pnjk_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "pnjk_stage_count": (ctypes.c_int, []),
+    "pnjk_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pnjk_stage_name": (ctypes.c_char_p, [ctypes.c_int]),
     "pnjk_stage_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "pnjk_footprint": (ctypes.c_uint64, [ctypes.c_void_p]),

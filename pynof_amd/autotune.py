@@ -1,0 +1,118 @@
+"""Tile-shape autotuner for the JK build stages.
+
+For one workload shape and precision it times every supported tile shape of
+each tunable stage (half_transform, quarter_transform, coulomb_jn, exchange_kn)
+with the engine's HIP-event stage timer, and keeps the fastest. Every output
+element's k-summation order is fixed by the MFMA sequence, so a tile shape
+changes no result bit (tests/test_gpu_tiles.py checks this); the tuned table is
+therefore free to differ between batch mixes without changing results.
+
+Tables are JSON: {"<precision>:<nbf>:<naux>:<nbf5>": {"<stage>": [fm, fn], ...}}.
+The "fp64" entry tunes the FP64 kernels, the "split" entry the BF16x3 kernels
+(the split_refine form uses the FP64 choice for its half- and quarter-transform
+and the split choice for J^n and K^n).
+``lookup`` returns the entry of the nearest tuned shape (log-distance in nbf,
+naux, nbf5), so instances of untuned sizes get the choice of their nearest
+tuned neighbour.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import math
+import os
+
+from . import _lib
+
+STAGES = {"half_transform": 1, "quarter_transform": 2, "coulomb_jn": 5, "exchange_kn": 6}
+CANDIDATES = {
+    "half_transform": [(2, 2), (4, 4), (5, 4), (4, 2), (2, 4), (5, 2)],
+    "quarter_transform": [(2, 2), (4, 2), (5, 2)],
+    "coulomb_jn": [(2, 2), (4, 4), (5, 4), (4, 2), (2, 4), (5, 2)],
+    "exchange_kn": [(2, 2), (4, 4)],
+}
+DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_tiles.json")
+
+
+def key(precision: str, nbf: int, naux: int, nbf5: int) -> str:
+    return f"{precision}:{nbf}:{naux}:{nbf5}"
+
+
+def stage_times(eng, build, reps: int = 2):
+    """Runs `build()` reps times with stage profiling on; returns the min time per stage (ms)."""
+    L = _lib.lib()
+    n = L.pnjk_stage_count()
+    names = [L.pnjk_stage_name(i).decode() for i in range(n)]
+    buf = (ctypes.c_double * n)()
+    best = {nm: float("inf") for nm in names}
+    _lib.check(L.pnjk_set_profiling(eng._h, 1))
+    try:
+        for _ in range(reps):
+            build()
+            _lib.check(L.pnjk_stage_times(eng._h, buf))
+            for i, nm in enumerate(names):
+                best[nm] = min(best[nm], buf[i])
+    finally:
+        _lib.check(L.pnjk_set_profiling(eng._h, 0))
+    return best
+
+
+ARITH = {"fp64": 0, "split": 1}
+
+
+def tune(eng, build, arith: str, reps: int = 2, stages=None, log=None):
+    """Tunes the tile of each stage of the `arith` kernels ("fp64" or "split") in
+    turn, others at their current choice. `build()` runs one build on `eng` in a
+    form that uses those kernels. Returns {stage: [fm, fn]}."""
+    L = _lib.lib()
+    a = ARITH[arith]
+    chosen = {}
+    for st in stages or STAGES:
+        idx = STAGES[st]
+        timings = {}
+        for fm, fn in CANDIDATES[st]:
+            if L.pnjk_set_tile(eng._h, a, idx, fm, fn) != _lib.OK:
+                continue
+            build()  # warm the shape's code object and workspace
+            timings[(fm, fn)] = stage_times(eng, build, reps)[st]
+        best = min(timings, key=timings.get)
+        _lib.check(L.pnjk_set_tile(eng._h, a, idx, *best))
+        chosen[st] = list(best)
+        if log:
+            log(f"{st}: " + ", ".join(f"{k}={v:.3f}ms" for k, v in sorted(timings.items())) + f" -> {best}")
+    return chosen
+
+
+def load(path: str = DEFAULT_TABLE) -> dict:
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def save(table: dict, path: str = DEFAULT_TABLE) -> None:
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+
+
+def lookup(table: dict, precision: str, nbf: int, naux: int, nbf5: int):
+    """Entry of the nearest tuned shape of the same precision, or None."""
+    best, dist = None, float("inf")
+    for k, v in table.items():
+        p, a, b_, c = k.split(":")
+        if p != precision:
+            continue
+        d = (math.log(int(a) / nbf) ** 2 + math.log(int(b_) / max(naux, 1)) ** 2
+             + math.log(int(c) / nbf5) ** 2)
+        if d < dist:
+            best, dist = v, d
+    return best
+
+
+def apply(eng, arith: str, entry) -> None:
+    """Sets the engine's stage tiles of the `arith` kernels from a table entry
+    (None restores the built-in choices)."""
+    L = _lib.lib()
+    for st, idx in STAGES.items():
+        fm, fn = (entry or {}).get(st, (0, 0))
+        _lib.check(L.pnjk_set_tile(eng._h, ARITH[arith], idx, int(fm), int(fn)))

@@ -33,6 +33,20 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
   return x * per + (x < rem ? x : rem) + i;
 }
 
+// bf16 bit pattern of f, rounded to nearest-even (inputs are finite).
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// x ~ hi + lo with hi = bf16(x), lo = bf16(x - hi).
+__device__ __forceinline__ void split2(double x, uint16_t& hi, uint16_t& lo) {
+  hi = bf16_rne((float)x);
+  lo = bf16_rne((float)(x - (double)bf16_to_f(hi)));
+}
+
 // Row index of the packed lower triangle: t = mu*(mu+1)/2 + nu for mu >= nu.
 __device__ __forceinline__ int64_t tri_index(int a, int b) {
   const int64_t hi = a > b ? a : b, lo = a > b ? b : a;

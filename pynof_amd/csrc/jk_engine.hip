@@ -83,10 +83,25 @@ bool is_device_ptr(const void* p) {
   return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
-TileF64 pick_ht_tile(int nbf5) {
-  if (nbf5 > 128) return {5, 4};
-  if (nbf5 > 64) return {4, 4};
-  return {2, 2};
+// Built-in tile shapes (from autotuning at nbf=1000, naux=3600, nbf5=300 on MI355X;
+// small problems keep 64 x 64 tiles so the grid fills the chip).
+TileF64 default_tile(int arith, int stage_kind, int nbf, int nbf5) {
+  // stage_kind: 0 HT, 1 QT, 2 JN, 3 K
+  const bool big = nbf5 > 128 && nbf >= 256;
+  if (arith == 0) {
+    switch (stage_kind) {
+      case 0: return big ? TileF64{5, 4} : TileF64{2, 2};
+      case 1: return TileF64{2, 2};
+      case 2: return big ? TileF64{2, 4} : TileF64{2, 2};
+      default: return TileF64{2, 2};
+    }
+  }
+  switch (stage_kind) {
+    case 0: return big ? TileF64{4, 4} : TileF64{2, 2};
+    case 1: return big ? TileF64{5, 2} : TileF64{2, 2};
+    case 2: return big ? TileF64{4, 4} : TileF64{2, 2};
+    default: return nbf >= 256 ? TileF64{4, 4} : TileF64{2, 2};
+  }
 }
 
 }  // namespace
@@ -113,6 +128,9 @@ struct pnjk_engine {
   DevBuf Cp, Hp, X, d, kpart, T, outs, stage, scal;
   DevBuf ct16h, ct16l, x16h, x16l, d16h, d16l;  // split workspace
   bool split_ready = false;
+  // Tile override per arithmetic (0: FP64 kernels, 1: BF16x3 kernels) and stage;
+  // {0, 0} selects the built-in choice.
+  TileF64 tiles[2][ST_COUNT] = {};
   DevBuf* all[22];
   int nall = 0;
   pnjk_engine() {
@@ -376,9 +394,15 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
                                     hipMemcpyHostToDevice, s));
   }
 
-  const TileF64 ht = pick_ht_tile(g.nbf5);
-  const TileF64 qt = g.nbf5 > 64 ? TileF64{4, 2} : TileF64{2, 2};
-  const TileF64 kt = g.nbf >= 256 ? TileF64{4, 4} : TileF64{2, 2};
+  // FP64 kernels run HT/QT in the fp64 and split_refine forms, J^n/K^n in fp64 only.
+  const int a_mo = precision == PNJK_SPLIT ? 1 : 0, a_jk = precision == PNJK_FP64 ? 0 : 1;
+  auto chosen = [&](int arith, int st, int kind) {
+    return e->tiles[arith][st].fm ? e->tiles[arith][st] : default_tile(arith, kind, g.nbf, g.nbf5);
+  };
+  const TileF64 ht = chosen(a_mo, ST_HT, 0);
+  const TileF64 qt = chosen(a_mo, ST_QT, 1);
+  const TileF64 jt = chosen(a_jk, ST_JN, 2);
+  const TileF64 kt = chosen(a_jk, ST_K, 3);
   int nt = 0;
   SplitGeom sg;
   sg.nbf = g.nbf;
@@ -401,6 +425,9 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
     PNJK_HIP_CHECK(launch_split_ct(Cp, g.ldcp, g.nbf, g.nbf5, g.nbf5, sg.ldk, ct_h, ct_l, s));
     PNJK_HIP_CHECK(launch_ht_x3(sg, ht, ct_h, ct_l, e->bht16h.as<uint16_t>(), e->bht16l.as<uint16_t>(), x_h, x_l,
                                 s));
+  } else if (precision == PNJK_SPLIT_REFINE && want_k) {
+    // FP64 X for the MO path, with its split planes for K^n written by the same epilogue.
+    PNJK_HIP_CHECK(launch_ht_f64(g, ht, Cp, e->btri.as<double>(), e->X.as<double>(), s, x_h, x_l));
   } else {
     PNJK_HIP_CHECK(launch_ht_f64(g, ht, Cp, e->btri.as<double>(), e->X.as<double>(), s));
   }
@@ -417,17 +444,18 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
   mark(ST_JMO);
   PNJK_HIP_CHECK(launch_syrk_f64(1, g.nbf5, g.naux_p, e->d.as<double>(), g.naux_p, 0, dJMO, g.nbf5, 0,
                                  TileF64{2, 2}, s));
-  if (precision == PNJK_SPLIT_REFINE && (want_j || want_k)) {
-    // J^n/K^n in split form from the exact X and d.
-    if (want_k) PNJK_HIP_CHECK(launch_split_btri(e->X.as<double>(), n5 * nbf * g.naux_p, x_h, x_l, s));
-    if (want_j) PNJK_HIP_CHECK(launch_split_btri(e->d.as<double>(), n5 * g.naux_p, d_h, d_l, s));
+  PNJK_HIP_CHECK(launch_symmetrize(1, g.nbf5, dJMO, g.nbf5, 0, s));
+  if (precision == PNJK_SPLIT_REFINE && want_j) {
+    // J^n in split form from the exact d.
+    PNJK_HIP_CHECK(launch_split_btri(e->d.as<double>(), n5 * g.naux_p, d_h, d_l, s));
   }
   mark(ST_JN);
   if (want_j) {
     if (precision == PNJK_FP64)
-      PNJK_HIP_CHECK(launch_jn_f64(g, ht, e->d.as<double>(), e->btri.as<double>(), dJ, s));
+      PNJK_HIP_CHECK(launch_jn_f64(g, jt, e->d.as<double>(), e->btri.as<double>(), dJ, s));
     else
-      PNJK_HIP_CHECK(launch_jn_x3(sg, ht, d_h, d_l, e->bt16h.as<uint16_t>(), e->bt16l.as<uint16_t>(), dJ, s));
+      PNJK_HIP_CHECK(launch_jn_x3(sg, jt, d_h, d_l, e->bt16h.as<uint16_t>(), e->bt16l.as<uint16_t>(), dJ, s));
+    PNJK_HIP_CHECK(launch_symmetrize(g.nbf5, g.nbf, dJ, nbf, nbf * nbf, s));
   }
   mark(ST_K);
   if (want_k) {
@@ -436,6 +464,7 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
                                      nbf * nbf, kt, s));
     else
       PNJK_HIP_CHECK(launch_kn_x3(sg, kt, x_h, x_l, dK, s));
+    PNJK_HIP_CHECK(launch_symmetrize(g.nbf5, g.nbf, dK, nbf, nbf * nbf, s));
   }
   mark(ST_HC);
   if (want_h) {
@@ -496,6 +525,26 @@ int pnjk_set_profiling(pnjk_engine* e, int on) {
 }
 
 int pnjk_stage_count(void) { return ST_COUNT; }
+
+int pnjk_set_tile(pnjk_engine* e, int arith, int stage, int fm, int fn) {
+  if (!e) return set_error(PNJK_EINVAL, "pnjk_set_tile: engine is NULL");
+  if (arith != 0 && arith != 1) return set_error(PNJK_EINVAL, "pnjk_set_tile: arith must be 0 (FP64) or 1 (BF16x3)");
+  if (stage != ST_HT && stage != ST_QT && stage != ST_JN && stage != ST_K)
+    return set_error(PNJK_EINVAL, "pnjk_set_tile: only half_transform, quarter_transform, coulomb_jn and "
+                                  "exchange_kn take a tile shape");
+  const bool reset = fm == 0 && fn == 0;
+  bool known = false;
+  for (const TileF64 t : {TileF64{2, 2}, TileF64{4, 4}, TileF64{5, 4}, TileF64{4, 2}, TileF64{2, 4}, TileF64{5, 2}})
+    known = known || (t.fm == fm && t.fn == fn);
+  if (!reset && !known) return set_error(PNJK_EINVAL, "pnjk_set_tile: unsupported tile shape");
+  if (!reset && stage == ST_QT && fn != 2)
+    return set_error(PNJK_EINVAL, "pnjk_set_tile: the quarter transform needs fn = 2 (64 aux columns per tile)");
+  if (!reset && stage == ST_K && fm != fn)
+    return set_error(PNJK_EINVAL, "pnjk_set_tile: the exchange SYRK needs square tiles (fm = fn)");
+  std::lock_guard<std::mutex> lk(e->mu);
+  e->tiles[arith][stage] = reset ? TileF64{0, 0} : TileF64{fm, fn};
+  return PNJK_OK;
+}
 
 const char* pnjk_stage_name(int i) { return (i >= 0 && i < ST_COUNT) ? kStageNames[i] : ""; }
 

@@ -18,7 +18,8 @@
 //
 // b is stored once, as the packed lower triangle btri[tri(mu,nu)][P] (b is
 // symmetric in mu,nu, integrals.py:35-49), halving its HBM footprint; the
-// symmetric outputs are computed on one triangle of tiles and mirrored.
+// symmetric outputs are computed on one triangle of tiles; a tiled transpose
+// (k_symmetrize) then fills the other triangle with coalesced reads and writes.
 #include "jk_common.h"
 #include "jk_kernels.h"
 
@@ -207,11 +208,16 @@ __device__ __forceinline__ void elem_coords(int lane, int wm, int wn, int i, int
   dn = wn + 16 * j + (lane & 15);
 }
 
+// kSplit: the epilogue also writes the bf16 split planes of X (same layout);
+// a separate instantiation, so the plain FP64 kernel keeps its register budget.
+template <bool kSplit>
 struct ProbHT {
   static constexpr int kA = KM, kB = KM;
   const double* Cp;
   const double* btri;
   double* X;
+  uint16_t* xhi;
+  uint16_t* xlo;
   int nbf, naux_p, nbf5, ldcp, nbf_p, mtiles;
   int64_t N;
   __device__ bool tile(int id, int BM, int BN, Tile& t) const {
@@ -237,7 +243,15 @@ struct ProbHT {
           elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int q = t.m0 + dm;
           const int64_t n = (int64_t)t.n0 + dn;
-          if (q < nbf5 && n < N) X[(int64_t)q * N + n] = acc[i][j][r];
+          if (q < nbf5 && n < N) {
+            X[(int64_t)q * N + n] = acc[i][j][r];
+            if (kSplit) {
+              uint16_t h, l;
+              split2(acc[i][j][r], h, l);
+              xhi[(int64_t)q * N + n] = h;
+              xlo[(int64_t)q * N + n] = l;
+            }
+          }
         }
   }
 };
@@ -339,7 +353,6 @@ struct ProbSYRK {
   template <int FM, int FN>
   __device__ void epilogue(const Tile& t, v4d (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
     double* o = O + (int64_t)t.batch * so;
-    const bool mirror = t.mt != t.nt;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -349,10 +362,7 @@ struct ProbSYRK {
           int dm, dn;
           elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int m = t.m0 + dm, n = t.n0 + dn;
-          if (m < M && n < M) {
-            o[(int64_t)m * ldo + n] = acc[i][j][r];
-            if (mirror) o[(int64_t)n * ldo + m] = acc[i][j][r];
-          }
+          if (m < M && n < M) o[(int64_t)m * ldo + n] = acc[i][j][r];
         }
   }
 };
@@ -392,11 +402,7 @@ struct ProbJN {
         for (int r = 0; r < 4; ++r) {
           elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int q = t.m0 + dm;
-          if (q < nbf5) {
-            double* jq = J + (int64_t)q * nn;
-            jq[(int64_t)mu * nbf + nu] = acc[i][j][r];
-            if (mu != nu) jq[(int64_t)nu * nbf + mu] = acc[i][j][r];
-          }
+          if (q < nbf5) J[(int64_t)q * nn + (int64_t)mu * nbf + nu] = acc[i][j][r];
         }
     }
   }
@@ -496,6 +502,28 @@ __global__ void k_pack_b(int nbf, int naux, int naux_p, int mu0, int mu1, const 
   if (amax > 0.0) atomic_max_nonneg(asym_max, amax);
 }
 
+// Fills the strict upper triangle of each n x n matrix from the lower one:
+// O[b][j][i] = O[b][i][j] for i > j. One 32 x 32 lower tile pair per workgroup,
+// read and written row-contiguously through LDS.
+__global__ void k_symmetrize(double* __restrict__ O, int n, int64_t ld, int64_t stride) {
+  __shared__ double tile[32][33];
+  const int64_t pair = blockIdx.x;
+  const int ti = tri_row(pair);
+  const int tj = (int)(pair - (int64_t)ti * (ti + 1) / 2);
+  double* o = O + (int64_t)blockIdx.y * stride;
+  for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
+    const int r = e >> 5, c = e & 31;
+    const int i = ti * 32 + r, j = tj * 32 + c;
+    tile[r][c] = (i < n && j < n) ? o[(int64_t)i * ld + j] : 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
+    const int r = e >> 5, c = e & 31;
+    const int j = tj * 32 + r, i = ti * 32 + c;
+    if (i < n && j < n && i > j) o[(int64_t)j * ld + i] = tile[c][r];
+  }
+}
+
 template <class Prob>
 hipError_t launch_prob(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s) {
   if (nblocks <= 0) return hipSuccess;
@@ -510,6 +538,7 @@ hipError_t launch_prob(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s)
   PNJK_CASE(5, 4, 8)
   PNJK_CASE(4, 2, 16)
   PNJK_CASE(2, 4, 16)
+  PNJK_CASE(5, 2, 16)
 #undef PNJK_CASE
   return hipErrorInvalidValue;
 }
@@ -518,12 +547,15 @@ inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
 }  // namespace
 
-hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* btri, double* X,
-                         hipStream_t s) {
-  ProbHT pr;
+template <bool kSplit>
+hipError_t launch_ht_f64_t(const BuildGeom& g, TileF64 tc, const double* Cp, const double* btri, double* X,
+                           hipStream_t s, uint16_t* x_hi, uint16_t* x_lo) {
+  ProbHT<kSplit> pr;
   pr.Cp = Cp;
   pr.btri = btri;
   pr.X = X;
+  pr.xhi = x_hi;
+  pr.xlo = x_lo;
   pr.nbf = g.nbf;
   pr.naux_p = g.naux_p;
   pr.nbf5 = g.nbf5;
@@ -533,6 +565,12 @@ hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
   pr.mtiles = cdiv(g.nbf5, BM);
   return launch_prob(pr, tc, pr.mtiles * cdiv(pr.N, BN), s);
+}
+
+hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* btri, double* X,
+                         hipStream_t s, uint16_t* x_hi, uint16_t* x_lo) {
+  if (x_hi && x_lo) return launch_ht_f64_t<true>(g, tc, Cp, btri, X, s, x_hi, x_lo);
+  return launch_ht_f64_t<false>(g, tc, Cp, btri, X, s, nullptr, nullptr);
 }
 
 hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* X, double* kpart,
@@ -555,6 +593,15 @@ hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const
   return launch_prob(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s);
 }
 
+hipError_t launch_symmetrize(int batch, int n, double* O, int64_t ld, int64_t stride, hipStream_t s) {
+  if (batch <= 0 || n <= 1) return hipSuccess;
+  const int64_t t = (n + 31) / 32;
+  const int64_t pairs = t * (t + 1) / 2;
+  if (batch > 65535 || pairs > 2147483647LL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_symmetrize, dim3((unsigned)pairs, (unsigned)batch), dim3(256), 0, s, O, n, ld, stride);
+  return hipGetLastError();
+}
+
 hipError_t launch_kmo_reduce(int nbf5, int nt, const double* kpart, double* kmo, hipStream_t s) {
   const int64_t n = (int64_t)nbf5 * nbf5;
   hipLaunchKernelGGL(k_kmo_reduce, dim3(cdiv(n, 256)), dim3(256), 0, s, nbf5, nt, kpart, kmo);
@@ -565,7 +612,7 @@ hipError_t launch_syrk_f64(int batch, int M, int K, const double* A, int64_t lda
                            int64_t ldo, int64_t so, TileF64 tc, hipStream_t s) {
   if (tc.fm != tc.fn) return hipErrorInvalidValue;
   const int BM = 32 * tc.fm;
-  const int BK = tc.fm == 2 ? 16 : 8;
+  const int BK = (tc.fm >= 4 && tc.fn >= 4) ? 8 : 16;
   if (K % BK != 0) return hipErrorInvalidValue;
   ProbSYRK pr;
   pr.A = A;

@@ -25,8 +25,9 @@ struct BuildGeom {
 };
 
 // X[q][nu*naux_p + P] = sum_mu Cp[mu][q] b_tri[tri(mu,nu)][P]   (half-transform)
+// When x_hi/x_lo are given, also writes the bf16 split planes of X (same layout).
 hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* btri, double* X,
-                         hipStream_t s);
+                         hipStream_t s, uint16_t* x_hi = nullptr, uint16_t* x_lo = nullptr);
 
 // Y_p[q][P] = sum_nu Cp[nu][q] X[p][nu][P] for q <= p; writes row sums of Y_p^2
 // per P-tile into kpart[(p*nbf5+q)*nt + tile] and d[p][P] = Y_p[p][P].
@@ -37,13 +38,17 @@ hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const
 // K_MO[p][q] = K_MO[q][p] = sum_t kpart[(p*nbf5+q)*nt + t]  (q <= p)
 hipError_t launch_kmo_reduce(int nbf5, int nt, const double* kpart, double* kmo, hipStream_t s);
 
-// O[b][i][j] = sum_k A[b][i][k] A[b][j][k] for i, j < M; full symmetric output.
+// O[b][i][j] = sum_k A[b][i][k] A[b][j][k] for i, j < M on the lower triangle of
+// tiles; the strict upper triangle is left for launch_symmetrize.
 hipError_t launch_syrk_f64(int batch, int M, int K, const double* A, int64_t lda, int64_t sa, double* O,
                            int64_t ldo, int64_t so, TileF64 tc, hipStream_t s);
 
-// J[q][mu][nu] = sum_P d[q][P] b_tri[tri(mu,nu)][P], full symmetric (nbf5, nbf, nbf) output.
+// J[q][mu][nu] = sum_P d[q][P] b_tri[tri(mu,nu)][P] for mu >= nu (lower triangle only).
 hipError_t launch_jn_f64(const BuildGeom& g, TileF64 tc, const double* d, const double* btri, double* J,
                          hipStream_t s);
+
+// O[b][j][i] = O[b][i][j] for i > j, b < batch (n x n matrices, row stride ld).
+hipError_t launch_symmetrize(int batch, int n, double* O, int64_t ld, int64_t stride, hipStream_t s);
 
 // T[mu][i] = sum_nu Hp[mu][nu] Cp[nu][i] (mu < nbf, i < ldcp), then
 // hcore[i] = sum_mu Cp[mu][i] T[mu][i] for i < nbf5.
@@ -82,10 +87,10 @@ hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
 hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
                         const uint16_t* x_hi, const uint16_t* x_lo, double* kpart, double* d, uint16_t* d_hi,
                         uint16_t* d_lo, int* ntiles, hipStream_t s);
-// J^n from split d and btri planes (FP64 output, mirrored).
+// J^n from split d and btri planes (FP64 output, mu >= nu only).
 hipError_t launch_jn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* d_hi, const uint16_t* d_lo,
                         const uint16_t* bt_hi, const uint16_t* bt_lo, double* J, hipStream_t s);
-// K^n from split X16 (FP64 output, mirrored).
+// K^n from split X16 (FP64 output, lower triangle of tiles only).
 hipError_t launch_kn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* x_hi, const uint16_t* x_lo, double* K,
                         hipStream_t s);
 

@@ -26,18 +26,6 @@ constexpr int kThreads = 256;
 constexpr int kBK = 32;
 constexpr int kLDK = kBK + 8;  // LDS row stride (elements): conflict-free 16-byte fragment reads
 
-__device__ __forceinline__ uint16_t bf16_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-
-__device__ __forceinline__ void split2(double x, uint16_t& hi, uint16_t& lo) {
-  hi = bf16_rne((float)x);
-  lo = bf16_rne((float)(x - (double)bf16_to_f(hi)));
-}
-
 // Output row of accumulator register r of `lane` inside a 16x16 f32 MFMA block.
 __device__ __forceinline__ int out_row(int lane, int r) { return 4 * (lane >> 4) + r; }
 
@@ -212,13 +200,13 @@ struct ProbHT {
   Planes ct, bht;
   uint16_t* xhi;
   uint16_t* xlo;
-  int nbf5, ldk, mtiles;
+  int nbf5, ldk, mtiles, m_base;  // rows [m_base, m_base + mtiles*BM) of this launch
   int64_t N;
   __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.batch = 0;
     t.mt = id % mtiles;
     t.nt = id / mtiles;
-    t.m0 = t.mt * BM;
+    t.m0 = m_base + t.mt * BM;
     t.n0 = t.nt * BN;
     return true;
   }
@@ -331,18 +319,19 @@ struct ProbQT {
   }
 };
 
-// J[q][mu][nu] = sum_P d16[q][P] btri16[t][P], full symmetric FP64 output
+// J[q][mu][nu] = sum_P d16[q][P] btri16[t][P] for mu >= nu (FP64 output; the
+// upper triangle is filled afterwards by launch_symmetrize)
 struct ProbJN {
   static constexpr int kA = RK, kB = RK;
   Planes dd, bt;
   double* J;
-  int nbf, naux_p, nbf5, mtiles;
+  int nbf, naux_p, nbf5, mtiles, m_base;  // rows [m_base, m_base + mtiles*BM) of this launch
   int64_t ntri;
   __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.batch = 0;
     t.mt = id % mtiles;
     t.nt = id / mtiles;
-    t.m0 = t.mt * BM;
+    t.m0 = m_base + t.mt * BM;
     t.n0 = t.nt * BN;
     return true;
   }
@@ -367,17 +356,15 @@ struct ProbJN {
           coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int q = t.m0 + dm;
           if (q < nbf5) {
-            double* jq = J + (int64_t)q * nn;
-            const double v = acc[i][j][r];
-            jq[(int64_t)mu * nbf + nu] = v;
-            if (mu != nu) jq[(int64_t)nu * nbf + mu] = v;
+            J[(int64_t)q * nn + (int64_t)mu * nbf + nu] = acc[i][j][r];
           }
         }
     }
   }
 };
 
-// K[b][i][j] = sum_P X16[b][i][P] X16[b][j][P], lower tile triangle, mirrored FP64 output
+// K[b][i][j] = sum_P X16[b][i][P] X16[b][j][P] on the lower triangle of tiles (FP64
+// output; the strict upper triangle is filled afterwards by launch_symmetrize)
 struct ProbSYRK {
   static constexpr int kA = RK, kB = RK;
   Planes x;
@@ -402,7 +389,6 @@ struct ProbSYRK {
   template <int FM, int FN>
   __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
     double* o = O + (int64_t)t.batch * so;
-    const bool mirror = t.mt != t.nt;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -413,9 +399,7 @@ struct ProbSYRK {
           coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int m = t.m0 + dm, n = t.n0 + dn;
           if (m < M && n < M) {
-            const double v = acc[i][j][r];
-            o[(int64_t)m * ldo + n] = v;
-            if (mirror) o[(int64_t)n * ldo + m] = v;
+            o[(int64_t)m * ldo + n] = acc[i][j][r];
           }
         }
   }
@@ -485,6 +469,8 @@ hipError_t launch_x3(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s) {
   PNJK_CASE(4, 4)
   PNJK_CASE(5, 4)
   PNJK_CASE(4, 2)
+  PNJK_CASE(5, 2)
+  PNJK_CASE(2, 4)
 #undef PNJK_CASE
   return hipErrorInvalidValue;
 }
@@ -525,6 +511,7 @@ hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
   pr.ldk = g.ldk;
   pr.N = (int64_t)g.nbf * g.naux_p;
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
+  pr.m_base = 0;
   pr.mtiles = cdiv(g.nbf5, BM);
   return launch_x3(pr, tc, pr.mtiles * cdiv(pr.N, BN), s);
 }
@@ -562,6 +549,7 @@ hipError_t launch_jn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* d_hi, co
   pr.naux_p = g.naux_p;
   pr.nbf5 = g.nbf5;
   pr.ntri = (int64_t)g.nbf * (g.nbf + 1) / 2;
+  pr.m_base = 0;
   pr.mtiles = cdiv(g.nbf5, BM);
   return launch_x3(pr, tc, pr.mtiles * cdiv(pr.ntri, BN), s);
 }

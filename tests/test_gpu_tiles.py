@@ -1,0 +1,41 @@
+"""Tile shapes change no result bit (GPU only): every stage, every supported
+shape, FP64 and split forms, against the built-in choice."""
+import numpy as np
+import pytest
+
+from pynof_amd import JKEngine, _lib, autotune, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_tiles_bitwise(gpu_required, precision):
+    nbf, naux, nbf5 = 150, 300, 110
+    print(f"case: tiles bitwise {precision} nbf={nbf} naux={naux} nbf5={nbf5}", flush=True)
+    b = synth.make_b(nbf, naux, 77, 0.05)
+    C = synth.make_C(nbf, 78)
+    H = synth.make_H(nbf, 79)
+    eng = JKEngine(0)
+    eng.upload_b(b)
+    ref = eng.build(C, nbf5, H=H, precision=precision)
+    L = _lib.lib()
+    a = autotune.ARITH[precision]
+    for st, idx in autotune.STAGES.items():
+        for fm, fn in autotune.CANDIDATES[st]:
+            _lib.check(L.pnjk_set_tile(eng._h, a, idx, fm, fn))
+            r = eng.build(C, nbf5, H=H, precision=precision)
+            for k in ref:
+                assert np.array_equal(r[k], ref[k]), (st, fm, fn, k)
+        _lib.check(L.pnjk_set_tile(eng._h, a, idx, 0, 0))
+    eng.close()
+
+
+def test_set_tile_rejects(gpu_required):
+    eng = JKEngine(0)
+    L = _lib.lib()
+    assert L.pnjk_set_tile(eng._h, 0, 2, 4, 4) == _lib.EINVAL   # quarter transform needs fn = 2
+    assert L.pnjk_set_tile(eng._h, 1, 6, 5, 4) == _lib.EINVAL   # exchange SYRK needs square tiles
+    assert L.pnjk_set_tile(eng._h, 0, 0, 2, 2) == _lib.EINVAL   # copy_in takes no tile
+    assert L.pnjk_set_tile(eng._h, 1, 1, 3, 3) == _lib.EINVAL   # unknown shape
+    assert L.pnjk_set_tile(eng._h, 2, 1, 2, 2) == _lib.EINVAL   # unknown arithmetic
+    eng.close()
