@@ -90,6 +90,26 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
                   int precision, uint32_t outputs, double* J, double* K, double* J_MO, double* K_MO,
                   double* H_core, void* stream);
 
+/* Grouped builds: many engines (one b each, different sizes allowed) built in
+ * one pass. Each GEMM stage of all instances runs as one persistent grouped
+ * launch per problem type and tile shape (tile shapes per instance from the
+ * engines' tile settings), so small instances share the device instead of
+ * each underfilling it. The reference has no batched form; it builds one
+ * system per call (integrals.py:172, :307). Arrays hold one entry per engine,
+ * in the order given to pnjk_group_create; NULL output arrays skip that output.
+ * Outputs must be device memory. use_graph = 1 captures the launch sequence
+ * in a hipGraph on the first call (and again whenever any pointer, size, tile
+ * or precision changes) and replays it afterwards; it needs device-resident C
+ * and H. Results are bitwise those of pnjk_jk_build on each engine alone. An
+ * engine must not be used elsewhere while its group builds. */
+typedef struct pnjk_group pnjk_group;
+int pnjk_group_create(pnjk_engine* const* engines, int count, pnjk_group** out);
+int pnjk_group_destroy(pnjk_group* g);
+int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, const int64_t* nbf5,
+                     const double* const* H, const int64_t* ldh, int precision, uint32_t outputs,
+                     double* const* J, double* const* K, double* const* J_MO, double* const* K_MO,
+                     double* const* H_core, int use_graph, void* stream);
+
 /* Per-stage timing with HIP events recorded on the build stream (no
  * counterpart in the reference, which has wall-clock prints only,
  * energy.py:29, 223-224). When on, every build records one event per stage;

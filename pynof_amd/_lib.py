@@ -41,6 +41,13 @@ _SIGS = {
     "pnjk_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pnjk_stage_name": (ctypes.c_char_p, [ctypes.c_int]),
     "pnjk_stage_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "pnjk_group_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "pnjk_group_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "pnjk_group_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "pnjk_footprint": (ctypes.c_uint64, [ctypes.c_void_p]),
     "pnjk_workspace_bytes": (ctypes.c_uint64, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int]),
 }

@@ -12,6 +12,18 @@ typedef double v2d __attribute__((ext_vector_type(2)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef int v4i __attribute__((ext_vector_type(4)));
 
+// Tile shape of one GEMM launch (FP64 or BF16x3 core): BM = 32*fm rows,
+// BN = 32*fn columns per workgroup.
+struct TileF64 {
+  int fm;
+  int fn;
+};
+
+// One output tile of a GEMM-shaped problem.
+struct Tile {
+  int batch, mt, nt, m0, n0;
+};
+
 // Records a message as the calling thread's last error; returns `code`.
 int set_error(int code, const std::string& msg);
 

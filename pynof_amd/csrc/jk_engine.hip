@@ -4,6 +4,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "jk_common.h"
 #include "jk_kernels.h"
@@ -29,14 +30,14 @@ struct DevBuf {
   int ensure(size_t want) {
     if (want <= bytes) return PNJK_OK;
     if (p) {
-      hipFree(p);
+      (void)hipFree(p);
       p = nullptr;
       bytes = 0;
     }
     if (want == 0) return PNJK_OK;
     hipError_t e = hipMalloc(&p, want);
     if (e != hipSuccess) {
-      hipGetLastError();
+      (void)hipGetLastError();
       p = nullptr;
       return set_error(PNJK_ENOMEM, "hipMalloc(" + std::to_string(want) + " bytes) failed: " + hipGetErrorString(e));
     }
@@ -44,7 +45,7 @@ struct DevBuf {
     return PNJK_OK;
   }
   void release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
   }
@@ -77,7 +78,7 @@ __global__ void k_copy2d(const double* __restrict__ src, int64_t lds, double* __
 bool is_device_ptr(const void* p) {
   hipPointerAttribute_t a;
   if (hipPointerGetAttributes(&a, p) != hipSuccess) {
-    hipGetLastError();
+    (void)hipGetLastError();
     return false;
   }
   return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
@@ -167,12 +168,12 @@ int pnjk_create(int device, pnjk_engine** out) {
 
 int pnjk_destroy(pnjk_engine* e) {
   if (!e) return PNJK_OK;
-  hipSetDevice(e->device);
-  hipStreamSynchronize(e->own);
+  (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->own);
   for (int i = 0; i < e->nall; ++i) e->all[i]->release();
   for (hipEvent_t& ev : e->ev)
-    if (ev) hipEventDestroy(ev);
-  hipStreamDestroy(e->own);
+    if (ev) (void)hipEventDestroy(ev);
+  (void)hipStreamDestroy(e->own);
   delete e;
   return PNJK_OK;
 }
@@ -304,211 +305,510 @@ int pnjk_upload_b_packed(pnjk_engine* e, const double* btri, int64_t nbf, int64_
   return PNJK_OK;
 }
 
-int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, const double* H, int64_t ldh,
-                  int precision, uint32_t outputs, double* J, double* K, double* J_MO, double* K_MO, double* H_core,
-                  void* stream) {
-  if (!e) return set_error(PNJK_EINVAL, "pnjk_jk_build: engine is NULL");
-  std::lock_guard<std::mutex> lk(e->mu);
-  if (e->nbf == 0) return set_error(PNJK_ESTATE, "pnjk_jk_build: no b uploaded");
-  if (!C) return set_error(PNJK_EINVAL, "pnjk_jk_build: C is NULL");
-  if (nbf5 < 1 || nbf5 > e->nbf)
-    return set_error(PNJK_EINVAL, "pnjk_jk_build: nbf5 must be in [1, nbf]; got " + std::to_string(nbf5));
-  if (ldc < nbf5) return set_error(PNJK_EINVAL, "pnjk_jk_build: ldc < nbf5");
-  if (outputs & ~PNJK_OUT_ALL) return set_error(PNJK_EINVAL, "pnjk_jk_build: unknown output bits");
-  if ((outputs & PNJK_OUT_J) && !J) return set_error(PNJK_EINVAL, "pnjk_jk_build: J requested but NULL");
-  if ((outputs & PNJK_OUT_K) && !K) return set_error(PNJK_EINVAL, "pnjk_jk_build: K requested but NULL");
-  if ((outputs & PNJK_OUT_JMO) && !J_MO) return set_error(PNJK_EINVAL, "pnjk_jk_build: J_MO requested but NULL");
-  if ((outputs & PNJK_OUT_KMO) && !K_MO) return set_error(PNJK_EINVAL, "pnjk_jk_build: K_MO requested but NULL");
-  if ((outputs & PNJK_OUT_HCORE) && (!H_core || !H))
-    return set_error(PNJK_EINVAL, "pnjk_jk_build: H_core requested but H or H_core is NULL");
-  if ((outputs & PNJK_OUT_HCORE) && ldh < e->nbf) return set_error(PNJK_EINVAL, "pnjk_jk_build: ldh < nbf");
-  if (precision != PNJK_FP64 && precision != PNJK_SPLIT && precision != PNJK_SPLIT_REFINE)
-    return set_error(PNJK_EUNSUPPORTED, "pnjk_jk_build: unknown precision " + std::to_string(precision));
-  PNJK_HIP_CHECK(hipSetDevice(e->device));
-  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : e->own;
-  auto mark = [&](int st) {
-    if (e->prof) hipEventRecord(e->ev[st], s);
-  };
+}  // extern "C"
 
+namespace {
+
+// Arguments of one build, as given at the C ABI.
+struct BuildArgs {
+  const double* C;
+  int64_t ldc, nbf5;
+  const double* H;
+  int64_t ldh;
+  int precision;
+  uint32_t outputs;
+  double *J, *K, *J_MO, *K_MO, *H_core;
+};
+
+// Per-build state between stages: geometry, tiles, device output pointers.
+struct BuildState {
+  BuildArgs a;
   BuildGeom g;
+  SplitGeom sg;
+  TileF64 ht, qt, jt, kt;
+  bool want_j, want_k, want_jmo, want_kmo, want_h;
+  bool hJ, hK, hJMO, hKMO, hH;  // output goes to host memory (staged)
+  bool c_dev = false, h_dev = false;  // C / H are device memory
+  double *dJ, *dK, *dJMO, *dKMO, *dH;
+  int nt = 0;  // P-tiles of the quarter transform
+};
+
+// Validates the arguments, sizes the workspace, resolves output staging and
+// tiles. Enqueues nothing.
+int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st) {
+  if (e->nbf == 0) return set_error(PNJK_ESTATE, "pnjk_jk_build: no b uploaded");
+  if (!a.C) return set_error(PNJK_EINVAL, "pnjk_jk_build: C is NULL");
+  if (a.nbf5 < 1 || a.nbf5 > e->nbf)
+    return set_error(PNJK_EINVAL, "pnjk_jk_build: nbf5 must be in [1, nbf]; got " + std::to_string(a.nbf5));
+  if (a.ldc < a.nbf5) return set_error(PNJK_EINVAL, "pnjk_jk_build: ldc < nbf5");
+  if (a.outputs & ~PNJK_OUT_ALL) return set_error(PNJK_EINVAL, "pnjk_jk_build: unknown output bits");
+  if ((a.outputs & PNJK_OUT_J) && !a.J) return set_error(PNJK_EINVAL, "pnjk_jk_build: J requested but NULL");
+  if ((a.outputs & PNJK_OUT_K) && !a.K) return set_error(PNJK_EINVAL, "pnjk_jk_build: K requested but NULL");
+  if ((a.outputs & PNJK_OUT_JMO) && !a.J_MO)
+    return set_error(PNJK_EINVAL, "pnjk_jk_build: J_MO requested but NULL");
+  if ((a.outputs & PNJK_OUT_KMO) && !a.K_MO)
+    return set_error(PNJK_EINVAL, "pnjk_jk_build: K_MO requested but NULL");
+  if ((a.outputs & PNJK_OUT_HCORE) && (!a.H_core || !a.H))
+    return set_error(PNJK_EINVAL, "pnjk_jk_build: H_core requested but H or H_core is NULL");
+  if ((a.outputs & PNJK_OUT_HCORE) && a.ldh < e->nbf) return set_error(PNJK_EINVAL, "pnjk_jk_build: ldh < nbf");
+  if (a.precision != PNJK_FP64 && a.precision != PNJK_SPLIT && a.precision != PNJK_SPLIT_REFINE)
+    return set_error(PNJK_EUNSUPPORTED, "pnjk_jk_build: unknown precision " + std::to_string(a.precision));
+  st.a = a;
+  st.c_dev = is_device_ptr(a.C);
+  st.h_dev = a.H && is_device_ptr(a.H);
+  BuildGeom& g = st.g;
   g.nbf = e->nbf;
   g.naux = e->naux;
   g.naux_p = e->naux_p;
-  g.nbf5 = (int)nbf5;
+  g.nbf5 = (int)a.nbf5;
   g.ntri = e->ntri;
   g.nbf_p = (int)roundup(e->nbf, 32);
-  g.ldcp = (int)roundup(nbf5, 32);
-  const int64_t nbf = g.nbf, n5 = nbf5;
+  g.ldcp = (int)roundup(a.nbf5, 32);
+  st.sg.nbf = g.nbf;
+  st.sg.naux_p = g.naux_p;
+  st.sg.nbf5 = g.nbf5;
+  st.sg.ldk = g.nbf_p;
+  const int64_t nbf = g.nbf, n5 = a.nbf5;
+  const int precision = a.precision;
 
   int rc;
   const int64_t cp_elems = (int64_t)(g.nbf_p + 8) * g.ldcp + 1024;
   if ((rc = e->Cp.ensure((size_t)cp_elems * 8))) return rc;
   if (precision != PNJK_SPLIT && (rc = e->X.ensure((size_t)(n5 * nbf * g.naux_p) * 8))) return rc;
-  const int64_t d_rows = roundup(n5, 160);
-  if ((rc = e->d.ensure((size_t)(d_rows * g.naux_p) * 8))) return rc;
-  const int qt_bn = 64;
-  if ((rc = e->kpart.ensure((size_t)(n5 * n5 * (g.naux_p / qt_bn)) * 8))) return rc;
+  if ((rc = e->d.ensure((size_t)(roundup(n5, 160) * g.naux_p) * 8))) return rc;
+  if ((rc = e->kpart.ensure((size_t)(n5 * n5 * (g.naux_p / 64)) * 8))) return rc;
 
-  // Output staging for host destinations.
-  const bool want_j = outputs & PNJK_OUT_J, want_k = outputs & PNJK_OUT_K, want_jmo = outputs & PNJK_OUT_JMO,
-             want_kmo = outputs & PNJK_OUT_KMO, want_h = outputs & PNJK_OUT_HCORE;
+  st.want_j = a.outputs & PNJK_OUT_J;
+  st.want_k = a.outputs & PNJK_OUT_K;
+  st.want_jmo = a.outputs & PNJK_OUT_JMO;
+  st.want_kmo = a.outputs & PNJK_OUT_KMO;
+  st.want_h = a.outputs & PNJK_OUT_HCORE;
   const int64_t jk_elems = n5 * nbf * nbf;
-  double* dJ = J;
-  double* dK = K;
-  double* dJMO = J_MO;
-  double* dKMO = K_MO;
-  double* dH = H_core;
-  const bool hJ = want_j && !is_device_ptr(J), hK = want_k && !is_device_ptr(K);
-  const bool hJMO = want_jmo && !is_device_ptr(J_MO), hKMO = !want_kmo || !is_device_ptr(K_MO);
-  const bool hH = want_h && !is_device_ptr(H_core);
+  st.dJ = a.J;
+  st.dK = a.K;
+  st.dJMO = a.J_MO;
+  st.dKMO = a.K_MO;
+  st.dH = a.H_core;
+  st.hJ = st.want_j && !is_device_ptr(a.J);
+  st.hK = st.want_k && !is_device_ptr(a.K);
+  st.hJMO = !st.want_jmo || !is_device_ptr(a.J_MO);  // J_MO and K_MO are always formed
+  st.hKMO = !st.want_kmo || !is_device_ptr(a.K_MO);
+  st.hH = st.want_h && !is_device_ptr(a.H_core);
   int64_t stage_elems = 0;
   const int64_t off_J = stage_elems;
-  if (hJ) stage_elems += jk_elems;
+  if (st.hJ) stage_elems += jk_elems;
   const int64_t off_K = stage_elems;
-  if (hK) stage_elems += jk_elems;
+  if (st.hK) stage_elems += jk_elems;
   const int64_t off_JMO = stage_elems;
-  stage_elems += n5 * n5;  // J_MO is always formed (cheap) for the energy path
+  if (st.hJMO) stage_elems += n5 * n5;
   const int64_t off_KMO = stage_elems;
-  if (hKMO) stage_elems += n5 * n5;
+  if (st.hKMO) stage_elems += n5 * n5;
   const int64_t off_H = stage_elems;
-  if (hH) stage_elems += n5;
+  if (st.hH) stage_elems += n5;
   if ((rc = e->outs.ensure((size_t)std::max<int64_t>(stage_elems, 1) * 8))) return rc;
   double* ob = e->outs.as<double>();
-  if (hJ) dJ = ob + off_J;
-  if (hK) dK = ob + off_K;
-  if (!want_jmo || hJMO) dJMO = ob + off_JMO;
-  if (hKMO) dKMO = ob + off_KMO;
-  if (hH) dH = ob + off_H;
+  if (st.hJ) st.dJ = ob + off_J;
+  if (st.hK) st.dK = ob + off_K;
+  if (st.hJMO) st.dJMO = ob + off_JMO;
+  if (st.hKMO) st.dKMO = ob + off_KMO;
+  if (st.hH) st.dH = ob + off_H;
 
-  // C -> Cp[nbf_p + slack][ldcp], zero elsewhere.
-  mark(ST_IN);
-  double* Cp = e->Cp.as<double>();
-  PNJK_HIP_CHECK(fill(Cp, cp_elems, 0.0, s));
-  if (is_device_ptr(C)) {
-    const int64_t n = nbf * n5;
-    hipLaunchKernelGGL(k_copy2d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, C, ldc, Cp, (int64_t)g.ldcp,
-                       (int)nbf, (int)n5);
-    PNJK_HIP_CHECK(hipGetLastError());
-  } else {
-    PNJK_HIP_CHECK(hipMemcpy2DAsync(Cp, (size_t)g.ldcp * 8, C, (size_t)ldc * 8, (size_t)n5 * 8, (size_t)nbf,
-                                    hipMemcpyHostToDevice, s));
-  }
-
-  // FP64 kernels run HT/QT in the fp64 and split_refine forms, J^n/K^n in fp64 only.
-  const int a_mo = precision == PNJK_SPLIT ? 1 : 0, a_jk = precision == PNJK_FP64 ? 0 : 1;
-  auto chosen = [&](int arith, int st, int kind) {
-    return e->tiles[arith][st].fm ? e->tiles[arith][st] : default_tile(arith, kind, g.nbf, g.nbf5);
-  };
-  const TileF64 ht = chosen(a_mo, ST_HT, 0);
-  const TileF64 qt = chosen(a_mo, ST_QT, 1);
-  const TileF64 jt = chosen(a_jk, ST_JN, 2);
-  const TileF64 kt = chosen(a_jk, ST_K, 3);
-  int nt = 0;
-  SplitGeom sg;
-  sg.nbf = g.nbf;
-  sg.naux_p = g.naux_p;
-  sg.nbf5 = g.nbf5;
-  sg.ldk = g.nbf_p;
   if (precision != PNJK_FP64) {
-    if ((rc = ensure_split_planes(e, s))) return rc;
-    const size_t ct_bytes = (size_t)(n5 * sg.ldk) * 2, x_bytes = (size_t)(n5 * nbf * g.naux_p) * 2,
+    const size_t ct_bytes = (size_t)(n5 * st.sg.ldk) * 2, x_bytes = (size_t)(n5 * nbf * g.naux_p) * 2,
                  d_bytes = (size_t)(n5 * g.naux_p) * 2;
     if ((rc = e->ct16h.ensure(ct_bytes)) || (rc = e->ct16l.ensure(ct_bytes))) return rc;
     if ((rc = e->x16h.ensure(x_bytes)) || (rc = e->x16l.ensure(x_bytes))) return rc;
     if ((rc = e->d16h.ensure(d_bytes)) || (rc = e->d16l.ensure(d_bytes))) return rc;
   }
-  uint16_t *ct_h = e->ct16h.as<uint16_t>(), *ct_l = e->ct16l.as<uint16_t>();
-  uint16_t *x_h = e->x16h.as<uint16_t>(), *x_l = e->x16l.as<uint16_t>();
-  uint16_t *d_h = e->d16h.as<uint16_t>(), *d_l = e->d16l.as<uint16_t>();
-  mark(ST_HT);
-  if (precision == PNJK_SPLIT) {
-    PNJK_HIP_CHECK(launch_split_ct(Cp, g.ldcp, g.nbf, g.nbf5, g.nbf5, sg.ldk, ct_h, ct_l, s));
-    PNJK_HIP_CHECK(launch_ht_x3(sg, ht, ct_h, ct_l, e->bht16h.as<uint16_t>(), e->bht16l.as<uint16_t>(), x_h, x_l,
-                                s));
-  } else if (precision == PNJK_SPLIT_REFINE && want_k) {
-    // FP64 X for the MO path, with its split planes for K^n written by the same epilogue.
-    PNJK_HIP_CHECK(launch_ht_f64(g, ht, Cp, e->btri.as<double>(), e->X.as<double>(), s, x_h, x_l));
-  } else {
-    PNJK_HIP_CHECK(launch_ht_f64(g, ht, Cp, e->btri.as<double>(), e->X.as<double>(), s));
-  }
-  mark(ST_QT);
-  if (precision == PNJK_SPLIT) {
-    PNJK_HIP_CHECK(launch_qt_x3(sg, qt, ct_h, ct_l, x_h, x_l, e->kpart.as<double>(), e->d.as<double>(), d_h, d_l,
-                                &nt, s));
-  } else {
-    PNJK_HIP_CHECK(
-        launch_qt_f64(g, qt, Cp, e->X.as<double>(), e->kpart.as<double>(), e->d.as<double>(), &nt, s));
-  }
-  mark(ST_KMO);
-  PNJK_HIP_CHECK(launch_kmo_reduce(g.nbf5, nt, e->kpart.as<double>(), dKMO, s));
-  mark(ST_JMO);
-  PNJK_HIP_CHECK(launch_syrk_f64(1, g.nbf5, g.naux_p, e->d.as<double>(), g.naux_p, 0, dJMO, g.nbf5, 0,
-                                 TileF64{2, 2}, s));
-  PNJK_HIP_CHECK(launch_symmetrize(1, g.nbf5, dJMO, g.nbf5, 0, s));
-  if (precision == PNJK_SPLIT_REFINE && want_j) {
-    // J^n in split form from the exact d.
-    PNJK_HIP_CHECK(launch_split_btri(e->d.as<double>(), n5 * g.naux_p, d_h, d_l, s));
-  }
-  mark(ST_JN);
-  if (want_j) {
-    if (precision == PNJK_FP64)
-      PNJK_HIP_CHECK(launch_jn_f64(g, jt, e->d.as<double>(), e->btri.as<double>(), dJ, s));
-    else
-      PNJK_HIP_CHECK(launch_jn_x3(sg, jt, d_h, d_l, e->bt16h.as<uint16_t>(), e->bt16l.as<uint16_t>(), dJ, s));
-    PNJK_HIP_CHECK(launch_symmetrize(g.nbf5, g.nbf, dJ, nbf, nbf * nbf, s));
-  }
-  mark(ST_K);
-  if (want_k) {
-    if (precision == PNJK_FP64)
-      PNJK_HIP_CHECK(launch_syrk_f64(g.nbf5, g.nbf, g.naux_p, e->X.as<double>(), g.naux_p, nbf * g.naux_p, dK, nbf,
-                                     nbf * nbf, kt, s));
-    else
-      PNJK_HIP_CHECK(launch_kn_x3(sg, kt, x_h, x_l, dK, s));
-    PNJK_HIP_CHECK(launch_symmetrize(g.nbf5, g.nbf, dK, nbf, nbf * nbf, s));
-  }
-  mark(ST_HC);
-  if (want_h) {
+  if (st.want_h) {
     if ((rc = e->Hp.ensure((size_t)(nbf * g.nbf_p) * 8))) return rc;
     if ((rc = e->T.ensure((size_t)(nbf * g.ldcp) * 8))) return rc;
-    double* Hp = e->Hp.as<double>();
-    PNJK_HIP_CHECK(fill(Hp, nbf * g.nbf_p, 0.0, s));
-    if (is_device_ptr(H)) {
-      const int64_t n = nbf * nbf;
-      hipLaunchKernelGGL(k_copy2d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, H, ldh, Hp,
-                         (int64_t)g.nbf_p, (int)nbf, (int)nbf);
-      PNJK_HIP_CHECK(hipGetLastError());
-    } else {
-      PNJK_HIP_CHECK(hipMemcpy2DAsync(Hp, (size_t)g.nbf_p * 8, H, (size_t)ldh * 8, (size_t)nbf * 8, (size_t)nbf,
-                                      hipMemcpyHostToDevice, s));
-    }
-    PNJK_HIP_CHECK(launch_hcore_f64(g, Hp, g.nbf_p, Cp, e->T.as<double>(), dH, s));
   }
+  // FP64 kernels run HT/QT in the fp64 and split_refine forms, J^n/K^n in fp64 only.
+  const int a_mo = precision == PNJK_SPLIT ? 1 : 0, a_jk = precision == PNJK_FP64 ? 0 : 1;
+  auto chosen = [&](int arith, int stg, int kind) {
+    return e->tiles[arith][stg].fm ? e->tiles[arith][stg] : default_tile(arith, kind, g.nbf, g.nbf5);
+  };
+  st.ht = chosen(a_mo, ST_HT, 0);
+  st.qt = chosen(a_mo, ST_QT, 1);
+  st.jt = chosen(a_jk, ST_JN, 2);
+  st.kt = chosen(a_jk, ST_K, 3);
+  return PNJK_OK;
+}
 
+// C (and H) into the padded device buffers; split planes of C^T.
+int stage_in(pnjk_engine* e, BuildState& st, hipStream_t s) {
+  if (st.a.precision != PNJK_FP64) {
+    int rc = ensure_split_planes(e, s);
+    if (rc) return rc;
+  }
+  const BuildGeom& g = st.g;
+  const int64_t nbf = g.nbf, n5 = g.nbf5;
+  double* Cp = e->Cp.as<double>();
+  PNJK_HIP_CHECK(fill(Cp, (int64_t)(g.nbf_p + 8) * g.ldcp + 1024, 0.0, s));
+  if (st.c_dev) {
+    const int64_t n = nbf * n5;
+    hipLaunchKernelGGL(k_copy2d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, st.a.C, st.a.ldc, Cp,
+                       (int64_t)g.ldcp, (int)nbf, (int)n5);
+    PNJK_HIP_CHECK(hipGetLastError());
+  } else {
+    PNJK_HIP_CHECK(hipMemcpy2DAsync(Cp, (size_t)g.ldcp * 8, st.a.C, (size_t)st.a.ldc * 8, (size_t)n5 * 8,
+                                    (size_t)nbf, hipMemcpyHostToDevice, s));
+  }
+  if (st.a.precision == PNJK_SPLIT)
+    PNJK_HIP_CHECK(launch_split_ct(Cp, g.ldcp, g.nbf, g.nbf5, g.nbf5, st.sg.ldk, e->ct16h.as<uint16_t>(),
+                                   e->ct16l.as<uint16_t>(), s));
+  return PNJK_OK;
+}
+
+int stage_ht(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col) {
+  const int prec = st.a.precision;
+  if (prec == PNJK_SPLIT) {
+    PNJK_HIP_CHECK(launch_ht_x3(st.sg, st.ht, e->ct16h.as<uint16_t>(), e->ct16l.as<uint16_t>(),
+                                e->bht16h.as<uint16_t>(), e->bht16l.as<uint16_t>(), e->x16h.as<uint16_t>(),
+                                e->x16l.as<uint16_t>(), s, col));
+  } else if (prec == PNJK_SPLIT_REFINE && st.want_k) {
+    // FP64 X for the MO path, with its split planes for K^n written by the same epilogue.
+    PNJK_HIP_CHECK(launch_ht_f64(st.g, st.ht, e->Cp.as<double>(), e->btri.as<double>(), e->X.as<double>(), s,
+                                 e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), col));
+  } else {
+    PNJK_HIP_CHECK(launch_ht_f64(st.g, st.ht, e->Cp.as<double>(), e->btri.as<double>(), e->X.as<double>(), s,
+                                 nullptr, nullptr, col));
+  }
+  return PNJK_OK;
+}
+
+int stage_qt(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col) {
+  if (st.a.precision == PNJK_SPLIT) {
+    PNJK_HIP_CHECK(launch_qt_x3(st.sg, st.qt, e->ct16h.as<uint16_t>(), e->ct16l.as<uint16_t>(),
+                                e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), e->kpart.as<double>(),
+                                e->d.as<double>(), e->d16h.as<uint16_t>(), e->d16l.as<uint16_t>(), &st.nt, s, col));
+  } else {
+    PNJK_HIP_CHECK(launch_qt_f64(st.g, st.qt, e->Cp.as<double>(), e->X.as<double>(), e->kpart.as<double>(),
+                                 e->d.as<double>(), &st.nt, s, col));
+  }
+  return PNJK_OK;
+}
+
+int stage_kmo(pnjk_engine* e, BuildState& st, hipStream_t s) {
+  PNJK_HIP_CHECK(launch_kmo_reduce(st.g.nbf5, st.nt, e->kpart.as<double>(), st.dKMO, s));
+  return PNJK_OK;
+}
+
+int stage_jmo(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col) {
+  PNJK_HIP_CHECK(launch_syrk_f64(1, st.g.nbf5, st.g.naux_p, e->d.as<double>(), st.g.naux_p, 0, st.dJMO, st.g.nbf5, 0,
+                                 TileF64{2, 2}, s, col));
+  return PNJK_OK;
+}
+
+int stage_jmo_post(pnjk_engine* e, BuildState& st, hipStream_t s) {
+  PNJK_HIP_CHECK(launch_symmetrize(1, st.g.nbf5, st.dJMO, st.g.nbf5, 0, s));
+  if (st.a.precision == PNJK_SPLIT_REFINE && st.want_j)  // J^n in split form from the exact d
+    PNJK_HIP_CHECK(launch_split_btri(e->d.as<double>(), (int64_t)st.g.nbf5 * st.g.naux_p, e->d16h.as<uint16_t>(),
+                                     e->d16l.as<uint16_t>(), s));
+  return PNJK_OK;
+}
+
+int stage_jn(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col) {
+  if (!st.want_j) return PNJK_OK;
+  if (st.a.precision == PNJK_FP64)
+    PNJK_HIP_CHECK(launch_jn_f64(st.g, st.jt, e->d.as<double>(), e->btri.as<double>(), st.dJ, s, col));
+  else
+    PNJK_HIP_CHECK(launch_jn_x3(st.sg, st.jt, e->d16h.as<uint16_t>(), e->d16l.as<uint16_t>(),
+                                e->bt16h.as<uint16_t>(), e->bt16l.as<uint16_t>(), st.dJ, s, col));
+  return PNJK_OK;
+}
+
+int stage_k(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col) {
+  if (!st.want_k) return PNJK_OK;
+  const int64_t nbf = st.g.nbf;
+  if (st.a.precision == PNJK_FP64)
+    PNJK_HIP_CHECK(launch_syrk_f64(st.g.nbf5, st.g.nbf, st.g.naux_p, e->X.as<double>(), st.g.naux_p,
+                                   nbf * st.g.naux_p, st.dK, nbf, nbf * nbf, st.kt, s, col));
+  else
+    PNJK_HIP_CHECK(launch_kn_x3(st.sg, st.kt, e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), st.dK, s, col));
+  return PNJK_OK;
+}
+
+int stage_sym(BuildState& st, double* O, hipStream_t s) {
+  const int64_t nbf = st.g.nbf;
+  PNJK_HIP_CHECK(launch_symmetrize(st.g.nbf5, st.g.nbf, O, nbf, nbf * nbf, s));
+  return PNJK_OK;
+}
+
+int stage_hcore(pnjk_engine* e, BuildState& st, hipStream_t s) {
+  if (!st.want_h) return PNJK_OK;
+  const BuildGeom& g = st.g;
+  const int64_t nbf = g.nbf;
+  double* Hp = e->Hp.as<double>();
+  PNJK_HIP_CHECK(fill(Hp, nbf * g.nbf_p, 0.0, s));
+  if (st.h_dev) {
+    const int64_t n = nbf * nbf;
+    hipLaunchKernelGGL(k_copy2d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, st.a.H, st.a.ldh, Hp,
+                       (int64_t)g.nbf_p, (int)nbf, (int)nbf);
+    PNJK_HIP_CHECK(hipGetLastError());
+  } else {
+    PNJK_HIP_CHECK(hipMemcpy2DAsync(Hp, (size_t)g.nbf_p * 8, st.a.H, (size_t)st.a.ldh * 8, (size_t)nbf * 8,
+                                    (size_t)nbf, hipMemcpyHostToDevice, s));
+  }
+  PNJK_HIP_CHECK(launch_hcore_f64(g, Hp, g.nbf_p, e->Cp.as<double>(), e->T.as<double>(), st.dH, s));
+  return PNJK_OK;
+}
+
+// Copies host-destined outputs back; returns whether a stream sync is needed.
+int stage_out(BuildState& st, hipStream_t s, bool* sync) {
+  const int64_t n5 = st.g.nbf5, nbf = st.g.nbf, jk = n5 * nbf * nbf;
+  if (st.hJ) PNJK_HIP_CHECK(hipMemcpyAsync(st.a.J, st.dJ, (size_t)jk * 8, hipMemcpyDeviceToHost, s));
+  if (st.hK) PNJK_HIP_CHECK(hipMemcpyAsync(st.a.K, st.dK, (size_t)jk * 8, hipMemcpyDeviceToHost, s));
+  if (st.want_jmo && st.hJMO)
+    PNJK_HIP_CHECK(hipMemcpyAsync(st.a.J_MO, st.dJMO, (size_t)(n5 * n5) * 8, hipMemcpyDeviceToHost, s));
+  if (st.want_kmo && st.hKMO)
+    PNJK_HIP_CHECK(hipMemcpyAsync(st.a.K_MO, st.dKMO, (size_t)(n5 * n5) * 8, hipMemcpyDeviceToHost, s));
+  if (st.hH) PNJK_HIP_CHECK(hipMemcpyAsync(st.a.H_core, st.dH, (size_t)n5 * 8, hipMemcpyDeviceToHost, s));
+  *sync = *sync || st.hJ || st.hK || (st.want_jmo && st.hJMO) || (st.want_kmo && st.hKMO) || st.hH;
+  return PNJK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, const double* H, int64_t ldh,
+                  int precision, uint32_t outputs, double* J, double* K, double* J_MO, double* K_MO, double* H_core,
+                  void* stream) {
+  if (!e) return set_error(PNJK_EINVAL, "pnjk_jk_build: engine is NULL");
+  std::lock_guard<std::mutex> lk(e->mu);
+  PNJK_HIP_CHECK(hipSetDevice(e->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : e->own;
+  auto mark = [&](int st) {
+    if (e->prof) (void)hipEventRecord(e->ev[st], s);
+  };
+  BuildArgs a{C, ldc, nbf5, H, ldh, precision, outputs, J, K, J_MO, K_MO, H_core};
+  BuildState st;
+  int rc;
+  if ((rc = build_prepare(e, a, st))) return rc;
+  mark(ST_IN);
+  if ((rc = stage_in(e, st, s))) return rc;
+  mark(ST_HT);
+  if ((rc = stage_ht(e, st, s, nullptr))) return rc;
+  mark(ST_QT);
+  if ((rc = stage_qt(e, st, s, nullptr))) return rc;
+  mark(ST_KMO);
+  if ((rc = stage_kmo(e, st, s))) return rc;
+  mark(ST_JMO);
+  if ((rc = stage_jmo(e, st, s, nullptr)) || (rc = stage_jmo_post(e, st, s))) return rc;
+  mark(ST_JN);
+  if ((rc = stage_jn(e, st, s, nullptr))) return rc;
+  if (st.want_j && (rc = stage_sym(st, st.dJ, s))) return rc;
+  mark(ST_K);
+  if ((rc = stage_k(e, st, s, nullptr))) return rc;
+  if (st.want_k && (rc = stage_sym(st, st.dK, s))) return rc;
+  mark(ST_HC);
+  if ((rc = stage_hcore(e, st, s))) return rc;
   mark(ST_OUT);
   bool sync = false;
-  if (hJ) {
-    PNJK_HIP_CHECK(hipMemcpyAsync(J, dJ, (size_t)jk_elems * 8, hipMemcpyDeviceToHost, s));
-    sync = true;
-  }
-  if (hK) {
-    PNJK_HIP_CHECK(hipMemcpyAsync(K, dK, (size_t)jk_elems * 8, hipMemcpyDeviceToHost, s));
-    sync = true;
-  }
-  if (want_jmo && hJMO) {
-    PNJK_HIP_CHECK(hipMemcpyAsync(J_MO, dJMO, (size_t)(n5 * n5) * 8, hipMemcpyDeviceToHost, s));
-    sync = true;
-  }
-  if (want_kmo && hKMO) {
-    PNJK_HIP_CHECK(hipMemcpyAsync(K_MO, dKMO, (size_t)(n5 * n5) * 8, hipMemcpyDeviceToHost, s));
-    sync = true;
-  }
-  if (hH) {
-    PNJK_HIP_CHECK(hipMemcpyAsync(H_core, dH, (size_t)n5 * 8, hipMemcpyDeviceToHost, s));
-    sync = true;
-  }
+  if ((rc = stage_out(st, s, &sync))) return rc;
   mark(ST_COUNT);
   if (e->prof) e->prof_recorded = true;
   if (sync) PNJK_HIP_CHECK(hipStreamSynchronize(s));
+  return PNJK_OK;
+}
+
+}  // extern "C"
+
+struct pnjk_group {
+  std::vector<pnjk_engine*> eng;
+  int device = 0;
+  std::mutex mu;
+  hipStream_t cap = nullptr;            // capture stream
+  std::vector<BuildState> st;
+  std::vector<char> sig;                // signature of the cached plan
+  std::vector<GroupCollector> plan;     // grouped stages: HT, QT, J_MO, J^n, K^n
+  DevBuf plan_dev;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool graph_valid = false;
+  ~pnjk_group() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
+    plan_dev.release();
+  }
+};
+
+namespace {
+
+enum { GS_HT = 0, GS_QT, GS_JMO, GS_JN, GS_K, GS_COUNT };
+
+int launch_plan(pnjk_group* g, int stage, hipStream_t s) {
+  const char* base = static_cast<const char*>(g->plan_dev.p);
+  for (const GroupBatch& b : g->plan[stage].batches) {
+    const int nt = (int)b.tiles.size();
+    if (nt == 0) continue;
+    const int grid = std::min(nt, 1024);
+    PNJK_HIP_CHECK(b.launch(base + b.probs_off, reinterpret_cast<const int2*>(base + b.tiles_off), nt, grid, s));
+  }
+  return PNJK_OK;
+}
+
+// The whole grouped build, enqueued on s (capturable: kernels only).
+int group_enqueue(pnjk_group* g, hipStream_t s) {
+  const size_t n = g->eng.size();
+  int rc;
+  for (size_t i = 0; i < n; ++i)
+    if ((rc = stage_in(g->eng[i], g->st[i], s))) return rc;
+  if ((rc = launch_plan(g, GS_HT, s)) || (rc = launch_plan(g, GS_QT, s))) return rc;
+  for (size_t i = 0; i < n; ++i)
+    if ((rc = stage_kmo(g->eng[i], g->st[i], s))) return rc;
+  if ((rc = launch_plan(g, GS_JMO, s))) return rc;
+  for (size_t i = 0; i < n; ++i)
+    if ((rc = stage_jmo_post(g->eng[i], g->st[i], s))) return rc;
+  if ((rc = launch_plan(g, GS_JN, s))) return rc;
+  for (size_t i = 0; i < n; ++i)
+    if (g->st[i].want_j && (rc = stage_sym(g->st[i], g->st[i].dJ, s))) return rc;
+  if ((rc = launch_plan(g, GS_K, s))) return rc;
+  for (size_t i = 0; i < n; ++i)
+    if (g->st[i].want_k && (rc = stage_sym(g->st[i], g->st[i].dK, s))) return rc;
+  for (size_t i = 0; i < n; ++i)
+    if ((rc = stage_hcore(g->eng[i], g->st[i], s))) return rc;
+  return PNJK_OK;
+}
+
+template <class T>
+void sig_put(std::vector<char>& v, const T& x) {
+  const char* r = reinterpret_cast<const char*>(&x);
+  v.insert(v.end(), r, r + sizeof(T));
+}
+
+}  // namespace
+
+extern "C" {
+
+int pnjk_group_create(pnjk_engine* const* engines, int count, pnjk_group** out) {
+  if (!out || !engines || count < 1) return set_error(PNJK_EINVAL, "pnjk_group_create: bad arguments");
+  *out = nullptr;
+  for (int i = 0; i < count; ++i) {
+    if (!engines[i]) return set_error(PNJK_EINVAL, "pnjk_group_create: NULL engine");
+    if (engines[i]->device != engines[0]->device)
+      return set_error(PNJK_EINVAL, "pnjk_group_create: engines must share one device");
+    for (int j = 0; j < i; ++j)
+      if (engines[j] == engines[i])
+        return set_error(PNJK_EINVAL, "pnjk_group_create: an engine appears twice (each instance needs its own)");
+  }
+  pnjk_group* g = new pnjk_group();
+  g->eng.assign(engines, engines + count);
+  g->device = engines[0]->device;
+  g->plan.resize(GS_COUNT);
+  hipError_t he = hipSetDevice(g->device);
+  if (he == hipSuccess) he = hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking);
+  if (he != hipSuccess) {
+    delete g;
+    return set_error(PNJK_EHIP, std::string("pnjk_group_create: ") + hipGetErrorString(he));
+  }
+  *out = g;
+  return PNJK_OK;
+}
+
+int pnjk_group_destroy(pnjk_group* g) {
+  if (!g) return PNJK_OK;
+  (void)hipSetDevice(g->device);
+  (void)hipStreamSynchronize(g->cap);
+  delete g;
+  return PNJK_OK;
+}
+
+int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, const int64_t* nbf5,
+                     const double* const* H, const int64_t* ldh, int precision, uint32_t outputs,
+                     double* const* J, double* const* K, double* const* J_MO, double* const* K_MO,
+                     double* const* H_core, int use_graph, void* stream) {
+  if (!g || !C || !ldc || !nbf5) return set_error(PNJK_EINVAL, "pnjk_group_build: null argument");
+  std::lock_guard<std::mutex> lk(g->mu);
+  const size_t n = g->eng.size();
+  std::vector<std::unique_lock<std::mutex>> locks;
+  locks.reserve(n);
+  for (pnjk_engine* e : g->eng) locks.emplace_back(e->mu);
+  PNJK_HIP_CHECK(hipSetDevice(g->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->eng[0]->own;
+  auto at = [](double* const* a, size_t i) { return a ? a[i] : nullptr; };
+  int rc;
+  g->st.resize(n);
+  std::vector<char> sig;
+  sig_put(sig, precision);
+  sig_put(sig, outputs);
+  for (size_t i = 0; i < n; ++i) {
+    BuildArgs a{C[i], ldc[i], nbf5[i], H ? H[i] : nullptr, ldh ? ldh[i] : 0, precision, outputs,
+                at(J, i), at(K, i), at(J_MO, i), at(K_MO, i), at(H_core, i)};
+    if ((rc = build_prepare(g->eng[i], a, g->st[i]))) return rc;
+    const BuildState& st = g->st[i];
+    if (st.hJ || st.hK || (st.want_jmo && st.hJMO) || (st.want_kmo && st.hKMO) || st.hH)
+      return set_error(PNJK_EINVAL, "pnjk_group_build: outputs of a grouped build must be device memory");
+    if (use_graph && (!st.c_dev || (st.want_h && !st.h_dev)))
+      return set_error(PNJK_EINVAL, "pnjk_group_build: graph mode needs device-resident C and H");
+    if (precision != PNJK_FP64 && (rc = ensure_split_planes(g->eng[i], s))) return rc;
+    sig_put(sig, st.a);
+    sig_put(sig, st.ht);
+    sig_put(sig, st.qt);
+    sig_put(sig, st.jt);
+    sig_put(sig, st.kt);
+    for (int k = 0; k < g->eng[i]->nall; ++k) sig_put(sig, g->eng[i]->all[k]->p);
+  }
+  if (sig != g->sig) {
+    // New plan: collect every instance's tiles per grouped stage, upload once.
+    for (GroupCollector& c : g->plan) c.batches.clear();
+    for (size_t i = 0; i < n; ++i) {
+      pnjk_engine* e = g->eng[i];
+      BuildState& st = g->st[i];
+      if ((rc = stage_ht(e, st, s, &g->plan[GS_HT])) || (rc = stage_qt(e, st, s, &g->plan[GS_QT])) ||
+          (rc = stage_jmo(e, st, s, &g->plan[GS_JMO])) || (rc = stage_jn(e, st, s, &g->plan[GS_JN])) ||
+          (rc = stage_k(e, st, s, &g->plan[GS_K])))
+        return rc;
+    }
+    size_t bytes = 0;
+    for (GroupCollector& c : g->plan)
+      for (GroupBatch& b : c.batches) {
+        b.probs_off = bytes;
+        bytes = roundup((int64_t)(bytes + b.probs.size()), 256);
+        b.tiles_off = bytes;
+        bytes = roundup((int64_t)(bytes + b.tiles.size() * sizeof(int2)), 256);
+      }
+    PNJK_HIP_CHECK(hipStreamSynchronize(s));  // the previous plan may still be in use
+    if ((rc = g->plan_dev.ensure(std::max<size_t>(bytes, 256)))) return rc;
+    char* base = static_cast<char*>(g->plan_dev.p);
+    for (GroupCollector& c : g->plan)
+      for (GroupBatch& b : c.batches) {
+        PNJK_HIP_CHECK(hipMemcpy(base + b.probs_off, b.probs.data(), b.probs.size(), hipMemcpyHostToDevice));
+        PNJK_HIP_CHECK(hipMemcpy(base + b.tiles_off, b.tiles.data(), b.tiles.size() * sizeof(int2),
+                                 hipMemcpyHostToDevice));
+      }
+    g->sig = sig;
+    g->graph_valid = false;
+  } else {
+    for (size_t i = 0; i < n; ++i) {  // quarter-transform tile counts for the K_MO reduction
+      BuildState& st = g->st[i];
+      st.nt = st.g.naux_p / (32 * st.qt.fn);
+    }
+  }
+  if (!use_graph) return group_enqueue(g, s);
+  if (!g->graph_valid) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    g->exec = nullptr;
+    g->graph = nullptr;
+    PNJK_HIP_CHECK(hipStreamSynchronize(s));
+    PNJK_HIP_CHECK(hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal));
+    rc = group_enqueue(g, g->cap);
+    hipGraph_t graph = nullptr;
+    const hipError_t he = hipStreamEndCapture(g->cap, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    PNJK_HIP_CHECK(he);
+    g->graph = graph;
+    PNJK_HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    g->graph_valid = true;
+  }
+  PNJK_HIP_CHECK(hipGraphLaunch(g->exec, s));
   return PNJK_OK;
 }
 

@@ -95,21 +95,13 @@ struct Loader<MK, ROWS, BK> {
   }
 };
 
-struct Tile {
-  int batch, mt, nt, m0, n0;
-};
-
 // The GEMM core. Accumulator element (fm, fn, r) of a lane is output element
 //   m = m0 + wm + 16*fm + lane/16 + 4*r,  n = n0 + wn + 16*fn + lane%16
 // (the v_mfma_f64_16x16x4f64 result layout).
 template <int FM, int FN, int BK, class Prob>
-__global__ void __launch_bounds__(kThreads) k_gemm_f64(const Prob prob) {
+__device__ __forceinline__ void gemm_tile_f64(const Prob& prob, const Tile& t, double* sA, double* sB) {
   constexpr int BM = 32 * FM, BN = 32 * FN;
-  __shared__ __align__(16) double sA[2][BK * (BM + kPad)];
-  __shared__ __align__(16) double sB[2][BK * (BN + kPad)];
-
-  Tile t;
-  if (!prob.tile(xcd_remap(blockIdx.x, gridDim.x), BM, BN, t)) return;
+  constexpr int SA = BK * (BM + kPad), SB = BK * (BN + kPad);
   const auto as = prob.a_src(t);
   const auto bs = prob.b_src(t);
   const int nk = prob.ksteps(BK);
@@ -121,8 +113,8 @@ __global__ void __launch_bounds__(kThreads) k_gemm_f64(const Prob prob) {
   Loader<Prob::kB, BN, BK> lb;
   la.load(as, t.m0, 0);
   lb.load(bs, t.n0, 0);
-  la.store(sA[0]);
-  lb.store(sB[0]);
+  la.store(sA);
+  lb.store(sB);
   __syncthreads();
 
   v4d acc[FM][FN];
@@ -138,8 +130,8 @@ __global__ void __launch_bounds__(kThreads) k_gemm_f64(const Prob prob) {
       la.load(as, t.m0, (kt + 1) * BK);
       lb.load(bs, t.n0, (kt + 1) * BK);
     }
-    const double* a_l = sA[cur] + wm + (lane & 15);
-    const double* b_l = sB[cur] + wn + (lane & 15);
+    const double* a_l = sA + cur * SA + wm + (lane & 15);
+    const double* b_l = sB + cur * SB + wn + (lane & 15);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kr = kk + (lane >> 4);
@@ -154,12 +146,41 @@ __global__ void __launch_bounds__(kThreads) k_gemm_f64(const Prob prob) {
         for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     if (more) {
-      la.store(sA[cur ^ 1]);
-      lb.store(sB[cur ^ 1]);
+      la.store(sA + (cur ^ 1) * SA);
+      lb.store(sB + (cur ^ 1) * SB);
     }
     __syncthreads();
   }
-  prob.template epilogue<FM, FN>(t, acc, wm, wn, lane, &sA[0][0]);
+  prob.template epilogue<FM, FN>(t, acc, wm, wn, lane, sA);
+}
+
+// One output tile per workgroup.
+template <int FM, int FN, int BK, class Prob>
+__global__ void __launch_bounds__(kThreads) k_gemm_f64(const Prob prob) {
+  constexpr int BM = 32 * FM, BN = 32 * FN;
+  __shared__ __align__(16) double sA[2 * BK * (BM + kPad)];
+  __shared__ __align__(16) double sB[2 * BK * (BN + kPad)];
+  Tile t;
+  if (!prob.tile(xcd_remap(blockIdx.x, gridDim.x), BM, BN, t)) return;
+  gemm_tile_f64<FM, FN, BK>(prob, t, sA, sB);
+}
+
+// Grouped persistent form: one launch over the tiles of many instances of the
+// same problem type. tiles[i] = (slot of the instance's problem in probs, local
+// tile id); each workgroup walks the table with stride gridDim.x.
+template <int FM, int FN, int BK, class Prob>
+__global__ void __launch_bounds__(kThreads)
+    k_gemm_f64_grouped(const Prob* __restrict__ probs, const int2* __restrict__ tiles, int ntiles) {
+  constexpr int BM = 32 * FM, BN = 32 * FN;
+  __shared__ __align__(16) double sA[2 * BK * (BM + kPad)];
+  __shared__ __align__(16) double sB[2 * BK * (BN + kPad)];
+  for (int id = blockIdx.x; id < ntiles; id += gridDim.x) {
+    const int2 tr = tiles[id];
+    const Prob prob = probs[tr.x];
+    Tile t;
+    if (prob.tile(tr.y, BM, BN, t)) gemm_tile_f64<FM, FN, BK>(prob, t, sA, sB);
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------- sources
@@ -220,7 +241,7 @@ struct ProbHT {
   uint16_t* xlo;
   int nbf, naux_p, nbf5, ldcp, nbf_p, mtiles;
   int64_t N;
-  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.batch = 0;
     t.mt = id % mtiles;
     t.nt = id / mtiles;
@@ -263,7 +284,7 @@ struct ProbQT {
   double* kpart;
   double* d;
   int nbf, naux_p, nbf5, ldcp, nbf_p, mtiles, ntiles;
-  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.mt = id % mtiles;
     const int rest = id / mtiles;
     t.nt = rest % ntiles;
@@ -335,7 +356,7 @@ struct ProbSYRK {
   double* O;
   int64_t lda, sa, ldo, so;
   int M, K, tiles_side, tiles_per_batch;
-  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.batch = id / tiles_per_batch;
     const int rem = id - t.batch * tiles_per_batch;
     int mt = (int)((sqrt(8.0 * rem + 1.0) - 1.0) * 0.5);
@@ -374,7 +395,7 @@ struct ProbJN {
   double* J;
   int nbf, naux_p, nbf5, mtiles;
   int64_t ntri;
-  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.batch = 0;
     t.mt = id % mtiles;
     t.nt = id / mtiles;
@@ -414,7 +435,7 @@ struct ProbGemmT {  // T[mu][i] = sum_nu Hp[mu][nu] Cp[nu][i]
   const double* Cp;
   double* T;
   int ldh, ldcp, nbf, nbf_p, mtiles;
-  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.batch = 0;
     t.mt = id % mtiles;
     t.nt = id / mtiles;
@@ -524,14 +545,28 @@ __global__ void k_symmetrize(double* __restrict__ O, int n, int64_t ld, int64_t 
   }
 }
 
+template <int FM, int FN, int BK, class Prob>
+hipError_t grouped_launch(const void* probs, const int2* tiles, int ntiles, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_gemm_f64_grouped<FM, FN, BK, Prob>), dim3(grid), dim3(kThreads), 0, s,
+                     static_cast<const Prob*>(probs), tiles, ntiles);
+  return hipGetLastError();
+}
+
+// Either launches `prob` over nblocks tiles now, or (col != null) appends it
+// and its valid tiles to the grouped launch for its problem type and tile.
 template <class Prob>
-hipError_t launch_prob(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s) {
+hipError_t launch_prob(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s, GroupCollector* col = nullptr) {
   if (nblocks <= 0) return hipSuccess;
   const dim3 grid(nblocks), block(kThreads);
-#define PNJK_CASE(FMV, FNV, BKV)                                                  \
-  if (tc.fm == FMV && tc.fn == FNV) {                                             \
-    hipLaunchKernelGGL((k_gemm_f64<FMV, FNV, BKV, Prob>), grid, block, 0, s, prob); \
-    return hipGetLastError();                                                     \
+#define PNJK_CASE(FMV, FNV, BKV)                                                                            \
+  if (tc.fm == FMV && tc.fn == FNV) {                                                                       \
+    if (col) {                                                                                              \
+      col->add(prob, tc, nblocks, 32 * FMV, 32 * FNV, &grouped_launch<FMV, FNV, BKV, Prob>,                 \
+               __PRETTY_FUNCTION__);                                                                        \
+      return hipSuccess;                                                                                    \
+    }                                                                                                       \
+    hipLaunchKernelGGL((k_gemm_f64<FMV, FNV, BKV, Prob>), grid, block, 0, s, prob);                         \
+    return hipGetLastError();                                                                               \
   }
   PNJK_CASE(2, 2, 16)
   PNJK_CASE(4, 4, 8)
@@ -549,7 +584,7 @@ inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
 template <bool kSplit>
 hipError_t launch_ht_f64_t(const BuildGeom& g, TileF64 tc, const double* Cp, const double* btri, double* X,
-                           hipStream_t s, uint16_t* x_hi, uint16_t* x_lo) {
+                           hipStream_t s, uint16_t* x_hi, uint16_t* x_lo, GroupCollector* col) {
   ProbHT<kSplit> pr;
   pr.Cp = Cp;
   pr.btri = btri;
@@ -564,17 +599,17 @@ hipError_t launch_ht_f64_t(const BuildGeom& g, TileF64 tc, const double* Cp, con
   pr.N = (int64_t)g.nbf * g.naux_p;
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
   pr.mtiles = cdiv(g.nbf5, BM);
-  return launch_prob(pr, tc, pr.mtiles * cdiv(pr.N, BN), s);
+  return launch_prob(pr, tc, pr.mtiles * cdiv(pr.N, BN), s, col);
 }
 
 hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* btri, double* X,
-                         hipStream_t s, uint16_t* x_hi, uint16_t* x_lo) {
-  if (x_hi && x_lo) return launch_ht_f64_t<true>(g, tc, Cp, btri, X, s, x_hi, x_lo);
-  return launch_ht_f64_t<false>(g, tc, Cp, btri, X, s, nullptr, nullptr);
+                         hipStream_t s, uint16_t* x_hi, uint16_t* x_lo, GroupCollector* col) {
+  if (x_hi && x_lo) return launch_ht_f64_t<true>(g, tc, Cp, btri, X, s, x_hi, x_lo, col);
+  return launch_ht_f64_t<false>(g, tc, Cp, btri, X, s, nullptr, nullptr, col);
 }
 
 hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* X, double* kpart,
-                         double* d, int* ntiles, hipStream_t s) {
+                         double* d, int* ntiles, hipStream_t s, GroupCollector* col) {
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
   if (g.naux_p % BN != 0) return hipErrorInvalidValue;
   ProbQT pr;
@@ -590,7 +625,7 @@ hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const
   pr.mtiles = cdiv(g.nbf5, BM);
   pr.ntiles = g.naux_p / BN;
   *ntiles = pr.ntiles;
-  return launch_prob(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s);
+  return launch_prob(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s, col);
 }
 
 hipError_t launch_symmetrize(int batch, int n, double* O, int64_t ld, int64_t stride, hipStream_t s) {
@@ -609,7 +644,7 @@ hipError_t launch_kmo_reduce(int nbf5, int nt, const double* kpart, double* kmo,
 }
 
 hipError_t launch_syrk_f64(int batch, int M, int K, const double* A, int64_t lda, int64_t sa, double* O,
-                           int64_t ldo, int64_t so, TileF64 tc, hipStream_t s) {
+                           int64_t ldo, int64_t so, TileF64 tc, hipStream_t s, GroupCollector* col) {
   if (tc.fm != tc.fn) return hipErrorInvalidValue;
   const int BM = 32 * tc.fm;
   const int BK = (tc.fm >= 4 && tc.fn >= 4) ? 8 : 16;
@@ -625,11 +660,11 @@ hipError_t launch_syrk_f64(int batch, int M, int K, const double* A, int64_t lda
   pr.K = K;
   pr.tiles_side = cdiv(M, BM);
   pr.tiles_per_batch = pr.tiles_side * (pr.tiles_side + 1) / 2;
-  return launch_prob(pr, tc, pr.tiles_per_batch * batch, s);
+  return launch_prob(pr, tc, pr.tiles_per_batch * batch, s, col);
 }
 
 hipError_t launch_jn_f64(const BuildGeom& g, TileF64 tc, const double* d, const double* btri, double* J,
-                         hipStream_t s) {
+                         hipStream_t s, GroupCollector* col) {
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
   ProbJN pr;
   pr.d = d;
@@ -640,7 +675,7 @@ hipError_t launch_jn_f64(const BuildGeom& g, TileF64 tc, const double* d, const 
   pr.nbf5 = g.nbf5;
   pr.ntri = g.ntri;
   pr.mtiles = cdiv(g.nbf5, BM);
-  return launch_prob(pr, tc, pr.mtiles * cdiv(g.ntri, BN), s);
+  return launch_prob(pr, tc, pr.mtiles * cdiv(g.ntri, BN), s, col);
 }
 
 hipError_t launch_hcore_f64(const BuildGeom& g, const double* Hp, int ldh, const double* Cp, double* T,

@@ -2,14 +2,53 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
 #include <cstdint>
+#include <string>
+#include <vector>
+
+#include "jk_common.h"
 
 namespace pnjk {
 
-// Tile shape of one FP64 MFMA GEMM launch: BM = 32*fm rows, BN = 32*fn columns.
-struct TileF64 {
-  int fm;
-  int fn;
+// ---------------------------------------------------------------- grouped launches
+// A grouped launch runs the tiles of many instances of one problem type and
+// tile shape in one persistent kernel. Launch functions given a collector do
+// not launch: they append the instance's problem struct and its valid tile ids.
+typedef hipError_t (*GroupedFn)(const void* probs, const int2* tiles, int ntiles, int grid, hipStream_t s);
+
+struct GroupBatch {
+  std::string key;        // problem type and tile shape
+  size_t prob_size = 0;
+  std::vector<char> probs;  // packed problem structs, one per instance slot
+  std::vector<int2> tiles;  // (slot, local tile id)
+  GroupedFn launch = nullptr;
+  size_t probs_off = 0, tiles_off = 0;  // offsets of the device copies in the plan buffer
+};
+
+struct GroupCollector {
+  std::vector<GroupBatch> batches;
+  template <class Prob>
+  void add(const Prob& pr, TileF64 tc, int nblocks, int BM, int BN, GroupedFn fn, const char* type) {
+    const std::string key = std::string(type) + "/" + std::to_string(tc.fm) + "x" + std::to_string(tc.fn);
+    GroupBatch* b = nullptr;
+    for (GroupBatch& x : batches)
+      if (x.key == key) b = &x;
+    if (!b) {
+      batches.emplace_back();
+      b = &batches.back();
+      b->key = key;
+      b->prob_size = sizeof(Prob);
+      b->launch = fn;
+    }
+    const int slot = (int)(b->probs.size() / sizeof(Prob));
+    const char* raw = reinterpret_cast<const char*>(&pr);
+    b->probs.insert(b->probs.end(), raw, raw + sizeof(Prob));
+    for (int id = 0; id < nblocks; ++id) {
+      Tile t;
+      if (pr.tile(id, BM, BN, t)) b->tiles.push_back(make_int2(slot, id));
+    }
+  }
 };
 
 // Geometry of one build. Row strides are in elements.
@@ -27,13 +66,14 @@ struct BuildGeom {
 // X[q][nu*naux_p + P] = sum_mu Cp[mu][q] b_tri[tri(mu,nu)][P]   (half-transform)
 // When x_hi/x_lo are given, also writes the bf16 split planes of X (same layout).
 hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* btri, double* X,
-                         hipStream_t s, uint16_t* x_hi = nullptr, uint16_t* x_lo = nullptr);
+                         hipStream_t s, uint16_t* x_hi = nullptr, uint16_t* x_lo = nullptr,
+                         GroupCollector* col = nullptr);
 
 // Y_p[q][P] = sum_nu Cp[nu][q] X[p][nu][P] for q <= p; writes row sums of Y_p^2
 // per P-tile into kpart[(p*nbf5+q)*nt + tile] and d[p][P] = Y_p[p][P].
 // Returns the number of P-tiles in *ntiles.
 hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* X, double* kpart,
-                         double* d, int* ntiles, hipStream_t s);
+                         double* d, int* ntiles, hipStream_t s, GroupCollector* col = nullptr);
 
 // K_MO[p][q] = K_MO[q][p] = sum_t kpart[(p*nbf5+q)*nt + t]  (q <= p)
 hipError_t launch_kmo_reduce(int nbf5, int nt, const double* kpart, double* kmo, hipStream_t s);
@@ -41,11 +81,11 @@ hipError_t launch_kmo_reduce(int nbf5, int nt, const double* kpart, double* kmo,
 // O[b][i][j] = sum_k A[b][i][k] A[b][j][k] for i, j < M on the lower triangle of
 // tiles; the strict upper triangle is left for launch_symmetrize.
 hipError_t launch_syrk_f64(int batch, int M, int K, const double* A, int64_t lda, int64_t sa, double* O,
-                           int64_t ldo, int64_t so, TileF64 tc, hipStream_t s);
+                           int64_t ldo, int64_t so, TileF64 tc, hipStream_t s, GroupCollector* col = nullptr);
 
 // J[q][mu][nu] = sum_P d[q][P] b_tri[tri(mu,nu)][P] for mu >= nu (lower triangle only).
 hipError_t launch_jn_f64(const BuildGeom& g, TileF64 tc, const double* d, const double* btri, double* J,
-                         hipStream_t s);
+                         hipStream_t s, GroupCollector* col = nullptr);
 
 // O[b][j][i] = O[b][i][j] for i > j, b < batch (n x n matrices, row stride ld).
 hipError_t launch_symmetrize(int batch, int n, double* O, int64_t ld, int64_t stride, hipStream_t s);
@@ -82,16 +122,17 @@ hipError_t launch_split_ct(const double* Cp, int ldcp, int nbf, int nbf5, int ro
 // X16[q][nu][P] = split(sum_mu C[mu][q] b[mu][nu][P]).
 hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
                         const uint16_t* bht_hi, const uint16_t* bht_lo, uint16_t* x_hi, uint16_t* x_lo,
-                        hipStream_t s);
+                        hipStream_t s, GroupCollector* col = nullptr);
 // Quarter transform from X16: K_MO row-sum partials, d (FP64 and split planes).
 hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
                         const uint16_t* x_hi, const uint16_t* x_lo, double* kpart, double* d, uint16_t* d_hi,
-                        uint16_t* d_lo, int* ntiles, hipStream_t s);
+                        uint16_t* d_lo, int* ntiles, hipStream_t s, GroupCollector* col = nullptr);
 // J^n from split d and btri planes (FP64 output, mu >= nu only).
 hipError_t launch_jn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* d_hi, const uint16_t* d_lo,
-                        const uint16_t* bt_hi, const uint16_t* bt_lo, double* J, hipStream_t s);
+                        const uint16_t* bt_hi, const uint16_t* bt_lo, double* J, hipStream_t s,
+                        GroupCollector* col = nullptr);
 // K^n from split X16 (FP64 output, lower triangle of tiles only).
 hipError_t launch_kn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* x_hi, const uint16_t* x_lo, double* K,
-                        hipStream_t s);
+                        hipStream_t s, GroupCollector* col = nullptr);
 
 }  // namespace pnjk

@@ -109,18 +109,12 @@ struct Ld<KR, ROWS> {
   }
 };
 
-struct Tile {
-  int batch, mt, nt, m0, n0;
-};
-
 template <int FM, int FN, class Prob>
-__global__ void __launch_bounds__(kThreads) k_gemm_x3(const Prob prob) {
+__device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, uint16_t* sAbase, uint16_t* sBbase) {
   constexpr int BM = 32 * FM, BN = 32 * FN;
-  __shared__ __align__(16) uint16_t sA[2][2][BM * kLDK];
-  __shared__ __align__(16) uint16_t sB[2][2][BN * kLDK];
-
-  Tile t;
-  if (!prob.tile(xcd_remap(blockIdx.x, gridDim.x), BM, BN, t)) return;
+  // LDS: [buffer][plane][rows * kLDK]
+  auto sA = [&](int buf, int plane) { return sAbase + (buf * 2 + plane) * BM * kLDK; };
+  auto sB = [&](int buf, int plane) { return sBbase + (buf * 2 + plane) * BN * kLDK; };
   const auto as = prob.a_src(t);
   const auto bs = prob.b_src(t);
   const int nk = prob.ksteps();
@@ -132,8 +126,8 @@ __global__ void __launch_bounds__(kThreads) k_gemm_x3(const Prob prob) {
   Ld<Prob::kB, BN> lb;
   la.load(as, t.m0, 0);
   lb.load(bs, t.n0, 0);
-  la.store(sA[0][0], sA[0][1]);
-  lb.store(sB[0][0], sB[0][1]);
+  la.store(sA(0, 0), sA(0, 1));
+  lb.store(sB(0, 0), sB(0, 1));
   __syncthreads();
 
   v4f acc[FM][FN];
@@ -150,18 +144,22 @@ __global__ void __launch_bounds__(kThreads) k_gemm_x3(const Prob prob) {
       la.load(as, t.m0, (kt + 1) * kBK);
       lb.load(bs, t.n0, (kt + 1) * kBK);
     }
+    const uint16_t* ah_s = sA(cur, 0);
+    const uint16_t* al_s = sA(cur, 1);
+    const uint16_t* bh_s = sB(cur, 0);
+    const uint16_t* bl_s = sB(cur, 1);
     bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
       const int o = (wm + 16 * i + frow) * kLDK + fk;
-      ah[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sA[cur][0][o]));
-      al[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sA[cur][1][o]));
+      ah[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(ah_s + o));
+      al[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(al_s + o));
     }
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const int o = (wn + 16 * j + frow) * kLDK + fk;
-      bh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sB[cur][0][o]));
-      bl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sB[cur][1][o]));
+      bh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(bh_s + o));
+      bl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(bl_s + o));
     }
 #pragma unroll
     for (int i = 0; i < FM; ++i)
@@ -172,12 +170,39 @@ __global__ void __launch_bounds__(kThreads) k_gemm_x3(const Prob prob) {
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
       }
     if (more) {
-      la.store(sA[cur ^ 1][0], sA[cur ^ 1][1]);
-      lb.store(sB[cur ^ 1][0], sB[cur ^ 1][1]);
+      la.store(sA(cur ^ 1, 0), sA(cur ^ 1, 1));
+      lb.store(sB(cur ^ 1, 0), sB(cur ^ 1, 1));
     }
     __syncthreads();
   }
-  prob.template epilogue<FM, FN>(t, acc, wm, wn, lane, reinterpret_cast<double*>(&sA[0][0][0]));
+  prob.template epilogue<FM, FN>(t, acc, wm, wn, lane, reinterpret_cast<double*>(sAbase));
+}
+
+// One output tile per workgroup.
+template <int FM, int FN, class Prob>
+__global__ void __launch_bounds__(kThreads) k_gemm_x3(const Prob prob) {
+  constexpr int BM = 32 * FM, BN = 32 * FN;
+  __shared__ __align__(16) uint16_t sA[2 * 2 * BM * kLDK];
+  __shared__ __align__(16) uint16_t sB[2 * 2 * BN * kLDK];
+  Tile t;
+  if (!prob.tile(xcd_remap(blockIdx.x, gridDim.x), BM, BN, t)) return;
+  gemm_tile_x3<FM, FN>(prob, t, sA, sB);
+}
+
+// Grouped persistent form over a (slot, local tile) table; see jk_kernels.h.
+template <int FM, int FN, class Prob>
+__global__ void __launch_bounds__(kThreads)
+    k_gemm_x3_grouped(const Prob* __restrict__ probs, const int2* __restrict__ tiles, int ntiles) {
+  constexpr int BM = 32 * FM, BN = 32 * FN;
+  __shared__ __align__(16) uint16_t sA[2 * 2 * BM * kLDK];
+  __shared__ __align__(16) uint16_t sB[2 * 2 * BN * kLDK];
+  for (int id = blockIdx.x; id < ntiles; id += gridDim.x) {
+    const int2 tr = tiles[id];
+    const Prob prob = probs[tr.x];
+    Tile t;
+    if (prob.tile(tr.y, BM, BN, t)) gemm_tile_x3<FM, FN>(prob, t, sA, sB);
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------- sources
@@ -202,7 +227,7 @@ struct ProbHT {
   uint16_t* xlo;
   int nbf5, ldk, mtiles, m_base;  // rows [m_base, m_base + mtiles*BM) of this launch
   int64_t N;
-  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.batch = 0;
     t.mt = id % mtiles;
     t.nt = id / mtiles;
@@ -252,7 +277,7 @@ struct ProbQT {
   uint16_t* dhi;
   uint16_t* dlo;
   int nbf, naux_p, nbf5, ldk, mtiles, ntiles;
-  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.mt = id % mtiles;
     const int rest = id / mtiles;
     t.nt = rest % ntiles;
@@ -327,7 +352,7 @@ struct ProbJN {
   double* J;
   int nbf, naux_p, nbf5, mtiles, m_base;  // rows [m_base, m_base + mtiles*BM) of this launch
   int64_t ntri;
-  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.batch = 0;
     t.mt = id % mtiles;
     t.nt = id / mtiles;
@@ -371,7 +396,7 @@ struct ProbSYRK {
   double* O;
   int64_t lda, sa, ldo, so;
   int M, K, tiles_per_batch;
-  __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.batch = id / tiles_per_batch;
     const int rem = id - t.batch * tiles_per_batch;
     int mt = (int)((sqrt(8.0 * rem + 1.0) - 1.0) * 0.5);
@@ -456,14 +481,25 @@ __global__ void k_split_ct(const double* __restrict__ Cp, int ldcp, int nbf, int
   lo[i] = l;
 }
 
+template <int FM, int FN, class Prob>
+hipError_t grouped_launch(const void* probs, const int2* tiles, int ntiles, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_gemm_x3_grouped<FM, FN, Prob>), dim3(grid), dim3(kThreads), 0, s,
+                     static_cast<const Prob*>(probs), tiles, ntiles);
+  return hipGetLastError();
+}
+
 template <class Prob>
-hipError_t launch_x3(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s) {
+hipError_t launch_x3(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s, GroupCollector* col = nullptr) {
   if (nblocks <= 0) return hipSuccess;
   const dim3 grid(nblocks), block(kThreads);
-#define PNJK_CASE(FMV, FNV)                                                   \
-  if (tc.fm == FMV && tc.fn == FNV) {                                         \
-    hipLaunchKernelGGL((k_gemm_x3<FMV, FNV, Prob>), grid, block, 0, s, prob); \
-    return hipGetLastError();                                                 \
+#define PNJK_CASE(FMV, FNV)                                                                                 \
+  if (tc.fm == FMV && tc.fn == FNV) {                                                                       \
+    if (col) {                                                                                              \
+      col->add(prob, tc, nblocks, 32 * FMV, 32 * FNV, &grouped_launch<FMV, FNV, Prob>, __PRETTY_FUNCTION__); \
+      return hipSuccess;                                                                                    \
+    }                                                                                                       \
+    hipLaunchKernelGGL((k_gemm_x3<FMV, FNV, Prob>), grid, block, 0, s, prob);                               \
+    return hipGetLastError();                                                                               \
   }
   PNJK_CASE(2, 2)
   PNJK_CASE(4, 4)
@@ -501,7 +537,7 @@ hipError_t launch_split_ct(const double* Cp, int ldcp, int nbf, int nbf5, int ro
 
 hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
                         const uint16_t* bht_hi, const uint16_t* bht_lo, uint16_t* x_hi, uint16_t* x_lo,
-                        hipStream_t s) {
+                        hipStream_t s, GroupCollector* col) {
   ProbHT pr;
   pr.ct = {ct_hi, ct_lo};
   pr.bht = {bht_hi, bht_lo};
@@ -513,12 +549,12 @@ hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
   pr.m_base = 0;
   pr.mtiles = cdiv(g.nbf5, BM);
-  return launch_x3(pr, tc, pr.mtiles * cdiv(pr.N, BN), s);
+  return launch_x3(pr, tc, pr.mtiles * cdiv(pr.N, BN), s, col);
 }
 
 hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
                         const uint16_t* x_hi, const uint16_t* x_lo, double* kpart, double* d, uint16_t* d_hi,
-                        uint16_t* d_lo, int* ntiles, hipStream_t s) {
+                        uint16_t* d_lo, int* ntiles, hipStream_t s, GroupCollector* col) {
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
   if (g.naux_p % BN != 0) return hipErrorInvalidValue;
   ProbQT pr;
@@ -535,11 +571,12 @@ hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
   pr.mtiles = cdiv(g.nbf5, BM);
   pr.ntiles = g.naux_p / BN;
   *ntiles = pr.ntiles;
-  return launch_x3(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s);
+  return launch_x3(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s, col);
 }
 
 hipError_t launch_jn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* d_hi, const uint16_t* d_lo,
-                        const uint16_t* bt_hi, const uint16_t* bt_lo, double* J, hipStream_t s) {
+                        const uint16_t* bt_hi, const uint16_t* bt_lo, double* J, hipStream_t s,
+                        GroupCollector* col) {
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
   ProbJN pr;
   pr.dd = {d_hi, d_lo};
@@ -551,11 +588,11 @@ hipError_t launch_jn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* d_hi, co
   pr.ntri = (int64_t)g.nbf * (g.nbf + 1) / 2;
   pr.m_base = 0;
   pr.mtiles = cdiv(g.nbf5, BM);
-  return launch_x3(pr, tc, pr.mtiles * cdiv(pr.ntri, BN), s);
+  return launch_x3(pr, tc, pr.mtiles * cdiv(pr.ntri, BN), s, col);
 }
 
 hipError_t launch_kn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* x_hi, const uint16_t* x_lo, double* K,
-                        hipStream_t s) {
+                        hipStream_t s, GroupCollector* col) {
   if (tc.fm != tc.fn) return hipErrorInvalidValue;
   const int BM = 32 * tc.fm;
   ProbSYRK pr;
@@ -569,7 +606,7 @@ hipError_t launch_kn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* x_hi, co
   pr.K = g.naux_p;
   const int side = cdiv(g.nbf, BM);
   pr.tiles_per_batch = side * (side + 1) / 2;
-  return launch_x3(pr, tc, pr.tiles_per_batch * g.nbf5, s);
+  return launch_x3(pr, tc, pr.tiles_per_batch * g.nbf5, s, col);
 }
 
 }  // namespace pnjk

@@ -1,0 +1,97 @@
+"""Grouped builds (GPU only): bitwise equal to building each instance alone, in
+fp64, split and split_refine forms, eagerly and from a replayed hipGraph."""
+import numpy as np
+import pytest
+
+from pynof_amd import JKEngine, synth
+from pynof_amd.batch import JKGroup
+
+pytestmark = pytest.mark.gpu
+SIZES = [(40, 144, 40), (97, 352, 60), (130, 472, 130), (64, 232, 50), (201, 728, 150)]
+OUTS = ("J", "K", "J_MO", "K_MO", "H_core")
+
+
+@pytest.fixture(scope="module")
+def setup(gpu_required):
+    import torch
+
+    engs, Cs, Hs, n5s = [], [], [], []
+    for i, (nbf, naux, n5) in enumerate(SIZES):
+        e = JKEngine(0)
+        e.upload_b(synth.make_b(nbf, naux, 900 + i, 0.05))
+        engs.append(e)
+        Cs.append(torch.from_numpy(synth.make_C(nbf, 950 + i)).cuda())
+        Hs.append(torch.from_numpy(synth.make_H(nbf, 980 + i)).cuda())
+        n5s.append(n5)
+    yield engs, Cs, Hs, n5s
+    for e in engs:
+        e.close()
+
+
+def alone(engs, Cs, Hs, n5s, prec):
+    import torch
+
+    res = []
+    for e, c, h, n5 in zip(engs, Cs, Hs, n5s):
+        r = e.build(c, n5, H=h, precision=prec)
+        torch.cuda.synchronize()
+        res.append({k: v.cpu().numpy() for k, v in r.items()})
+    return res
+
+
+@pytest.mark.parametrize("prec", ["fp64", "split", "split_refine"])
+def test_group_equals_alone(setup, prec):
+    import torch
+
+    engs, Cs, Hs, n5s = setup
+    print(f"case: group vs alone {prec} sizes {SIZES}", flush=True)
+    ref = alone(engs, Cs, Hs, n5s, prec)
+    g = JKGroup(engs, tuned={})
+    out = g.build(Cs, n5s, Hs, precision=prec)
+    torch.cuda.synchronize()
+    for i, (o, r) in enumerate(zip(out, ref)):
+        for k in OUTS:
+            assert np.array_equal(o[k].cpu().numpy(), r[k]), (prec, i, k)
+    g.close()
+
+
+def test_group_graph_replays(setup):
+    """Capture once, then replay with new C contents written in place; every
+    replay matches an eager build of the same C, with eager work in between."""
+    import torch
+
+    engs, Cs, Hs, n5s = setup
+    print("case: group graph replays split", flush=True)
+    Cdev = [c.clone() for c in Cs]
+    g = JKGroup(engs, tuned={})
+    outs = g.alloc_outputs(n5s)
+    for rep in range(3):
+        for i, c in enumerate(Cdev):  # rotate C in place: same addresses, new values
+            R = torch.from_numpy(synth.rotation(c.shape[0], 7000 + 10 * rep + i)).cuda()
+            c.copy_(Cs[i] @ R)
+        torch.cuda.synchronize()
+        g.build(Cdev, n5s, Hs, precision="split", outs=outs, use_graph=True)
+        torch.cuda.synchronize()
+        got = [{k: v.cpu().numpy() for k, v in o.items()} for o in outs]
+        ref = alone(engs, Cdev, Hs, n5s, "split")  # eager work between replays
+        for i in range(len(engs)):
+            for k in OUTS:
+                assert np.array_equal(got[i][k], ref[i][k]), (rep, i, k)
+    g.close()
+
+
+def test_group_rejects_host_outputs(setup):
+    import ctypes
+
+    from pynof_amd import _lib
+
+    engs, Cs, Hs, n5s = setup
+    g = JKGroup(engs[:1], tuned={})
+    J = np.empty((n5s[0], engs[0].nbf, engs[0].nbf))
+    arr = (ctypes.c_void_p * 1)(J.ctypes.data)
+    Carr = (ctypes.c_void_p * 1)(Cs[0].data_ptr())
+    i64 = ctypes.c_int64 * 1
+    rc = _lib.lib().pnjk_group_build(g._h, Carr, i64(Cs[0].shape[1]), i64(n5s[0]), None, None, 0, 1, arr, None,
+                                     None, None, None, 0, None)
+    assert rc == _lib.EINVAL
+    g.close()
