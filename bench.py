@@ -85,7 +85,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="large", choices=["water", "mid", "large"])
+    ap.add_argument("--workload", default="large", choices=["water", "mid", "large", "batch64", "serving"])
+    ap.add_argument("--requests", type=int, default=120, help="serving: requests per run")
     ap.add_argument("--form-steps", type=int, default=5, help="builds timed per single form")
     ap.add_argument("--cpu-sample-naux", type=int, default=72)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -94,6 +95,11 @@ def main():
         print("bench.py measures one GPU: every box here has one MI355X, so --gpus must be 1 "
               "(a multi-GPU build would shard the orbitals q over GPUs and all-gather d)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch64":
+        return bench_batch(args)
+    if args.workload == "serving":
+        return bench_serving(args)
 
     import torch
 
@@ -261,6 +267,130 @@ def main():
         line["cpu_baseline"] = cpu_baseline(w, min(args.cpu_sample_naux, w.naux))
     else:
         line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+def bench_batch(args):
+    """64 independent builds, nbf ~ U[40, 400] (seeded), naux = ceil8(3.6 nbf), nbf5 = nbf:
+    sequential single-instance builds against one grouped pass (eager, and replayed
+    from a captured hipGraph), FP64 and split forms."""
+    import torch
+
+    from pynof_amd import JKEngine, synth, workcount
+    from pynof_amd.batch import JKGroup
+
+    ws = synth.batch_workloads(64)
+    dev = torch.device("cuda", 0)
+    log(f"batch64: {len(ws)} instances, nbf {min(w.nbf for w in ws)}..{max(w.nbf for w in ws)}")
+    engs, Cs, Hs = [], [], []
+    for w in ws:
+        e = JKEngine(0)
+        bt = synth.device_b_packed(w.nbf, w.naux, w.seed_b, w.lam, JKEngine.packed_ld(w.nbf, w.naux))
+        e.upload_b_packed(bt, w.nbf, w.naux)
+        del bt
+        engs.append(e)
+        Cs.append(torch.from_numpy(synth.make_C(w.nbf, w.seed_c)).to(dev))
+        Hs.append(torch.from_numpy(synth.make_H(w.nbf, w.seed_h)).to(dev))
+    torch.cuda.empty_cache()
+    n5s = [w.nbf5 for w in ws]
+    flops = sum(workcount.build_flops(w.nbf, w.naux, w.nbf5) for w in ws)
+    grp = JKGroup(engs)
+    outs = grp.alloc_outputs(n5s)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    res = {}
+    for form in ("fp64", "split"):
+        def seq():
+            for e, c, h, o, n5 in zip(engs, Cs, Hs, outs, n5s):
+                e.build(c, n5, H=h, precision=form, out=o, stream=stream)
+
+        def grouped(graph):
+            grp.build(Cs, n5s, Hs, precision=form, outs=outs, use_graph=graph)
+
+        for name, fn in (("sequential", seq), ("grouped", lambda: grouped(False)),
+                         ("grouped_graph", lambda: grouped(True))):
+            for _ in range(max(1, args.warmup)):
+                fn()
+            torch.cuda.synchronize()
+            reps = 5
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / reps
+            res[f"{form}_{name}"] = {"ms_per_pass": round(1e3 * dt, 3), "builds_per_s": round(len(ws) / dt, 2),
+                                     "useful_tflops": round(flops / dt / 1e12, 2)}
+            log(f"{form} {name}: {1e3 * dt:.2f} ms per 64 builds")
+    line = {
+        "metric": "batched JK builds/s, 64 uneven instances in one grouped persistent pass",
+        "value": res["split_grouped_graph"]["builds_per_s"],
+        "unit": "builds/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "bf16x3 (fp32 accumulate)",
+        "data": "synthetic: device-drawn b, seeded C/H; nbf ~ U[40,400] (seed 4000)",
+        "config": {"workload": "batch64 nbf~U[40,400] naux=ceil8(3.6 nbf) nbf5=nbf",
+                   "sizes": [w.nbf for w in ws], "useful_flops_per_pass": flops},
+        "results": res,
+        "grouped_graph_speedup_vs_sequential": {
+            f: round(res[f"{f}_sequential"]["ms_per_pass"] / res[f"{f}_grouped_graph"]["ms_per_pass"], 3)
+            for f in ("fp64", "split")},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_serving(args):
+    """Continuous-batching serving loop: saturation throughput, then p50/p99
+    latency under Poisson arrivals at 50%, 80% and 100% of it."""
+    import torch
+
+    from pynof_amd.serving import DeviceExecutor, Server, poisson_requests
+
+    n = args.requests
+
+    def run(rate):
+        rs = poisson_requests(n, rate=rate, seed=5000)
+        srv = Server(DeviceExecutor(0, with_jk=True), hbm_budget=200e9, max_active=64)
+        for r in rs:
+            srv.submit(r)
+        t0 = time.perf_counter()
+        srv.run(timeout=600)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        st["wall_s"] = round(wall, 3)
+        st["requests_per_s"] = round(st["finished"] / wall, 3)
+        st["builds_per_s"] = round(20 * st["finished"] / wall, 2)
+        if "p50_s" in st:
+            st["p99_over_p50"] = round(st["p99_s"] / st["p50_s"], 3)
+        torch.cuda.empty_cache()
+        return st
+
+    log("serving: warmup (8 requests)")
+    small = poisson_requests(8, rate=0.0, seed=77)
+    srv = Server(DeviceExecutor(0), hbm_budget=200e9)
+    for r in small:
+        srv.submit(r)
+    srv.run(timeout=300)
+    log(f"serving: saturation ({n} requests at t=0)")
+    sat = run(0.0)
+    sat_rate = sat["requests_per_s"]
+    loads = {}
+    for frac in (0.5, 0.8, 1.0):
+        log(f"serving: Poisson arrivals at {frac:.0%} of saturation ({frac * sat_rate:.2f} req/s)")
+        loads[f"{int(frac * 100)}%"] = run(frac * sat_rate)
+    line = {
+        "metric": "serving: orbital-optimisation requests (20 JK builds each), throughput and p50/p99 latency",
+        "value": sat_rate,
+        "unit": "requests/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "bf16x3 (fp32 accumulate) + f64 refinement for 1e-9 Eh requests",
+        "data": "synthetic: Poisson arrivals (seed 5000), nbf lognormal median 150 in [24,800], naux=ceil8(3.6 nbf)",
+        "config": {"workload": "serving", "requests": n, "builds_per_request": 20, "hbm_budget_gb": 200,
+                   "max_active": 64},
+        "saturation": sat,
+        "loads": loads,
+        "p99_under_2x_p50_at_80pct": bool(loads["80%"].get("p99_over_p50", 99) < 2.0),
+    }
     print(json.dumps(line), flush=True)
 
 

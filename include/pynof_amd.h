@@ -83,9 +83,10 @@ int pnjk_upload_b_packed(pnjk_engine* e, const double* btri_dev, int64_t nbf, in
  *   J_MO     (nbf5, nbf5)                          integrals.py:356
  *   K_MO     (nbf5, nbf5)                          integrals.py:358
  *   H_core   (nbf5)                                integrals.py:360
- * Output pointers may be host or device memory. `stream` is a hipStream_t (NULL:
- * the engine's own stream). The call returns after the outputs are complete
- * when any output is host memory; otherwise it is asynchronous on `stream`. */
+ * Output pointers may be host or device memory. `stream` is a hipStream_t
+ * (NULL: the null stream, which is also torch's default stream). The call
+ * returns after the outputs are complete when any output is host memory;
+ * otherwise it is asynchronous on `stream`. */
 int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, const double* H, int64_t ldh,
                   int precision, uint32_t outputs, double* J, double* K, double* J_MO, double* K_MO,
                   double* H_core, void* stream);

@@ -116,3 +116,47 @@ def apply(eng, arith: str, entry) -> None:
     for st, idx in STAGES.items():
         fm, fn = (entry or {}).get(st, (0, 0))
         _lib.check(L.pnjk_set_tile(eng._h, ARITH[arith], idx, int(fm), int(fn)))
+
+
+def tune_sizes(sizes, out_path: str, reps: int = 2, log=print) -> dict:
+    """Tunes both arithmetics at each (nbf, naux, nbf5) with device-drawn synthetic
+    b and seeded C, H; writes the table to out_path and returns it."""
+    import torch
+
+    from .engine import JKEngine
+    from .synth import device_b_packed, make_C, make_H
+
+    table = {}
+    for nbf, naux, nbf5 in sizes:
+        eng = JKEngine(0)
+        bt = device_b_packed(nbf, naux, 4242, 0.05, JKEngine.packed_ld(nbf, naux))
+        eng.upload_b_packed(bt, nbf, naux)
+        del bt
+        C = torch.from_numpy(make_C(nbf, 4243)).cuda()
+        H = torch.from_numpy(make_H(nbf, 4244)).cuda()
+        for arith, form in (("split", "split"), ("fp64", "fp64")):
+            out = None
+
+            def build():
+                nonlocal out
+                out = eng.build(C, nbf5, H=H, precision=form, out=out)
+
+            build()
+            table[key(arith, nbf, naux, nbf5)] = tune(eng, build, arith, reps=reps,
+                                                      log=lambda m: log(f"  {arith} {nbf}/{naux}/{nbf5} {m}"))
+            out = None
+        eng.close()
+        torch.cuda.empty_cache()
+        save(table, out_path)
+    return table
+
+
+if __name__ == "__main__":
+    import sys
+
+    from .synth import ceil8
+
+    path = sys.argv[1] if len(sys.argv) > 1 else DEFAULT_TABLE
+    grid = [(n, ceil8(3.6 * n), n) for n in (48, 96, 160, 240, 320, 400, 560, 800)]
+    grid += [(280, 1100, 280), (1000, 3600, 300)]
+    tune_sizes(grid, path, log=lambda m: print(m, flush=True))

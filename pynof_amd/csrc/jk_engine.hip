@@ -579,7 +579,7 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
   if (!e) return set_error(PNJK_EINVAL, "pnjk_jk_build: engine is NULL");
   std::lock_guard<std::mutex> lk(e->mu);
   PNJK_HIP_CHECK(hipSetDevice(e->device));
-  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : e->own;
+  hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
   auto mark = [&](int st) {
     if (e->prof) (void)hipEventRecord(e->ev[st], s);
   };
@@ -728,7 +728,7 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
   locks.reserve(n);
   for (pnjk_engine* e : g->eng) locks.emplace_back(e->mu);
   PNJK_HIP_CHECK(hipSetDevice(g->device));
-  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->eng[0]->own;
+  hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
   auto at = [](double* const* a, size_t i) { return a ? a[i] : nullptr; };
   int rc;
   g->st.resize(n);

@@ -174,7 +174,18 @@ __global__ void __launch_bounds__(kThreads)
   constexpr int BM = 32 * FM, BN = 32 * FN;
   __shared__ __align__(16) double sA[2 * BK * (BM + kPad)];
   __shared__ __align__(16) double sB[2 * BK * (BN + kPad)];
-  for (int id = blockIdx.x; id < ntiles; id += gridDim.x) {
+  // Workgroup b runs on XCD b % 8: give each XCD one contiguous run of the
+  // table, so tiles that share an operand (adjacent ids) stay in one L2.
+  // (With fewer than 8 workgroups, every workgroup strides the whole table.)
+  constexpr int kXcd = 8;
+  const bool split = gridDim.x >= kXcd;
+  const int x = blockIdx.x % kXcd;
+  const int wi = split ? blockIdx.x / kXcd : blockIdx.x;
+  const int nw = split ? (gridDim.x - x + kXcd - 1) / kXcd : gridDim.x;  // workgroups sharing the range
+  const int per = ntiles / kXcd, rem = ntiles % kXcd;
+  const int lo = split ? x * per + (x < rem ? x : rem) : 0;
+  const int hi = split ? lo + per + (x < rem ? 1 : 0) : ntiles;
+  for (int id = lo + wi; id < hi; id += nw) {
     const int2 tr = tiles[id];
     const Prob prob = probs[tr.x];
     Tile t;

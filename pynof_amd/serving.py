@@ -1,0 +1,274 @@
+"""Continuous-batching serving loop for orbital-optimisation requests.
+
+A request is one orbital optimisation as PyNOF's ADAM loop runs it
+(minimization.py:200-253): a fixed number of JK builds (20 here) with the
+orbitals rotated between builds, C <- C exp(kappa) (utils.py:550-565). The
+server:
+
+* admits requests first-come-first-served while the device-memory budget
+  holds their b tensor, its split planes and their workspace (footprint from
+  the C ABI's pnjk_workspace_bytes); a request that could never fit is
+  rejected, one that does not fit now waits until memory frees up;
+* keeps each admitted request's b, C, H and occupations resident on the device
+  until it completes;
+* runs continuous batching: every scheduler step advances every active request
+  by one build, all of them together in grouped launches (one per form), and
+  requests that arrive while a step runs join at the next step;
+* picks each request's form from its energy tolerance: tol >= 1e-6 Eh runs
+  every build split; a tighter tolerance runs the final build, whose energy
+  the request returns, as split_refine (FP64 MO path).
+
+The scheduler is independent of the device: `Executor` is the interface, with
+`DeviceExecutor` driving pynof_amd on the GPU.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+from collections import deque
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    nbf: int
+    naux: int
+    nbf5: int
+    ndoc: int
+    tol: float = 1e-9
+    n_builds: int = 20
+    seed: int = 0
+    arrival: float = 0.0
+    # filled by the server
+    admitted: float | None = None
+    done: float | None = None
+    builds_done: int = 0
+    energies: list = dataclasses.field(default_factory=list)
+    error: str | None = None
+    footprint: int = 0
+
+    def form_for_build(self, i: int) -> str:
+        if self.tol < 1e-6 and i == self.n_builds - 1:
+            return "split_refine"
+        return "split"
+
+    @property
+    def latency(self):
+        return None if self.done is None else self.done - self.arrival
+
+
+class Executor:
+    """Device side of the server."""
+
+    def footprint(self, req: Request) -> int:
+        raise NotImplementedError
+
+    def admit(self, req: Request) -> None:
+        raise NotImplementedError
+
+    def step(self, reqs: list[Request]) -> None:
+        """Runs build number req.builds_done of every request in `reqs` and appends
+        each request's energy to req.energies."""
+        raise NotImplementedError
+
+    def release(self, req: Request) -> None:
+        raise NotImplementedError
+
+
+class Server:
+    def __init__(self, executor: Executor, hbm_budget: float = 200e9, max_active: int = 64, clock=time.perf_counter,
+                 sleep=time.sleep):
+        self.ex = executor
+        self.budget = float(hbm_budget)
+        self.max_active = int(max_active)
+        self.clock = clock
+        self.sleep = sleep
+        self.pending: deque[Request] = deque()
+        self.active: list[Request] = []
+        self.finished: list[Request] = []
+        self.rejected: list[Request] = []
+        self.in_use = 0
+        self.steps = 0
+        self.step_sizes: list[int] = []
+
+    def submit(self, req: Request) -> None:
+        req.footprint = int(self.ex.footprint(req))
+        if req.footprint > self.budget:
+            req.error = f"footprint {req.footprint} B exceeds the device budget {int(self.budget)} B"
+            self.rejected.append(req)
+            return
+        self.pending.append(req)
+
+    def _admit(self, now: float) -> None:
+        while self.pending and self.pending[0].arrival <= now and len(self.active) < self.max_active:
+            r = self.pending[0]
+            if self.in_use + r.footprint > self.budget:
+                break  # first-come-first-served: later requests wait behind it
+            self.pending.popleft()
+            try:
+                self.ex.admit(r)
+            except Exception as exc:  # allocation or upload failure: fail this request only
+                r.error = f"admission failed: {exc}"
+                self.rejected.append(r)
+                continue
+            r.admitted = self.clock()
+            self.in_use += r.footprint
+            self.active.append(r)
+
+    def run(self, t0: float | None = None, timeout: float = 1e9) -> None:
+        """Serves until every submitted request is done or rejected. Arrival times
+        are relative to t0 (default: now)."""
+        base = self.clock() if t0 is None else t0
+        for r in self.pending:
+            r.arrival += base
+        end = base + timeout
+        while self.pending or self.active:
+            now = self.clock()
+            if now > end:
+                raise TimeoutError("serving loop exceeded its time limit")
+            self._admit(now)
+            if not self.active:
+                if self.pending:
+                    self.sleep(max(0.0, min(self.pending[0].arrival - now, 0.05)))
+                continue
+            batch = list(self.active)
+            self.ex.step(batch)
+            self.steps += 1
+            self.step_sizes.append(len(batch))
+            t = self.clock()
+            for r in batch:
+                r.builds_done += 1
+                if r.builds_done == r.n_builds:
+                    r.done = t
+                    self.active.remove(r)
+                    self.ex.release(r)
+                    self.in_use -= r.footprint
+                    self.finished.append(r)
+
+    def stats(self) -> dict:
+        lat = np.array([r.latency for r in self.finished]) if self.finished else np.zeros(0)
+        out = {"finished": len(self.finished), "rejected": len(self.rejected), "steps": self.steps,
+               "mean_batch": float(np.mean(self.step_sizes)) if self.step_sizes else 0.0}
+        if lat.size:
+            out.update(p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                       mean_s=float(lat.mean()))
+        return out
+
+
+def poisson_requests(count: int, rate: float, seed: int = 5000, median_nbf: int = 150, lo: int = 24, hi: int = 800,
+                     sigma: float = 0.6, tol_tight_fraction: float = 0.5, n_builds: int = 20) -> list[Request]:
+    """Requests with Poisson arrivals (rate per second) and lognormal nbf (median
+    `median_nbf`, clipped to [lo, hi]); naux = 3.6 nbf rounded up to a multiple of
+    8, nbf5 = nbf; half the requests ask for 1e-9 Eh, half for 1e-6 Eh."""
+    from .synth import ceil8, default_ndoc
+
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        nbf = int(np.clip(round(median_nbf * math.exp(sigma * rng.standard_normal())), lo, hi))
+        tol = 1e-9 if rng.random() < tol_tight_fraction else 1e-6
+        out.append(Request(rid=i, nbf=nbf, naux=ceil8(3.6 * nbf), nbf5=nbf, ndoc=default_ndoc(nbf), tol=tol,
+                           n_builds=n_builds, seed=6000 + i, arrival=t))
+    return out
+
+
+class DeviceExecutor(Executor):
+    """Runs requests on one GPU: one JKEngine per request (its b resident), grouped
+    builds per step. C is rotated on the device by exp(kappa_i) with a seeded
+    antisymmetric kappa_i, ||kappa_i||_F = 1e-2."""
+
+    def __init__(self, device: int = 0, with_jk: bool = True, keep_final_C: bool = False):
+        import torch
+
+        self.torch = torch
+        self.dev = torch.device("cuda", device)
+        self.device = device
+        self.with_jk = with_jk
+        self.keep_final_C = keep_final_C
+        self.state: dict[int, dict] = {}
+        self._groups: dict = {}
+
+    def footprint(self, req: Request) -> int:
+        from . import _lib
+
+        L = _lib.lib()
+        ld = L.pnjk_packed_ld(req.nbf, req.naux)
+        ntri = req.nbf * (req.nbf + 1) // 2
+        b_bytes = ntri * ld * 8
+        ws = L.pnjk_workspace_bytes(req.nbf, req.naux, req.nbf5, 2)  # split_refine needs the most
+        outs = (2 * req.nbf5 * req.nbf * req.nbf + 2 * req.nbf5 * req.nbf5 + req.nbf5) * 8
+        return int(b_bytes + ws + outs + 3 * req.nbf * req.nbf * 8)
+
+    def admit(self, req: Request) -> None:
+        from .engine import JKEngine
+        from .synth import device_b_packed, make_H, make_n
+
+        torch = self.torch
+        eng = JKEngine(self.device)
+        btri = device_b_packed(req.nbf, req.naux, req.seed, 0.05, JKEngine.packed_ld(req.nbf, req.naux), self.dev)
+        eng.upload_b_packed(btri, req.nbf, req.naux)
+        del btri
+        g = torch.Generator(device=self.dev)
+        g.manual_seed(req.seed)
+        C = torch.linalg.qr(torch.randn((req.nbf, req.nbf), dtype=torch.float64, device=self.dev, generator=g))[0]
+        H = torch.from_numpy(make_H(req.nbf, req.seed + 1)).to(self.dev)
+        n = make_n(req.nbf5, req.ndoc, req.seed + 2)
+        n5, nbf = req.nbf5, req.nbf
+        outs = {k: torch.empty(s, dtype=torch.float64, device=self.dev) for k, s in
+                (("J_MO", (n5, n5)), ("K_MO", (n5, n5)), ("H_core", (n5,)))}
+        if self.with_jk:
+            outs["J"] = torch.empty((n5, nbf, nbf), dtype=torch.float64, device=self.dev)
+            outs["K"] = torch.empty((n5, nbf, nbf), dtype=torch.float64, device=self.dev)
+        self.state[req.rid] = {"eng": eng, "C": C, "H": H, "n": torch.from_numpy(n).to(self.dev), "outs": outs,
+                               "gen": g}
+
+    def _group(self, reqs):
+        from .batch import JKGroup
+
+        key = tuple(r.rid for r in reqs)
+        grp = self._groups.get(key)
+        if grp is None:
+            for k in list(self._groups):
+                if not set(k) <= set(self.state):
+                    self._groups.pop(k).close()
+            grp = JKGroup([self.state[r.rid]["eng"] for r in reqs])
+            self._groups[key] = grp
+        return grp
+
+    def step(self, reqs):
+        torch = self.torch
+        by_form: dict[str, list] = {}
+        for r in reqs:
+            by_form.setdefault(r.form_for_build(r.builds_done), []).append(r)
+        outputs = ("J", "K", "J_MO", "K_MO", "H_core") if self.with_jk else ("J_MO", "K_MO", "H_core")
+        for form, rs in by_form.items():
+            st = [self.state[r.rid] for r in rs]
+            self._group(rs).build([s["C"] for s in st], [r.nbf5 for r in rs], [s["H"] for s in st],
+                                  precision=form, outputs=outputs, outs=[s["outs"] for s in st])
+        for r in reqs:  # energies (FP64, on the device), then the orbital rotation for the next build
+            s = self.state[r.rid]
+            o, n = s["outs"], s["n"]
+            e = 2.0 * torch.dot(n, o["H_core"]) + n @ (2.0 * o["J_MO"] - o["K_MO"]) @ n
+            r.energies.append(e)
+            if r.builds_done + 1 < r.n_builds:
+                a = torch.randn((r.nbf, r.nbf), dtype=torch.float64, device=self.dev, generator=s["gen"])
+                k = a - a.T
+                k *= 1e-2 / torch.linalg.norm(k)
+                s["C"] = s["C"] @ torch.linalg.matrix_exp(k)
+        torch.cuda.current_stream(self.dev).synchronize()
+
+    def release(self, req: Request) -> None:
+        s = self.state.pop(req.rid)
+        req.energies = [float(e) for e in req.energies]
+        if self.keep_final_C:
+            req.final_C = s["C"].cpu().numpy()
+            req.final_outs = {k: v.cpu().numpy() for k, v in s["outs"].items()}
+        for k in list(self._groups):
+            if req.rid in k:
+                self._groups.pop(k).close()
+        s["eng"].close()

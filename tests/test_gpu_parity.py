@@ -186,3 +186,22 @@ def test_large_workload_properties(gpu_required):
     e.close()
     del r
     torch.cuda.empty_cache()
+
+
+def test_device_outputs_ordered_on_current_stream(eng):
+    """A build with device outputs is ordered on torch's current stream: torch ops
+    issued right after it (no explicit sync) read the complete outputs."""
+    import torch
+
+    print("case: stream ordering nbf=120 naux=400 nbf5=120", flush=True)
+    nbf, naux, nbf5 = 120, 400, 120
+    b = synth.make_b(nbf, naux, 31, 0.05)
+    C = synth.make_C(nbf, 32)
+    eng.upload_b(b)
+    ref = eng.build(C, nbf5, outputs=("J_MO", "K_MO"))
+    Cd = torch.from_numpy(C).cuda()
+    want = float((2.0 * torch.from_numpy(ref["J_MO"]).cuda() - torch.from_numpy(ref["K_MO"]).cuda()).sum())
+    for _ in range(3):
+        out = eng.build(Cd, nbf5, outputs=("J", "K", "J_MO", "K_MO"))
+        s = float((2.0 * out["J_MO"] - out["K_MO"]).sum())  # no synchronize before this
+        assert s == want

@@ -1,0 +1,136 @@
+"""Serving-loop scheduler on the CPU (simulated clock, fake executor): admission by
+device-memory budget, FIFO order, rejection of requests that can never fit,
+failure of one admission not affecting others, continuous batching, and the
+per-request choice of form."""
+import pytest
+
+from pynof_amd.serving import Executor, Request, Server, poisson_requests
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+    def sleep(self, dt):
+        self.t += max(dt, 1e-3)
+
+
+class FakeExecutor(Executor):
+    def __init__(self, clock, cost_per_build=0.01, fail_rids=()):
+        self.clock = clock
+        self.cost = cost_per_build
+        self.fail = set(fail_rids)
+        self.admitted, self.released, self.forms = [], [], []
+        self.live = set()
+
+    def footprint(self, req):
+        return req.nbf * 1_000_000
+
+    def admit(self, req):
+        if req.rid in self.fail:
+            raise MemoryError("injected allocation failure")
+        self.admitted.append(req.rid)
+        self.live.add(req.rid)
+
+    def step(self, reqs):
+        assert all(r.rid in self.live for r in reqs)
+        self.forms.append({r.rid: r.form_for_build(r.builds_done) for r in reqs})
+        for r in reqs:
+            r.energies.append(float(r.rid))
+        self.clock.t += self.cost * len(reqs)
+
+    def release(self, req):
+        self.live.discard(req.rid)
+        self.released.append(req.rid)
+
+
+def make(rid, nbf, arrival=0.0, tol=1e-9, n=5):
+    return Request(rid=rid, nbf=nbf, naux=8, nbf5=nbf, ndoc=1, tol=tol, n_builds=n, arrival=arrival)
+
+
+def test_budget_and_fifo():
+    clk = Clock()
+    ex = FakeExecutor(clk)
+    srv = Server(ex, hbm_budget=250e6, clock=clk, sleep=clk.sleep)
+    for i in range(4):
+        srv.submit(make(i, 100))
+    peak = []
+    orig = ex.step
+
+    def step(reqs):
+        peak.append(len(reqs))
+        orig(reqs)
+
+    ex.step = step
+    srv.run(t0=0.0)
+    assert [r.rid for r in srv.finished] == [0, 1, 2, 3]
+    assert ex.admitted == [0, 1, 2, 3]
+    assert max(peak) == 2  # 2 x 100 MB fit the 250 MB budget, a third does not
+    assert srv.in_use == 0 and not ex.live
+
+
+def test_oversized_rejected_others_served():
+    clk = Clock()
+    ex = FakeExecutor(clk)
+    srv = Server(ex, hbm_budget=150e6, clock=clk, sleep=clk.sleep)
+    srv.submit(make(0, 100))
+    srv.submit(make(1, 500))
+    srv.submit(make(2, 100))
+    srv.run(t0=0.0)
+    assert [r.rid for r in srv.rejected] == [1] and "exceeds" in srv.rejected[0].error
+    assert sorted(r.rid for r in srv.finished) == [0, 2]
+
+
+def test_admission_failure_isolated():
+    clk = Clock()
+    ex = FakeExecutor(clk, fail_rids=(1,))
+    srv = Server(ex, hbm_budget=1e9, clock=clk, sleep=clk.sleep)
+    for i in range(3):
+        srv.submit(make(i, 50))
+    srv.run(t0=0.0)
+    assert [r.rid for r in srv.rejected] == [1] and "injected" in srv.rejected[0].error
+    assert sorted(r.rid for r in srv.finished) == [0, 2]
+    assert srv.in_use == 0
+
+
+def test_continuous_batching_joins_mid_flight():
+    clk = Clock()
+    ex = FakeExecutor(clk, cost_per_build=0.01)
+    srv = Server(ex, hbm_budget=1e9, clock=clk, sleep=clk.sleep)
+    srv.submit(make(0, 50, arrival=0.0, n=10))
+    srv.submit(make(1, 50, arrival=0.035, n=4))
+    srv.run(t0=0.0)
+    assert srv.step_sizes[0] == 1 and 2 in srv.step_sizes
+    first_joint = srv.step_sizes.index(2)
+    assert 3 <= first_joint <= 5  # joined at the first step after its arrival
+    r1 = next(r for r in srv.finished if r.rid == 1)
+    assert r1.done < next(r for r in srv.finished if r.rid == 0).done  # finished while 0 was still running
+
+
+def test_form_per_tolerance():
+    assert [make(0, 10, tol=1e-9, n=3).form_for_build(i) for i in range(3)] == ["split", "split", "split_refine"]
+    assert [make(0, 10, tol=1e-6, n=3).form_for_build(i) for i in range(3)] == ["split"] * 3
+
+
+def test_timeout():
+    clk = Clock()
+    ex = FakeExecutor(clk, cost_per_build=1.0)
+    srv = Server(ex, hbm_budget=1e9, clock=clk, sleep=clk.sleep)
+    srv.submit(make(0, 10, n=100))
+    with pytest.raises(TimeoutError):
+        srv.run(t0=0.0, timeout=5.0)
+
+
+def test_poisson_requests_shape():
+    rs = poisson_requests(200, rate=10.0, seed=5000)
+    assert all(24 <= r.nbf <= 800 for r in rs)
+    assert all(r.naux % 8 == 0 and r.naux >= 3.6 * r.nbf for r in rs)
+    arr = [r.arrival for r in rs]
+    assert arr == sorted(arr)
+    assert rs[0].nbf == poisson_requests(1, rate=10.0, seed=5000)[0].nbf
+    med = sorted(r.nbf for r in rs)[100]
+    assert 100 <= med <= 220
+    assert {r.tol for r in rs} == {1e-6, 1e-9}
