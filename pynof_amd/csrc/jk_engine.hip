@@ -755,7 +755,15 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
   if (sig != g->sig) {
     // New plan: collect every instance's tiles per grouped stage, upload once.
     for (GroupCollector& c : g->plan) c.batches.clear();
-    for (size_t i = 0; i < n; ++i) {
+    // Largest instances first: the persistent walk then deals big tiles before
+    // small ones (longest-first balance); an instance's tiles stay adjacent.
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+      const BuildGeom &gx = g->st[x].g, &gy = g->st[y].g;
+      return (double)gx.nbf * gx.nbf * gx.naux > (double)gy.nbf * gy.nbf * gy.naux;
+    });
+    for (size_t i : order) {
       pnjk_engine* e = g->eng[i];
       BuildState& st = g->st[i];
       if ((rc = stage_ht(e, st, s, &g->plan[GS_HT])) || (rc = stage_qt(e, st, s, &g->plan[GS_QT])) ||

@@ -118,10 +118,14 @@ __device__ __forceinline__ void gemm_tile_f64(const Prob& prob, const Tile& t, d
   __syncthreads();
 
   v4d acc[FM][FN];
+  uint32_t active = 0;  // 16x16 blocks whose outputs are needed (wave-uniform)
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < FN; ++j) {
+      acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+      if (prob.block_active(t, t.m0 + wm + 16 * i, t.n0 + wn + 16 * j)) active |= 1u << (i * FN + j);
+    }
 
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
@@ -143,7 +147,9 @@ __device__ __forceinline__ void gemm_tile_f64(const Prob& prob, const Tile& t, d
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < FN; ++j)
+          if (active & (1u << (i * FN + j)))
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     if (more) {
       la.store(sA + (cur ^ 1) * SA);
@@ -186,8 +192,12 @@ __global__ void __launch_bounds__(kThreads)
   const int lo = split ? x * per + (x < rem ? x : rem) : 0;
   const int hi = split ? lo + per + (x < rem ? 1 : 0) : ntiles;
   for (int id = lo + wi; id < hi; id += nw) {
-    const int2 tr = tiles[id];
-    const Prob prob = probs[tr.x];
+    // The entry is uniform over the workgroup: readfirstlane lets the problem
+    // struct come in through scalar loads.
+    const int2 e = tiles[id];
+    const int slot = __builtin_amdgcn_readfirstlane(e.x), local = __builtin_amdgcn_readfirstlane(e.y);
+    const Prob prob = probs[slot];
+    const int2 tr = make_int2(slot, local);
     Tile t;
     if (prob.tile(tr.y, BM, BN, t)) gemm_tile_f64<FM, FN, BK>(prob, t, sA, sB);
     __syncthreads();
@@ -260,6 +270,7 @@ struct ProbHT {
     t.n0 = t.nt * BN;
     return true;
   }
+  __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf5; }
   __device__ int ksteps(int BK) const { return nbf_p / BK; }
   __device__ SrcKMDense a_src(const Tile&) const { return {Cp, ldcp}; }
   __device__ SrcKMBtri b_src(const Tile&) const { return {btri, nbf, naux_p}; }
@@ -304,6 +315,8 @@ struct ProbQT {
     t.n0 = t.nt * BN;
     return t.m0 <= t.batch;  // only q <= p is needed (Y_p[q] = Y_q[p])
   }
+  // only q <= p is needed (Y_p[q] = Y_q[p])
+  __device__ bool block_active(const Tile& t, int r0, int) const { return r0 <= t.batch && r0 < nbf5; }
   __device__ int ksteps(int BK) const { return nbf_p / BK; }
   __device__ SrcKMDense a_src(const Tile&) const { return {Cp, ldcp}; }
   __device__ SrcKMRows b_src(const Tile& t) const {
@@ -379,6 +392,10 @@ struct ProbSYRK {
     t.n0 = t.nt * BN;
     return true;
   }
+  // blocks strictly above the diagonal of a diagonal tile are filled by symmetrize
+  __device__ bool block_active(const Tile& t, int r0, int c0) const {
+    return r0 < M && c0 < M && !(t.mt == t.nt && r0 + 16 <= c0);
+  }
   __device__ int ksteps(int BK) const { return K / BK; }
   __device__ SrcMKDense a_src(const Tile& t) const { return {A + (int64_t)t.batch * sa, lda, M}; }
   __device__ SrcMKDense b_src(const Tile& t) const { return {A + (int64_t)t.batch * sa, lda, M}; }
@@ -414,6 +431,7 @@ struct ProbJN {
     t.n0 = t.nt * BN;
     return true;
   }
+  __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf5; }
   __device__ int ksteps(int BK) const { return naux_p / BK; }
   __device__ SrcMKDense a_src(const Tile&) const { return {d, naux_p, nbf5}; }
   __device__ SrcMKDense b_src(const Tile&) const { return {btri, naux_p, (int)(ntri < (1 << 30) ? ntri : (1 << 30))}; }
@@ -454,6 +472,7 @@ struct ProbGemmT {  // T[mu][i] = sum_nu Hp[mu][nu] Cp[nu][i]
     t.n0 = t.nt * BN;
     return true;
   }
+  __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf; }
   __device__ int ksteps(int BK) const { return nbf_p / BK; }
   __device__ SrcMKDense a_src(const Tile&) const { return {Hp, ldh, nbf}; }
   __device__ SrcKMDense b_src(const Tile&) const { return {Cp, ldcp}; }

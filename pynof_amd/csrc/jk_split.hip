@@ -208,8 +208,12 @@ __global__ void __launch_bounds__(kThreads)
   const int lo = split ? x * per + (x < rem ? x : rem) : 0;
   const int hi = split ? lo + per + (x < rem ? 1 : 0) : ntiles;
   for (int id = lo + wi; id < hi; id += nw) {
-    const int2 tr = tiles[id];
-    const Prob prob = probs[tr.x];
+    // The entry is uniform over the workgroup: readfirstlane lets the problem
+    // struct come in through scalar loads.
+    const int2 e = tiles[id];
+    const int slot = __builtin_amdgcn_readfirstlane(e.x), local = __builtin_amdgcn_readfirstlane(e.y);
+    const Prob prob = probs[slot];
+    const int2 tr = make_int2(slot, local);
     Tile t;
     if (prob.tile(tr.y, BM, BN, t)) gemm_tile_x3<FM, FN>(prob, t, sA, sB);
     __syncthreads();
