@@ -93,14 +93,14 @@ TileF64 default_tile(int arith, int stage_kind, int nbf, int nbf5) {
     switch (stage_kind) {
       case 0: return big ? TileF64{5, 4} : TileF64{2, 2};
       case 1: return TileF64{2, 2};
-      case 2: return big ? TileF64{2, 4} : TileF64{2, 2};
-      default: return TileF64{2, 2};
+      case 2: return big ? TileF64{5, 4} : TileF64{2, 2};
+      default: return big ? TileF64{4, 4} : TileF64{2, 2};
     }
   }
   switch (stage_kind) {
-    case 0: return big ? TileF64{4, 4} : TileF64{2, 2};
-    case 1: return big ? TileF64{5, 2} : TileF64{2, 2};
-    case 2: return big ? TileF64{4, 4} : TileF64{2, 2};
+    case 0: return big ? TileF64{5, 4} : TileF64{2, 2};
+    case 1: return big ? TileF64{4, 2} : TileF64{2, 2};
+    case 2: return big ? TileF64{5, 4} : TileF64{2, 2};
     default: return nbf >= 256 ? TileF64{4, 4} : TileF64{2, 2};
   }
 }

@@ -24,7 +24,13 @@ typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kThreads = 256;
 constexpr int kBK = 32;
-constexpr int kLDK = kBK + 8;  // LDS row stride (elements): conflict-free 16-byte fragment reads
+constexpr int kLDK = kBK;  // LDS row: 32 bf16 = 64 bytes = four 16-byte chunks, unpadded
+
+// LDS placement of 16-byte chunk `c` (0..3) of tile row `r`: XOR-swizzled so
+// that 16 lanes reading the same logical chunk of 16 consecutive rows (one
+// MFMA fragment read) and 4 lanes storing the 4 chunks of each of 4 rows (the
+// loaders) touch distinct banks, for 32 and for 64 4-byte banks alike.
+__device__ __forceinline__ int swz(int r, int c) { return r * kLDK + 8 * (c ^ (((r >> 1) ^ (r >> 2)) & 3)); }
 
 // Output row of accumulator register r of `lane` inside a 16x16 f32 MFMA block.
 __device__ __forceinline__ int out_row(int lane, int r) { return 4 * (lane >> 4) + r; }
@@ -65,9 +71,9 @@ struct Ld<RK, ROWS> {
     for (int i = 0; i < kPer; ++i) {
       const int c = threadIdx.x + i * kThreads;
       if (kChunks % kThreads == 0 || c < kChunks) {
-        const int r = c >> 2, kc = (c & 3) * 8;
-        *reinterpret_cast<u16x8*>(sh + r * kLDK + kc) = h[i];
-        *reinterpret_cast<u16x8*>(sl + r * kLDK + kc) = l[i];
+        const int r = c >> 2;
+        *reinterpret_cast<u16x8*>(sh + swz(r, c & 3)) = h[i];
+        *reinterpret_cast<u16x8*>(sl + swz(r, c & 3)) = l[i];
       }
     }
   }
@@ -101,8 +107,9 @@ struct Ld<KR, ROWS> {
         const int k = c % kBK, rc = (c / kBK) * 8;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-          sh[(rc + e) * kLDK + k] = h[i][e];
-          sl[(rc + e) * kLDK + k] = l[i][e];
+          const int o = swz(rc + e, k >> 3) + (k & 7);
+          sh[o] = h[i][e];
+          sl[o] = l[i][e];
         }
       }
     }
@@ -151,13 +158,13 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
     bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
-      const int o = (wm + 16 * i + frow) * kLDK + fk;
+      const int o = swz(wm + 16 * i + frow, fk >> 3);
       ah[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(ah_s + o));
       al[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(al_s + o));
     }
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
-      const int o = (wn + 16 * j + frow) * kLDK + fk;
+      const int o = swz(wn + 16 * j + frow, fk >> 3);
       bh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(bh_s + o));
       bl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(bl_s + o));
     }
