@@ -467,10 +467,6 @@ int stage_ht(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col)
     PNJK_HIP_CHECK(launch_ht_x3(st.sg, st.ht, e->ct16h.as<uint16_t>(), e->ct16l.as<uint16_t>(),
                                 e->bht16h.as<uint16_t>(), e->bht16l.as<uint16_t>(), e->x16h.as<uint16_t>(),
                                 e->x16l.as<uint16_t>(), s, col));
-  } else if (prec == PNJK_SPLIT_REFINE && st.want_k) {
-    // FP64 X for the MO path, with its split planes for K^n written by the same epilogue.
-    PNJK_HIP_CHECK(launch_ht_f64(st.g, st.ht, e->Cp.as<double>(), e->btri.as<double>(), e->X.as<double>(), s,
-                                 e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), col));
   } else {
     PNJK_HIP_CHECK(launch_ht_f64(st.g, st.ht, e->Cp.as<double>(), e->btri.as<double>(), e->X.as<double>(), s,
                                  nullptr, nullptr, col));
@@ -503,9 +499,14 @@ int stage_jmo(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col
 
 int stage_jmo_post(pnjk_engine* e, BuildState& st, hipStream_t s) {
   PNJK_HIP_CHECK(launch_symmetrize(1, st.g.nbf5, st.dJMO, st.g.nbf5, 0, s));
-  if (st.a.precision == PNJK_SPLIT_REFINE && st.want_j)  // J^n in split form from the exact d
-    PNJK_HIP_CHECK(launch_split_btri(e->d.as<double>(), (int64_t)st.g.nbf5 * st.g.naux_p, e->d16h.as<uint16_t>(),
-                                     e->d16l.as<uint16_t>(), s));
+  if (st.a.precision == PNJK_SPLIT_REFINE) {  // J^n, K^n in split form from the exact d and X
+    if (st.want_j)
+      PNJK_HIP_CHECK(launch_split_btri(e->d.as<double>(), (int64_t)st.g.nbf5 * st.g.naux_p, e->d16h.as<uint16_t>(),
+                                       e->d16l.as<uint16_t>(), s));
+    if (st.want_k)
+      PNJK_HIP_CHECK(launch_split_btri(e->X.as<double>(), (int64_t)st.g.nbf5 * st.g.nbf * st.g.naux_p,
+                                       e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), s));
+  }
   return PNJK_OK;
 }
 
