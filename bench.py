@@ -35,6 +35,12 @@ PEAK_FP64 = 78.6      # TFLOP/s, MI355X dense FP64 MFMA
 PEAK_BF16 = 2500.0    # TFLOP/s, MI355X dense BF16 MFMA
 REQUEST = 20          # builds per orbital-optimisation request
 OUTS = ("J", "K", "J_MO", "K_MO", "H_core")
+# rocprofv3 summary of this workload (kernel times and --pmc HBM traffic per dispatch)
+PROFILE = os.path.join(ROOT, "profiles", "jk_build_large_kernels.md")
+SPLIT_KERNEL = {"half_transform": "k_gemm_x3<5, 4, ProbHT>", "quarter_transform": "k_gemm_x3<5, 4, ProbQT>",
+                "coulomb_jn": "k_gemm_x3<5, 4, ProbJN>", "exchange_kn": "k_gemm_x3<4, 4, ProbSYRK>"}
+FP64_KERNEL = {"half_transform": "k_gemm_f64<5, 4, 8, ProbHT<false> >", "quarter_transform": "k_gemm_f64<2, 2, 16, ProbQT>",
+               "coulomb_jn": "k_gemm_f64<5, 4, 8, ProbJN>", "exchange_kn": "k_gemm_f64<4, 4, 8, ProbSYRK>"}
 
 
 def cpu_model() -> str:
@@ -74,6 +80,27 @@ def cpu_baseline(w, sample_naux: int):
                    f"nbf={w.nbf}, nbf5={w.nbf5}, naux={sample_naux} of {w.naux}: {dt:.2f} s measured, "
                    f"scaled x{scale:.0f} (linear in naux); BLAS threads={cores}; CPU: {cpu_model()}"),
     }
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc summary:
+    FETCH_SIZE scaled by the copy-kernel calibration plus WRITE_SIZE. None when the
+    summary or the kernel's row is absent."""
+    try:
+        with open(PROFILE) as f:
+            text = f.read()
+    except OSError:
+        return None
+    for row in text.splitlines():
+        cells = [c.strip() for c in row.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[0] == f"`{kernel}`":
+            try:
+                fetch, write = float(cells[2]), float(cells[3])
+            except ValueError:
+                continue
+            return {"bytes": round((fetch + write) * 2**30), "fetch_GiB": fetch, "write_GiB": write,
+                    "source": "profiles/jk_build_large_kernels.md (rocprofv3 --pmc FETCH_SIZE x2 calibration, WRITE_SIZE)"}
+    return None
 
 
 def log(msg):
@@ -215,15 +242,19 @@ def main():
     sms = {names[i]: float(split_stages[i]) for i in range(nst)}
     dom = max((n for n in names if n in flops), key=lambda n: sms[n])
     achieved = flops[dom] / (sms[dom] * 1e-3) / 1e12
+    tr = profiled_traffic(SPLIT_KERNEL.get(dom, ""))
     roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_BF16,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16, 4),
+                "traffic": tr["bytes"] if tr else None, "traffic_detail": tr,
                 "note": "BF16x3: each useful multiply-add issues 3 bf16 MFMA products; "
                         f"issued rate {3 * achieved:.1f} TFLOP/s = {3 * achieved / PEAK_BF16:.3f} of peak",
                 "per_stage_tflops": {n: round(flops[n] / (sms[n] * 1e-3) / 1e12, 3) for n in flops if sms[n] > 0}}
     fp64_stages = forms["fp64"]["stage_ms"]
     fd = max((n for n in flops), key=lambda n: fp64_stages[n])
+    tr64 = profiled_traffic(FP64_KERNEL.get(fd, ""))
     roofline_fp64 = {"bound": "mfma", "kernel": fd, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                     "achieved": round(flops[fd] / (fp64_stages[fd] * 1e-3) / 1e12, 3)}
+                     "achieved": round(flops[fd] / (fp64_stages[fd] * 1e-3) / 1e12, 3),
+                     "traffic": tr64["bytes"] if tr64 else None}
     roofline_fp64["frac"] = round(roofline_fp64["achieved"] / PEAK_FP64, 4)
 
     line = {

@@ -27,7 +27,7 @@ from . import _lib
 STAGES = {"half_transform": 1, "quarter_transform": 2, "coulomb_jn": 5, "exchange_kn": 6}
 CANDIDATES = {
     "half_transform": [(2, 2), (4, 4), (5, 4), (4, 2), (2, 4), (5, 2)],
-    "quarter_transform": [(2, 2), (4, 2), (5, 2)],
+    "quarter_transform": [(2, 2), (4, 2), (5, 2), (2, 4), (4, 4), (5, 4)],  # fn = 4: BF16x3 only
     "coulomb_jn": [(2, 2), (4, 4), (5, 4), (4, 2), (2, 4), (5, 2)],
     "exchange_kn": [(2, 2), (4, 4)],
 }
@@ -71,7 +71,7 @@ def tune(eng, build, arith: str, reps: int = 2, stages=None, log=None):
         idx = STAGES[st]
         timings = {}
         for fm, fn in CANDIDATES[st]:
-            if L.pnjk_set_tile(eng._h, a, idx, fm, fn) != _lib.OK:
+            if L.pnjk_set_tile(eng._h, a, idx, fm, fn) != _lib.OK:  # shape not offered by this arithmetic
                 continue
             build()  # warm the shape's code object and workspace
             timings[(fm, fn)] = stage_times(eng, build, reps)[st]

@@ -34,6 +34,20 @@ int set_error(int code, const std::string& msg);
       return ::pnjk::set_error(2, std::string(#expr) + ": " + hipGetErrorString(_e)); \
   } while (0)
 
+// Pointers that a kernel reads out of a problem table in device memory (the
+// grouped launches) carry no address space, so the compiler would emit flat_*
+// accesses; flat loads also count against lgkmcnt, which makes every LDS wait a
+// wait for the HBM loads as well. gmem() restores the global address space.
+#define PNJK_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const PNJK_GLOBAL T* gmem(const T* p) {
+  return (const PNJK_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ PNJK_GLOBAL T* gmem(T* p) {
+  return (PNJK_GLOBAL T*)p;
+}
+
 // Workgroup -> XCD placement on MI355X: consecutive workgroup ids are dealt
 // round-robin over the 8 XCDs. This maps a workgroup id to a logical tile id so
 // that each XCD receives one contiguous run of logical tiles, keeping tiles that

@@ -99,7 +99,7 @@ TileF64 default_tile(int arith, int stage_kind, int nbf, int nbf5) {
   }
   switch (stage_kind) {
     case 0: return big ? TileF64{5, 4} : TileF64{2, 2};
-    case 1: return big ? TileF64{4, 2} : TileF64{2, 2};
+    case 1: return big ? TileF64{5, 4} : TileF64{2, 2};
     case 2: return big ? TileF64{5, 4} : TileF64{2, 2};
     default: return nbf >= 256 ? TileF64{4, 4} : TileF64{2, 2};
   }
@@ -205,7 +205,7 @@ uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int preci
     s += (uint64_t)(ntri * naux_p) * 4;                      // btri16 hi/lo
     s += (uint64_t)(nbf * naux_p * nbf_p) * 4;               // bht16 hi/lo
     s += (uint64_t)(nbf5 * nbf_p) * 4;                       // Ct16 hi/lo
-    s += (uint64_t)(nbf5 * nbf * naux_p) * 4;                // X16 hi/lo
+    s += (uint64_t)(nbf5 * nbf * naux_p + 64) * 4;           // X16 hi/lo (+ slack)
     s += (uint64_t)(nbf5 * naux_p) * 4;                      // d16 hi/lo
   }
   return s;
@@ -414,7 +414,8 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st) {
   if (st.hH) st.dH = ob + off_H;
 
   if (precision != PNJK_FP64) {
-    const size_t ct_bytes = (size_t)(n5 * st.sg.ldk) * 2, x_bytes = (size_t)(n5 * nbf * g.naux_p) * 2,
+    // X16 slack: a 128-column quarter-transform tile reads up to 64 columns past naux_p.
+    const size_t ct_bytes = (size_t)(n5 * st.sg.ldk) * 2, x_bytes = (size_t)(n5 * nbf * g.naux_p + 64) * 2,
                  d_bytes = (size_t)(n5 * g.naux_p) * 2;
     if ((rc = e->ct16h.ensure(ct_bytes)) || (rc = e->ct16l.ensure(ct_bytes))) return rc;
     if ((rc = e->x16h.ensure(x_bytes)) || (rc = e->x16l.ensure(x_bytes))) return rc;
@@ -846,8 +847,8 @@ int pnjk_set_tile(pnjk_engine* e, int arith, int stage, int fm, int fn) {
   for (const TileF64 t : {TileF64{2, 2}, TileF64{4, 4}, TileF64{5, 4}, TileF64{4, 2}, TileF64{2, 4}, TileF64{5, 2}})
     known = known || (t.fm == fm && t.fn == fn);
   if (!reset && !known) return set_error(PNJK_EINVAL, "pnjk_set_tile: unsupported tile shape");
-  if (!reset && stage == ST_QT && fn != 2)
-    return set_error(PNJK_EINVAL, "pnjk_set_tile: the quarter transform needs fn = 2 (64 aux columns per tile)");
+  if (!reset && stage == ST_QT && fn != 2 && !(arith == 1 && fn == 4))
+    return set_error(PNJK_EINVAL, "pnjk_set_tile: the quarter transform needs fn = 2 (FP64) or fn = 2, 4 (BF16x3)");
   if (!reset && stage == ST_K && fm != fn)
     return set_error(PNJK_EINVAL, "pnjk_set_tile: the exchange SYRK needs square tiles (fm = fn)");
   std::lock_guard<std::mutex> lk(e->mu);

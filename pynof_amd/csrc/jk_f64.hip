@@ -49,7 +49,7 @@ struct Loader<KM, ROWS, BK> {
       const int c = threadIdx.x + i * kThreads;
       if (kChunks % kThreads == 0 || c < kChunks) {
         const int k = c / (ROWS / 2), rp = (c % (ROWS / 2)) * 2;
-        v[i] = *reinterpret_cast<const v2d*>(s.at(k0 + k, r0 + rp));
+        v[i] = *gmem(reinterpret_cast<const v2d*>(s.at(k0 + k, r0 + rp)));
       }
     }
   }
@@ -78,7 +78,7 @@ struct Loader<MK, ROWS, BK> {
       const int c = threadIdx.x + i * kThreads;
       if (kChunks % kThreads == 0 || c < kChunks) {
         const int r = c % ROWS, kp = (c / ROWS) * 2;
-        v[i] = *reinterpret_cast<const v2d*>(s.row(r0 + r) + k0 + kp);
+        v[i] = *gmem(reinterpret_cast<const v2d*>(s.row(r0 + r) + k0 + kp));
       }
     }
   }
@@ -106,6 +106,9 @@ __device__ __forceinline__ void gemm_tile_f64(const Prob& prob, const Tile& t, d
   const auto bs = prob.b_src(t);
   const int nk = prob.ksteps(BK);
 
+  // (The wave index is left as a vector value here: making it scalar with
+  // readfirstlane turns the MFMA block predicates into scalar branches, which
+  // measured 6% slower on the half-transform and 11% on the exchange SYRK.)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
 
@@ -127,7 +130,11 @@ __device__ __forceinline__ void gemm_tile_f64(const Prob& prob, const Tile& t, d
       if (prob.block_active(t, t.m0 + wm + 16 * i, t.n0 + wn + 16 * j)) active |= 1u << (i * FN + j);
     }
 
-  for (int kt = 0; kt < nk; ++kt) {
+  // Every problem has at least one k-step (k extents are padded to >= 32). A
+  // do-while gives the accumulators no zero-trip path, which would otherwise make
+  // the compiler copy them out of AGPRs at the loop exit.
+  int kt = 0;
+  do {
     const int cur = kt & 1;
     const bool more = kt + 1 < nk;
     if (more) {
@@ -156,7 +163,7 @@ __device__ __forceinline__ void gemm_tile_f64(const Prob& prob, const Tile& t, d
       lb.store(sB + (cur ^ 1) * SB);
     }
     __syncthreads();
-  }
+  } while (++kt < nk);
   prob.template epilogue<FM, FN>(t, acc, wm, wn, lane, sA);
 }
 
@@ -287,12 +294,12 @@ struct ProbHT {
           const int q = t.m0 + dm;
           const int64_t n = (int64_t)t.n0 + dn;
           if (q < nbf5 && n < N) {
-            X[(int64_t)q * N + n] = acc[i][j][r];
+            gmem(X)[(int64_t)q * N + n] = acc[i][j][r];
             if (kSplit) {
               uint16_t h, l;
               split2(acc[i][j][r], h, l);
-              xhi[(int64_t)q * N + n] = h;
-              xlo[(int64_t)q * N + n] = l;
+              gmem(xhi)[(int64_t)q * N + n] = h;
+              gmem(xlo)[(int64_t)q * N + n] = l;
             }
           }
         }
@@ -335,7 +342,7 @@ struct ProbQT {
           for (int r = 0; r < 4; ++r) {
             int dm, dn;
             elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
-            if (t.m0 + dm == p) d[(int64_t)p * naux_p + t.n0 + dn] = acc[i][j][r];
+            if (t.m0 + dm == p) gmem(d)[(int64_t)p * naux_p + t.n0 + dn] = acc[i][j][r];
           }
     }
     // Row sums of squares over this P-tile, reduced over the 16 column lanes,
@@ -352,7 +359,7 @@ struct ProbQT {
         for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
         s[i][r] = v;
       }
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if ((wave & 1) && (lane & 15) == 0) {
 #pragma unroll
       for (int i = 0; i < FM; ++i)
@@ -368,7 +375,7 @@ struct ProbQT {
           const int dm = wm + 16 * i + (lane >> 4) + 4 * r;
           const int q = t.m0 + dm;
           if (q <= p && q < nbf5)
-            kpart[((int64_t)p * nbf5 + q) * ntiles + t.nt] = s[i][r] + lds[dm];
+            gmem(kpart)[((int64_t)p * nbf5 + q) * ntiles + t.nt] = s[i][r] + lds[dm];
         }
     }
   }
@@ -411,7 +418,7 @@ struct ProbSYRK {
           int dm, dn;
           elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int m = t.m0 + dm, n = t.n0 + dn;
-          if (m < M && n < M) o[(int64_t)m * ldo + n] = acc[i][j][r];
+          if (m < M && n < M) gmem(o)[(int64_t)m * ldo + n] = acc[i][j][r];
         }
   }
 };
@@ -452,7 +459,7 @@ struct ProbJN {
         for (int r = 0; r < 4; ++r) {
           elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int q = t.m0 + dm;
-          if (q < nbf5) J[(int64_t)q * nn + (int64_t)mu * nbf + nu] = acc[i][j][r];
+          if (q < nbf5) gmem(J)[(int64_t)q * nn + (int64_t)mu * nbf + nu] = acc[i][j][r];
         }
     }
   }
@@ -487,7 +494,7 @@ struct ProbGemmT {  // T[mu][i] = sum_nu Hp[mu][nu] Cp[nu][i]
           int dm, dn;
           elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int m = t.m0 + dm, n = t.n0 + dn;
-          if (m < nbf && n < ldcp) T[(int64_t)m * ldcp + n] = acc[i][j][r];
+          if (m < nbf && n < ldcp) gmem(T)[(int64_t)m * ldcp + n] = acc[i][j][r];
         }
   }
 };

@@ -61,8 +61,8 @@ struct Ld<RK, ROWS> {
       if (kChunks % kThreads == 0 || c < kChunks) {
         const int r = c >> 2, kc = (c & 3) * 8;
         const int64_t o = s.off(r0 + r) + k0 + kc;
-        h[i] = *reinterpret_cast<const u16x8*>(s.p.hi + o);
-        l[i] = *reinterpret_cast<const u16x8*>(s.p.lo + o);
+        h[i] = *gmem(reinterpret_cast<const u16x8*>(s.p.hi + o));
+        l[i] = *gmem(reinterpret_cast<const u16x8*>(s.p.lo + o));
       }
     }
   }
@@ -94,8 +94,8 @@ struct Ld<KR, ROWS> {
       if (kChunks % kThreads == 0 || c < kChunks) {
         const int k = c % kBK, rc = (c / kBK) * 8;
         const int64_t o = s.off(k0 + k) + r0 + rc;
-        h[i] = *reinterpret_cast<const u16x8*>(s.p.hi + o);
-        l[i] = *reinterpret_cast<const u16x8*>(s.p.lo + o);
+        h[i] = *gmem(reinterpret_cast<const u16x8*>(s.p.hi + o));
+        l[i] = *gmem(reinterpret_cast<const u16x8*>(s.p.lo + o));
       }
     }
   }
@@ -126,7 +126,9 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
   const auto bs = prob.b_src(t);
   const int nk = prob.ksteps();
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // The wave index is uniform over the wave; readfirstlane makes that visible to
+  // the compiler, so per-wave offsets and block predicates stay scalar.
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
 
   Ld<Prob::kA, BM> la;
@@ -275,15 +277,18 @@ struct ProbHT {
           if (q < nbf5 && n < N) {
             const float v = acc[i][j][r];
             const uint16_t h = bf16_rne(v);
-            xhi[(int64_t)q * N + n] = h;
-            xlo[(int64_t)q * N + n] = bf16_rne(v - bf16_to_f(h));
+            gmem(xhi)[(int64_t)q * N + n] = h;
+            gmem(xlo)[(int64_t)q * N + n] = bf16_rne(v - bf16_to_f(h));
           }
         }
   }
 };
 
 // Batch p: Y_p[q][P] = sum_nu Ct[q][nu] X16[p][nu][P] for q <= p; writes d[p][P]
-// (FP64 and split planes) and per-P-tile row sums of Y_p^2 (FP64) into kpart.
+// (FP64 and split planes) and per-64-column row sums of Y_p^2 (FP64) into kpart.
+// Tiles are 64 or 128 aux columns wide (FN = 2 or 4); both give the same kpart
+// bits: a 64-column partial is always S(first 32 columns) + S(last 32 columns),
+// S being one wave's sum over two 16-column blocks reduced across 16 lanes.
 struct SrcXT {  // element (k = nu, r = P) at p*nbf*naux_p + min(nu, nbf-1)*naux_p + P
   Planes p;
   int64_t base, ld;
@@ -298,7 +303,7 @@ struct ProbQT {
   double* d;
   uint16_t* dhi;
   uint16_t* dlo;
-  int nbf, naux_p, nbf5, ldk, mtiles, ntiles;
+  int nbf, naux_p, nbf5, ldk, mtiles, ntiles, nt64;  // nt64 = naux_p / 64 (kpart stride)
   __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.mt = id % mtiles;
     const int rest = id / mtiles;
@@ -313,6 +318,7 @@ struct ProbQT {
   __device__ SrcXT b_src(const Tile& t) const { return {x, (int64_t)t.batch * nbf * naux_p, naux_p, nbf}; }
   template <int FM, int FN>
   __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double* lds) const {
+    static_assert(FN == 2 || FN == 4, "quarter-transform tiles are 64 or 128 aux columns wide");
     const int p = t.batch;
     if (p >= t.m0 && p < t.m0 + 32 * FM) {
 #pragma unroll
@@ -323,13 +329,13 @@ struct ProbQT {
           for (int r = 0; r < 4; ++r) {
             int dm, dn;
             coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
-            if (t.m0 + dm == p) {
+            if (t.m0 + dm == p && t.n0 + dn < naux_p) {
               const int64_t o = (int64_t)p * naux_p + t.n0 + dn;
               const float v = acc[i][j][r];
-              d[o] = v;
+              gmem(d)[o] = v;
               const uint16_t h = bf16_rne(v);
-              dhi[o] = h;
-              dlo[o] = bf16_rne(v - bf16_to_f(h));
+              gmem(dhi)[o] = h;
+              gmem(dlo)[o] = bf16_rne(v - bf16_to_f(h));
             }
           }
     }
@@ -338,30 +344,50 @@ struct ProbQT {
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        double v = 0.0;
+        double tot = 0.0;
 #pragma unroll
-        for (int j = 0; j < FN; ++j) v += (double)acc[i][j][r] * (double)acc[i][j][r];
+        for (int h = 0; h < FN / 2; ++h) {
+          double v = 0.0;
 #pragma unroll
-        for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
-        s[i][r] = v;
-      }
-    const int wave = threadIdx.x >> 6;
-    if ((wave & 1) && (lane & 15) == 0) {
+          for (int j = 2 * h; j < 2 * h + 2; ++j) v += (double)acc[i][j][r] * (double)acc[i][j][r];
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) lds[wm + 16 * i + out_row(lane, r)] = s[i][r];
-    }
-    __syncthreads();
-    if (!(wave & 1) && (lane & 15) == 0) {
-#pragma unroll
-      for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int dm = wm + 16 * i + out_row(lane, r);
-          const int q = t.m0 + dm;
-          if (q <= p && q < nbf5) kpart[((int64_t)p * nbf5 + q) * ntiles + t.nt] = s[i][r] + lds[dm];
+          for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
+          tot = h == 0 ? v : tot + v;
         }
+        s[i][r] = tot;
+      }
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if constexpr (FN == 4) {  // each wave covers one 64-column block
+      const int c0 = t.n0 + wn;
+      if ((lane & 15) == 0 && c0 < naux_p) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int q = t.m0 + wm + 16 * i + out_row(lane, r);
+            if (q <= p && q < nbf5) gmem(kpart)[((int64_t)p * nbf5 + q) * nt64 + c0 / 64] = s[i][r];
+          }
+      }
+      (void)wave;
+      (void)lds;
+    } else {  // the two waves of a row pair cover one 64-column block: add through LDS
+      if ((wave & 1) && (lane & 15) == 0) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) lds[wm + 16 * i + out_row(lane, r)] = s[i][r];
+      }
+      __syncthreads();
+      if (!(wave & 1) && (lane & 15) == 0) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int dm = wm + 16 * i + out_row(lane, r);
+            const int q = t.m0 + dm;
+            if (q <= p && q < nbf5) gmem(kpart)[((int64_t)p * nbf5 + q) * nt64 + t.n0 / 64] = s[i][r] + lds[dm];
+          }
+      }
     }
   }
 };
@@ -403,7 +429,7 @@ struct ProbJN {
           coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int q = t.m0 + dm;
           if (q < nbf5) {
-            J[(int64_t)q * nn + (int64_t)mu * nbf + nu] = acc[i][j][r];
+            gmem(J)[(int64_t)q * nn + (int64_t)mu * nbf + nu] = acc[i][j][r];
           }
         }
     }
@@ -446,7 +472,7 @@ struct ProbSYRK {
           coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int m = t.m0 + dm, n = t.n0 + dn;
           if (m < M && n < M) {
-            o[(int64_t)m * ldo + n] = acc[i][j][r];
+            gmem(o)[(int64_t)m * ldo + n] = acc[i][j][r];
           }
         }
   }
@@ -577,8 +603,10 @@ hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
 hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
                         const uint16_t* x_hi, const uint16_t* x_lo, double* kpart, double* d, uint16_t* d_hi,
                         uint16_t* d_lo, int* ntiles, hipStream_t s, GroupCollector* col) {
+  // naux_p is a multiple of 64; a 128-column tile may cover 64 columns past it,
+  // which the B loader reads from the workspace slack and the epilogue drops.
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
-  if (g.naux_p % BN != 0) return hipErrorInvalidValue;
+  if (tc.fn != 2 && tc.fn != 4) return hipErrorInvalidValue;
   ProbQT pr;
   pr.ct = {ct_hi, ct_lo};
   pr.x = {x_hi, x_lo};
@@ -591,8 +619,9 @@ hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
   pr.nbf5 = g.nbf5;
   pr.ldk = g.ldk;
   pr.mtiles = cdiv(g.nbf5, BM);
-  pr.ntiles = g.naux_p / BN;
-  *ntiles = pr.ntiles;
+  pr.ntiles = cdiv(g.naux_p, BN);
+  pr.nt64 = g.naux_p / 64;
+  *ntiles = pr.nt64;
   return launch_x3(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s, col);
 }
 

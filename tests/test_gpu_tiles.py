@@ -22,6 +22,9 @@ def test_tiles_bitwise(gpu_required, precision):
     a = autotune.ARITH[precision]
     for st, idx in autotune.STAGES.items():
         for fm, fn in autotune.CANDIDATES[st]:
+            if st == "quarter_transform" and fn == 4 and precision == "fp64":
+                assert L.pnjk_set_tile(eng._h, a, idx, fm, fn) == _lib.EINVAL  # 128-column QT tiles: BF16x3 only
+                continue
             _lib.check(L.pnjk_set_tile(eng._h, a, idx, fm, fn))
             r = eng.build(C, nbf5, H=H, precision=precision)
             for k in ref:
@@ -33,7 +36,9 @@ def test_tiles_bitwise(gpu_required, precision):
 def test_set_tile_rejects(gpu_required):
     eng = JKEngine(0)
     L = _lib.lib()
-    assert L.pnjk_set_tile(eng._h, 0, 2, 4, 4) == _lib.EINVAL   # quarter transform needs fn = 2
+    assert L.pnjk_set_tile(eng._h, 0, 2, 4, 4) == _lib.EINVAL   # FP64 quarter transform needs fn = 2
+    assert L.pnjk_set_tile(eng._h, 1, 2, 4, 4) == _lib.OK       # BF16x3 takes fn = 2 or 4
+    assert L.pnjk_set_tile(eng._h, 1, 2, 0, 0) == _lib.OK
     assert L.pnjk_set_tile(eng._h, 1, 6, 5, 4) == _lib.EINVAL   # exchange SYRK needs square tiles
     assert L.pnjk_set_tile(eng._h, 0, 0, 2, 2) == _lib.EINVAL   # copy_in takes no tile
     assert L.pnjk_set_tile(eng._h, 1, 1, 3, 3) == _lib.EINVAL   # unknown shape
