@@ -377,10 +377,11 @@ def bench_serving(args):
     from pynof_amd.serving import DeviceExecutor, Server, poisson_requests
 
     n = args.requests
+    alone = {}  # rid -> service time of the request run by itself (upload + 20 builds)
 
-    def run(rate):
+    def run(rate, max_active=64):
         rs = poisson_requests(n, rate=rate, seed=5000)
-        srv = Server(DeviceExecutor(0, with_jk=True), hbm_budget=200e9, max_active=64)
+        srv = Server(DeviceExecutor(0, with_jk=True), hbm_budget=200e9, max_active=max_active)
         for r in rs:
             srv.submit(r)
         t0 = time.perf_counter()
@@ -392,8 +393,14 @@ def bench_serving(args):
         st["builds_per_s"] = round(20 * st["finished"] / wall, 2)
         if "p50_s" in st:
             st["p99_over_p50"] = round(st["p99_s"] / st["p50_s"], 3)
+        if alone:
+            # slowdown = latency / the request's own service time alone: what the
+            # scheduler adds, with the spread of request sizes divided out
+            sd = np.array([r.latency / alone[r.rid] for r in srv.finished])
+            st["slowdown_p50"] = round(float(np.percentile(sd, 50)), 3)
+            st["slowdown_p99"] = round(float(np.percentile(sd, 99)), 3)
         torch.cuda.empty_cache()
-        return st
+        return st, srv.finished
 
     log("serving: warmup (8 requests)")
     small = poisson_requests(8, rate=0.0, seed=77)
@@ -401,13 +408,21 @@ def bench_serving(args):
     for r in small:
         srv.submit(r)
     srv.run(timeout=300)
+    log(f"serving: each request alone ({n} requests, one at a time)")
+    _, fin = run(0.0, max_active=1)
+    for r in fin:
+        alone[r.rid] = r.done - r.started
+    svc = np.array(list(alone.values()))
+    isolated = {"p50_s": round(float(np.percentile(svc, 50)), 4), "p99_s": round(float(np.percentile(svc, 99)), 4),
+                "p99_over_p50": round(float(np.percentile(svc, 99) / np.percentile(svc, 50)), 3),
+                "total_s": round(float(svc.sum()), 3)}
     log(f"serving: saturation ({n} requests at t=0)")
-    sat = run(0.0)
+    sat, _ = run(0.0)
     sat_rate = sat["requests_per_s"]
     loads = {}
     for frac in (0.5, 0.8, 1.0):
         log(f"serving: Poisson arrivals at {frac:.0%} of saturation ({frac * sat_rate:.2f} req/s)")
-        loads[f"{int(frac * 100)}%"] = run(frac * sat_rate)
+        loads[f"{int(frac * 100)}%"], _ = run(frac * sat_rate)
     line = {
         "metric": "serving: orbital-optimisation requests (20 JK builds each), throughput and p50/p99 latency",
         "value": sat_rate,
@@ -418,6 +433,7 @@ def bench_serving(args):
         "data": "synthetic: Poisson arrivals (seed 5000), nbf lognormal median 150 in [24,800], naux=ceil8(3.6 nbf)",
         "config": {"workload": "serving", "requests": n, "builds_per_request": 20, "hbm_budget_gb": 200,
                    "max_active": 64},
+        "isolated_service": isolated,
         "saturation": sat,
         "loads": loads,
         "p99_under_2x_p50_at_80pct": bool(loads["80%"].get("p99_over_p50", 99) < 2.0),

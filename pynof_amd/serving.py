@@ -43,6 +43,7 @@ class Request:
     seed: int = 0
     arrival: float = 0.0
     # filled by the server
+    started: float | None = None   # admission (b upload) began
     admitted: float | None = None
     done: float | None = None
     builds_done: int = 0
@@ -108,6 +109,7 @@ class Server:
             if self.in_use + r.footprint > self.budget:
                 break  # first-come-first-served: later requests wait behind it
             self.pending.popleft()
+            r.started = self.clock()
             try:
                 self.ex.admit(r)
             except Exception as exc:  # allocation or upload failure: fail this request only
@@ -162,14 +164,17 @@ def poisson_requests(count: int, rate: float, seed: int = 5000, median_nbf: int 
                      sigma: float = 0.6, tol_tight_fraction: float = 0.5, n_builds: int = 20) -> list[Request]:
     """Requests with Poisson arrivals (rate per second) and lognormal nbf (median
     `median_nbf`, clipped to [lo, hi]); naux = 3.6 nbf rounded up to a multiple of
-    8, nbf5 = nbf; half the requests ask for 1e-9 Eh, half for 1e-6 Eh."""
+    8, nbf5 = nbf; half the requests ask for 1e-9 Eh, half for 1e-6 Eh. Sizes and
+    tolerances come from their own random stream, so every rate (including 0, all
+    at t = 0) serves the same requests."""
     from .synth import ceil8, default_ndoc
 
     rng = np.random.default_rng(seed)
+    arrivals = np.random.default_rng([seed, 1])
     t = 0.0
     out = []
     for i in range(count):
-        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        t += arrivals.exponential(1.0 / rate) if rate > 0 else 0.0
         nbf = int(np.clip(round(median_nbf * math.exp(sigma * rng.standard_normal())), lo, hi))
         tol = 1e-9 if rng.random() < tol_tight_fraction else 1e-6
         out.append(Request(rid=i, nbf=nbf, naux=ceil8(3.6 * nbf), nbf5=nbf, ndoc=default_ndoc(nbf), tol=tol,

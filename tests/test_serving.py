@@ -134,3 +134,9 @@ def test_poisson_requests_shape():
     med = sorted(r.nbf for r in rs)[100]
     assert 100 <= med <= 220
     assert {r.tol for r in rs} == {1e-6, 1e-9}
+
+
+def test_poisson_requests_same_requests_at_every_rate():
+    a, b = poisson_requests(50, rate=0.0, seed=5000), poisson_requests(50, rate=7.5, seed=5000)
+    assert [(r.nbf, r.tol, r.seed) for r in a] == [(r.nbf, r.tol, r.seed) for r in b]
+    assert all(r.arrival == 0.0 for r in a) and b[-1].arrival > 0.0
