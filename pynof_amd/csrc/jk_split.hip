@@ -13,6 +13,8 @@
 //                      a transposing LDS loader)
 // Contractions (reference lines as in the FP64 kernels): HT integrals.py:211,
 // QT :354 (-> d :212, K_MO :358), JN :214, K^n :216.
+#include <type_traits>
+
 #include "jk_common.h"
 #include "jk_kernels.h"
 
@@ -47,33 +49,32 @@ enum LdKind { RK = 0, KR = 1 };
 template <int KIND, int ROWS>
 struct Ld;
 
-// Global element (r, k) at off(r) + k, k contiguous: 16-byte vector copies.
-template <int ROWS>
-struct Ld<RK, ROWS> {
-  static constexpr int kChunks = ROWS * (kBK / 8);
-  static constexpr int kPer = (kChunks + kThreads - 1) / kThreads;
-  u16x8 h[kPer], l[kPer];
+// Direct HBM -> LDS copy of both planes of a ROWS x 32 tile whose contraction
+// index k is contiguous in memory (global element (r, k) at off(r) + k), with
+// global_load_lds_dwordx4: lane L of one wave instruction writes 16 bytes at the
+// LDS base + 16 L, so one instruction fills 16 tile rows (1 KB). Lane L fetches
+// the global chunk that belongs in swizzled slot L & 3 of row L >> 2. The
+// instructions are dealt round-robin over the NW waves of the workgroup. No registers are
+// staged and no LDS store is issued; the loads complete under vmcnt.
+template <int ROWS, int NW>
+struct Dma {
+  static_assert(ROWS % 16 == 0, "a tile is whole 16-row blocks");
+  static constexpr int kInst = ROWS / 16;
   template <class S>
-  __device__ __forceinline__ void load(const S& s, int r0, int k0) {
+  static __device__ __forceinline__ void issue(const S& s, int r0, int k0, uint16_t* sh, uint16_t* sl, int wave,
+                                               int lane) {
+    const int rr = lane >> 2, slot = lane & 3;
 #pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-      const int c = threadIdx.x + i * kThreads;
-      if (kChunks % kThreads == 0 || c < kChunks) {
-        const int r = c >> 2, kc = (c & 3) * 8;
-        const int64_t o = s.off(r0 + r) + k0 + kc;
-        h[i] = *gmem(reinterpret_cast<const u16x8*>(s.p.hi + o));
-        l[i] = *gmem(reinterpret_cast<const u16x8*>(s.p.lo + o));
-      }
-    }
-  }
-  __device__ __forceinline__ void store(uint16_t* sh, uint16_t* sl) const {
-#pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-      const int c = threadIdx.x + i * kThreads;
-      if (kChunks % kThreads == 0 || c < kChunks) {
-        const int r = c >> 2;
-        *reinterpret_cast<u16x8*>(sh + swz(r, c & 3)) = h[i];
-        *reinterpret_cast<u16x8*>(sl + swz(r, c & 3)) = l[i];
+    for (int j = 0; j < (kInst + NW - 1) / NW; ++j) {
+      const int i = NW * j + wave;  // wave-uniform
+      if (kInst % NW == 0 || i < kInst) {
+        const int r = 16 * i + rr;
+        const int c = slot ^ (((r >> 1) ^ (r >> 2)) & 3);
+        const int64_t o = s.off(r0 + r) + k0 + 8 * c;
+        __builtin_amdgcn_global_load_lds((const PNJK_GLOBAL void*)(s.p.hi + o),
+                                         (__attribute__((address_space(3))) void*)(sh + 16 * i * kLDK), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const PNJK_GLOBAL void*)(s.p.lo + o),
+                                         (__attribute__((address_space(3))) void*)(sl + 16 * i * kLDK), 16, 0, 0);
       }
     }
   }
@@ -85,58 +86,72 @@ template <int ROWS>
 struct Ld<KR, ROWS> {
   static constexpr int kChunks = kBK * (ROWS / 8);
   static constexpr int kPer = (kChunks + kThreads - 1) / kThreads;
+  // Every lane loads and stores (no branches in the k-loop); when kChunks is not
+  // a multiple of kThreads the surplus lanes repeat chunk c % kChunks and store
+  // identical bytes to the same LDS address.
+  static __device__ __forceinline__ int chunk(int i) {
+    const int c = threadIdx.x + i * kThreads;
+    return kChunks % kThreads == 0 ? c : c % kChunks;
+  }
   u16x8 h[kPer], l[kPer];
   template <class S>
   __device__ __forceinline__ void load(const S& s, int r0, int k0) {
 #pragma unroll
     for (int i = 0; i < kPer; ++i) {
-      const int c = threadIdx.x + i * kThreads;
-      if (kChunks % kThreads == 0 || c < kChunks) {
-        const int k = c % kBK, rc = (c / kBK) * 8;
-        const int64_t o = s.off(k0 + k) + r0 + rc;
-        h[i] = *gmem(reinterpret_cast<const u16x8*>(s.p.hi + o));
-        l[i] = *gmem(reinterpret_cast<const u16x8*>(s.p.lo + o));
-      }
+      const int c = chunk(i);
+      const int k = c % kBK, rc = (c / kBK) * 8;
+      const int64_t o = s.off(k0 + k) + r0 + rc;
+      h[i] = *gmem(reinterpret_cast<const u16x8*>(s.p.hi + o));
+      l[i] = *gmem(reinterpret_cast<const u16x8*>(s.p.lo + o));
     }
   }
   __device__ __forceinline__ void store(uint16_t* sh, uint16_t* sl) const {
 #pragma unroll
     for (int i = 0; i < kPer; ++i) {
-      const int c = threadIdx.x + i * kThreads;
-      if (kChunks % kThreads == 0 || c < kChunks) {
-        const int k = c % kBK, rc = (c / kBK) * 8;
+      const int c = chunk(i);
+      const int k = c % kBK, rc = (c / kBK) * 8;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int o = swz(rc + e, k >> 3) + (k & 7);
-          sh[o] = h[i][e];
-          sl[o] = l[i][e];
-        }
+      for (int e = 0; e < 8; ++e) {
+        const int o = swz(rc + e, k >> 3) + (k & 7);
+        sh[o] = h[i][e];
+        sl[o] = l[i][e];
       }
     }
   }
 };
 
-template <int FM, int FN, class Prob>
-__device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, uint16_t* sAbase, uint16_t* sBbase) {
-  constexpr int BM = 32 * FM, BN = 32 * FN;
-  // LDS: [buffer][plane][rows * kLDK]
-  auto sA = [&](int buf, int plane) { return sAbase + (buf * 2 + plane) * BM * kLDK; };
-  auto sB = [&](int buf, int plane) { return sBbase + (buf * 2 + plane) * BN * kLDK; };
+// One output tile of (16 FM WGM) x (16 FN WGN): WGM x WGN waves, each with an
+// FM x FN grid of 16x16 MFMA blocks. LDS: two buffers per operand (separate
+// arrays, so the compiler sees that fragment reads of one never alias copies
+// into the other), each holding the hi plane then the lo plane. k-step kt+1 is
+// fetched into the other buffer while k-step kt computes; one barrier per k-step.
+template <int FM, int FN, int WGM, int WGN, class Prob>
+__device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, uint16_t* sA0, uint16_t* sA1,
+                                             uint16_t* sB0, uint16_t* sB1) {
+  constexpr int BM = 16 * FM * WGM, BN = 16 * FN * WGN, NW = WGM * WGN;
+  static_assert(Prob::kA == RK, "the A operand is k-contiguous");
+  constexpr bool kBdma = Prob::kB == RK;
+  static_assert(kBdma || NW == 4, "the transposing register loader is written for four waves");
   const auto as = prob.a_src(t);
   const auto bs = prob.b_src(t);
   const int nk = prob.ksteps();
 
   // The wave index is uniform over the wave; readfirstlane makes that visible to
-  // the compiler, so per-wave offsets and block predicates stay scalar.
+  // the compiler, so per-wave offsets stay scalar.
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
+  const int wm = (wave / WGN) * (16 * FM), wn = (wave % WGN) * (16 * FN);
 
-  Ld<Prob::kA, BM> la;
-  Ld<Prob::kB, BN> lb;
-  la.load(as, t.m0, 0);
-  lb.load(bs, t.n0, 0);
-  la.store(sA(0, 0), sA(0, 1));
-  lb.store(sB(0, 0), sB(0, 1));
+  Ld<KR, BN> lb;  // register path of a transposed B operand (the quarter transform)
+  auto fetch = [&](int k0, uint16_t* a, uint16_t* b) {
+    Dma<BM, NW>::issue(as, t.m0, k0, a, a + BM * kLDK, wave, lane);
+    if constexpr (kBdma)
+      Dma<BN, NW>::issue(bs, t.n0, k0, b, b + BN * kLDK, wave, lane);
+    else
+      lb.load(bs, t.n0, k0);
+  };
+  fetch(0, sA0, sB0);
+  if constexpr (!kBdma) lb.store(sB0, sB0 + BN * kLDK);
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
 
   v4f acc[FM][FN];
@@ -146,29 +161,26 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
     for (int j = 0; j < FN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
   const int frow = lane & 15, fk = 8 * (lane >> 4);
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    const bool more = kt + 1 < nk;
-    if (more) {
-      la.load(as, t.m0, (kt + 1) * kBK);
-      lb.load(bs, t.n0, (kt + 1) * kBK);
-    }
-    const uint16_t* ah_s = sA(cur, 0);
-    const uint16_t* al_s = sA(cur, 1);
-    const uint16_t* bh_s = sB(cur, 0);
-    const uint16_t* bl_s = sB(cur, 1);
+  auto step = [&](const int kt, auto stage, auto more) {
+    constexpr int cur = decltype(stage)::value;
+    constexpr bool kMore = decltype(more)::value;
+    uint16_t* const a_c = cur ? sA1 : sA0;
+    uint16_t* const b_c = cur ? sB1 : sB0;
+    uint16_t* const a_n = cur ? sA0 : sA1;
+    uint16_t* const b_n = cur ? sB0 : sB1;
+    if constexpr (kMore) fetch((kt + 1) * kBK, a_n, b_n);
     bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
       const int o = swz(wm + 16 * i + frow, fk >> 3);
-      ah[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(ah_s + o));
-      al[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(al_s + o));
+      ah[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(a_c + o));
+      al[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(a_c + BM * kLDK + o));
     }
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const int o = swz(wn + 16 * j + frow, fk >> 3);
-      bh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(bh_s + o));
-      bl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(bl_s + o));
+      bh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(b_c + o));
+      bl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(b_c + BN * kLDK + o));
     }
 #pragma unroll
     for (int i = 0; i < FM; ++i)
@@ -178,33 +190,47 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
       }
-    if (more) {
-      la.store(sA(cur ^ 1, 0), sA(cur ^ 1, 1));
-      lb.store(sB(cur ^ 1, 0), sB(cur ^ 1, 1));
-    }
+    if constexpr (kMore && !kBdma) lb.store(b_n, b_n + BN * kLDK);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies into the next buffers have landed
     __syncthreads();
+  };
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  using Y = std::true_type;
+  using N = std::false_type;
+  int kt = 0;
+  for (; kt + 2 < nk; kt += 2) {
+    step(kt, S0{}, Y{});
+    step(kt + 1, S1{}, Y{});
   }
-  prob.template epilogue<FM, FN>(t, acc, wm, wn, lane, reinterpret_cast<double*>(sAbase));
+  if (nk - kt == 2) {  // kt is even: one or two k-steps remain
+    step(kt, S0{}, Y{});
+    step(kt + 1, S1{}, N{});
+  } else {
+    step(kt, S0{}, N{});
+  }
+  prob.template epilogue<FM, FN>(t, acc, wm, wn, lane, reinterpret_cast<double*>(sA0));
 }
 
-// One output tile per workgroup.
-template <int FM, int FN, class Prob>
-__global__ void __launch_bounds__(kThreads) k_gemm_x3(const Prob prob) {
-  constexpr int BM = 32 * FM, BN = 32 * FN;
-  __shared__ __align__(16) uint16_t sA[2 * 2 * BM * kLDK];
-  __shared__ __align__(16) uint16_t sB[2 * 2 * BN * kLDK];
+// One output tile per workgroup. Two waves per SIMD: four-wave workgroups run
+// two per CU, eight-wave workgroups one.
+template <int FM, int FN, int WGM, int WGN, class Prob>
+__global__ void __launch_bounds__(64 * WGM * WGN, 2) k_gemm_x3(const Prob prob) {
+  constexpr int BM = 16 * FM * WGM, BN = 16 * FN * WGN;
+  __shared__ __align__(16) uint16_t sA0[2 * BM * kLDK], sA1[2 * BM * kLDK];
+  __shared__ __align__(16) uint16_t sB0[2 * BN * kLDK], sB1[2 * BN * kLDK];
   Tile t;
   if (!prob.tile(xcd_remap(blockIdx.x, gridDim.x), BM, BN, t)) return;
-  gemm_tile_x3<FM, FN>(prob, t, sA, sB);
+  gemm_tile_x3<FM, FN, WGM, WGN>(prob, t, sA0, sA1, sB0, sB1);
 }
 
 // Grouped persistent form over a (slot, local tile) table; see jk_kernels.h.
-template <int FM, int FN, class Prob>
-__global__ void __launch_bounds__(kThreads)
+template <int FM, int FN, int WGM, int WGN, class Prob>
+__global__ void __launch_bounds__(64 * WGM * WGN, 2)
     k_gemm_x3_grouped(const Prob* __restrict__ probs, const int2* __restrict__ tiles, int ntiles) {
-  constexpr int BM = 32 * FM, BN = 32 * FN;
-  __shared__ __align__(16) uint16_t sA[2 * 2 * BM * kLDK];
-  __shared__ __align__(16) uint16_t sB[2 * 2 * BN * kLDK];
+  constexpr int BM = 16 * FM * WGM, BN = 16 * FN * WGN;
+  __shared__ __align__(16) uint16_t sA0[2 * BM * kLDK], sA1[2 * BM * kLDK];
+  __shared__ __align__(16) uint16_t sB0[2 * BN * kLDK], sB1[2 * BN * kLDK];
   // Workgroup b runs on XCD b % 8: give each XCD one contiguous run of the
   // table, so tiles that share an operand (adjacent ids) stay in one L2.
   // (With fewer than 8 workgroups, every workgroup strides the whole table.)
@@ -224,7 +250,7 @@ __global__ void __launch_bounds__(kThreads)
     const Prob prob = probs[slot];
     const int2 tr = make_int2(slot, local);
     Tile t;
-    if (prob.tile(tr.y, BM, BN, t)) gemm_tile_x3<FM, FN>(prob, t, sA, sB);
+    if (prob.tile(tr.y, BM, BN, t)) gemm_tile_x3<FM, FN, WGM, WGN>(prob, t, sA0, sA1, sB0, sB1);
     __syncthreads();
   }
 }
@@ -529,32 +555,40 @@ __global__ void k_split_ct(const double* __restrict__ Cp, int ldcp, int nbf, int
   lo[i] = l;
 }
 
-template <int FM, int FN, class Prob>
+template <int FM, int FN, int WGM, int WGN, class Prob>
 hipError_t grouped_launch(const void* probs, const int2* tiles, int ntiles, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_gemm_x3_grouped<FM, FN, Prob>), dim3(grid), dim3(kThreads), 0, s,
+  hipLaunchKernelGGL((k_gemm_x3_grouped<FM, FN, WGM, WGN, Prob>), dim3(grid), dim3(64 * WGM * WGN), 0, s,
                      static_cast<const Prob*>(probs), tiles, ntiles);
   return hipGetLastError();
 }
 
+// Tile shapes are named (fm, fn) for a BM x BN = (32 fm) x (32 fn) output tile.
+// All but (10, 4) use 2 x 2 waves of (fm, fn) MFMA blocks; (10, 4) is 4 x 2 waves
+// of (5, 4) blocks (8 waves), offered to the half-transform and J^n, whose M
+// extent (nbf5) it covers in one tile for nbf5 <= 320.
 template <class Prob>
 hipError_t launch_x3(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s, GroupCollector* col = nullptr) {
   if (nblocks <= 0) return hipSuccess;
-  const dim3 grid(nblocks), block(kThreads);
-#define PNJK_CASE(FMV, FNV)                                                                                 \
-  if (tc.fm == FMV && tc.fn == FNV) {                                                                       \
-    if (col) {                                                                                              \
-      col->add(prob, tc, nblocks, 32 * FMV, 32 * FNV, &grouped_launch<FMV, FNV, Prob>, __PRETTY_FUNCTION__); \
-      return hipSuccess;                                                                                    \
-    }                                                                                                       \
-    hipLaunchKernelGGL((k_gemm_x3<FMV, FNV, Prob>), grid, block, 0, s, prob);                               \
-    return hipGetLastError();                                                                               \
+  const dim3 grid(nblocks);
+#define PNJK_CASE(FMV, FNV, WFM, WFN, WGMV, WGNV)                                                          \
+  if (tc.fm == FMV && tc.fn == FNV) {                                                                      \
+    if (col) {                                                                                             \
+      col->add(prob, tc, nblocks, 32 * FMV, 32 * FNV, &grouped_launch<WFM, WFN, WGMV, WGNV, Prob>,        \
+               __PRETTY_FUNCTION__);                                                                       \
+      return hipSuccess;                                                                                   \
+    }                                                                                                      \
+    hipLaunchKernelGGL((k_gemm_x3<WFM, WFN, WGMV, WGNV, Prob>), grid, dim3(64 * WGMV * WGNV), 0, s, prob); \
+    return hipGetLastError();                                                                              \
   }
-  PNJK_CASE(2, 2)
-  PNJK_CASE(4, 4)
-  PNJK_CASE(5, 4)
-  PNJK_CASE(4, 2)
-  PNJK_CASE(5, 2)
-  PNJK_CASE(2, 4)
+  PNJK_CASE(2, 2, 2, 2, 2, 2)
+  PNJK_CASE(4, 4, 4, 4, 2, 2)
+  PNJK_CASE(5, 4, 5, 4, 2, 2)
+  PNJK_CASE(4, 2, 4, 2, 2, 2)
+  PNJK_CASE(5, 2, 5, 2, 2, 2)
+  PNJK_CASE(2, 4, 2, 4, 2, 2)
+  if constexpr (std::is_same<Prob, ProbHT>::value || std::is_same<Prob, ProbJN>::value) {
+    PNJK_CASE(10, 4, 5, 4, 4, 2)
+  }
 #undef PNJK_CASE
   return hipErrorInvalidValue;
 }

@@ -22,8 +22,8 @@ def test_tiles_bitwise(gpu_required, precision):
     a = autotune.ARITH[precision]
     for st, idx in autotune.STAGES.items():
         for fm, fn in autotune.CANDIDATES[st]:
-            if st == "quarter_transform" and fn == 4 and precision == "fp64":
-                assert L.pnjk_set_tile(eng._h, a, idx, fm, fn) == _lib.EINVAL  # 128-column QT tiles: BF16x3 only
+            if precision == "fp64" and ((st == "quarter_transform" and fn == 4) or (fm, fn) == (10, 4)):
+                assert L.pnjk_set_tile(eng._h, a, idx, fm, fn) == _lib.EINVAL  # BF16x3-only shapes
                 continue
             _lib.check(L.pnjk_set_tile(eng._h, a, idx, fm, fn))
             r = eng.build(C, nbf5, H=H, precision=precision)
@@ -39,6 +39,9 @@ def test_set_tile_rejects(gpu_required):
     assert L.pnjk_set_tile(eng._h, 0, 2, 4, 4) == _lib.EINVAL   # FP64 quarter transform needs fn = 2
     assert L.pnjk_set_tile(eng._h, 1, 2, 4, 4) == _lib.OK       # BF16x3 takes fn = 2 or 4
     assert L.pnjk_set_tile(eng._h, 1, 2, 0, 0) == _lib.OK
+    assert L.pnjk_set_tile(eng._h, 1, 1, 10, 4) == _lib.OK      # eight-wave tile: BF16x3 HT and J^n
+    assert L.pnjk_set_tile(eng._h, 1, 6, 10, 4) == _lib.EINVAL  # not the exchange SYRK
+    assert L.pnjk_set_tile(eng._h, 1, 2, 10, 4) == _lib.EINVAL  # nor the quarter transform
     assert L.pnjk_set_tile(eng._h, 1, 6, 5, 4) == _lib.EINVAL   # exchange SYRK needs square tiles
     assert L.pnjk_set_tile(eng._h, 0, 0, 2, 2) == _lib.EINVAL   # copy_in takes no tile
     assert L.pnjk_set_tile(eng._h, 1, 1, 3, 3) == _lib.EINVAL   # unknown shape
