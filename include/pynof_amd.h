@@ -91,6 +91,20 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
                   int precision, uint32_t outputs, double* J, double* K, double* J_MO, double* K_MO,
                   double* H_core, void* stream);
 
+/* Orbital-gradient MO integrals of the RI path: replaces JKH_MO_tmp with
+ * p.RI (integrals.py:665-682), i.e. Integrals_MO_RI_CPU / _GPU
+ * (integrals.py:703-716), which computeLagrange2 (utils.py:262-265) calls for
+ * every orbital gradient:
+ *   H_mat[i][j]   = sum_mn C[m][i] H[m][n] C[n][j]       i < nbf, j < nbf5
+ *   b_MO[p][q][P] = sum_mn C[m][p] C[n][q] b[m][n][P]    p < nbf, q < nbf5
+ * b_MO is C-order (nbf, nbf5, naux), H_mat (nbf, nbf5). C is (nbf, ldc >= nbf),
+ * row-major; every column is used. precision: PNJK_FP64 or PNJK_SPLIT (BF16x3,
+ * FP32 accumulation; b_MO then carries ~1e-5 relative error). Either output may
+ * be NULL (not computed); H_mat needs H (ldh >= nbf). Outputs may be host or
+ * device memory; with a host output the call returns after the copy back. */
+int pnjk_mo_integrals(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, const double* H, int64_t ldh,
+                      int precision, double* b_MO, double* H_mat, void* stream);
+
 /* Grouped builds: many engines (one b each, different sizes allowed) built in
  * one pass. Each GEMM stage of all instances runs as one persistent grouped
  * launch per problem type and tile shape (tile shapes per instance from the

@@ -11,7 +11,8 @@ Pinning (see tests/test_oracle.py): every function below is checked against
 I = b b^T through the reference's ``JKj_Full``/``JKH_MO_Full`` formulas,
 (3) closed forms at C = identity and naux = 1, (4) invariants (symmetry,
 diag K_MO = diag J_MO, J_MO PSD, contraction of J^n/K^n with D_p reproducing
-J_MO/K_MO, orbital invariance of sum_q J^q and sum_q K^q), and (5) the
+J_MO/K_MO, orbital invariance of sum_q J^q and sum_q K^q, the MO integrals'
+nbf5 x nbf5 block reproducing J_MO, K_MO and H_core), and (5) the
 committed golden fixtures of the water-sized workload. The reference ships
 no tests or stored outputs, so no reference-held vector exists to pin against.
 """
@@ -148,6 +149,51 @@ def jkh_mo_loops(C, H, b_mnl, nbf5):
                 J_MO[p, q] += bpq[p, p, l] * bpq[q, q, l]
                 K_MO[p, q] += bpq[p, q, l] * bpq[p, q, l]
     return J_MO, K_MO, H_core
+
+
+def integrals_mo_ri(C: np.ndarray, H: np.ndarray, b_mnl: np.ndarray, nbf5: int):
+    """(H_mat, b_MO) of the orbital gradient, Integrals_MO_RI_CPU (integrals.py:703-707),
+    reached through JKH_MO_tmp (:665-682) from computeLagrange2 (utils.py:262-265):
+
+    H_mat[i,j]   = sum_mn C[m,i] H[m,n] C[n,j]         i < nbf, j < nbf5   (:705)
+    b_MO[p,q,P]  = sum_mn C[m,p] C[n,q] b[m,n,P]       p < nbf, q < nbf5   (:706)
+    """
+    Cq = C[:, 0:nbf5]
+    H_mat = np.einsum("mi,mn,nj->ij", C, H, Cq, optimize=True)
+    b_MO = np.einsum("mp,nq,mnl->pql", C, Cq, b_mnl, optimize=True)
+    return H_mat, b_MO
+
+
+def integrals_mo_loops(C, H, b_mnl, nbf5):
+    """The same (H_mat, b_MO) by explicit loops over every index (tiny inputs only):
+    an independent route that pins integrals_mo_ri."""
+    nbf, naux = b_mnl.shape[0], b_mnl.shape[2]
+    H_mat = np.zeros((nbf, nbf5))
+    b_MO = np.zeros((nbf, nbf5, naux))
+    for i in range(nbf):
+        for j in range(nbf5):
+            s = 0.0
+            for m in range(nbf):
+                for n in range(nbf):
+                    s += C[m, i] * H[m, n] * C[n, j]
+            H_mat[i, j] = s
+            for P in range(naux):
+                t = 0.0
+                for m in range(nbf):
+                    for n in range(nbf):
+                        t += C[m, i] * C[n, j] * b_mnl[m, n, P]
+                b_MO[i, j, P] = t
+    return H_mat, b_MO
+
+
+def integrals_mo_full(C, H, I, nbf5):
+    """Integrals_MO_Full_CPU (integrals.py:685-690): H_mat and the 4-index MO integrals
+    I_MO[p,q,r,t] = sum C[m,p] C[n,q] I[m,n,s,l] C[s,r] C[l,t] (q, r, t < nbf5). With
+    I = b b^T this equals sum_P b_MO[p,q,P] b_MO[r,t,P]."""
+    Cq = C[:, 0:nbf5]
+    H_mat = np.einsum("mi,mn,nj->ij", C, H, Cq, optimize=True)
+    I_MO = np.einsum("mp,nq,mnsl,sr,lt->pqrt", C, Cq, I, Cq, Cq, optimize=True)
+    return H_mat, I_MO
 
 
 def energy(n, J_MO, K_MO, H_core):

@@ -36,6 +36,9 @@ _SIGS = {
                                      ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_uint32,
                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_void_p]),
+    "pnjk_mo_integrals": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                         ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p]),
     "pnjk_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "pnjk_stage_count": (ctypes.c_int, []),
     "pnjk_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),

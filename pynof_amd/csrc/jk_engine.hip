@@ -128,15 +128,16 @@ struct pnjk_engine {
   DevBuf bt16h, bt16l, bht16h, bht16l;  // split planes of b, built on the first split build
   DevBuf Cp, Hp, X, d, kpart, T, outs, stage, scal;
   DevBuf ct16h, ct16l, x16h, x16l, d16h, d16l;  // split workspace
+  DevBuf Cf, ctf16h, ctf16l, hc;                 // MO integrals: all columns of C; H_core scratch
   bool split_ready = false;
   // Tile override per arithmetic (0: FP64 kernels, 1: BF16x3 kernels) and stage;
   // {0, 0} selects the built-in choice.
   TileF64 tiles[2][ST_COUNT] = {};
-  DevBuf* all[22];
+  DevBuf* all[26];
   int nall = 0;
   pnjk_engine() {
     for (DevBuf* b : {&btri, &bt16h, &bt16l, &bht16h, &bht16l, &Cp, &Hp, &X, &d, &kpart, &T, &outs, &stage, &scal,
-                      &ct16h, &ct16l, &x16h, &x16l, &d16h, &d16l})
+                      &ct16h, &ct16l, &x16h, &x16l, &d16h, &d16l, &Cf, &ctf16h, &ctf16l, &hc})
       all[nall++] = b;
   }
 };
@@ -613,6 +614,83 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
   mark(ST_COUNT);
   if (e->prof) e->prof_recorded = true;
   if (sync) PNJK_HIP_CHECK(hipStreamSynchronize(s));
+  return PNJK_OK;
+}
+
+int pnjk_mo_integrals(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, const double* H, int64_t ldh,
+                      int precision, double* b_MO, double* H_mat, void* stream) {
+  if (!e) return set_error(PNJK_EINVAL, "pnjk_mo_integrals: engine is NULL");
+  std::lock_guard<std::mutex> lk(e->mu);
+  PNJK_HIP_CHECK(hipSetDevice(e->device));
+  hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
+  if (e->nbf == 0) return set_error(PNJK_ESTATE, "pnjk_mo_integrals: no b uploaded");
+  if (!C) return set_error(PNJK_EINVAL, "pnjk_mo_integrals: C is NULL");
+  if (ldc < e->nbf) return set_error(PNJK_EINVAL, "pnjk_mo_integrals: ldc < nbf (every column of C is used)");
+  if (precision != PNJK_FP64 && precision != PNJK_SPLIT)
+    return set_error(PNJK_EUNSUPPORTED, "pnjk_mo_integrals: precision must be PNJK_FP64 or PNJK_SPLIT");
+  if (!b_MO && !H_mat) return set_error(PNJK_EINVAL, "pnjk_mo_integrals: neither b_MO nor H_mat requested");
+  if (H_mat && (!H || ldh < e->nbf))
+    return set_error(PNJK_EINVAL, "pnjk_mo_integrals: H_mat needs H with ldh >= nbf");
+  BuildArgs a{C, ldc, nbf5, H, ldh, precision, 0u, nullptr, nullptr, nullptr, nullptr, nullptr};
+  BuildState st;
+  int rc;
+  if ((rc = build_prepare(e, a, st))) return rc;
+  const BuildGeom& g = st.g;
+  const int64_t nbf = g.nbf, n5 = g.nbf5, naux = g.naux;
+  // Full-width C, k-major: Cf[nu][p] for nu < nbf, p < nbf; zero rows to nbf_p + 8 and
+  // columns to ldcf, which covers every M-tile (at most 160 rows) past nbf.
+  const int ldcf = (int)roundup(nbf, 32) + 160;
+  const int64_t cf_elems = (int64_t)(g.nbf_p + 8) * ldcf;
+  if ((rc = e->Cf.ensure((size_t)cf_elems * 8))) return rc;
+  const bool host_b = b_MO && !is_device_ptr(b_MO), host_h = H_mat && !is_device_ptr(H_mat);
+  const int64_t b_elems = nbf * n5 * naux, h_elems = nbf * n5;
+  if ((host_b || host_h) &&
+      (rc = e->outs.ensure((size_t)((host_b ? b_elems : 0) + (host_h ? h_elems : 0)) * 8)))
+    return rc;
+  double* dB = host_b ? e->outs.as<double>() : b_MO;
+  double* dHm = host_h ? e->outs.as<double>() + (host_b ? b_elems : 0) : H_mat;
+  if (H_mat) {  // T = H C[:, :nbf5] (through the H_core stage), then H_mat = Cf^T T
+    st.want_h = true;
+    if ((rc = e->Hp.ensure((size_t)(nbf * g.nbf_p) * 8)) || (rc = e->T.ensure((size_t)(nbf * g.ldcp + 64) * 8)) ||
+        (rc = e->hc.ensure((size_t)n5 * 8)))
+      return rc;
+    st.dH = e->hc.as<double>();
+    st.hH = false;
+  }
+
+  if ((rc = stage_in(e, st, s))) return rc;
+  double* Cf = e->Cf.as<double>();
+  PNJK_HIP_CHECK(fill(Cf, cf_elems, 0.0, s));
+  if (st.c_dev) {
+    const int64_t n = nbf * nbf;
+    hipLaunchKernelGGL(k_copy2d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, C, ldc, Cf, (int64_t)ldcf,
+                       (int)nbf, (int)nbf);
+    PNJK_HIP_CHECK(hipGetLastError());
+  } else {
+    PNJK_HIP_CHECK(hipMemcpy2DAsync(Cf, (size_t)ldcf * 8, C, (size_t)ldc * 8, (size_t)nbf * 8, (size_t)nbf,
+                                    hipMemcpyHostToDevice, s));
+  }
+  if (b_MO) {
+    if ((rc = stage_ht(e, st, s, nullptr))) return rc;
+    const bool big = nbf >= 256;
+    if (precision == PNJK_SPLIT) {
+      const size_t ct_bytes = (size_t)(nbf * st.sg.ldk) * 2;
+      if ((rc = e->ctf16h.ensure(ct_bytes)) || (rc = e->ctf16l.ensure(ct_bytes))) return rc;
+      PNJK_HIP_CHECK(launch_split_ct(Cf, ldcf, (int)nbf, (int)nbf, (int)nbf, st.sg.ldk, e->ctf16h.as<uint16_t>(),
+                                     e->ctf16l.as<uint16_t>(), s));
+      PNJK_HIP_CHECK(launch_mo_x3(st.sg, (int)naux, big ? TileF64{5, 4} : TileF64{2, 2}, e->ctf16h.as<uint16_t>(),
+                                  e->ctf16l.as<uint16_t>(), e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), dB, s));
+    } else {
+      PNJK_HIP_CHECK(launch_mo_f64(g, big ? TileF64{5, 2} : TileF64{2, 2}, Cf, ldcf, e->X.as<double>(), dB, s));
+    }
+  }
+  if (H_mat) {
+    if ((rc = stage_hcore(e, st, s))) return rc;
+    PNJK_HIP_CHECK(launch_hmat_f64(g, Cf, ldcf, e->T.as<double>(), dHm, s));
+  }
+  if (host_b) PNJK_HIP_CHECK(hipMemcpyAsync(b_MO, dB, (size_t)b_elems * 8, hipMemcpyDeviceToHost, s));
+  if (host_h) PNJK_HIP_CHECK(hipMemcpyAsync(H_mat, dHm, (size_t)h_elems * 8, hipMemcpyDeviceToHost, s));
+  if (host_b || host_h) PNJK_HIP_CHECK(hipStreamSynchronize(s));
   return PNJK_OK;
 }
 

@@ -499,6 +499,81 @@ struct ProbGemmT {  // T[mu][i] = sum_nu Hp[mu][nu] Cp[nu][i]
   }
 };
 
+// Orbital-gradient MO integrals (integrals.py:703-716, Integrals_MO_RI_*): batch q,
+//   b_MO[p][q][P] = sum_nu Cf[nu][p] X[q][nu][P]   for p < nbf, q < nbf5, P < naux,
+// i.e. the quarter transform over every row p instead of q <= p. Cf holds all nbf
+// columns of C (k-major, zero rows past nbf, zero columns past nbf).
+struct ProbMO {
+  static constexpr int kA = KM, kB = KM;
+  const double* Cf;
+  const double* X;
+  double* out;
+  int nbf, naux, naux_p, nbf5, ldcf, nbf_p, mtiles, ntiles;
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.mt = id % mtiles;
+    const int rest = id / mtiles;
+    t.nt = rest % ntiles;
+    t.batch = rest / ntiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf; }
+  __device__ int ksteps(int BK) const { return nbf_p / BK; }
+  __device__ SrcKMDense a_src(const Tile&) const { return {Cf, ldcf}; }
+  __device__ SrcKMRows b_src(const Tile& t) const { return {X + (int64_t)t.batch * nbf * naux_p, naux_p, nbf}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4d (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int m = t.m0 + dm, n = t.n0 + dn;
+          if (m < nbf && n < naux) gmem(out)[((int64_t)m * nbf5 + t.batch) * naux + n] = acc[i][j][r];
+        }
+  }
+};
+
+// H_mat[i][j] = sum_mu Cf[mu][i] T[mu][j] = (C^T H C)[i][j] for i < nbf, j < nbf5
+// (integrals.py:705), with T = H C[:, :nbf5] from ProbGemmT.
+struct ProbHM {
+  static constexpr int kA = KM, kB = KM;
+  const double* Cf;
+  const double* T;
+  double* out;
+  int nbf, nbf5, ldcf, ldcp, nbf_p, mtiles;
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.batch = 0;
+    t.mt = id % mtiles;
+    t.nt = id / mtiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf; }
+  __device__ int ksteps(int BK) const { return nbf_p / BK; }
+  __device__ SrcKMDense a_src(const Tile&) const { return {Cf, ldcf}; }
+  __device__ SrcKMRows b_src(const Tile&) const { return {T, ldcp, nbf}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4d (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int m = t.m0 + dm, n = t.n0 + dn;
+          if (m < nbf && n < nbf5) gmem(out)[(int64_t)m * nbf5 + n] = acc[i][j][r];
+        }
+  }
+};
+
 // ---------------------------------------------------------------- small kernels
 __global__ void k_kmo_reduce(int nbf5, int nt, const double* __restrict__ kpart, double* __restrict__ kmo) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -713,6 +788,42 @@ hipError_t launch_jn_f64(const BuildGeom& g, TileF64 tc, const double* d, const 
   pr.ntri = g.ntri;
   pr.mtiles = cdiv(g.nbf5, BM);
   return launch_prob(pr, tc, pr.mtiles * cdiv(g.ntri, BN), s, col);
+}
+
+hipError_t launch_mo_f64(const BuildGeom& g, TileF64 tc, const double* Cf, int ldcf, const double* X, double* b_mo,
+                         hipStream_t s) {
+  const int BM = 32 * tc.fm, BN = 32 * tc.fn;
+  if (g.naux_p % BN != 0 || ldcf < cdiv(g.nbf, BM) * BM) return hipErrorInvalidValue;
+  ProbMO pr;
+  pr.Cf = Cf;
+  pr.X = X;
+  pr.out = b_mo;
+  pr.nbf = g.nbf;
+  pr.naux = g.naux;
+  pr.naux_p = g.naux_p;
+  pr.nbf5 = g.nbf5;
+  pr.ldcf = ldcf;
+  pr.nbf_p = g.nbf_p;
+  pr.mtiles = cdiv(g.nbf, BM);
+  pr.ntiles = g.naux_p / BN;
+  return launch_prob(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s);
+}
+
+hipError_t launch_hmat_f64(const BuildGeom& g, const double* Cf, int ldcf, const double* T, double* hmat,
+                           hipStream_t s) {
+  const TileF64 tc{2, 2};  // 64 x 64 tiles: BN spans at most ldcp + 32 columns of T (slack)
+  if (ldcf < cdiv(g.nbf, 64) * 64) return hipErrorInvalidValue;
+  ProbHM pr;
+  pr.Cf = Cf;
+  pr.T = T;
+  pr.out = hmat;
+  pr.nbf = g.nbf;
+  pr.nbf5 = g.nbf5;
+  pr.ldcf = ldcf;
+  pr.ldcp = g.ldcp;
+  pr.nbf_p = g.nbf_p;
+  pr.mtiles = cdiv(g.nbf, 64);
+  return launch_prob(pr, tc, pr.mtiles * cdiv(g.nbf5, 64), s);
 }
 
 hipError_t launch_hcore_f64(const BuildGeom& g, const double* Hp, int ldh, const double* Cp, double* T,

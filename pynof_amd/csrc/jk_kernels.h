@@ -76,6 +76,12 @@ hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const
                          double* d, int* ntiles, hipStream_t s, GroupCollector* col = nullptr);
 
 // K_MO[p][q] = K_MO[q][p] = sum_t kpart[(p*nbf5+q)*nt + t]  (q <= p)
+// Orbital-gradient MO integrals: b_MO[p][q][P] (p < nbf, q < nbf5, P < naux) from X
+// and the full-width C (Cf, ldcf >= nbf rounded up to the tile rows); H_mat = Cf^T T.
+hipError_t launch_mo_f64(const BuildGeom& g, TileF64 tc, const double* Cf, int ldcf, const double* X, double* b_mo,
+                         hipStream_t s);
+hipError_t launch_hmat_f64(const BuildGeom& g, const double* Cf, int ldcf, const double* T, double* hmat,
+                           hipStream_t s);
 hipError_t launch_kmo_reduce(int nbf5, int nt, const double* kpart, double* kmo, hipStream_t s);
 
 // O[b][i][j] = sum_k A[b][i][k] A[b][j][k] for i, j < M on the lower triangle of
@@ -124,6 +130,10 @@ hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
                         const uint16_t* bht_hi, const uint16_t* bht_lo, uint16_t* x_hi, uint16_t* x_lo,
                         hipStream_t s, GroupCollector* col = nullptr);
 // Quarter transform from X16: K_MO row-sum partials, d (FP64 and split planes).
+// BF16x3 orbital-gradient MO integrals from X16 and the split planes of C^T over
+// all nbf columns (rows clamped at nbf); FP64 output b_MO[p][q][P].
+hipError_t launch_mo_x3(const SplitGeom& g, int naux, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
+                        const uint16_t* x_hi, const uint16_t* x_lo, double* b_mo, hipStream_t s);
 hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
                         const uint16_t* x_hi, const uint16_t* x_lo, double* kpart, double* d, uint16_t* d_hi,
                         uint16_t* d_lo, int* ntiles, hipStream_t s, GroupCollector* col = nullptr);

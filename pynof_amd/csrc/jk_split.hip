@@ -418,6 +418,42 @@ struct ProbQT {
   }
 };
 
+// Orbital-gradient MO integrals in BF16x3 (integrals.py:703-716): batch q,
+//   b_MO[p][q][P] = sum_nu Ctf16[p][nu] X16[q][nu][P]   for p < nbf, q < nbf5,
+// with Ctf16 the split planes of C^T over all nbf columns; FP64 output.
+struct ProbMO {
+  static constexpr int kA = RK, kB = KR;
+  Planes ct, x;
+  double* out;
+  int nbf, naux, naux_p, nbf5, ldk, mtiles, ntiles;
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.mt = id % mtiles;
+    const int rest = id / mtiles;
+    t.nt = rest % ntiles;
+    t.batch = rest / ntiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ int ksteps() const { return ldk / kBK; }
+  __device__ SrcRows a_src(const Tile&) const { return {ct, 0, ldk, nbf}; }
+  __device__ SrcXT b_src(const Tile& t) const { return {x, (int64_t)t.batch * nbf * naux_p, naux_p, nbf}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int m = t.m0 + dm, n = t.n0 + dn;
+          if (m < nbf && n < naux) gmem(out)[((int64_t)m * nbf5 + t.batch) * naux + n] = acc[i][j][r];
+        }
+  }
+};
+
 // J[q][mu][nu] = sum_P d16[q][P] btri16[t][P] for mu >= nu (FP64 output; the
 // upper triangle is filled afterwards by launch_symmetrize)
 struct ProbJN {
@@ -657,6 +693,25 @@ hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
   pr.nt64 = g.naux_p / 64;
   *ntiles = pr.nt64;
   return launch_x3(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s, col);
+}
+
+hipError_t launch_mo_x3(const SplitGeom& g, int naux, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
+                        const uint16_t* x_hi, const uint16_t* x_lo, double* b_mo, hipStream_t s) {
+  // A 128-column tile may read up to 64 columns past naux_p (X16 slack) and drops them.
+  const int BM = 32 * tc.fm, BN = 32 * tc.fn;
+  if (tc.fn != 2 && tc.fn != 4) return hipErrorInvalidValue;
+  ProbMO pr;
+  pr.ct = {ct_hi, ct_lo};
+  pr.x = {x_hi, x_lo};
+  pr.out = b_mo;
+  pr.nbf = g.nbf;
+  pr.naux = naux;
+  pr.naux_p = g.naux_p;
+  pr.nbf5 = g.nbf5;
+  pr.ldk = g.ldk;
+  pr.mtiles = cdiv(g.nbf, BM);
+  pr.ntiles = cdiv(g.naux_p, BN);
+  return launch_x3(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s);
 }
 
 hipError_t launch_jn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* d_hi, const uint16_t* d_lo,

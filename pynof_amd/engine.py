@@ -159,3 +159,53 @@ class JKEngine:
                                             _ptr(Hi) if Hi is not None else None, self.nbf if Hi is not None else 0,
                                             prec, mask, *ptrs, stream))
         return res
+
+    def mo_integrals(self, C, nbf5: int, H=None, precision: str = "fp64", outputs=("b_MO", "H_mat"), out=None,
+                     stream=None):
+        """Orbital-gradient MO integrals (JKH_MO_tmp with RI, integrals.py:665-716):
+        b_MO (nbf, nbf5, naux) = C^T b C[:, :nbf5] per aux function and H_mat
+        (nbf, nbf5) = C^T H C[:, :nbf5]. C: (nbf, nbf), every column used.
+        precision: "fp64" or "split". Inputs and outputs follow `build`."""
+        if self.nbf == 0:
+            raise _lib.PnjkError(_lib.ESTATE, "no b uploaded")
+        prec = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        nbf5 = int(nbf5)
+        Ci = _as_input(C, "C")
+        if Ci.ndim != 2 or tuple(Ci.shape) != (self.nbf, self.nbf):
+            raise ValueError(f"C must have shape ({self.nbf}, {self.nbf}); got {tuple(Ci.shape)}")
+        want = [n for n in outputs]
+        for n in want:
+            if n not in ("b_MO", "H_mat"):
+                raise ValueError(f"unknown output {n!r}")
+        Hi = None
+        if "H_mat" in want:
+            if H is None:
+                raise ValueError("H_mat requested but H is None")
+            Hi = _as_input(H, "H")
+            if tuple(Hi.shape) != (self.nbf, self.nbf):
+                raise ValueError(f"H must have shape ({self.nbf}, {self.nbf}); got {tuple(Hi.shape)}")
+        on_device = _is_torch(Ci) and Ci.is_cuda
+        shapes = {"b_MO": (self.nbf, nbf5, self.naux), "H_mat": (self.nbf, nbf5)}
+        res = {}
+        for name in want:
+            if out is not None and name in out:
+                a = out[name]
+                if tuple(a.shape) != shapes[name]:
+                    raise ValueError(f"out[{name}] must have shape {shapes[name]}")
+                if _is_torch(a):
+                    if a.dtype != _torch().float64 or not a.is_contiguous():
+                        raise ValueError(f"out[{name}] must be contiguous float64")
+                elif a.dtype != np.float64 or not a.flags.c_contiguous:
+                    raise ValueError(f"out[{name}] must be C-contiguous float64")
+                res[name] = a
+            elif on_device:
+                res[name] = _torch().empty(shapes[name], dtype=_torch().float64, device=Ci.device)
+            else:
+                res[name] = np.empty(shapes[name], dtype=np.float64)
+        if stream is None and on_device:
+            stream = _torch().cuda.current_stream(Ci.device).cuda_stream
+        _lib.check(_lib.lib().pnjk_mo_integrals(
+            self._h, _ptr(Ci), Ci.shape[1], nbf5, _ptr(Hi) if Hi is not None else None,
+            self.nbf if Hi is not None else 0, prec, _ptr(res["b_MO"]) if "b_MO" in res else None,
+            _ptr(res["H_mat"]) if "H_mat" in res else None, stream))
+        return res

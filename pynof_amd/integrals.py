@@ -1,9 +1,11 @@
 """Drop-in replacements for PyNOF's RI JK builds.
 
-``computeJKj(C, I, b_mnl, p)`` and ``computeJKH_MO(C, H, I, b_mnl, p)`` keep the
-reference's signatures (src/pynof/integrals.py:172 and :307), argument meaning
-and return values: host FP64 arrays J, K of shape (nbf5, nbf, nbf), and J_MO,
-K_MO (nbf5, nbf5) with H_core (nbf5,). ``I`` is ignored (it is ``[]`` in RI mode,
+``computeJKj(C, I, b_mnl, p)``, ``computeJKH_MO(C, H, I, b_mnl, p)`` and
+``JKH_MO_tmp(C, H, I, b_mnl, p)`` keep the reference's signatures
+(src/pynof/integrals.py:172, :307 and :665), argument meaning and return values:
+host FP64 arrays J, K of shape (nbf5, nbf, nbf); J_MO, K_MO (nbf5, nbf5) with
+H_core (nbf5,); and the orbital-gradient pair H_mat (nbf, nbf5), b_MO
+(nbf, nbf5, naux). ``I`` is ignored (it is ``[]`` in RI mode,
 integrals.py:28-33). Both run on the MI355X through the HIP library; there is no
 CPU path. ``p`` must provide ``nbf5``; ``p.RI`` must be true. ``p.precision``
 (optional, default "fp64") selects "fp64", "split" or "split_refine".
@@ -62,6 +64,19 @@ def computeJKH_MO(C, H, I, b_mnl, p):
     r = eng.build(C, p.nbf5, H=H, precision=getattr(p, "precision", "fp64"),
                   outputs=("J_MO", "K_MO", "H_core"))
     return r["J_MO"], r["K_MO"], r["H_core"]
+
+
+def JKH_MO_tmp(C, H, I, b_mnl, p):
+    """(H_mat, b_MO) for the orbital gradient (integrals.py:665-682 with p.RI, i.e.
+    Integrals_MO_RI_CPU/_GPU at :703-716): H_mat = C^T H C[:, :nbf5] (nbf, nbf5) and
+    b_MO[p, q, P] = sum_mn C[m, p] C[n, q] b[m, n, P] (nbf, nbf5, naux). Host FP64
+    arrays, as the reference returns on its CPU path. ``p.precision`` "split"
+    gives BF16x3 b_MO; "fp64" and "split_refine" give FP64."""
+    _check_p(p)
+    eng = _engine_for(b_mnl, p)
+    prec = "split" if getattr(p, "precision", "fp64") == "split" else "fp64"
+    r = eng.mo_integrals(C, p.nbf5, H=H, precision=prec)
+    return r["H_mat"], r["b_MO"]
 
 
 def release_cached_engines() -> None:

@@ -159,3 +159,63 @@ def test_planted_mistakes_fail():
     Jw, _ = O.jkj_ri(C, _mutant_dropped_sqrt2(b), nbf5)
     Jf, _ = O.jkj_full(C, I4, nbf5)
     assert rel(Jw, Jf) > 0.5
+
+
+# ---- orbital-gradient MO integrals (Integrals_MO_RI_CPU, integrals.py:703-707)
+
+@pytest.mark.parametrize("nbf,naux,nbf5", [(5, 7, 3), (4, 3, 4), (6, 2, 1)])
+def test_mo_integrals_loops(nbf, naux, nbf5):
+    b, C, H, _ = tiny(nbf, naux, nbf5, seed=71)
+    Hm, bm = O.integrals_mo_ri(C, H, b, nbf5)
+    Hl, bl = O.integrals_mo_loops(C, H, b, nbf5)
+    assert Hm.shape == (nbf, nbf5) and bm.shape == (nbf, nbf5, naux)
+    assert rel(Hm, Hl) < 1e-13 and rel(bm, bl) < 1e-13
+
+
+def test_mo_integrals_four_index_route():
+    nbf, naux, nbf5 = 6, 8, 4
+    b, C, H, _ = tiny(nbf, naux, nbf5, seed=72)
+    Hm, bm = O.integrals_mo_ri(C, H, b, nbf5)
+    Hf, I_MO = O.integrals_mo_full(C, H, O.four_index(b), nbf5)
+    assert rel(Hm, Hf) < 1e-13
+    # I_MO[p,q,r,t] = sum_P b_MO[p,q,P] b_MO[r,t,P] for r, t < nbf5
+    assert rel(np.einsum("pqP,rtP->pqrt", bm, bm[:nbf5]), I_MO) < 1e-12
+
+
+def test_mo_integrals_permutation_closed_form():
+    nbf, naux, nbf5 = 6, 5, 4
+    b, _, H, _ = tiny(nbf, naux, nbf5, seed=73)
+    perm = np.roll(np.arange(nbf), 3)  # column p of C is e_perm[p]
+    C = np.zeros((nbf, nbf))
+    C[perm, np.arange(nbf)] = 1.0
+    Hm, bm = O.integrals_mo_ri(C, H, b, nbf5)
+    for p in range(nbf):
+        for q in range(nbf5):
+            assert np.array_equal(bm[p, q], b[perm[p], perm[q]])
+            assert Hm[p, q] == H[perm[p], perm[q]]
+
+
+def test_mo_integrals_reproduce_build_outputs():
+    """The nbf5 x nbf5 block of b_MO is Y of the JK build: J_MO = sum_P b_pp b_qq,
+    K_MO = sum_P b_pq^2, H_core = diag H_mat; that block is symmetric in (p, q)."""
+    b, C, H, nbf5 = tiny(7, 9, 5, seed=74)
+    Hm, bm = O.integrals_mo_ri(C, H, b, nbf5)
+    JM, KM, Hc = O.jkh_mo_loops(C, H, b, nbf5)
+    blk = bm[:nbf5]
+    dg = np.einsum("ppP->pP", blk)
+    assert rel(dg @ dg.T, JM) < 1e-13
+    assert rel(np.einsum("pqP,pqP->pq", blk, blk), KM) < 1e-13
+    assert rel(np.diag(Hm[:nbf5]), Hc) < 1e-13
+    assert rel(blk, blk.transpose(1, 0, 2)) < 1e-13
+
+
+def test_mo_integrals_planted_mistakes_fail():
+    b, C, H, nbf5 = tiny(5, 6, 3, seed=75)
+    Hl, bl = O.integrals_mo_loops(C, H, b, nbf5)
+    Cq = C[:, :nbf5]
+    # C instead of C^T on the first index
+    wrong = np.einsum("pm,nq,mnl->pql", C, Cq, b)
+    assert rel(wrong, bl) > 1e-3
+    # C transposed on the right of H_mat: C[j, n] instead of C[n, j]
+    wrong_h = np.einsum("mi,mn,jn->ij", C, H, C[:nbf5, :])
+    assert rel(wrong_h, Hl) > 1e-3
