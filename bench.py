@@ -257,6 +257,34 @@ def main():
                      "traffic": tr64["bytes"] if tr64 else None}
     roofline_fp64["frac"] = round(roofline_fp64["achieved"] / PEAK_FP64, 4)
 
+    # ---- orbital-gradient MO integrals (JKH_MO_tmp): HT + full-row quarter transform + H_mat
+    mo = {}
+    mo_flops = 2.0 * nbf5 * nbf * nbf * naux + 2.0 * nbf * nbf5 * nbf * naux + 2.0 * nbf * nbf * nbf5 * 2
+    mo_out = {"b_MO": torch.empty((nbf, nbf5, naux), dtype=torch.float64, device=dev),
+              "H_mat": torch.empty((nbf, nbf5), dtype=torch.float64, device=dev)}
+    mo_ref = None
+    for form in ("fp64", "split"):
+        log(f"MO integrals {form}")
+        eng.mo_integrals(Cd[0], nbf5, H=Hd, precision=form, out=mo_out, stream=stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        reps = max(1, args.form_steps)
+        for _ in range(reps):
+            eng.mo_integrals(Cd[0], nbf5, H=Hd, precision=form, out=mo_out, stream=stream)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        mo[form] = {"ms": round(ms, 3), "useful_tflops": round(mo_flops / (ms * 1e-3) / 1e12, 2)}
+        if form == "fp64":
+            mo_ref = mo_out["b_MO"].clone()
+        else:
+            mo[form]["rel_err_b_MO_vs_fp64"] = float(torch.linalg.norm(mo_out["b_MO"] - mo_ref) /
+                                                     torch.linalg.norm(mo_ref))
+    mo["flops"] = mo_flops
+    mo["speedup_split_vs_fp64"] = round(mo["fp64"]["ms"] / mo["split"]["ms"], 3)
+    del mo_out, mo_ref
+    torch.cuda.empty_cache()
+
     line = {
         "metric": METRIC,
         "value": round(1e3 / ms_per_step, 4),
@@ -282,6 +310,7 @@ def main():
         "fp64_build_tflops": round(total_flops * forms["fp64"]["builds_per_s"] / 1e12, 3),
         "roofline": roofline,
         "roofline_fp64": roofline_fp64,
+        "mo_integrals": mo,
     }
     eng.close()
     del out, Cd, Hd

@@ -197,7 +197,7 @@ uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int preci
   uint64_t s = 0;
   s += (uint64_t)((nbf_p + 8) * ldcp + 1024) * 8;           // Cp
   s += (uint64_t)(nbf * nbf_p) * 8;                          // Hp
-  s += (uint64_t)(nbf5 * nbf * naux_p) * 8;                  // X
+  s += (uint64_t)(nbf5 * nbf * naux_p + 64) * 8;             // X (+ slack)
   s += (uint64_t)(roundup(nbf5, 160) * naux_p) * 8;          // d
   s += (uint64_t)(nbf5 * nbf5 * (naux_p / 64)) * 8;          // kpart
   s += (uint64_t)(nbf * ldcp) * 8;                           // T
@@ -375,7 +375,8 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st) {
   int rc;
   const int64_t cp_elems = (int64_t)(g.nbf_p + 8) * g.ldcp + 1024;
   if ((rc = e->Cp.ensure((size_t)cp_elems * 8))) return rc;
-  if (precision != PNJK_SPLIT && (rc = e->X.ensure((size_t)(n5 * nbf * g.naux_p) * 8))) return rc;
+  // X slack: a 128-column quarter-transform tile reads up to 64 columns past naux_p.
+  if (precision != PNJK_SPLIT && (rc = e->X.ensure((size_t)(n5 * nbf * g.naux_p + 64) * 8))) return rc;
   if ((rc = e->d.ensure((size_t)(roundup(n5, 160) * g.naux_p) * 8))) return rc;
   if ((rc = e->kpart.ensure((size_t)(n5 * n5 * (g.naux_p / 64)) * 8))) return rc;
 
@@ -927,8 +928,8 @@ int pnjk_set_tile(pnjk_engine* e, int arith, int stage, int fm, int fn) {
   // The eight-wave 320 x 128 tile exists for the BF16x3 half-transform and J^n only.
   if (arith == 1 && (stage == ST_HT || stage == ST_JN) && fm == 10 && fn == 4) known = true;
   if (!reset && !known) return set_error(PNJK_EINVAL, "pnjk_set_tile: unsupported tile shape");
-  if (!reset && stage == ST_QT && fn != 2 && !(arith == 1 && fn == 4))
-    return set_error(PNJK_EINVAL, "pnjk_set_tile: the quarter transform needs fn = 2 (FP64) or fn = 2, 4 (BF16x3)");
+  if (!reset && stage == ST_QT && fn != 2 && fn != 4)
+    return set_error(PNJK_EINVAL, "pnjk_set_tile: the quarter transform needs fn = 2 or 4 (64 or 128 aux columns)");
   if (!reset && stage == ST_K && fm != fn)
     return set_error(PNJK_EINVAL, "pnjk_set_tile: the exchange SYRK needs square tiles (fm = fn)");
   std::lock_guard<std::mutex> lk(e->mu);

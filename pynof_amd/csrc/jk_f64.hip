@@ -312,7 +312,7 @@ struct ProbQT {
   const double* X;
   double* kpart;
   double* d;
-  int nbf, naux_p, nbf5, ldcp, nbf_p, mtiles, ntiles;
+  int nbf, naux_p, nbf5, ldcp, nbf_p, mtiles, ntiles, nt64;  // nt64 = naux_p / 64 (kpart stride)
   __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.mt = id % mtiles;
     const int rest = id / mtiles;
@@ -342,41 +342,64 @@ struct ProbQT {
           for (int r = 0; r < 4; ++r) {
             int dm, dn;
             elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
-            if (t.m0 + dm == p) gmem(d)[(int64_t)p * naux_p + t.n0 + dn] = acc[i][j][r];
+            if (t.m0 + dm == p && t.n0 + dn < naux_p) gmem(d)[(int64_t)p * naux_p + t.n0 + dn] = acc[i][j][r];
           }
     }
-    // Row sums of squares over this P-tile, reduced over the 16 column lanes,
-    // then over the two column waves through LDS (fixed order: deterministic).
+    // Row sums of squares per 64 aux columns: a 64-column partial is always
+    // S(first 32 columns) + S(last 32 columns), S being one wave's sum over two
+    // 16-column blocks reduced across the 16 column lanes, so 64- and 128-column
+    // tiles (FN = 2, 4) give the same bits (fixed order: deterministic).
+    static_assert(FN == 2 || FN == 4, "quarter-transform tiles are 64 or 128 aux columns wide");
     double s[FM][4];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        double v = 0.0;
+        double tot = 0.0;
 #pragma unroll
-        for (int j = 0; j < FN; ++j) v += acc[i][j][r] * acc[i][j][r];
+        for (int h = 0; h < FN / 2; ++h) {
+          double v = 0.0;
 #pragma unroll
-        for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
-        s[i][r] = v;
+          for (int j = 2 * h; j < 2 * h + 2; ++j) v += acc[i][j][r] * acc[i][j][r];
+#pragma unroll
+          for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
+          tot = h == 0 ? v : tot + v;
+        }
+        s[i][r] = tot;
       }
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if ((wave & 1) && (lane & 15) == 0) {
+    if constexpr (FN == 4) {  // each wave covers one 64-column block
+      const int c0 = t.n0 + wn;
+      if ((lane & 15) == 0 && c0 < naux_p) {
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+        for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) lds[wm + 16 * i + (lane >> 4) + 4 * r] = s[i][r];
-    }
-    __syncthreads();
-    if (!(wave & 1) && (lane & 15) == 0) {
+          for (int r = 0; r < 4; ++r) {
+            const int q = t.m0 + wm + 16 * i + (lane >> 4) + 4 * r;
+            if (q <= p && q < nbf5) gmem(kpart)[((int64_t)p * nbf5 + q) * nt64 + c0 / 64] = s[i][r];
+          }
+      }
+      (void)wave;
+      (void)lds;
+    } else {  // the two waves of a row pair cover one 64-column block: add through LDS
+      if ((wave & 1) && (lane & 15) == 0) {
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+        for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int dm = wm + 16 * i + (lane >> 4) + 4 * r;
-          const int q = t.m0 + dm;
-          if (q <= p && q < nbf5)
-            gmem(kpart)[((int64_t)p * nbf5 + q) * ntiles + t.nt] = s[i][r] + lds[dm];
-        }
+          for (int r = 0; r < 4; ++r) lds[wm + 16 * i + (lane >> 4) + 4 * r] = s[i][r];
+      }
+      __syncthreads();
+      if (!(wave & 1) && (lane & 15) == 0) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int dm = wm + 16 * i + (lane >> 4) + 4 * r;
+            const int q = t.m0 + dm;
+            if (q <= p && q < nbf5)
+              gmem(kpart)[((int64_t)p * nbf5 + q) * nt64 + t.n0 / 64] = s[i][r] + lds[dm];
+          }
+      }
     }
   }
 };
@@ -722,8 +745,10 @@ hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const
 
 hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* X, double* kpart,
                          double* d, int* ntiles, hipStream_t s, GroupCollector* col) {
+  // naux_p is a multiple of 64; a 128-column tile may cover 64 columns past it,
+  // which the B loader reads from the workspace slack of X and the epilogue drops.
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
-  if (g.naux_p % BN != 0) return hipErrorInvalidValue;
+  if (tc.fn != 2 && tc.fn != 4) return hipErrorInvalidValue;
   ProbQT pr;
   pr.Cp = Cp;
   pr.X = X;
@@ -735,8 +760,9 @@ hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const
   pr.ldcp = g.ldcp;
   pr.nbf_p = g.nbf_p;
   pr.mtiles = cdiv(g.nbf5, BM);
-  pr.ntiles = g.naux_p / BN;
-  *ntiles = pr.ntiles;
+  pr.ntiles = cdiv(g.naux_p, BN);
+  pr.nt64 = g.naux_p / 64;
+  *ntiles = pr.nt64;
   return launch_prob(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s, col);
 }
 

@@ -22,7 +22,7 @@ def test_tiles_bitwise(gpu_required, precision):
     a = autotune.ARITH[precision]
     for st, idx in autotune.STAGES.items():
         for fm, fn in autotune.CANDIDATES[st]:
-            if precision == "fp64" and ((st == "quarter_transform" and fn == 4) or (fm, fn) == (10, 4)):
+            if precision == "fp64" and (fm, fn) == (10, 4):
                 assert L.pnjk_set_tile(eng._h, a, idx, fm, fn) == _lib.EINVAL  # BF16x3-only shapes
                 continue
             _lib.check(L.pnjk_set_tile(eng._h, a, idx, fm, fn))
@@ -36,8 +36,9 @@ def test_tiles_bitwise(gpu_required, precision):
 def test_set_tile_rejects(gpu_required):
     eng = JKEngine(0)
     L = _lib.lib()
-    assert L.pnjk_set_tile(eng._h, 0, 2, 4, 4) == _lib.EINVAL   # FP64 quarter transform needs fn = 2
-    assert L.pnjk_set_tile(eng._h, 1, 2, 4, 4) == _lib.OK       # BF16x3 takes fn = 2 or 4
+    assert L.pnjk_set_tile(eng._h, 0, 2, 5, 1) == _lib.EINVAL   # not an offered shape
+    assert L.pnjk_set_tile(eng._h, 0, 2, 4, 4) == _lib.OK       # 128-column tiles, FP64 and BF16x3
+    assert L.pnjk_set_tile(eng._h, 1, 2, 4, 4) == _lib.OK
     assert L.pnjk_set_tile(eng._h, 1, 2, 0, 0) == _lib.OK
     assert L.pnjk_set_tile(eng._h, 1, 1, 10, 4) == _lib.OK      # eight-wave tile: BF16x3 HT and J^n
     assert L.pnjk_set_tile(eng._h, 1, 6, 10, 4) == _lib.EINVAL  # not the exchange SYRK
