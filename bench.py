@@ -1,6 +1,6 @@
 """Benchmark of the MI355X density-fitted JK build.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|mid|water]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|mid|water|batch64|serving|gemmref]
 
 One step is one fused JK build (J^n, K^n, J_MO, K_MO, H_core for one set of
 orbitals C) against a device-resident b, with C already on the device.
@@ -112,7 +112,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="large", choices=["water", "mid", "large", "batch64", "serving"])
+    ap.add_argument("--workload", default="large", choices=["water", "mid", "large", "batch64", "serving", "gemmref"])
     ap.add_argument("--requests", type=int, default=120, help="serving: requests per run")
     ap.add_argument("--form-steps", type=int, default=5, help="builds timed per single form")
     ap.add_argument("--cpu-sample-naux", type=int, default=72)
@@ -127,6 +127,8 @@ def main():
         return bench_batch(args)
     if args.workload == "serving":
         return bench_serving(args)
+    if args.workload == "gemmref":
+        return bench_gemmref(args)
 
     import torch
 
@@ -396,6 +398,42 @@ def bench_batch(args):
             for f in ("fp64", "split")},
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_gemmref(args):
+    """Vendor-library GEMM rates on this GPU (hipBLASLt/rocBLAS through torch.matmul),
+    as a practical reference for the hand-written cores: large square GEMMs and the
+    half-transform's own shape (M = 320 rows of C^T, N = nbf * naux_p, K = 1024),
+    bf16 and fp64. Prints one JSON line."""
+    import torch
+
+    dev = torch.device("cuda", 0)
+
+    def rate(M, N, K, dtype, reps):
+        a = torch.randn(M, K, device=dev, dtype=dtype)
+        b = torch.randn(K, N, device=dev, dtype=dtype)
+        for _ in range(2):
+            a @ b
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            a @ b
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        del a, b
+        torch.cuda.empty_cache()
+        res = {"M": M, "N": N, "K": K, "dtype": str(dtype).replace("torch.", ""), "ms": round(ms, 3),
+               "tflops": round(2.0 * M * N * K / (ms * 1e-3) / 1e12, 1)}
+        log(json.dumps(res))
+        return res
+
+    rows = [rate(8192, 8192, 8192, torch.bfloat16, 10), rate(16384, 16384, 16384, torch.bfloat16, 5),
+            rate(320, 3648000, 1024, torch.bfloat16, 5), rate(8192, 8192, 8192, torch.float64, 5),
+            rate(16384, 16384, 16384, torch.float64, 2), rate(320, 3648000, 1024, torch.float64, 2)]
+    print(json.dumps({"metric": "vendor GEMM reference rates (torch.matmul: hipBLASLt / rocBLAS)", "unit": "TFLOP/s",
+                      "n_gpus": 1, "results": rows}), flush=True)
 
 
 def bench_serving(args):
