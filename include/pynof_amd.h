@@ -105,6 +105,25 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
 int pnjk_mo_integrals(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, const double* H, int64_t ldh,
                       int precision, double* b_MO, double* H_mat, void* stream);
 
+/* Orbital-gradient Lagrangian of the RI path, closed shell: replaces
+ * computeLagrange2 with p.RI and p.MSpin == 0 (utils.py:262-270, :328-355),
+ * which calcorbg / calcorbeg (pnof.py:1114-1154) call on every orbital step.
+ * With B = b_MO and Hm = H_mat of pnjk_mo_integrals (kept on the device):
+ *   elag[a][b] = n[b] Hm[a][b]
+ *              + w_b n[b] sum_P B[a][b][P] B[b][b][P]   (w_b = 1 for b < nbeta or b >= nalpha)
+ *              + sum_P B[a][b][P] sum_q cj12[b][q] B[q][q][P]
+ *              - sum_q ck12[b][q] sum_P B[a][q][P] B[b][q][P]       for b < nbf5,
+ *   elag[a][b] = 0 for b >= nbf5;  a < nbf.
+ * n (nbf5), cj12 and ck12 (nbf5 x nbf5, row-major) are read with their
+ * diagonals taken as zero (the reference zeroes the caller's arrays in place,
+ * utils.py:269-270; this call leaves its inputs unchanged). elag is (nbf, nbf),
+ * H_mat (nbf, nbf5) optional; host or device memory, as in pnjk_jk_build.
+ * precision (PNJK_FP64 or PNJK_SPLIT) applies to b_MO; the contractions are FP64.
+ * The orbital gradient is 4 (elag - elag^T) (pnof.py:1120). */
+int pnjk_lagrangian(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, const double* H, int64_t ldh,
+                    const double* n, const double* cj12, const double* ck12, int64_t nbeta, int64_t nalpha,
+                    int precision, double* elag, double* H_mat, void* stream);
+
 /* Grouped builds: many engines (one b each, different sizes allowed) built in
  * one pass. Each GEMM stage of all instances runs as one persistent grouped
  * launch per problem type and tile shape (tile shapes per instance from the

@@ -196,6 +196,79 @@ def integrals_mo_full(C, H, I, nbf5):
     return H_mat, I_MO
 
 
+def lagrange2_ri(n, cj12, ck12, C, H, b_mnl, nbf5, nbeta, nalpha, MSpin=0):
+    """(elag, Hmat) of computeLagrange2 with p.RI on the CPU (utils.py:262-270, :328-355).
+
+    elag[:, b] (b < nbf5; zero for b >= nbf5), with cj, ck = cj12, ck12 with zeroed
+    diagonals (the reference zeroes the caller's arrays in place, :269-270):
+      n_b Hmat[a,b]                                    (:334)
+      + n_b sum_k b_MO[a,b,k] b_MO[b,b,k]   for b < nbeta and nalpha <= b < nbf5 (:337-346)
+      + sum_k b_MO[a,b,k] sum_q cj[b,q] b_MO[q,q,k]    (:349-350)
+      - sum_q ck[b,q] sum_k b_MO[a,q,k] b_MO[b,q,k]    (:353-355)
+    For MSpin != 0 the reference's RI CPU branch leaves elag at zero."""
+    Hmat, b_MO = integrals_mo_ri(C, H, b_mnl, nbf5)
+    cj = np.array(cj12, dtype=np.float64, copy=True)
+    ck = np.array(ck12, dtype=np.float64, copy=True)
+    np.fill_diagonal(cj, 0.0)
+    np.fill_diagonal(ck, 0.0)
+    nbf = C.shape[0]
+    elag = np.zeros((nbf, nbf))
+    if MSpin == 0:
+        elag[:, :nbf5] += np.einsum("b,ab->ab", n, Hmat[:, :nbf5], optimize=True)
+        elag[:, :nbeta] += np.einsum("b,abk,bbk->ab", n[:nbeta], b_MO[:, :nbeta, :], b_MO[:nbeta, :nbeta, :],
+                                     optimize=True)
+        elag[:, nalpha:nbf5] += np.einsum("b,abk,bbk->ab", n[nalpha:nbf5], b_MO[:, nalpha:nbf5, :],
+                                          b_MO[nalpha:nbf5, nalpha:nbf5, :], optimize=True)
+        tmp = np.einsum("bq,qqk->bk", cj, b_MO[:nbf5, :nbf5, :], optimize=True)
+        elag[:, :nbf5] += np.einsum("abk,bk->ab", b_MO[:, :nbf5, :], tmp, optimize=True)
+        elag[:, :nbf5] -= np.einsum("bq,aqk,bqk->ab", ck, b_MO[:, :nbf5, :], b_MO[:nbf5, :nbf5, :], optimize=True)
+    return elag, Hmat
+
+
+def lagrange2_loops(n, cj12, ck12, C, H, b_mnl, nbf5, nbeta, nalpha):
+    """The MSpin = 0 elag of lagrange2_ri by explicit loops (tiny inputs only), from
+    b_MO and Hmat of integrals_mo_loops: an independent route."""
+    Hmat, B = integrals_mo_loops(C, H, b_mnl, nbf5)
+    nbf, naux = b_mnl.shape[0], b_mnl.shape[2]
+    elag = np.zeros((nbf, nbf))
+    for a in range(nbf):
+        for bb in range(nbf5):
+            v = n[bb] * Hmat[a, bb]
+            if bb < nbeta or nalpha <= bb:
+                v += n[bb] * sum(B[a, bb, k] * B[bb, bb, k] for k in range(naux))
+            for q in range(nbf5):
+                if q == bb:
+                    continue
+                v += cj12[bb, q] * sum(B[a, bb, k] * B[q, q, k] for k in range(naux))
+                v -= ck12[bb, q] * sum(B[a, q, k] * B[bb, q, k] for k in range(naux))
+            elag[a, bb] = v
+    return elag, Hmat
+
+
+def orbital_gradient(elag, nbf5):
+    """calcorbg's gradient vector (pnof.py:1114-1131): 4 (elag - elag^T)[i, j] for
+    i < nbf5, j > i, in rotate_orbital's variable order (utils.py:550-565)."""
+    g = 4.0 * (elag - elag.T)
+    nbf = elag.shape[0]
+    return np.array([g[i, j] for i in range(nbf5) for j in range(i + 1, nbf)])
+
+
+def rotate_orbital(y, C, nbf5):
+    """C exp(kappa(y)), kappa[i, j] = y, kappa[j, i] = -y for i < nbf5, j > i
+    (utils.py:550-565)."""
+    from scipy.linalg import expm
+
+    nbf = C.shape[0]
+    kap = np.zeros((nbf, nbf))
+    m = 0
+    for i in range(nbf5):
+        for j in range(i + 1, nbf):
+            kap[i, j] = y[m]
+            kap[j, i] = -y[m]
+            m += 1
+    return C @ expm(kap)
+
+
 def energy(n, J_MO, K_MO, H_core):
     """E_1 + E_2 with E_1 = 2 sum n H_core and E_2 = sum_pq n_p n_q (2J_pq - K_pq)."""
     e1 = 2.0 * float(np.dot(n, H_core))

@@ -3,11 +3,12 @@
 Entry points:
   computeJKj, computeJKH_MO   drop-in replacements of PyNOF's RI JK builds
   JKH_MO_tmp                  drop-in for the orbital-gradient MO integrals (RI)
+  computeLagrange2            drop-in for the orbital-gradient Lagrangian (RI, closed shell)
   JKEngine                    device-resident b and fused builds
 """
 from ._lib import PnjkError  # noqa: F401
 from .engine import JKEngine  # noqa: F401
-from .integrals import JKH_MO_tmp, computeJKj, computeJKH_MO  # noqa: F401
+from .integrals import JKH_MO_tmp, computeJKj, computeJKH_MO, computeLagrange2  # noqa: F401
 from .parameters import param  # noqa: F401
 
-__all__ = ["JKEngine", "computeJKj", "computeJKH_MO", "JKH_MO_tmp", "param", "PnjkError"]
+__all__ = ["JKEngine", "computeJKj", "computeJKH_MO", "JKH_MO_tmp", "computeLagrange2", "param", "PnjkError"]

@@ -129,15 +129,17 @@ struct pnjk_engine {
   DevBuf Cp, Hp, X, d, kpart, T, outs, stage, scal;
   DevBuf ct16h, ct16l, x16h, x16l, d16h, d16l;  // split workspace
   DevBuf Cf, ctf16h, ctf16l, hc;                 // MO integrals: all columns of C; H_core scratch
+  DevBuf bmo, hm, lagw, lagg, lagp, lagc;        // Lagrangian workspace: b_MO, H_mat, W, G, partials, n/C^J/C^K
   bool split_ready = false;
   // Tile override per arithmetic (0: FP64 kernels, 1: BF16x3 kernels) and stage;
   // {0, 0} selects the built-in choice.
   TileF64 tiles[2][ST_COUNT] = {};
-  DevBuf* all[26];
+  DevBuf* all[32];
   int nall = 0;
   pnjk_engine() {
     for (DevBuf* b : {&btri, &bt16h, &bt16l, &bht16h, &bht16l, &Cp, &Hp, &X, &d, &kpart, &T, &outs, &stage, &scal,
-                      &ct16h, &ct16l, &x16h, &x16l, &d16h, &d16l, &Cf, &ctf16h, &ctf16l, &hc})
+                      &ct16h, &ct16l, &x16h, &x16l, &d16h, &d16l, &Cf, &ctf16h, &ctf16l, &hc,
+                      &bmo, &hm, &lagw, &lagg, &lagp, &lagc})
       all[nall++] = b;
   }
 };
@@ -618,6 +620,72 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
   return PNJK_OK;
 }
 
+}  // extern "C"
+
+// The MO integrals into device buffers: b_MO rows of stride ldrow (dB, may be
+// NULL) and H_mat (dHm, may be NULL). `st` comes from build_prepare with no outputs.
+static int mo_compute(pnjk_engine* e, BuildState& st, double* dB, int64_t ldrow, double* dHm, hipStream_t s) {
+  const BuildGeom& g = st.g;
+  const int64_t nbf = g.nbf, n5 = g.nbf5;
+  const int precision = st.a.precision;
+  int rc;
+  // Full-width C, k-major: Cf[nu][p] for nu < nbf, p < nbf; zero rows to nbf_p + 8 and
+  // columns to ldcf, which covers every M-tile (at most 160 rows) past nbf.
+  const int ldcf = (int)roundup(nbf, 32) + 160;
+  const int64_t cf_elems = (int64_t)(g.nbf_p + 8) * ldcf;
+  if ((rc = e->Cf.ensure((size_t)cf_elems * 8))) return rc;
+  if (dHm) {  // T = H C[:, :nbf5] (through the H_core stage), then H_mat = Cf^T T
+    st.want_h = true;
+    if ((rc = e->Hp.ensure((size_t)(nbf * g.nbf_p) * 8)) || (rc = e->T.ensure((size_t)(nbf * g.ldcp + 64) * 8)) ||
+        (rc = e->hc.ensure((size_t)n5 * 8)))
+      return rc;
+    st.dH = e->hc.as<double>();
+    st.hH = false;
+  }
+  if ((rc = stage_in(e, st, s))) return rc;
+  double* Cf = e->Cf.as<double>();
+  PNJK_HIP_CHECK(fill(Cf, cf_elems, 0.0, s));
+  if (st.c_dev) {
+    const int64_t n = nbf * nbf;
+    hipLaunchKernelGGL(k_copy2d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, st.a.C, st.a.ldc, Cf,
+                       (int64_t)ldcf, (int)nbf, (int)nbf);
+    PNJK_HIP_CHECK(hipGetLastError());
+  } else {
+    PNJK_HIP_CHECK(hipMemcpy2DAsync(Cf, (size_t)ldcf * 8, st.a.C, (size_t)st.a.ldc * 8, (size_t)nbf * 8,
+                                    (size_t)nbf, hipMemcpyHostToDevice, s));
+  }
+  if (dB) {
+    if ((rc = stage_ht(e, st, s, nullptr))) return rc;
+    const bool big = nbf >= 256;
+    if (precision == PNJK_SPLIT) {
+      const size_t ct_bytes = (size_t)(nbf * st.sg.ldk) * 2;
+      if ((rc = e->ctf16h.ensure(ct_bytes)) || (rc = e->ctf16l.ensure(ct_bytes))) return rc;
+      PNJK_HIP_CHECK(launch_split_ct(Cf, ldcf, (int)nbf, (int)nbf, (int)nbf, st.sg.ldk, e->ctf16h.as<uint16_t>(),
+                                     e->ctf16l.as<uint16_t>(), s));
+      PNJK_HIP_CHECK(launch_mo_x3(st.sg, g.naux, big ? TileF64{5, 4} : TileF64{2, 2}, e->ctf16h.as<uint16_t>(),
+                                  e->ctf16l.as<uint16_t>(), e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), dB, ldrow,
+                                  s));
+    } else {
+      PNJK_HIP_CHECK(
+          launch_mo_f64(g, big ? TileF64{5, 2} : TileF64{2, 2}, Cf, ldcf, e->X.as<double>(), dB, ldrow, s));
+    }
+  }
+  if (dHm) {
+    if ((rc = stage_hcore(e, st, s))) return rc;
+    PNJK_HIP_CHECK(launch_hmat_f64(g, Cf, ldcf, e->T.as<double>(), dHm, s));
+  }
+  return PNJK_OK;
+}
+
+// Copies n doubles from host or device memory into a device buffer.
+static int to_device(const double* src, double* dst, int64_t n, hipStream_t s) {
+  PNJK_HIP_CHECK(hipMemcpyAsync(dst, src, (size_t)n * 8, is_device_ptr(src) ? hipMemcpyDeviceToDevice
+                                                                            : hipMemcpyHostToDevice, s));
+  return PNJK_OK;
+}
+
+extern "C" {
+
 int pnjk_mo_integrals(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, const double* H, int64_t ldh,
                       int precision, double* b_MO, double* H_mat, void* stream) {
   if (!e) return set_error(PNJK_EINVAL, "pnjk_mo_integrals: engine is NULL");
@@ -636,13 +704,7 @@ int pnjk_mo_integrals(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5
   BuildState st;
   int rc;
   if ((rc = build_prepare(e, a, st))) return rc;
-  const BuildGeom& g = st.g;
-  const int64_t nbf = g.nbf, n5 = g.nbf5, naux = g.naux;
-  // Full-width C, k-major: Cf[nu][p] for nu < nbf, p < nbf; zero rows to nbf_p + 8 and
-  // columns to ldcf, which covers every M-tile (at most 160 rows) past nbf.
-  const int ldcf = (int)roundup(nbf, 32) + 160;
-  const int64_t cf_elems = (int64_t)(g.nbf_p + 8) * ldcf;
-  if ((rc = e->Cf.ensure((size_t)cf_elems * 8))) return rc;
+  const int64_t nbf = st.g.nbf, n5 = st.g.nbf5, naux = st.g.naux;
   const bool host_b = b_MO && !is_device_ptr(b_MO), host_h = H_mat && !is_device_ptr(H_mat);
   const int64_t b_elems = nbf * n5 * naux, h_elems = nbf * n5;
   if ((host_b || host_h) &&
@@ -650,48 +712,67 @@ int pnjk_mo_integrals(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5
     return rc;
   double* dB = host_b ? e->outs.as<double>() : b_MO;
   double* dHm = host_h ? e->outs.as<double>() + (host_b ? b_elems : 0) : H_mat;
-  if (H_mat) {  // T = H C[:, :nbf5] (through the H_core stage), then H_mat = Cf^T T
-    st.want_h = true;
-    if ((rc = e->Hp.ensure((size_t)(nbf * g.nbf_p) * 8)) || (rc = e->T.ensure((size_t)(nbf * g.ldcp + 64) * 8)) ||
-        (rc = e->hc.ensure((size_t)n5 * 8)))
-      return rc;
-    st.dH = e->hc.as<double>();
-    st.hH = false;
-  }
-
-  if ((rc = stage_in(e, st, s))) return rc;
-  double* Cf = e->Cf.as<double>();
-  PNJK_HIP_CHECK(fill(Cf, cf_elems, 0.0, s));
-  if (st.c_dev) {
-    const int64_t n = nbf * nbf;
-    hipLaunchKernelGGL(k_copy2d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, C, ldc, Cf, (int64_t)ldcf,
-                       (int)nbf, (int)nbf);
-    PNJK_HIP_CHECK(hipGetLastError());
-  } else {
-    PNJK_HIP_CHECK(hipMemcpy2DAsync(Cf, (size_t)ldcf * 8, C, (size_t)ldc * 8, (size_t)nbf * 8, (size_t)nbf,
-                                    hipMemcpyHostToDevice, s));
-  }
-  if (b_MO) {
-    if ((rc = stage_ht(e, st, s, nullptr))) return rc;
-    const bool big = nbf >= 256;
-    if (precision == PNJK_SPLIT) {
-      const size_t ct_bytes = (size_t)(nbf * st.sg.ldk) * 2;
-      if ((rc = e->ctf16h.ensure(ct_bytes)) || (rc = e->ctf16l.ensure(ct_bytes))) return rc;
-      PNJK_HIP_CHECK(launch_split_ct(Cf, ldcf, (int)nbf, (int)nbf, (int)nbf, st.sg.ldk, e->ctf16h.as<uint16_t>(),
-                                     e->ctf16l.as<uint16_t>(), s));
-      PNJK_HIP_CHECK(launch_mo_x3(st.sg, (int)naux, big ? TileF64{5, 4} : TileF64{2, 2}, e->ctf16h.as<uint16_t>(),
-                                  e->ctf16l.as<uint16_t>(), e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), dB, s));
-    } else {
-      PNJK_HIP_CHECK(launch_mo_f64(g, big ? TileF64{5, 2} : TileF64{2, 2}, Cf, ldcf, e->X.as<double>(), dB, s));
-    }
-  }
-  if (H_mat) {
-    if ((rc = stage_hcore(e, st, s))) return rc;
-    PNJK_HIP_CHECK(launch_hmat_f64(g, Cf, ldcf, e->T.as<double>(), dHm, s));
-  }
+  if ((rc = mo_compute(e, st, dB, n5 * naux, dHm, s))) return rc;
   if (host_b) PNJK_HIP_CHECK(hipMemcpyAsync(b_MO, dB, (size_t)b_elems * 8, hipMemcpyDeviceToHost, s));
   if (host_h) PNJK_HIP_CHECK(hipMemcpyAsync(H_mat, dHm, (size_t)h_elems * 8, hipMemcpyDeviceToHost, s));
   if (host_b || host_h) PNJK_HIP_CHECK(hipStreamSynchronize(s));
+  return PNJK_OK;
+}
+
+int pnjk_lagrangian(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, const double* H, int64_t ldh,
+                    const double* n, const double* cj12, const double* ck12, int64_t nbeta, int64_t nalpha,
+                    int precision, double* elag, double* H_mat, void* stream) {
+  if (!e) return set_error(PNJK_EINVAL, "pnjk_lagrangian: engine is NULL");
+  std::lock_guard<std::mutex> lk(e->mu);
+  PNJK_HIP_CHECK(hipSetDevice(e->device));
+  hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
+  if (e->nbf == 0) return set_error(PNJK_ESTATE, "pnjk_lagrangian: no b uploaded");
+  if (!C || !H || !n || !cj12 || !ck12 || !elag)
+    return set_error(PNJK_EINVAL, "pnjk_lagrangian: C, H, n, cj12, ck12 and elag are required");
+  if (ldc < e->nbf || ldh < e->nbf) return set_error(PNJK_EINVAL, "pnjk_lagrangian: ldc and ldh must be >= nbf");
+  if (precision != PNJK_FP64 && precision != PNJK_SPLIT)
+    return set_error(PNJK_EUNSUPPORTED, "pnjk_lagrangian: precision must be PNJK_FP64 or PNJK_SPLIT");
+  if (nbeta < 0 || nalpha < nbeta || nalpha > nbf5)
+    return set_error(PNJK_EINVAL, "pnjk_lagrangian: need 0 <= nbeta <= nalpha <= nbf5");
+  BuildArgs a{C, ldc, nbf5, H, ldh, precision, 0u, nullptr, nullptr, nullptr, nullptr, nullptr};
+  BuildState st;
+  int rc;
+  if ((rc = build_prepare(e, a, st))) return rc;
+  const BuildGeom& g = st.g;
+  const int64_t nbf = g.nbf, n5 = g.nbf5, naux = g.naux;
+  // Split-K chunking of k' = (q, P) for the exchange GEMM: about 2048 tiles of 128 x 64.
+  const int64_t kfull = n5 * naux, tiles = ((nbf + 127) / 128) * ((n5 + 63) / 64);
+  const int64_t want = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
+  const int64_t kc = roundup(std::max<int64_t>(1, (kfull + want - 1) / want), 64);
+  const int64_t nchunks = (kfull + kc - 1) / kc, ldrow = nchunks * kc;
+  if ((rc = e->bmo.ensure((size_t)(nbf * ldrow) * 8)) || (rc = e->hm.ensure((size_t)(nbf * n5) * 8)) ||
+      (rc = e->lagw.ensure((size_t)(n5 * ldrow) * 8)) || (rc = e->lagg.ensure((size_t)(n5 * naux) * 8)) ||
+      (rc = e->lagp.ensure((size_t)(nchunks * nbf * n5) * 8)) || (rc = e->lagc.ensure((size_t)(3 * n5 * n5) * 8)))
+    return rc;
+  const bool host_e = !is_device_ptr(elag), host_h = H_mat && !is_device_ptr(H_mat);
+  if (host_e && (rc = e->outs.ensure((size_t)(nbf * nbf) * 8))) return rc;
+  double* dE = host_e ? e->outs.as<double>() : elag;
+  double* B = e->bmo.as<double>();
+  double* Hm = e->hm.as<double>();
+  double* cf = e->lagc.as<double>();  // [n5 * n5 C^J | n5 * n5 C^K | n5 occupations]
+  double* cjd = cf;
+  double* ckd = cf + n5 * n5;
+  double* nd = cf + 2 * n5 * n5;
+  if ((rc = mo_compute(e, st, B, ldrow, Hm, s))) return rc;
+  PNJK_HIP_CHECK(launch_fill_cols(B, (int)nbf, ldrow, kfull, ldrow, s));
+  // Coefficients: copy, then clear the diagonal (utils.py:269-270) on the device.
+  if ((rc = to_device(cj12, e->lagw.as<double>(), n5 * n5, s))) return rc;
+  PNJK_HIP_CHECK(launch_lag_coef(e->lagw.as<double>(), cjd, (int)n5, s));
+  if ((rc = to_device(ck12, e->lagw.as<double>(), n5 * n5, s))) return rc;
+  PNJK_HIP_CHECK(launch_lag_coef(e->lagw.as<double>(), ckd, (int)n5, s));
+  if ((rc = to_device(n, nd, n5, s))) return rc;
+  PNJK_HIP_CHECK(launch_lag_f64(g, B, ldrow, kc, (int)nchunks, nd, cjd, ckd, (int)nbeta, (int)nalpha, Hm,
+                                e->lagg.as<double>(), e->lagw.as<double>(), e->lagp.as<double>(), dE, s));
+  if (host_e) PNJK_HIP_CHECK(hipMemcpyAsync(elag, dE, (size_t)(nbf * nbf) * 8, hipMemcpyDeviceToHost, s));
+  if (H_mat)
+    PNJK_HIP_CHECK(hipMemcpyAsync(H_mat, Hm, (size_t)(nbf * n5) * 8,
+                                  host_h ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+  if (host_e || host_h) PNJK_HIP_CHECK(hipStreamSynchronize(s));
   return PNJK_OK;
 }
 

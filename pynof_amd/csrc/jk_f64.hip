@@ -531,6 +531,7 @@ struct ProbMO {
   const double* Cf;
   const double* X;
   double* out;
+  int64_t ldrow;  // output row stride (nbf5 * naux for the reference layout)
   int nbf, naux, naux_p, nbf5, ldcf, nbf_p, mtiles, ntiles;
   __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.mt = id % mtiles;
@@ -556,7 +557,7 @@ struct ProbMO {
           int dm, dn;
           elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int m = t.m0 + dm, n = t.n0 + dn;
-          if (m < nbf && n < naux) gmem(out)[((int64_t)m * nbf5 + t.batch) * naux + n] = acc[i][j][r];
+          if (m < nbf && n < naux) gmem(out)[(int64_t)m * ldrow + (int64_t)t.batch * naux + n] = acc[i][j][r];
         }
   }
 };
@@ -593,6 +594,47 @@ struct ProbHM {
           elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int m = t.m0 + dm, n = t.n0 + dn;
           if (m < nbf && n < nbf5) gmem(out)[(int64_t)m * nbf5 + n] = acc[i][j][r];
+        }
+  }
+};
+
+// Exchange term of the orbital-gradient Lagrangian (utils.py:353-355), split over
+// k' = (q, P) in chunks of kc:
+//   part[c][a][b] = sum_{k' in chunk c} B[a][k'] W[b][k'],   W[b][q naux + P] = ck[b][q] B[b][q][P],
+// a < nbf, b < nbf5, with B = b_MO rows of stride ldrow (zero past nbf5 * naux).
+struct ProbLagX {
+  static constexpr int kA = MK, kB = MK;
+  const double* B;
+  const double* W;
+  double* part;
+  int64_t ldrow, kc;
+  int nbf, nbf5, mtiles, ntiles;
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.mt = id % mtiles;
+    const int rest = id / mtiles;
+    t.nt = rest % ntiles;
+    t.batch = rest / ntiles;  // k chunk
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf; }
+  __device__ int ksteps(int BK) const { return (int)(kc / BK); }
+  __device__ SrcMKDense a_src(const Tile& t) const { return {B + (int64_t)t.batch * kc, ldrow, nbf}; }
+  __device__ SrcMKDense b_src(const Tile& t) const { return {W + (int64_t)t.batch * kc, ldrow, nbf5}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4d (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int m = t.m0 + dm, n = t.n0 + dn;
+          if (m < nbf && n < nbf5)
+            gmem(part)[((int64_t)t.batch * nbf + m) * nbf5 + n] = acc[i][j][r];
         }
   }
 };
@@ -816,14 +858,84 @@ hipError_t launch_jn_f64(const BuildGeom& g, TileF64 tc, const double* d, const 
   return launch_prob(pr, tc, pr.mtiles * cdiv(g.ntri, BN), s, col);
 }
 
+// Lagrangian helpers (utils.py:262-355).
+// Copies an nbf5 x nbf5 coefficient matrix with its diagonal set to zero (:269-270).
+__global__ void k_lag_coef(const double* __restrict__ src, double* __restrict__ dst, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * n) return;
+  dst[i] = (i / n == i % n) ? 0.0 : src[i];
+}
+
+// G[b][P] = w_b n_b d[b][P] + sum_q cj[b][q] d[q][P], d[q][P] = B[q][q][P], w_b = 1 for
+// b < nbeta or b >= nalpha (:337-350).
+__global__ void k_lag_g(const double* __restrict__ B, int64_t ldrow, int naux, int nbf5, int nbeta, int nalpha,
+                        const double* __restrict__ nocc, const double* __restrict__ cj, double* __restrict__ G) {
+  const int P = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (P >= naux) return;
+  double v = 0.0;
+  for (int q = 0; q < nbf5; ++q) v += cj[(int64_t)b * nbf5 + q] * B[(int64_t)q * ldrow + (int64_t)q * naux + P];
+  if (b < nbeta || b >= nalpha) v += nocc[b] * B[(int64_t)b * ldrow + (int64_t)b * naux + P];
+  G[(int64_t)b * naux + P] = v;
+}
+
+// W[b][k'] = ck[b][q] B[b][k'] for k' = q naux + P < nbf5 naux, zero up to ldrow (:353-355).
+__global__ void k_lag_w(const double* __restrict__ B, int64_t ldrow, int naux, int nbf5,
+                        const double* __restrict__ ck, double* __restrict__ W) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)nbf5 * ldrow) return;
+  const int b = (int)(i / ldrow);
+  const int64_t kp = i - (int64_t)b * ldrow;
+  const int q = (int)(kp / naux);
+  W[i] = q < nbf5 ? ck[(int64_t)b * nbf5 + q] * B[i] : 0.0;
+}
+
+// Zeroes columns [c0, c1) of `rows` rows of stride ld.
+__global__ void k_fill_cols(double* __restrict__ p, int rows, int64_t ld, int64_t c0, int64_t c1) {
+  const int64_t w = c1 - c0;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)rows * w) return;
+  p[(i / w) * ld + c0 + i % w] = 0.0;
+}
+
+// elag[a][b] = n_b Hm[a][b] + sum_P B[a][b][P] G[b][P] - sum_c part[c][a][b] for b < nbf5,
+// zero for b >= nbf5. One wave per (a, b); fixed summation order (deterministic).
+__global__ void __launch_bounds__(256) k_lag_final(const double* __restrict__ B, int64_t ldrow, int naux, int nbf,
+                                                   int nbf5, const double* __restrict__ G,
+                                                   const double* __restrict__ Hm, const double* __restrict__ nocc,
+                                                   const double* __restrict__ part, int nchunks,
+                                                   double* __restrict__ elag) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= (int64_t)nbf * nbf) return;
+  const int a = (int)(w / nbf), b = (int)(w % nbf);
+  if (b >= nbf5) {
+    if (lane == 0) elag[w] = 0.0;
+    return;
+  }
+  const double* row = B + (int64_t)a * ldrow + (int64_t)b * naux;
+  const double* g = G + (int64_t)b * naux;
+  double v = 0.0;
+  for (int P = lane; P < naux; P += 64) v += row[P] * g[P];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (lane == 0) {
+    double x = 0.0;
+    for (int c = 0; c < nchunks; ++c) x += part[((int64_t)c * nbf + a) * nbf5 + b];
+    elag[w] = nocc[b] * Hm[(int64_t)a * nbf5 + b] + v - x;
+  }
+}
+
 hipError_t launch_mo_f64(const BuildGeom& g, TileF64 tc, const double* Cf, int ldcf, const double* X, double* b_mo,
-                         hipStream_t s) {
+                         int64_t ldrow, hipStream_t s) {
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
-  if (g.naux_p % BN != 0 || ldcf < cdiv(g.nbf, BM) * BM) return hipErrorInvalidValue;
+  if (g.naux_p % BN != 0 || ldcf < cdiv(g.nbf, BM) * BM || ldrow < (int64_t)g.nbf5 * g.naux)
+    return hipErrorInvalidValue;
   ProbMO pr;
   pr.Cf = Cf;
   pr.X = X;
   pr.out = b_mo;
+  pr.ldrow = ldrow;
   pr.nbf = g.nbf;
   pr.naux = g.naux;
   pr.naux_p = g.naux_p;
@@ -850,6 +962,46 @@ hipError_t launch_hmat_f64(const BuildGeom& g, const double* Cf, int ldcf, const
   pr.nbf_p = g.nbf_p;
   pr.mtiles = cdiv(g.nbf, 64);
   return launch_prob(pr, tc, pr.mtiles * cdiv(g.nbf5, 64), s);
+}
+
+hipError_t launch_lag_f64(const BuildGeom& g, const double* B, int64_t ldrow, int64_t kc, int nchunks,
+                          const double* nocc, const double* cj, const double* ck, int nbeta, int nalpha,
+                          const double* Hm, double* G, double* W, double* part, double* elag, hipStream_t s) {
+  const int nbf = g.nbf, n5 = g.nbf5, naux = g.naux;
+  if (kc % 16 != 0 || (int64_t)nchunks * kc != ldrow || ldrow < (int64_t)n5 * naux) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_lag_g, dim3(cdiv(naux, 256), n5), dim3(256), 0, s, B, ldrow, naux, n5, nbeta, nalpha, nocc, cj,
+                     G);
+  const int64_t nw = (int64_t)n5 * ldrow;
+  hipLaunchKernelGGL(k_lag_w, dim3((unsigned)cdiv(nw, 256)), dim3(256), 0, s, B, ldrow, naux, n5, ck, W);
+  const TileF64 tc{4, 2};
+  ProbLagX pr;
+  pr.B = B;
+  pr.W = W;
+  pr.part = part;
+  pr.ldrow = ldrow;
+  pr.kc = kc;
+  pr.nbf = nbf;
+  pr.nbf5 = n5;
+  pr.mtiles = cdiv(nbf, 32 * tc.fm);
+  pr.ntiles = cdiv(n5, 32 * tc.fn);
+  hipError_t e = launch_prob(pr, tc, pr.mtiles * pr.ntiles * nchunks, s);
+  if (e != hipSuccess) return e;
+  const int64_t waves = (int64_t)nbf * nbf;
+  hipLaunchKernelGGL(k_lag_final, dim3((unsigned)cdiv(waves, 4)), dim3(256), 0, s, B, ldrow, naux, nbf, n5, G, Hm, nocc,
+                     part, nchunks, elag);
+  return hipGetLastError();
+}
+
+hipError_t launch_lag_coef(const double* src, double* dst, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_lag_coef, dim3(cdiv(n * n, 256)), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_cols(double* p, int rows, int64_t ld, int64_t c0, int64_t c1, hipStream_t s) {
+  if (c1 <= c0 || rows <= 0) return hipSuccess;
+  const int64_t n = (int64_t)rows * (c1 - c0);
+  hipLaunchKernelGGL(k_fill_cols, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, p, rows, ld, c0, c1);
+  return hipGetLastError();
 }
 
 hipError_t launch_hcore_f64(const BuildGeom& g, const double* Hp, int ldh, const double* Cp, double* T,

@@ -79,7 +79,15 @@ hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const
 // Orbital-gradient MO integrals: b_MO[p][q][P] (p < nbf, q < nbf5, P < naux) from X
 // and the full-width C (Cf, ldcf >= nbf rounded up to the tile rows); H_mat = Cf^T T.
 hipError_t launch_mo_f64(const BuildGeom& g, TileF64 tc, const double* Cf, int ldcf, const double* X, double* b_mo,
-                         hipStream_t s);
+                         int64_t ldrow, hipStream_t s);
+// Orbital-gradient Lagrangian from b_MO (rows of stride ldrow = nchunks * kc, zero past
+// nbf5 * naux), Hm = H_mat, occupations and zero-diagonal C^J, C^K (device memory):
+// elag (nbf x nbf). G (nbf5 x naux), W (nbf5 x ldrow), part (nchunks x nbf x nbf5) are workspace.
+hipError_t launch_lag_f64(const BuildGeom& g, const double* B, int64_t ldrow, int64_t kc, int nchunks,
+                          const double* nocc, const double* cj, const double* ck, int nbeta, int nalpha,
+                          const double* Hm, double* G, double* W, double* part, double* elag, hipStream_t s);
+hipError_t launch_lag_coef(const double* src, double* dst, int n, hipStream_t s);
+hipError_t launch_fill_cols(double* p, int rows, int64_t ld, int64_t c0, int64_t c1, hipStream_t s);
 hipError_t launch_hmat_f64(const BuildGeom& g, const double* Cf, int ldcf, const double* T, double* hmat,
                            hipStream_t s);
 hipError_t launch_kmo_reduce(int nbf5, int nt, const double* kpart, double* kmo, hipStream_t s);
@@ -133,7 +141,7 @@ hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
 // BF16x3 orbital-gradient MO integrals from X16 and the split planes of C^T over
 // all nbf columns (rows clamped at nbf); FP64 output b_MO[p][q][P].
 hipError_t launch_mo_x3(const SplitGeom& g, int naux, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
-                        const uint16_t* x_hi, const uint16_t* x_lo, double* b_mo, hipStream_t s);
+                        const uint16_t* x_hi, const uint16_t* x_lo, double* b_mo, int64_t ldrow, hipStream_t s);
 hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
                         const uint16_t* x_hi, const uint16_t* x_lo, double* kpart, double* d, uint16_t* d_hi,
                         uint16_t* d_lo, int* ntiles, hipStream_t s, GroupCollector* col = nullptr);

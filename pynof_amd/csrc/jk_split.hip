@@ -425,6 +425,7 @@ struct ProbMO {
   static constexpr int kA = RK, kB = KR;
   Planes ct, x;
   double* out;
+  int64_t ldrow;  // output row stride (nbf5 * naux for the reference layout)
   int nbf, naux, naux_p, nbf5, ldk, mtiles, ntiles;
   __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.mt = id % mtiles;
@@ -449,7 +450,7 @@ struct ProbMO {
           int dm, dn;
           coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int m = t.m0 + dm, n = t.n0 + dn;
-          if (m < nbf && n < naux) gmem(out)[((int64_t)m * nbf5 + t.batch) * naux + n] = acc[i][j][r];
+          if (m < nbf && n < naux) gmem(out)[(int64_t)m * ldrow + (int64_t)t.batch * naux + n] = acc[i][j][r];
         }
   }
 };
@@ -696,14 +697,15 @@ hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
 }
 
 hipError_t launch_mo_x3(const SplitGeom& g, int naux, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
-                        const uint16_t* x_hi, const uint16_t* x_lo, double* b_mo, hipStream_t s) {
+                        const uint16_t* x_hi, const uint16_t* x_lo, double* b_mo, int64_t ldrow, hipStream_t s) {
   // A 128-column tile may read up to 64 columns past naux_p (X16 slack) and drops them.
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
-  if (tc.fn != 2 && tc.fn != 4) return hipErrorInvalidValue;
+  if ((tc.fn != 2 && tc.fn != 4) || ldrow < (int64_t)g.nbf5 * naux) return hipErrorInvalidValue;
   ProbMO pr;
   pr.ct = {ct_hi, ct_lo};
   pr.x = {x_hi, x_lo};
   pr.out = b_mo;
+  pr.ldrow = ldrow;
   pr.nbf = g.nbf;
   pr.naux = naux;
   pr.naux_p = g.naux_p;

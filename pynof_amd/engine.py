@@ -209,3 +209,37 @@ class JKEngine:
             self.nbf if Hi is not None else 0, prec, _ptr(res["b_MO"]) if "b_MO" in res else None,
             _ptr(res["H_mat"]) if "H_mat" in res else None, stream))
         return res
+
+    def lagrangian(self, C, nbf5: int, H, n, cj12, ck12, nbeta: int, nalpha: int, precision: str = "fp64",
+                   want_hmat: bool = True, out=None, stream=None):
+        """Closed-shell orbital-gradient Lagrangian (computeLagrange2 with RI, utils.py:262-355):
+        returns {"elag": (nbf, nbf), "H_mat": (nbf, nbf5)}. n (nbf5,), cj12 and ck12
+        (nbf5, nbf5) with diagonals taken as zero; inputs are not modified. b_MO stays on
+        the device. precision "fp64" or "split" (of b_MO); contractions are FP64."""
+        if self.nbf == 0:
+            raise _lib.PnjkError(_lib.ESTATE, "no b uploaded")
+        prec = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        nbf5 = int(nbf5)
+        Ci, Hi = _as_input(C, "C"), _as_input(H, "H")
+        if tuple(Ci.shape) != (self.nbf, self.nbf) or tuple(Hi.shape) != (self.nbf, self.nbf):
+            raise ValueError(f"C and H must have shape ({self.nbf}, {self.nbf})")
+        ni, cji, cki = _as_input(n, "n"), _as_input(cj12, "cj12"), _as_input(ck12, "ck12")
+        if tuple(ni.shape) != (nbf5,) or tuple(cji.shape) != (nbf5, nbf5) or tuple(cki.shape) != (nbf5, nbf5):
+            raise ValueError(f"n must have shape ({nbf5},), cj12 and ck12 ({nbf5}, {nbf5})")
+        on_device = _is_torch(Ci) and Ci.is_cuda
+        shapes = {"elag": (self.nbf, self.nbf), "H_mat": (self.nbf, nbf5)}
+        res = {}
+        for name in ("elag", "H_mat") if want_hmat else ("elag",):
+            if out is not None and name in out:
+                res[name] = out[name]
+            elif on_device:
+                res[name] = _torch().empty(shapes[name], dtype=_torch().float64, device=Ci.device)
+            else:
+                res[name] = np.empty(shapes[name], dtype=np.float64)
+        if stream is None and on_device:
+            stream = _torch().cuda.current_stream(Ci.device).cuda_stream
+        _lib.check(_lib.lib().pnjk_lagrangian(
+            self._h, _ptr(Ci), Ci.shape[1], nbf5, _ptr(Hi), Hi.shape[1], _ptr(ni), _ptr(cji), _ptr(cki),
+            int(nbeta), int(nalpha), prec, _ptr(res["elag"]), _ptr(res["H_mat"]) if "H_mat" in res else None,
+            stream))
+        return res

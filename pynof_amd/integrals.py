@@ -1,11 +1,12 @@
 """Drop-in replacements for PyNOF's RI JK builds.
 
-``computeJKj(C, I, b_mnl, p)``, ``computeJKH_MO(C, H, I, b_mnl, p)`` and
-``JKH_MO_tmp(C, H, I, b_mnl, p)`` keep the reference's signatures
-(src/pynof/integrals.py:172, :307 and :665), argument meaning and return values:
+``computeJKj(C, I, b_mnl, p)``, ``computeJKH_MO(C, H, I, b_mnl, p)``,
+``JKH_MO_tmp(C, H, I, b_mnl, p)`` and ``computeLagrange2(n, cj12, ck12, C, H, I,
+b_mnl, p)`` keep the reference's signatures (src/pynof/integrals.py:172, :307,
+:665; utils.py:262), argument meaning and return values:
 host FP64 arrays J, K of shape (nbf5, nbf, nbf); J_MO, K_MO (nbf5, nbf5) with
-H_core (nbf5,); and the orbital-gradient pair H_mat (nbf, nbf5), b_MO
-(nbf, nbf5, naux). ``I`` is ignored (it is ``[]`` in RI mode,
+H_core (nbf5,); the orbital-gradient pair H_mat (nbf, nbf5), b_MO
+(nbf, nbf5, naux); and the Lagrangian elag (nbf, nbf) with Hmat. ``I`` is ignored (it is ``[]`` in RI mode,
 integrals.py:28-33). Both run on the MI355X through the HIP library; there is no
 CPU path. ``p`` must provide ``nbf5``; ``p.RI`` must be true. ``p.precision``
 (optional, default "fp64") selects "fp64", "split" or "split_refine".
@@ -77,6 +78,29 @@ def JKH_MO_tmp(C, H, I, b_mnl, p):
     prec = "split" if getattr(p, "precision", "fp64") == "split" else "fp64"
     r = eng.mo_integrals(C, p.nbf5, H=H, precision=prec)
     return r["H_mat"], r["b_MO"]
+
+
+def computeLagrange2(n, cj12, ck12, C, H, I, b_mnl, p):
+    """(elag, Hmat) of the orbital gradient (utils.py:262-382 with p.RI): host FP64
+    elag (nbf, nbf) and Hmat (nbf, nbf5). As in the reference, the diagonals of the
+    caller's cj12 and ck12 are set to zero in place (:269-270), and for p.MSpin != 0
+    elag is returned as zeros (the RI branch fills it for MSpin == 0 only). b_MO is
+    formed and contracted on the device and never copied to the host."""
+    _check_p(p)
+    import numpy as np
+
+    np.fill_diagonal(cj12, 0)
+    np.fill_diagonal(ck12, 0)
+    eng = _engine_for(b_mnl, p)
+    prec = "split" if getattr(p, "precision", "fp64") == "split" else "fp64"
+    nbf5 = p.nbf5
+    if getattr(p, "MSpin", 0) != 0:
+        r = eng.mo_integrals(C, nbf5, H=H, precision=prec, outputs=("H_mat",))
+        return np.zeros((eng.nbf, eng.nbf)), r["H_mat"]
+    nbeta = int(getattr(p, "nbeta", 0))
+    nalpha = int(getattr(p, "nalpha", nbeta))
+    r = eng.lagrangian(C, nbf5, H, np.asarray(n)[:nbf5], cj12, ck12, nbeta, nalpha, precision=prec)
+    return r["elag"], r["H_mat"]
 
 
 def release_cached_engines() -> None:

@@ -219,3 +219,72 @@ def test_mo_integrals_planted_mistakes_fail():
     # C transposed on the right of H_mat: C[j, n] instead of C[n, j]
     wrong_h = np.einsum("mi,mn,jn->ij", C, H, C[:nbf5, :])
     assert rel(wrong_h, Hl) > 1e-3
+
+
+# ---- orbital-gradient Lagrangian (computeLagrange2 with RI, utils.py:262-355)
+
+def lag_inputs(nbf=6, naux=7, nbf5=4, seed=81):
+    b, C, H, _ = tiny(nbf, naux, nbf5, seed=seed)
+    rng = np.random.default_rng(seed + 3)
+    n = rng.uniform(0.05, 0.95, nbf5)
+    cj = rng.standard_normal((nbf5, nbf5))
+    ck = rng.standard_normal((nbf5, nbf5))
+    return b, C, H, n, cj + cj.T, ck + ck.T
+
+
+def fd_gradient(n, cj, ck, C, H, b, nbf5, nbeta, nalpha, eps=1e-5):
+    def E(Cx):
+        JM, KM, Hc = O.jkh_mo_ri(Cx, H, b, nbf5)
+        return O.calce_closed_shell(n, cj, ck, JM, KM, Hc, nbeta, nalpha, nbf5)
+    nv = sum(C.shape[0] - i - 1 for i in range(nbf5))
+    g = np.zeros(nv)
+    for m in range(nv):
+        y = np.zeros(nv)
+        y[m] = eps
+        g[m] = (E(O.rotate_orbital(y, C, nbf5)) - E(O.rotate_orbital(-y, C, nbf5))) / (2 * eps)
+    return g
+
+
+@pytest.mark.parametrize("nbeta,nalpha", [(2, 2), (1, 3), (4, 4)])
+def test_lagrangian_loops_and_energy_gradient(nbeta, nalpha):
+    """elag matches explicit loops, and 4 (elag - elag^T) is the derivative of the
+    closed-shell energy (pnof.py:905-922) under C -> C exp(kappa) (utils.py:550-565):
+    an independent check of every term and sign."""
+    b, C, H, n, cj, ck = lag_inputs()
+    nbf5 = 4
+    el, Hm = O.lagrange2_ri(n, cj, ck, C, H, b, nbf5, nbeta, nalpha)
+    ell, _ = O.lagrange2_loops(n, cj, ck, C, H, b, nbf5, nbeta, nalpha)
+    assert rel(el, ell) < 1e-13
+    assert np.all(el[:, nbf5:] == 0.0)
+    g = O.orbital_gradient(el, nbf5)
+    fd = fd_gradient(n, cj, ck, C, H, b, nbf5, nbeta, nalpha)
+    assert np.max(np.abs(g - fd)) < 1e-7 * max(1.0, np.max(np.abs(g)))
+
+
+def test_lagrangian_does_not_mutate_and_open_shell_is_zero():
+    b, C, H, n, cj, ck = lag_inputs(seed=85)
+    cj0, ck0 = cj.copy(), ck.copy()
+    el, Hm = O.lagrange2_ri(n, cj, ck, C, H, b, 4, 2, 2, MSpin=1)
+    assert np.array_equal(cj, cj0) and np.array_equal(ck, ck0)
+    assert np.all(el == 0.0) and Hm.shape == (6, 4)
+
+
+def test_lagrangian_planted_mistakes_fail():
+    b, C, H, n, cj, ck = lag_inputs(seed=86)
+    nbf5, nbeta, nalpha = 4, 2, 2
+    fd = fd_gradient(n, cj, ck, C, H, b, nbf5, nbeta, nalpha)
+    Hm, B = O.integrals_mo_ri(C, H, b, nbf5)
+    el, _ = O.lagrange2_ri(n, cj, ck, C, H, b, nbf5, nbeta, nalpha)
+    # the exchange term with the wrong sign
+    cjz, ckz = cj.copy(), ck.copy()
+    np.fill_diagonal(cjz, 0.0)
+    np.fill_diagonal(ckz, 0.0)
+    xterm = np.einsum("bq,aqk,bqk->ab", ckz, B[:, :nbf5], B[:nbf5, :nbf5])
+    wrong = el.copy()
+    wrong[:, :nbf5] += 2.0 * xterm
+    assert np.max(np.abs(O.orbital_gradient(wrong, nbf5) - fd)) > 1e-3
+    # the dJ_pp term dropped
+    djj = np.einsum("b,abk,bbk->ab", n[:nbeta], B[:, :nbeta], B[:nbeta, :nbeta])
+    wrong2 = el.copy()
+    wrong2[:, :nbeta] -= djj
+    assert np.max(np.abs(O.orbital_gradient(wrong2, nbf5) - fd)) > 1e-3
