@@ -287,6 +287,34 @@ def main():
     del mo_out, mo_ref
     torch.cuda.empty_cache()
 
+    # ---- orbital-gradient Lagrangian (computeLagrange2): MO integrals + contractions on the device
+    rng = np.random.default_rng(7000)
+    cj = rng.standard_normal((nbf5, nbf5))
+    ck = rng.standard_normal((nbf5, nbf5))
+    lag_in = [torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in (n_occ, cj + cj.T, ck + ck.T)]
+    nb = w.ndoc
+    lag = {}
+    lag_ref = None
+    for form in ("fp64", "split"):
+        log(f"Lagrangian {form}")
+        lo = eng.lagrangian(Cd[0], nbf5, Hd, *lag_in, nb, nb, precision=form, stream=stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        reps = max(1, args.form_steps)
+        for _ in range(reps):
+            lo = eng.lagrangian(Cd[0], nbf5, Hd, *lag_in, nb, nb, precision=form, out=lo, stream=stream)
+        e1.record()
+        torch.cuda.synchronize()
+        lag[form] = {"ms": round(e0.elapsed_time(e1) / reps, 3)}
+        if form == "fp64":
+            lag_ref = lo["elag"].clone()
+        else:
+            lag[form]["rel_err_elag_vs_fp64"] = float(torch.linalg.norm(lo["elag"] - lag_ref) /
+                                                      torch.linalg.norm(lag_ref))
+    lag["exchange_flops"] = 2.0 * nbf * nbf5 * nbf5 * naux
+    del lag_in, lag_ref, lo
+    torch.cuda.empty_cache()
+
     line = {
         "metric": METRIC,
         "value": round(1e3 / ms_per_step, 4),
@@ -313,6 +341,7 @@ def main():
         "roofline": roofline,
         "roofline_fp64": roofline_fp64,
         "mo_integrals": mo,
+        "lagrangian": lag,
     }
     eng.close()
     del out, Cd, Hd

@@ -130,16 +130,17 @@ struct pnjk_engine {
   DevBuf ct16h, ct16l, x16h, x16l, d16h, d16l;  // split workspace
   DevBuf Cf, ctf16h, ctf16l, hc;                 // MO integrals: all columns of C; H_core scratch
   DevBuf bmo, hm, lagw, lagg, lagp, lagc;        // Lagrangian workspace: b_MO, H_mat, W, G, partials, n/C^J/C^K
+  DevBuf bm16h, bm16l, w16h, w16l;               // split form of the Lagrangian's exchange operands
   bool split_ready = false;
   // Tile override per arithmetic (0: FP64 kernels, 1: BF16x3 kernels) and stage;
   // {0, 0} selects the built-in choice.
   TileF64 tiles[2][ST_COUNT] = {};
-  DevBuf* all[32];
+  DevBuf* all[36];
   int nall = 0;
   pnjk_engine() {
     for (DevBuf* b : {&btri, &bt16h, &bt16l, &bht16h, &bht16l, &Cp, &Hp, &X, &d, &kpart, &T, &outs, &stage, &scal,
                       &ct16h, &ct16l, &x16h, &x16l, &d16h, &d16l, &Cf, &ctf16h, &ctf16l, &hc,
-                      &bmo, &hm, &lagw, &lagg, &lagp, &lagc})
+                      &bmo, &hm, &lagw, &lagg, &lagp, &lagc, &bm16h, &bm16l, &w16h, &w16l})
       all[nall++] = b;
   }
 };
@@ -746,7 +747,9 @@ int pnjk_lagrangian(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, 
   const int64_t kc = roundup(std::max<int64_t>(1, (kfull + want - 1) / want), 64);
   const int64_t nchunks = (kfull + kc - 1) / kc, ldrow = nchunks * kc;
   if ((rc = e->bmo.ensure((size_t)(nbf * ldrow) * 8)) || (rc = e->hm.ensure((size_t)(nbf * n5) * 8)) ||
-      (rc = e->lagw.ensure((size_t)(n5 * ldrow) * 8)) || (rc = e->lagg.ensure((size_t)(n5 * naux) * 8)) ||
+      // lagw: the FP64 W of the exchange (FP64 form), and staging for C^J, C^K in both forms
+      (rc = e->lagw.ensure((size_t)(precision == PNJK_SPLIT ? n5 * n5 : n5 * ldrow) * 8)) ||
+      (rc = e->lagg.ensure((size_t)(n5 * naux) * 8)) ||
       (rc = e->lagp.ensure((size_t)(nchunks * nbf * n5) * 8)) || (rc = e->lagc.ensure((size_t)(3 * n5 * n5) * 8)))
     return rc;
   const bool host_e = !is_device_ptr(elag), host_h = H_mat && !is_device_ptr(H_mat);
@@ -766,8 +769,18 @@ int pnjk_lagrangian(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, 
   if ((rc = to_device(ck12, e->lagw.as<double>(), n5 * n5, s))) return rc;
   PNJK_HIP_CHECK(launch_lag_coef(e->lagw.as<double>(), ckd, (int)n5, s));
   if ((rc = to_device(n, nd, n5, s))) return rc;
+  const bool split = precision == PNJK_SPLIT;
+  if (split) {  // exchange in BF16x3
+    const size_t bb = (size_t)(nbf * ldrow) * 2, wb = (size_t)(n5 * ldrow) * 2;
+    if ((rc = e->bm16h.ensure(bb)) || (rc = e->bm16l.ensure(bb)) || (rc = e->w16h.ensure(wb)) ||
+        (rc = e->w16l.ensure(wb)))
+      return rc;
+    PNJK_HIP_CHECK(launch_lag_x3((int)nbf, (int)n5, (int)naux, B, ldrow, kc, (int)nchunks, ckd,
+                                 e->bm16h.as<uint16_t>(), e->bm16l.as<uint16_t>(), e->w16h.as<uint16_t>(),
+                                 e->w16l.as<uint16_t>(), e->lagp.as<double>(), s));
+  }
   PNJK_HIP_CHECK(launch_lag_f64(g, B, ldrow, kc, (int)nchunks, nd, cjd, ckd, (int)nbeta, (int)nalpha, Hm,
-                                e->lagg.as<double>(), e->lagw.as<double>(), e->lagp.as<double>(), dE, s));
+                                e->lagg.as<double>(), e->lagw.as<double>(), e->lagp.as<double>(), dE, !split, s));
   if (host_e) PNJK_HIP_CHECK(hipMemcpyAsync(elag, dE, (size_t)(nbf * nbf) * 8, hipMemcpyDeviceToHost, s));
   if (H_mat)
     PNJK_HIP_CHECK(hipMemcpyAsync(H_mat, Hm, (size_t)(nbf * n5) * 8,

@@ -915,15 +915,15 @@ __global__ void __launch_bounds__(256) k_lag_final(const double* __restrict__ B,
   }
   const double* row = B + (int64_t)a * ldrow + (int64_t)b * naux;
   const double* g = G + (int64_t)b * naux;
-  double v = 0.0;
+  double v = 0.0, x = 0.0;
   for (int P = lane; P < naux; P += 64) v += row[P] * g[P];
+  for (int c = lane; c < nchunks; c += 64) x += part[((int64_t)c * nbf + a) * nbf5 + b];
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  if (lane == 0) {
-    double x = 0.0;
-    for (int c = 0; c < nchunks; ++c) x += part[((int64_t)c * nbf + a) * nbf5 + b];
-    elag[w] = nocc[b] * Hm[(int64_t)a * nbf5 + b] + v - x;
+  for (int off = 32; off > 0; off >>= 1) {
+    v += __shfl_xor(v, off, 64);
+    x += __shfl_xor(x, off, 64);
   }
+  if (lane == 0) elag[w] = nocc[b] * Hm[(int64_t)a * nbf5 + b] + v - x;
 }
 
 hipError_t launch_mo_f64(const BuildGeom& g, TileF64 tc, const double* Cf, int ldcf, const double* X, double* b_mo,
@@ -966,26 +966,29 @@ hipError_t launch_hmat_f64(const BuildGeom& g, const double* Cf, int ldcf, const
 
 hipError_t launch_lag_f64(const BuildGeom& g, const double* B, int64_t ldrow, int64_t kc, int nchunks,
                           const double* nocc, const double* cj, const double* ck, int nbeta, int nalpha,
-                          const double* Hm, double* G, double* W, double* part, double* elag, hipStream_t s) {
+                          const double* Hm, double* G, double* W, double* part, double* elag, bool exchange,
+                          hipStream_t s) {
   const int nbf = g.nbf, n5 = g.nbf5, naux = g.naux;
   if (kc % 16 != 0 || (int64_t)nchunks * kc != ldrow || ldrow < (int64_t)n5 * naux) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_lag_g, dim3(cdiv(naux, 256), n5), dim3(256), 0, s, B, ldrow, naux, n5, nbeta, nalpha, nocc, cj,
                      G);
-  const int64_t nw = (int64_t)n5 * ldrow;
-  hipLaunchKernelGGL(k_lag_w, dim3((unsigned)cdiv(nw, 256)), dim3(256), 0, s, B, ldrow, naux, n5, ck, W);
-  const TileF64 tc{4, 2};
-  ProbLagX pr;
-  pr.B = B;
-  pr.W = W;
-  pr.part = part;
-  pr.ldrow = ldrow;
-  pr.kc = kc;
-  pr.nbf = nbf;
-  pr.nbf5 = n5;
-  pr.mtiles = cdiv(nbf, 32 * tc.fm);
-  pr.ntiles = cdiv(n5, 32 * tc.fn);
-  hipError_t e = launch_prob(pr, tc, pr.mtiles * pr.ntiles * nchunks, s);
-  if (e != hipSuccess) return e;
+  if (exchange) {  // FP64 exchange partials (otherwise the caller formed them)
+    const int64_t nw = (int64_t)n5 * ldrow;
+    hipLaunchKernelGGL(k_lag_w, dim3((unsigned)cdiv(nw, 256)), dim3(256), 0, s, B, ldrow, naux, n5, ck, W);
+    const TileF64 tc{4, 2};
+    ProbLagX pr;
+    pr.B = B;
+    pr.W = W;
+    pr.part = part;
+    pr.ldrow = ldrow;
+    pr.kc = kc;
+    pr.nbf = nbf;
+    pr.nbf5 = n5;
+    pr.mtiles = cdiv(nbf, 32 * tc.fm);
+    pr.ntiles = cdiv(n5, 32 * tc.fn);
+    hipError_t e = launch_prob(pr, tc, pr.mtiles * pr.ntiles * nchunks, s);
+    if (e != hipSuccess) return e;
+  }
   const int64_t waves = (int64_t)nbf * nbf;
   hipLaunchKernelGGL(k_lag_final, dim3((unsigned)cdiv(waves, 4)), dim3(256), 0, s, B, ldrow, naux, nbf, n5, G, Hm, nocc,
                      part, nchunks, elag);

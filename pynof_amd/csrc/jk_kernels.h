@@ -85,7 +85,13 @@ hipError_t launch_mo_f64(const BuildGeom& g, TileF64 tc, const double* Cf, int l
 // elag (nbf x nbf). G (nbf5 x naux), W (nbf5 x ldrow), part (nchunks x nbf x nbf5) are workspace.
 hipError_t launch_lag_f64(const BuildGeom& g, const double* B, int64_t ldrow, int64_t kc, int nchunks,
                           const double* nocc, const double* cj, const double* ck, int nbeta, int nalpha,
-                          const double* Hm, double* G, double* W, double* part, double* elag, hipStream_t s);
+                          const double* Hm, double* G, double* W, double* part, double* elag, bool exchange,
+                          hipStream_t s);
+// BF16x3 exchange partials of the Lagrangian (split form): converts B to planes (bh, bl:
+// nbf x ldrow) and forms W16 (wh, wl: nbf5 x ldrow); part as in launch_lag_f64.
+hipError_t launch_lag_x3(int nbf, int nbf5, int naux, const double* B, int64_t ldrow, int64_t kc, int nchunks,
+                         const double* ck, uint16_t* bh, uint16_t* bl, uint16_t* wh, uint16_t* wl, double* part,
+                         hipStream_t s);
 hipError_t launch_lag_coef(const double* src, double* dst, int n, hipStream_t s);
 hipError_t launch_fill_cols(double* p, int rows, int64_t ld, int64_t c0, int64_t c1, hipStream_t s);
 hipError_t launch_hmat_f64(const BuildGeom& g, const double* Cf, int ldcf, const double* T, double* hmat,

@@ -455,6 +455,43 @@ struct ProbMO {
   }
 };
 
+// Exchange term of the orbital-gradient Lagrangian in BF16x3 (utils.py:353-355), split
+// over k' = (q, P) in chunks of kc: part[c][a][b] = sum_{k' in chunk c} B16[a][k'] W16[b][k'],
+// W = ck[b][q] B[b][k'], both operands rows of stride ldrow (zero past nbf5 * naux).
+struct ProbLagX {
+  static constexpr int kA = RK, kB = RK;
+  Planes bm, w;
+  double* part;
+  int64_t ldrow, kc;
+  int nbf, nbf5, mtiles, ntiles;
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.mt = id % mtiles;
+    const int rest = id / mtiles;
+    t.nt = rest % ntiles;
+    t.batch = rest / ntiles;  // k chunk
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ int ksteps() const { return (int)(kc / kBK); }
+  __device__ SrcRows a_src(const Tile& t) const { return {bm, (int64_t)t.batch * kc, ldrow, nbf}; }
+  __device__ SrcRows b_src(const Tile& t) const { return {w, (int64_t)t.batch * kc, ldrow, nbf5}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int m = t.m0 + dm, n = t.n0 + dn;
+          if (m < nbf && n < nbf5) gmem(part)[((int64_t)t.batch * nbf + m) * nbf5 + n] = acc[i][j][r];
+        }
+  }
+};
+
 // J[q][mu][nu] = sum_P d16[q][P] btri16[t][P] for mu >= nu (FP64 output; the
 // upper triangle is filled afterwards by launch_symmetrize)
 struct ProbJN {
@@ -637,6 +674,40 @@ inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 hipError_t launch_split_btri(const double* btri, int64_t n, uint16_t* hi, uint16_t* lo, hipStream_t s) {
   hipLaunchKernelGGL(k_split_packed, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, btri, n, hi, lo);
   return hipGetLastError();
+}
+
+// W16[b][k'] = split(ck[b][q] B[b][k']) for k' = q naux + P < nbf5 naux, zero up to ldrow.
+__global__ void k_lag_w16(const double* __restrict__ B, int64_t ldrow, int naux, int nbf5,
+                          const double* __restrict__ ck, uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)nbf5 * ldrow) return;
+  const int b = (int)(i / ldrow);
+  const int q = (int)((i - (int64_t)b * ldrow) / naux);
+  uint16_t h, l;
+  split2(q < nbf5 ? ck[(int64_t)b * nbf5 + q] * B[i] : 0.0, h, l);
+  hi[i] = h;
+  lo[i] = l;
+}
+
+hipError_t launch_lag_x3(int nbf, int nbf5, int naux, const double* B, int64_t ldrow, int64_t kc, int nchunks,
+                         const double* ck, uint16_t* bh, uint16_t* bl, uint16_t* wh, uint16_t* wl, double* part,
+                         hipStream_t s) {
+  if (kc % kBK != 0 || (int64_t)nchunks * kc != ldrow) return hipErrorInvalidValue;
+  const int64_t nb = (int64_t)nbf * ldrow, nw = (int64_t)nbf5 * ldrow;
+  hipLaunchKernelGGL(k_split_packed, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, s, B, nb, bh, bl);
+  hipLaunchKernelGGL(k_lag_w16, dim3((unsigned)cdiv(nw, 256)), dim3(256), 0, s, B, ldrow, naux, nbf5, ck, wh, wl);
+  const TileF64 tc{5, 2};
+  ProbLagX pr;
+  pr.bm = {bh, bl};
+  pr.w = {wh, wl};
+  pr.part = part;
+  pr.ldrow = ldrow;
+  pr.kc = kc;
+  pr.nbf = nbf;
+  pr.nbf5 = nbf5;
+  pr.mtiles = cdiv(nbf, 32 * tc.fm);
+  pr.ntiles = cdiv(nbf5, 32 * tc.fn);
+  return launch_x3(pr, tc, pr.mtiles * pr.ntiles * nchunks, s);
 }
 
 hipError_t launch_build_bht(const double* btri, int nbf, int naux_p, int ldk, uint16_t* hi, uint16_t* lo,
