@@ -245,6 +245,53 @@ def lagrange2_loops(n, cj12, ck12, C, H, b_mnl, nbf5, nbeta, nalpha):
     return elag, Hmat
 
 
+def fock_rc(J, K, n, H, cj12, ck12, nbeta, nalpha, no1=0):
+    """Generalised Fock matrices of the closed-shell case, computeF_RC_CPU
+    (utils.py:25-54): F[i] = n_i H + w_i n_i J[i] + sum_j cj'[i,j] J[j] - sum_j ck'[i,j] K[j]
+    with w_i = 1 for ini <= i < nbeta or nalpha <= i < nbf5, ini = no1 if no1 > 1 else 0,
+    and cj', ck' the coefficients with the diagonal of their [ini:, ini:] block zeroed (the
+    reference zeroes it in the caller's arrays, :45, :50; this copy leaves them alone)."""
+    nbf5 = J.shape[0]
+    ini = no1 if no1 > 1 else 0
+    cj = np.array(cj12, dtype=np.float64, copy=True)
+    ck = np.array(ck12, dtype=np.float64, copy=True)
+    np.fill_diagonal(cj[ini:, ini:], 0)
+    np.fill_diagonal(ck[ini:, ini:], 0)
+    F = np.einsum("i,mn->imn", n, H, optimize=True)
+    F[ini:nbeta] += np.einsum("i,imn->imn", n[ini:nbeta], J[ini:nbeta], optimize=True)
+    F[nalpha:nbf5] += np.einsum("i,imn->imn", n[nalpha:nbf5], J[nalpha:nbf5], optimize=True)
+    F += np.einsum("ij,jmn->imn", cj, J, optimize=True)
+    F -= np.einsum("ij,jmn->imn", ck, K, optimize=True)
+    return F
+
+
+def lagrange_fock(F, C, noptorb):
+    """Lagrange multipliers from the Fock matrices, computeLagrange (utils.py:249-259):
+    G[m, i] = sum_n F[i, m, n] C[n, i]; elag[i, j] = sum_m C[m, i] G[m, j] for i < noptorb,
+    j < nbf5; zero elsewhere."""
+    nbf5, nbf = F.shape[0], F.shape[1]
+    G = np.einsum("imn,ni->mi", F, C[:, :nbf5], optimize=True)
+    elag = np.zeros((nbf, nbf))
+    elag[:noptorb, :nbf5] = np.einsum("mi,mj->ij", C[:, :noptorb], G, optimize=True)[:noptorb, :nbf5]
+    return elag
+
+
+def fock_loops(J, K, n, H, cj12, ck12, nbeta, nalpha, no1=0):
+    """fock_rc by explicit loops over i, j (tiny inputs only)."""
+    nbf5, nbf = J.shape[0], J.shape[1]
+    ini = no1 if no1 > 1 else 0
+    F = np.zeros((nbf5, nbf, nbf))
+    for i in range(nbf5):
+        F[i] = n[i] * H
+        if ini <= i < nbeta or nalpha <= i < nbf5:
+            F[i] += n[i] * J[i]
+        for j in range(nbf5):
+            if i == j and i >= ini:
+                continue
+            F[i] += cj12[i, j] * J[j] - ck12[i, j] * K[j]
+    return F
+
+
 def orbital_gradient(elag, nbf5):
     """calcorbg's gradient vector (pnof.py:1114-1131): 4 (elag - elag^T)[i, j] for
     i < nbf5, j > i, in rotate_orbital's variable order (utils.py:550-565)."""

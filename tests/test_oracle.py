@@ -288,3 +288,43 @@ def test_lagrangian_planted_mistakes_fail():
     wrong2 = el.copy()
     wrong2[:, :nbeta] -= djj
     assert np.max(np.abs(O.orbital_gradient(wrong2, nbf5) - fd)) > 1e-3
+
+
+# ---- generalised Fock and Lagrange multipliers (computeF_RC_CPU, computeLagrange)
+
+@pytest.mark.parametrize("nbeta,nalpha,no1", [(2, 2, 0), (1, 3, 0), (3, 3, 2)])
+def test_fock_loops(nbeta, nalpha, no1):
+    b, C, H, n, cj, ck = lag_inputs(seed=90)
+    nbf5 = 4
+    J, K = O.jkj_ri(C, b, nbf5)
+    F = O.fock_rc(J, K, n, H, cj, ck, nbeta, nalpha, no1)
+    Fl = O.fock_loops(J, K, n, H, cj, ck, nbeta, nalpha, no1)
+    assert F.shape == (nbf5, 6, 6) and rel(F, Fl) < 1e-13
+
+
+@pytest.mark.parametrize("nbeta,nalpha", [(2, 2), (1, 3)])
+def test_fock_lagrangian_equals_mo_lagrangian(nbeta, nalpha):
+    """The Lagrangian from the Fock matrices (utils.py:249-259) equals the one from the
+    MO integrals (utils.py:262-355), which the energy's finite-difference gradient pins:
+    C_a^T J[j] C_b = sum_P d[j,P] b_MO[a,b,P] and C_a^T K[j] C_b = sum_P b_MO[a,j,P] b_MO[b,j,P]."""
+    b, C, H, n, cj, ck = lag_inputs(seed=91)
+    nbf5 = 4
+    J, K = O.jkj_ri(C, b, nbf5)
+    F = O.fock_rc(J, K, n, H, cj, ck, nbeta, nalpha)
+    el = O.lagrange_fock(F, C, C.shape[0])
+    el2, _ = O.lagrange2_ri(n, cj, ck, C, H, b, nbf5, nbeta, nalpha)
+    assert rel(el, el2) < 1e-12
+    # noptorb < nbf keeps only the first rows
+    el3 = O.lagrange_fock(F, C, 3)
+    assert np.array_equal(el3[:3], el[:3]) and np.all(el3[3:] == 0.0)
+
+
+def test_fock_planted_mistake_fails():
+    b, C, H, n, cj, ck = lag_inputs(seed=92)
+    nbf5 = 4
+    J, K = O.jkj_ri(C, b, nbf5)
+    F = O.fock_rc(J, K, n, H, cj, ck, 2, 2)
+    el2, _ = O.lagrange2_ri(n, cj, ck, C, H, b, nbf5, 2, 2)
+    # C^K applied to J instead of K
+    wrong = F + np.einsum("ij,jmn->imn", ck - np.diag(np.diag(ck)), K - J)
+    assert rel(O.lagrange_fock(wrong, C, 6), el2) > 1e-3
