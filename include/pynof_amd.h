@@ -124,6 +124,31 @@ int pnjk_lagrangian(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, 
                     const double* n, const double* cj12, const double* ck12, int64_t nbeta, int64_t nalpha,
                     int precision, double* elag, double* H_mat, void* stream);
 
+/* Generalised Fock matrices, closed shell: replaces computeF_RC_driver /
+ * computeF_RC_CPU / computeF_RC_GPU (utils.py:15-54, :58-84), which ENERGY1r
+ * (utils.py:415-427) and the orbital optimisers call when p.no1 > 0:
+ *   F[i] = n[i] H + w_i n[i] J[i] + sum_j cj12'[i][j] J[j] - sum_j ck12'[i][j] K[j]
+ * for i < nbf5, with w_i = 1 for ini <= i < nbeta or nalpha <= i < nbf5,
+ * ini = no1 if no1 > 1 else 0 (utils.py:31-33), and cj12', ck12' the
+ * coefficients with the diagonal of their [ini:, ini:] block taken as zero (the
+ * reference zeroes it in the caller's arrays, utils.py:45, :50; this call leaves
+ * its inputs unchanged). J, K, F are (nbf5, nbf, nbf) C-order; H (nbf, nbf) of
+ * row stride ldh; n (nbf5); cj12, ck12 (nbf5, nbf5). Arrays may be host or
+ * device memory; the engine needs no b. One FP64 GEMM over the stacked J and K
+ * (MFMA f64) with n[i] H added in its epilogue. Needs nbf^2 <= 2^30. */
+int pnjk_fock_rc(pnjk_engine* e, const double* J, const double* K, int64_t nbf, int64_t nbf5, const double* n,
+                 const double* H, int64_t ldh, const double* cj12, const double* ck12, int64_t no1, int64_t nbeta,
+                 int64_t nalpha, double* F, void* stream);
+
+/* Lagrange multipliers from the Fock matrices: replaces computeLagrange
+ * (utils.py:249-259):
+ *   G[m][i]    = sum_n F[i][m][n] C[n][i]                 i < nbf5
+ *   elag[i][j] = sum_m C[m][i] G[m][j]   for i < noptorb, j < nbf5; 0 elsewhere.
+ * F (nbf5, nbf, nbf) C-order, C (nbf, ldc >= nbf) row-major, elag (nbf, nbf).
+ * Host or device memory; FP64 throughout; the engine needs no b. */
+int pnjk_lagrange_fock(pnjk_engine* e, const double* F, int64_t nbf, int64_t nbf5, const double* C, int64_t ldc,
+                       int64_t noptorb, double* elag, void* stream);
+
 /* Grouped builds: many engines (one b each, different sizes allowed) built in
  * one pass. Each GEMM stage of all instances runs as one persistent grouped
  * launch per problem type and tile shape (tile shapes per instance from the

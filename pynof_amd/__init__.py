@@ -8,7 +8,14 @@ Entry points:
 """
 from ._lib import PnjkError  # noqa: F401
 from .engine import JKEngine  # noqa: F401
-from .integrals import JKH_MO_tmp, computeJKj, computeJKH_MO, computeLagrange2  # noqa: F401
+from .integrals import (  # noqa: F401
+    JKH_MO_tmp,
+    computeF_RC_driver,
+    computeJKH_MO,
+    computeJKj,
+    computeLagrange,
+    computeLagrange2,
+)
 from .parameters import param  # noqa: F401
 
 __all__ = ["JKEngine", "computeJKj", "computeJKH_MO", "JKH_MO_tmp", "computeLagrange2", "param", "PnjkError"]

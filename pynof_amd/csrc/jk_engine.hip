@@ -131,16 +131,17 @@ struct pnjk_engine {
   DevBuf Cf, ctf16h, ctf16l, hc;                 // MO integrals: all columns of C; H_core scratch
   DevBuf bmo, hm, lagw, lagg, lagp, lagc;        // Lagrangian workspace: b_MO, H_mat, W, G, partials, n/C^J/C^K
   DevBuf bm16h, bm16l, w16h, w16l;               // split form of the Lagrangian's exchange operands
+  DevBuf fjk, fh, fco, ff;                       // Fock: staged J/K, H, coefficients, F
   bool split_ready = false;
   // Tile override per arithmetic (0: FP64 kernels, 1: BF16x3 kernels) and stage;
   // {0, 0} selects the built-in choice.
   TileF64 tiles[2][ST_COUNT] = {};
-  DevBuf* all[36];
+  DevBuf* all[40];
   int nall = 0;
   pnjk_engine() {
     for (DevBuf* b : {&btri, &bt16h, &bt16l, &bht16h, &bht16l, &Cp, &Hp, &X, &d, &kpart, &T, &outs, &stage, &scal,
                       &ct16h, &ct16l, &x16h, &x16l, &d16h, &d16l, &Cf, &ctf16h, &ctf16l, &hc,
-                      &bmo, &hm, &lagw, &lagg, &lagp, &lagc, &bm16h, &bm16l, &w16h, &w16l})
+                      &bmo, &hm, &lagw, &lagg, &lagp, &lagc, &bm16h, &bm16l, &w16h, &w16l, &fjk, &fh, &fco, &ff})
       all[nall++] = b;
   }
 };
@@ -623,6 +624,30 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
 
 }  // extern "C"
 
+// Full-width C, k-major: Cf[nu][p] for nu < nbf, p < nbf (from C, row stride ldc, host or
+// device); zero rows to nbf_p + 8 and columns to *ldcf, which covers every M-tile (at most
+// 160 rows) past nbf.
+static int load_cf(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf, int64_t nbf_p, hipStream_t s,
+                   int* ldcf_out) {
+  const int ldcf = (int)roundup(nbf, 32) + 160;
+  const int64_t cf_elems = (nbf_p + 8) * ldcf;
+  int rc;
+  if ((rc = e->Cf.ensure((size_t)cf_elems * 8))) return rc;
+  double* Cf = e->Cf.as<double>();
+  PNJK_HIP_CHECK(fill(Cf, cf_elems, 0.0, s));
+  if (is_device_ptr(C)) {
+    const int64_t n = nbf * nbf;
+    hipLaunchKernelGGL(k_copy2d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, C, ldc, Cf, (int64_t)ldcf,
+                       (int)nbf, (int)nbf);
+    PNJK_HIP_CHECK(hipGetLastError());
+  } else {
+    PNJK_HIP_CHECK(hipMemcpy2DAsync(Cf, (size_t)ldcf * 8, C, (size_t)ldc * 8, (size_t)nbf * 8, (size_t)nbf,
+                                    hipMemcpyHostToDevice, s));
+  }
+  *ldcf_out = ldcf;
+  return PNJK_OK;
+}
+
 // The MO integrals into device buffers: b_MO rows of stride ldrow (dB, may be
 // NULL) and H_mat (dHm, may be NULL). `st` comes from build_prepare with no outputs.
 static int mo_compute(pnjk_engine* e, BuildState& st, double* dB, int64_t ldrow, double* dHm, hipStream_t s) {
@@ -630,11 +655,7 @@ static int mo_compute(pnjk_engine* e, BuildState& st, double* dB, int64_t ldrow,
   const int64_t nbf = g.nbf, n5 = g.nbf5;
   const int precision = st.a.precision;
   int rc;
-  // Full-width C, k-major: Cf[nu][p] for nu < nbf, p < nbf; zero rows to nbf_p + 8 and
-  // columns to ldcf, which covers every M-tile (at most 160 rows) past nbf.
-  const int ldcf = (int)roundup(nbf, 32) + 160;
-  const int64_t cf_elems = (int64_t)(g.nbf_p + 8) * ldcf;
-  if ((rc = e->Cf.ensure((size_t)cf_elems * 8))) return rc;
+  int ldcf = 0;
   if (dHm) {  // T = H C[:, :nbf5] (through the H_core stage), then H_mat = Cf^T T
     st.want_h = true;
     if ((rc = e->Hp.ensure((size_t)(nbf * g.nbf_p) * 8)) || (rc = e->T.ensure((size_t)(nbf * g.ldcp + 64) * 8)) ||
@@ -644,17 +665,8 @@ static int mo_compute(pnjk_engine* e, BuildState& st, double* dB, int64_t ldrow,
     st.hH = false;
   }
   if ((rc = stage_in(e, st, s))) return rc;
+  if ((rc = load_cf(e, st.a.C, st.a.ldc, nbf, g.nbf_p, s, &ldcf))) return rc;
   double* Cf = e->Cf.as<double>();
-  PNJK_HIP_CHECK(fill(Cf, cf_elems, 0.0, s));
-  if (st.c_dev) {
-    const int64_t n = nbf * nbf;
-    hipLaunchKernelGGL(k_copy2d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, st.a.C, st.a.ldc, Cf,
-                       (int64_t)ldcf, (int)nbf, (int)nbf);
-    PNJK_HIP_CHECK(hipGetLastError());
-  } else {
-    PNJK_HIP_CHECK(hipMemcpy2DAsync(Cf, (size_t)ldcf * 8, st.a.C, (size_t)st.a.ldc * 8, (size_t)nbf * 8,
-                                    (size_t)nbf, hipMemcpyHostToDevice, s));
-  }
   if (dB) {
     if ((rc = stage_ht(e, st, s, nullptr))) return rc;
     const bool big = nbf >= 256;
@@ -786,6 +798,112 @@ int pnjk_lagrangian(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, 
     PNJK_HIP_CHECK(hipMemcpyAsync(H_mat, Hm, (size_t)(nbf * n5) * 8,
                                   host_h ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
   if (host_e || host_h) PNJK_HIP_CHECK(hipStreamSynchronize(s));
+  return PNJK_OK;
+}
+
+// Copies a rows x cols block (source row stride lds, host or device) to device memory of
+// row stride ldd.
+static int to_device_2d(const double* src, int64_t lds, double* dst, int64_t ldd, int64_t rows, int64_t cols,
+                        hipStream_t s) {
+  PNJK_HIP_CHECK(hipMemcpy2DAsync(dst, (size_t)ldd * 8, src, (size_t)lds * 8, (size_t)cols * 8, (size_t)rows,
+                                  is_device_ptr(src) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  return PNJK_OK;
+}
+
+int pnjk_fock_rc(pnjk_engine* e, const double* J, const double* K, int64_t nbf, int64_t nbf5, const double* n,
+                 const double* H, int64_t ldh, const double* cj12, const double* ck12, int64_t no1, int64_t nbeta,
+                 int64_t nalpha, double* F, void* stream) {
+  if (!e) return set_error(PNJK_EINVAL, "pnjk_fock_rc: engine is NULL");
+  std::lock_guard<std::mutex> lk(e->mu);
+  PNJK_HIP_CHECK(hipSetDevice(e->device));
+  hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
+  if (!J || !K || !n || !H || !cj12 || !ck12 || !F)
+    return set_error(PNJK_EINVAL, "pnjk_fock_rc: J, K, n, H, cj12, ck12 and F are required");
+  if (nbf < 1 || nbf5 < 1 || nbf5 > nbf || nbf * nbf > (int64_t)1 << 30)
+    return set_error(PNJK_EINVAL, "pnjk_fock_rc: need 1 <= nbf5 <= nbf and nbf^2 <= 2^30");
+  if (ldh < nbf) return set_error(PNJK_EINVAL, "pnjk_fock_rc: ldh < nbf");
+  if (no1 < 0 || nbeta < 0 || nalpha < nbeta || nalpha > nbf5)
+    return set_error(PNJK_EINVAL, "pnjk_fock_rc: need no1 >= 0 and 0 <= nbeta <= nalpha <= nbf5");
+  const int64_t n5 = nbf5, nn = nbf * nbf;
+  const int ini = no1 > 1 ? (int)no1 : 0;  // utils.py:31-33
+  // J and K feed the GEMM in place when they are device rows with 16-byte aligned pairs;
+  // otherwise they are staged with an even row stride.
+  const bool direct = is_device_ptr(J) && is_device_ptr(K) && nn % 2 == 0 && (uintptr_t)J % 16 == 0 &&
+                      (uintptr_t)K % 16 == 0;
+  const int64_t ld = direct ? nn : roundup(nn, 2);
+  const int64_t kp = roundup(2 * n5, 16);
+  int rc;
+  if ((!direct && (rc = e->fjk.ensure((size_t)(2 * n5 * ld) * 8))) || (rc = e->fh.ensure((size_t)nn * 8)) ||
+      (rc = e->fco.ensure((size_t)(n5 * kp + 2 * n5 * n5 + n5) * 8)))
+    return rc;
+  const bool host_f = !is_device_ptr(F);
+  if (host_f && (rc = e->ff.ensure((size_t)(n5 * nn) * 8))) return rc;
+  const double* dJ = J;
+  const double* dK = K;
+  if (!direct) {
+    double* jk = e->fjk.as<double>();
+    if ((rc = to_device_2d(J, nn, jk, ld, n5, nn, s)) || (rc = to_device_2d(K, nn, jk + n5 * ld, ld, n5, nn, s)))
+      return rc;
+    dJ = jk;
+    dK = jk + n5 * ld;
+  }
+  double* A = e->fco.as<double>();
+  double* cjd = A + n5 * kp;
+  double* ckd = cjd + n5 * n5;
+  double* nd = ckd + n5 * n5;
+  if ((rc = to_device_2d(H, ldh, e->fh.as<double>(), nbf, nbf, nbf, s)) || (rc = to_device(cj12, cjd, n5 * n5, s)) ||
+      (rc = to_device(ck12, ckd, n5 * n5, s)) || (rc = to_device(n, nd, n5, s)))
+    return rc;
+  double* dF = host_f ? e->ff.as<double>() : F;
+  PNJK_HIP_CHECK(launch_fock_f64((int)nbf, (int)n5, dJ, dK, ld, e->fh.as<double>(), nd, cjd, ckd, ini, (int)nbeta,
+                                 (int)nalpha, A, dF, s));
+  if (host_f) {
+    PNJK_HIP_CHECK(hipMemcpyAsync(F, dF, (size_t)(n5 * nn) * 8, hipMemcpyDeviceToHost, s));
+    PNJK_HIP_CHECK(hipStreamSynchronize(s));
+  } else if (!is_device_ptr(J) || !is_device_ptr(K) || !is_device_ptr(H) || !is_device_ptr(n) ||
+             !is_device_ptr(cj12) || !is_device_ptr(ck12)) {
+    // Host inputs were copied asynchronously from pageable memory the caller may reuse.
+    PNJK_HIP_CHECK(hipStreamSynchronize(s));
+  }
+  return PNJK_OK;
+}
+
+int pnjk_lagrange_fock(pnjk_engine* e, const double* F, int64_t nbf, int64_t nbf5, const double* C, int64_t ldc,
+                       int64_t noptorb, double* elag, void* stream) {
+  if (!e) return set_error(PNJK_EINVAL, "pnjk_lagrange_fock: engine is NULL");
+  std::lock_guard<std::mutex> lk(e->mu);
+  PNJK_HIP_CHECK(hipSetDevice(e->device));
+  hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
+  if (!F || !C || !elag) return set_error(PNJK_EINVAL, "pnjk_lagrange_fock: F, C and elag are required");
+  if (nbf < 1 || nbf5 < 1 || nbf5 > nbf || nbf5 > 65535 || nbf * nbf > (int64_t)1 << 30)
+    return set_error(PNJK_EINVAL, "pnjk_lagrange_fock: need 1 <= nbf5 <= min(nbf, 65535) and nbf^2 <= 2^30");
+  if (ldc < nbf) return set_error(PNJK_EINVAL, "pnjk_lagrange_fock: ldc < nbf (every column of C is used)");
+  if (noptorb < 0 || noptorb > nbf) return set_error(PNJK_EINVAL, "pnjk_lagrange_fock: need 0 <= noptorb <= nbf");
+  const int64_t n5 = nbf5, nn = nbf * nbf;
+  BuildGeom g{};
+  g.nbf = (int)nbf;
+  g.nbf5 = (int)n5;
+  g.nbf_p = (int)roundup(nbf, 32);
+  g.ldcp = (int)roundup(n5, 32);
+  int rc;
+  const bool host_f = !is_device_ptr(F), host_e = !is_device_ptr(elag);
+  if ((host_f && (rc = e->ff.ensure((size_t)(n5 * nn) * 8))) ||
+      (rc = e->T.ensure((size_t)(nbf * g.ldcp + 64) * 8)) || (rc = e->hm.ensure((size_t)(nbf * n5) * 8)) ||
+      (host_e && (rc = e->outs.ensure((size_t)nn * 8))))
+    return rc;
+  const double* dF = F;
+  if (host_f) {
+    if ((rc = to_device(F, e->ff.as<double>(), n5 * nn, s))) return rc;
+    dF = e->ff.as<double>();
+  }
+  int ldcf = 0;
+  if ((rc = load_cf(e, C, ldc, nbf, g.nbf_p, s, &ldcf))) return rc;
+  double* G = e->T.as<double>();
+  PNJK_HIP_CHECK(fill(G, nbf * g.ldcp + 64, 0.0, s));
+  double* dE = host_e ? e->outs.as<double>() : elag;
+  PNJK_HIP_CHECK(launch_lagrange_fock_f64(g, dF, e->Cf.as<double>(), ldcf, G, e->hm.as<double>(), (int)noptorb, dE, s));
+  if (host_e) PNJK_HIP_CHECK(hipMemcpyAsync(elag, dE, (size_t)nn * 8, hipMemcpyDeviceToHost, s));
+  if (host_e || host_f || !is_device_ptr(C)) PNJK_HIP_CHECK(hipStreamSynchronize(s));
   return PNJK_OK;
 }
 

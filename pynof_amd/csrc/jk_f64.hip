@@ -639,6 +639,61 @@ struct ProbLagX {
   }
 };
 
+// Generalised Fock matrices, closed shell (utils.py:25-54), as one GEMM over the stacked
+// J^n and K^n: F[i][mn] = sum_k A[i][k] JK[k][mn] + n_i H[mn], with A = [C^J' + diag(w n) | -C^K']
+// (nbf5 x kp, zero past 2 nbf5) and JK rows J[0..nbf5) then K[0..nbf5) of length nn = nbf^2
+// and stride ld (even; pairs at the row end are clamped inside the row).
+struct SrcKMJK {
+  const double* J;
+  const double* K;
+  int64_t ld;
+  int n5;
+  __device__ __forceinline__ const double* at(int k, int r) const {
+    const int kk = k < 2 * n5 ? k : 2 * n5 - 1;
+    const double* row = kk < n5 ? J + (int64_t)kk * ld : K + (int64_t)(kk - n5) * ld;
+    return row + (r < ld - 1 ? (int64_t)r : ld - 2);
+  }
+};
+
+struct ProbFock {
+  static constexpr int kA = MK, kB = KM;
+  const double* A;
+  const double* J;
+  const double* K;
+  const double* H;  // nbf x nbf, row stride nbf
+  const double* nocc;
+  double* F;
+  int64_t ld, nn;
+  int nbf, nbf5, kp, mtiles;
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.batch = 0;
+    t.mt = id % mtiles;
+    t.nt = id / mtiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf5; }
+  __device__ int ksteps(int BK) const { return kp / BK; }
+  __device__ SrcMKDense a_src(const Tile&) const { return {A, kp, nbf5}; }
+  __device__ SrcKMJK b_src(const Tile&) const { return {J, K, ld, nbf5}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4d (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int m = t.m0 + dm;
+          const int64_t n = (int64_t)t.n0 + dn;
+          if (m < nbf5 && n < nn) gmem(F)[(int64_t)m * nn + n] = acc[i][j][r] + nocc[m] * H[n];
+        }
+  }
+};
+
 // ---------------------------------------------------------------- small kernels
 __global__ void k_kmo_reduce(int nbf5, int nt, const double* __restrict__ kpart, double* __restrict__ kmo) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1004,6 +1059,113 @@ hipError_t launch_fill_cols(double* p, int rows, int64_t ld, int64_t c0, int64_t
   if (c1 <= c0 || rows <= 0) return hipSuccess;
   const int64_t n = (int64_t)rows * (c1 - c0);
   hipLaunchKernelGGL(k_fill_cols, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, p, rows, ld, c0, c1);
+  return hipGetLastError();
+}
+
+// A[i][k] of the Fock GEMM (utils.py:34-52): k < nbf5: C^J[i][k] (zero on the diagonal for
+// i >= ini) + n_i for k == i when ini <= i < nbeta or i >= nalpha; nbf5 <= k < 2 nbf5: -C^K[i][k - nbf5]
+// (same diagonal rule); zero up to kp.
+__global__ void k_fock_coef(const double* __restrict__ cj, const double* __restrict__ ck,
+                            const double* __restrict__ nocc, int n5, int ini, int nbeta, int nalpha, int kp,
+                            double* __restrict__ A) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n5 * kp) return;
+  const int i = idx / kp, k = idx % kp;
+  double v = 0.0;
+  if (k < n5) {
+    v = (k == i && i >= ini) ? 0.0 : cj[(int64_t)i * n5 + k];
+    if (k == i && ((i >= ini && i < nbeta) || i >= nalpha)) v += nocc[i];
+  } else if (k < 2 * n5) {
+    const int j = k - n5;
+    v = (j == i && i >= ini) ? 0.0 : -ck[(int64_t)i * n5 + j];
+  }
+  A[idx] = v;
+}
+
+// G[m][i] = sum_n F[i][m][n] C[n][i] (utils.py:251), i < nbf5, row stride ldg. Block
+// (x, y) takes orbital i = y and rows m of 32 x .. 32 x + 31, eight per wave; column i of C
+// (Cf, row stride ldcf) is staged through LDS in chunks. Each lane sums n = lane + 64 j in
+// increasing order, then the wave reduces by xor-shuffle: the order is fixed.
+__global__ void __launch_bounds__(256) k_fock_g(const double* __restrict__ F, const double* __restrict__ Cf,
+                                                int ldcf, int nbf, int ldg, double* __restrict__ G) {
+  constexpr int kRows = 8, kChunk = 2048;
+  __shared__ double c[kChunk];
+  const int i = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int mw = blockIdx.x * 32 + wave * kRows;
+  double acc[kRows];
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) acc[r] = 0.0;
+  for (int n0 = 0; n0 < nbf; n0 += kChunk) {
+    const int len = nbf - n0 < kChunk ? nbf - n0 : kChunk;
+    __syncthreads();
+    for (int t = threadIdx.x; t < len; t += 256) c[t] = Cf[(int64_t)(n0 + t) * ldcf + i];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int m = mw + r;
+      if (m < nbf) {
+        const double* row = F + ((int64_t)i * nbf + m) * nbf + n0;
+        double v = acc[r];
+        for (int n = lane; n < len; n += 64) v += row[n] * c[n];
+        acc[r] = v;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    double v = acc[r];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int m = mw + r;
+    if (lane == 0 && m < nbf) G[(int64_t)m * ldg + i] = v;
+  }
+}
+
+// elag[i][j] = src[i][j] (nbf x nbf5) for i < noptorb, j < nbf5; zero elsewhere (utils.py:254-257).
+__global__ void k_elag_scatter(const double* __restrict__ src, int nbf, int nbf5, int noptorb,
+                               double* __restrict__ elag) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)nbf * nbf) return;
+  const int i = (int)(idx / nbf), j = (int)(idx % nbf);
+  elag[idx] = (i < noptorb && j < nbf5) ? src[(int64_t)i * nbf5 + j] : 0.0;
+}
+
+hipError_t launch_fock_f64(int nbf, int n5, const double* J, const double* K, int64_t ld, const double* Hd,
+                           const double* nocc, const double* cj, const double* ck, int ini, int nbeta, int nalpha,
+                           double* A, double* F, hipStream_t s) {
+  const int64_t nn = (int64_t)nbf * nbf;
+  if (ld % 2 != 0 || ld < nn || ld < 2) return hipErrorInvalidValue;
+  const int kp = (2 * n5 + 15) / 16 * 16;
+  hipLaunchKernelGGL(k_fock_coef, dim3(cdiv((int64_t)n5 * kp, 256)), dim3(256), 0, s, cj, ck, nocc, n5, ini, nbeta,
+                     nalpha, kp, A);
+  const bool big = n5 > 64 && nn >= (1 << 16);
+  const TileF64 tc = big ? TileF64{5, 4} : TileF64{2, 2};
+  ProbFock pr;
+  pr.A = A;
+  pr.J = J;
+  pr.K = K;
+  pr.H = Hd;
+  pr.nocc = nocc;
+  pr.F = F;
+  pr.ld = ld;
+  pr.nn = nn;
+  pr.nbf = nbf;
+  pr.nbf5 = n5;
+  pr.kp = kp;
+  pr.mtiles = cdiv(n5, 32 * tc.fm);
+  const int64_t tiles = (int64_t)pr.mtiles * cdiv(nn, 32 * tc.fn);
+  if (tiles > 2147483647LL) return hipErrorInvalidValue;
+  return launch_prob(pr, tc, (int)tiles, s);
+}
+
+hipError_t launch_lagrange_fock_f64(const BuildGeom& g, const double* F, const double* Cf, int ldcf, double* G,
+                                    double* tmp, int noptorb, double* elag, hipStream_t s) {
+  hipLaunchKernelGGL(k_fock_g, dim3(cdiv(g.nbf, 32), g.nbf5), dim3(256), 0, s, F, Cf, ldcf, g.nbf, g.ldcp, G);
+  hipError_t e = launch_hmat_f64(g, Cf, ldcf, G, tmp, s);
+  if (e != hipSuccess) return e;
+  const int64_t n = (int64_t)g.nbf * g.nbf;
+  hipLaunchKernelGGL(k_elag_scatter, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, tmp, g.nbf, g.nbf5, noptorb,
+                     elag);
   return hipGetLastError();
 }
 

@@ -243,3 +243,55 @@ class JKEngine:
             int(nbeta), int(nalpha), prec, _ptr(res["elag"]), _ptr(res["H_mat"]) if "H_mat" in res else None,
             stream))
         return res
+
+    def fock_rc(self, J, K, n, H, cj12, ck12, no1: int, nbeta: int, nalpha: int, out=None, stream=None):
+        """Closed-shell generalised Fock matrices (computeF_RC_CPU/_GPU, utils.py:25-84):
+        F (nbf5, nbf, nbf) from J, K (nbf5, nbf, nbf), n (nbf5,), H (nbf, nbf) and
+        cj12, ck12 (nbf5, nbf5), whose [ini:, ini:] diagonals (ini = no1 if no1 > 1 else 0)
+        are taken as zero; inputs are not modified. Needs no b. Device (torch) J gives a
+        device F; host inputs give a host F."""
+        Ji, Ki, Hi = _as_input(J, "J"), _as_input(K, "K"), _as_input(H, "H")
+        if Ji.ndim != 3 or Ji.shape[1] != Ji.shape[2] or tuple(Ki.shape) != tuple(Ji.shape):
+            raise ValueError(f"J and K must have one shape (nbf5, nbf, nbf); got {tuple(Ji.shape)}, {tuple(Ki.shape)}")
+        nbf5, nbf = int(Ji.shape[0]), int(Ji.shape[1])
+        if tuple(Hi.shape) != (nbf, nbf):
+            raise ValueError(f"H must have shape ({nbf}, {nbf}); got {tuple(Hi.shape)}")
+        ni, cji, cki = _as_input(n, "n"), _as_input(cj12, "cj12"), _as_input(ck12, "ck12")
+        if tuple(ni.shape) != (nbf5,) or tuple(cji.shape) != (nbf5, nbf5) or tuple(cki.shape) != (nbf5, nbf5):
+            raise ValueError(f"n must have shape ({nbf5},), cj12 and ck12 ({nbf5}, {nbf5})")
+        on_device = _is_torch(Ji) and Ji.is_cuda
+        if out is not None:
+            F = out
+        elif on_device:
+            F = _torch().empty((nbf5, nbf, nbf), dtype=_torch().float64, device=Ji.device)
+        else:
+            F = np.empty((nbf5, nbf, nbf), dtype=np.float64)
+        if stream is None and on_device:
+            stream = _torch().cuda.current_stream(Ji.device).cuda_stream
+        _lib.check(_lib.lib().pnjk_fock_rc(
+            self._h, _ptr(Ji), _ptr(Ki), nbf, nbf5, _ptr(ni), _ptr(Hi), nbf, _ptr(cji), _ptr(cki), int(no1),
+            int(nbeta), int(nalpha), _ptr(F), stream))
+        return F
+
+    def lagrange_fock(self, F, C, noptorb: int, out=None, stream=None):
+        """Lagrange multipliers from the Fock matrices (computeLagrange, utils.py:249-259):
+        elag (nbf, nbf) with elag[i, j] = sum_mn C[m, i] F[j, m, n] C[n, j] for
+        i < noptorb, j < nbf5, zero elsewhere. F (nbf5, nbf, nbf), C (nbf, nbf). Needs no b."""
+        Fi, Ci = _as_input(F, "F"), _as_input(C, "C")
+        if Fi.ndim != 3 or Fi.shape[1] != Fi.shape[2]:
+            raise ValueError(f"F must have shape (nbf5, nbf, nbf); got {tuple(Fi.shape)}")
+        nbf5, nbf = int(Fi.shape[0]), int(Fi.shape[1])
+        if tuple(Ci.shape) != (nbf, nbf):
+            raise ValueError(f"C must have shape ({nbf}, {nbf}); got {tuple(Ci.shape)}")
+        on_device = _is_torch(Fi) and Fi.is_cuda
+        if out is not None:
+            elag = out
+        elif on_device:
+            elag = _torch().empty((nbf, nbf), dtype=_torch().float64, device=Fi.device)
+        else:
+            elag = np.empty((nbf, nbf), dtype=np.float64)
+        if stream is None and on_device:
+            stream = _torch().cuda.current_stream(Fi.device).cuda_stream
+        _lib.check(_lib.lib().pnjk_lagrange_fock(self._h, _ptr(Fi), nbf, nbf5, _ptr(Ci), nbf, int(noptorb),
+                                                  _ptr(elag), stream))
+        return elag

@@ -1,12 +1,14 @@
 """Drop-in replacements for PyNOF's RI JK builds.
 
 ``computeJKj(C, I, b_mnl, p)``, ``computeJKH_MO(C, H, I, b_mnl, p)``,
-``JKH_MO_tmp(C, H, I, b_mnl, p)`` and ``computeLagrange2(n, cj12, ck12, C, H, I,
-b_mnl, p)`` keep the reference's signatures (src/pynof/integrals.py:172, :307,
-:665; utils.py:262), argument meaning and return values:
+``JKH_MO_tmp(C, H, I, b_mnl, p)``, ``computeLagrange2(n, cj12, ck12, C, H, I,
+b_mnl, p)``, ``computeF_RC_driver(J, K, n, H, cj12, ck12, p)`` and
+``computeLagrange(F, C, p)`` keep the reference's signatures (src/pynof/integrals.py:172,
+:307, :665; utils.py:262, :15, :249), argument meaning and return values:
 host FP64 arrays J, K of shape (nbf5, nbf, nbf); J_MO, K_MO (nbf5, nbf5) with
 H_core (nbf5,); the orbital-gradient pair H_mat (nbf, nbf5), b_MO
-(nbf, nbf5, naux); and the Lagrangian elag (nbf, nbf) with Hmat. ``I`` is ignored (it is ``[]`` in RI mode,
+(nbf, nbf5, naux); the Lagrangian elag (nbf, nbf) with Hmat; the generalised Fock
+matrices F (nbf5, nbf, nbf). ``I`` is ignored (it is ``[]`` in RI mode,
 integrals.py:28-33). Both run on the MI355X through the HIP library; there is no
 CPU path. ``p`` must provide ``nbf5``; ``p.RI`` must be true. ``p.precision``
 (optional, default "fp64") selects "fp64", "split" or "split_refine".
@@ -103,9 +105,49 @@ def computeLagrange2(n, cj12, ck12, C, H, I, b_mnl, p):
     return r["elag"], r["H_mat"]
 
 
+_plain_engines: dict[int, JKEngine] = {}
+
+
+def _plain_engine(p) -> JKEngine:
+    """An engine with no b, per device, for the Fock-matrix calls."""
+    dev = int(getattr(p, "device", 0))
+    with _cache_lock:
+        eng = _plain_engines.get(dev)
+        if eng is None:
+            eng = _plain_engines[dev] = JKEngine(dev)
+        return eng
+
+
+def computeF_RC_driver(J, K, n, H, cj12, ck12, p):
+    """Closed-shell generalised Fock matrices F (nbf5, nbf, nbf), host FP64
+    (computeF_RC_driver, utils.py:15-54; the GPU form :58-84 returns the same host
+    array). As in the reference, the diagonals of the caller's cj12[ini:, ini:] and
+    ck12[ini:, ini:] are set to zero in place (:45, :50), ini = p.no1 if p.no1 > 1
+    else 0. Device (torch) J and K give a device F."""
+    import numpy as np
+
+    ini = p.no1 if p.no1 > 1 else 0
+    for c in (cj12, ck12):
+        if isinstance(c, np.ndarray):
+            np.fill_diagonal(c[ini:, ini:], 0)
+        else:
+            c[ini:, ini:].fill_diagonal_(0)
+    nbf5 = int(p.nbf5)
+    return _plain_engine(p).fock_rc(J, K, n[:nbf5], H, cj12, ck12, p.no1, p.nbeta, p.nalpha)
+
+
+def computeLagrange(F, C, p):
+    """Lagrange multipliers elag (nbf, nbf) from the Fock matrices (computeLagrange,
+    utils.py:249-259): rows past p.noptorb and columns past p.nbf5 are zero."""
+    return _plain_engine(p).lagrange_fock(F, C, p.noptorb)
+
+
 def release_cached_engines() -> None:
-    """Drops every cached device copy of b."""
+    """Drops every cached device copy of b and the per-device engines."""
     with _cache_lock:
         for _, _, eng in _engines.values():
             eng.close()
         _engines.clear()
+        for eng in _plain_engines.values():
+            eng.close()
+        _plain_engines.clear()
