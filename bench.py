@@ -312,7 +312,30 @@ def main():
             lag[form]["rel_err_elag_vs_fp64"] = float(torch.linalg.norm(lo["elag"] - lag_ref) /
                                                       torch.linalg.norm(lag_ref))
     lag["exchange_flops"] = 2.0 * nbf * nbf5 * nbf5 * naux
-    del lag_in, lag_ref, lo
+
+    # ---- generalised Fock matrices from J^n, K^n (computeF_RC) and elag from F (computeLagrange)
+    log("Fock matrices and Lagrangian from F")
+    build(0, "fp64")
+    F = eng.fock_rc(out["J"], out["K"], lag_in[0], Hd, lag_in[1], lag_in[2], 0, nb, nb, stream=stream)
+    elag_f = eng.lagrange_fock(F, Cd[0], nbf, stream=stream)
+    reps = max(1, args.form_steps)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record()
+    for _ in range(reps):
+        eng.fock_rc(out["J"], out["K"], lag_in[0], Hd, lag_in[1], lag_in[2], 0, nb, nb, out=F, stream=stream)
+    ev[1].record()
+    for _ in range(reps):
+        eng.lagrange_fock(F, Cd[0], nbf, out=elag_f, stream=stream)
+    ev[2].record()
+    torch.cuda.synchronize()
+    f_ms, l_ms = ev[0].elapsed_time(ev[1]) / reps, ev[1].elapsed_time(ev[2]) / reps
+    f_flops = 2.0 * nbf5 * (2 * nbf5) * nbf * nbf
+    fock = {"fock_ms": round(f_ms, 3), "fock_flops": f_flops,
+            "fock_tflops": round(f_flops / (f_ms * 1e-3) / 1e12, 2),
+            "fock_bytes_min": 8.0 * 3 * nbf5 * nbf * nbf,
+            "lagrange_from_F_ms": round(l_ms, 3),
+            "lagrange_from_F_GBps": round(8.0 * nbf5 * nbf * nbf / (l_ms * 1e-3) / 1e9, 1)}
+    del lag_in, lag_ref, lo, F, elag_f
     torch.cuda.empty_cache()
 
     line = {
@@ -342,6 +365,7 @@ def main():
         "roofline_fp64": roofline_fp64,
         "mo_integrals": mo,
         "lagrangian": lag,
+        "fock": fock,
     }
     eng.close()
     del out, Cd, Hd
