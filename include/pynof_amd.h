@@ -149,6 +149,52 @@ int pnjk_fock_rc(pnjk_engine* e, const double* J, const double* K, int64_t nbf, 
 int pnjk_lagrange_fock(pnjk_engine* e, const double* F, int64_t nbf, int64_t nbf5, const double* C, int64_t ldc,
                        int64_t noptorb, double* elag, void* stream);
 
+/* ---- occupation step (pnof.py) ----
+ * The PNOF functional and occupation parametrisation, the fields of PyNOF's param
+ * (parameters.py:46-110) that the occupation step reads. nbf5 must equal
+ * no1 + ndoc (1 + ncwo) + nsoc (parameters.py:65); nbeta = no1 + ndoc,
+ * nalpha = nbeta + nsoc, nv = ndoc ncwo are derived. */
+#define PNJK_OCC_SOFTMAX 0         /* p.occ_method == "Softmax" (pnof.py:770-810) */
+#define PNJK_OCC_TRIGONOMETRIC 1   /* p.occ_method == "Trigonometric" (pnof.py:704-768) */
+typedef struct pnjk_pnof {
+  int32_t ipnof;      /* p.ipnof: 5 (PNOF5), 7 (PNOF7) or 8 (GNOF) */
+  int32_t ista;       /* p.ista: PNOF7 Phi = sqrt(n(1-n)) for 0, else 2n(1-n) */
+  int32_t occ_method; /* PNJK_OCC_SOFTMAX or PNJK_OCC_TRIGONOMETRIC */
+  int32_t high_spin;  /* p.HighSpin: singly occupied n = 1 instead of 0.5 */
+  int32_t mspin;      /* p.MSpin: only 0 (closed shell) is supported */
+  int32_t no1, ndoc, nsoc, ncwo, nbf5;
+} pnjk_pnof;
+
+/* One occupation step from the parameters gamma (nv): replaces ocupacion
+ * (pnof.py:692-810), PNOFi_selector (:657-667, CJCKD5/7/8 :155-422), calcocce
+ * (:994-1001, calce :905-921 with MSpin = 0), der_PNOFi_selector (:670-689,
+ * der_CJCKD5/7/8 :206-525) and calcoccg (:1004-1025, MSpin = 0), which the
+ * occupation optimiser (minimization.py:86-88) calls on every iteration.
+ * Outputs, each written when not NULL: E (1) the energy of calcocce; grad (nv)
+ * of calcoccg; n (nbf5) and dn_dgamma (nbf5, nv) of ocupacion; cj12, ck12
+ * (nbf5, nbf5) of PNOFi_selector. E and grad need J_MO, K_MO (nbf5, nbf5) and
+ * H_core (nbf5), e.g. the device outputs of pnjk_jk_build. Arrays may be host
+ * or device memory; with a host output the call returns after the copy back.
+ * The reference's softmax writes the strong occupations at index i < ndoc
+ * (pnof.py:792, :800), not no1 + i; this is reproduced as it is. */
+int pnjk_occupation_step(pnjk_engine* e, const pnjk_pnof* p, const double* gamma, const double* J_MO,
+                         const double* K_MO, const double* H_core, double* E, double* grad, double* n,
+                         double* dn_dgamma, double* cj12, double* ck12, void* stream);
+
+/* C^J, C^K (nbf5, nbf5) for given occupations n (nbf5): replaces PNOFi_selector
+ * (pnof.py:657-667) for PNOF5, PNOF7 and GNOF. Either output may be NULL. */
+int pnjk_pnof_coefficients(pnjk_engine* e, const pnjk_pnof* p, const double* n, double* cj12, double* ck12,
+                           void* stream);
+
+/* Closed-shell energy for given n, cj12, ck12: replaces calce with MSpin = 0
+ * (pnof.py:905-921): 2 sum n H_core + sum n_i J_ii over i < nbeta and
+ * nalpha <= i < nbf5 + sum' cj12 J_MO - sum' ck12 K_MO, diagonals excluded (the
+ * reference zeroes them in the caller's arrays, :913, :917; this call leaves its
+ * inputs unchanged). E is one double, host or device. */
+int pnjk_energy(pnjk_engine* e, int64_t nbf5, int64_t nbeta, int64_t nalpha, const double* n, const double* cj12,
+                const double* ck12, const double* J_MO, const double* K_MO, const double* H_core, double* E,
+                void* stream);
+
 /* Grouped builds: many engines (one b each, different sizes allowed) built in
  * one pass. Each GEMM stage of all instances runs as one persistent grouped
  * launch per problem type and tile shape (tile shapes per instance from the

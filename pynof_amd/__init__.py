@@ -17,5 +17,6 @@ from .integrals import (  # noqa: F401
     computeLagrange2,
 )
 from .parameters import param  # noqa: F401
+from .pnof import PNOFi_selector, calce, calcocce, calcoccg, ocupacion  # noqa: F401
 
 __all__ = ["JKEngine", "computeJKj", "computeJKH_MO", "JKH_MO_tmp", "computeLagrange2", "param", "PnjkError"]

@@ -19,6 +19,14 @@ OUT_J, OUT_K, OUT_JMO, OUT_KMO, OUT_HCORE = 1, 2, 4, 8, 16
 OUT_ALL = 31
 
 PRECISIONS = {"fp64": FP64, "split": SPLIT, "split_refine": SPLIT_REFINE}
+OCC_SOFTMAX, OCC_TRIGONOMETRIC = 0, 1
+OCC_METHODS = {"Softmax": OCC_SOFTMAX, "Trigonometric": OCC_TRIGONOMETRIC}
+
+
+class Pnof(ctypes.Structure):
+    """struct pnjk_pnof: the PNOF functional and occupation parametrisation."""
+    _fields_ = [(name, ctypes.c_int32) for name in
+                ("ipnof", "ista", "occ_method", "high_spin", "mspin", "no1", "ndoc", "nsoc", "ncwo", "nbf5")]
 
 # name -> (restype, argtypes)
 _SIGS = {
@@ -62,6 +70,10 @@ _SIGS = {
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "pnjk_occupation_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_void_p] * 11),
+    "pnjk_pnof_coefficients": (ctypes.c_int, [ctypes.c_void_p] * 6),
+    "pnjk_energy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]
+                    + [ctypes.c_void_p] * 8),
     "pnjk_footprint": (ctypes.c_uint64, [ctypes.c_void_p]),
     "pnjk_workspace_bytes": (ctypes.c_uint64, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int]),
 }

@@ -132,16 +132,17 @@ struct pnjk_engine {
   DevBuf bmo, hm, lagw, lagg, lagp, lagc;        // Lagrangian workspace: b_MO, H_mat, W, G, partials, n/C^J/C^K
   DevBuf bm16h, bm16l, w16h, w16l;               // split form of the Lagrangian's exchange operands
   DevBuf fjk, fh, fco, ff;                       // Fock: staged J/K, H, coefficients, F
+  DevBuf occ;                                    // occupation step: every array of the step
   bool split_ready = false;
   // Tile override per arithmetic (0: FP64 kernels, 1: BF16x3 kernels) and stage;
   // {0, 0} selects the built-in choice.
   TileF64 tiles[2][ST_COUNT] = {};
-  DevBuf* all[40];
+  DevBuf* all[41];
   int nall = 0;
   pnjk_engine() {
     for (DevBuf* b : {&btri, &bt16h, &bt16l, &bht16h, &bht16l, &Cp, &Hp, &X, &d, &kpart, &T, &outs, &stage, &scal,
                       &ct16h, &ct16l, &x16h, &x16l, &d16h, &d16l, &Cf, &ctf16h, &ctf16l, &hc,
-                      &bmo, &hm, &lagw, &lagg, &lagp, &lagc, &bm16h, &bm16l, &w16h, &w16l, &fjk, &fh, &fco, &ff})
+                      &bmo, &hm, &lagw, &lagg, &lagp, &lagc, &bm16h, &bm16l, &w16h, &w16l, &fjk, &fh, &fco, &ff, &occ})
       all[nall++] = b;
   }
 };
@@ -904,6 +905,176 @@ int pnjk_lagrange_fock(pnjk_engine* e, const double* F, int64_t nbf, int64_t nbf
   PNJK_HIP_CHECK(launch_lagrange_fock_f64(g, dF, e->Cf.as<double>(), ldcf, G, e->hm.as<double>(), (int)noptorb, dE, s));
   if (host_e) PNJK_HIP_CHECK(hipMemcpyAsync(elag, dE, (size_t)nn * 8, hipMemcpyDeviceToHost, s));
   if (host_e || host_f || !is_device_ptr(C)) PNJK_HIP_CHECK(hipStreamSynchronize(s));
+  return PNJK_OK;
+}
+
+// ---------------------------------------------------------------- occupation step
+// Checks a pnjk_pnof against the relations of parameters.py:46-65, 94 and fills the
+// kernel-side copy.
+static int pnof_from(const pnjk_pnof* p, PnofDev* d, const char* who) {
+  const std::string w(who);
+  if (!p) return set_error(PNJK_EINVAL, w + ": pnof parameters are NULL");
+  if (p->mspin != 0) return set_error(PNJK_EUNSUPPORTED, w + ": only MSpin = 0 (closed shell) is supported");
+  if (p->ipnof != 5 && p->ipnof != 7 && p->ipnof != 8)
+    return set_error(PNJK_EUNSUPPORTED, w + ": ipnof must be 5, 7 or 8");
+  if (p->occ_method != PNJK_OCC_SOFTMAX && p->occ_method != PNJK_OCC_TRIGONOMETRIC)
+    return set_error(PNJK_EUNSUPPORTED, w + ": occ_method must be PNJK_OCC_SOFTMAX or PNJK_OCC_TRIGONOMETRIC");
+  if (p->no1 < 0 || p->ndoc < 0 || p->nsoc < 0 || p->ncwo < 0)
+    return set_error(PNJK_EINVAL, w + ": no1, ndoc, nsoc, ncwo must be >= 0");
+  const int64_t nbf5 = (int64_t)p->no1 + (int64_t)p->ndoc * (1 + (int64_t)p->ncwo) + p->nsoc;
+  if (p->nbf5 != nbf5 || nbf5 < 1 || nbf5 > 16384)
+    return set_error(PNJK_EINVAL, w + ": nbf5 must equal no1 + ndoc (1 + ncwo) + nsoc, in [1, 16384]");
+  d->ipnof = p->ipnof;
+  d->ista = p->ista;
+  d->method = p->occ_method;
+  d->high_spin = p->high_spin ? 1 : 0;
+  d->no1 = p->no1;
+  d->ndoc = p->ndoc;
+  d->nsoc = p->nsoc;
+  d->ncwo = p->ncwo;
+  d->nbf5 = (int)nbf5;
+  d->nv = p->ndoc * p->ncwo;
+  d->nbeta = p->no1 + p->ndoc;
+  d->nalpha = d->nbeta + p->nsoc;
+  d->ndns = p->ndoc + p->nsoc;
+  return PNJK_OK;
+}
+
+// Device arrays of one occupation step, carved from the engine's `occ` buffer.
+struct OccBufs {
+  double *x, *n, *dn, *fi, *nd, *nd12, *dfi, *dnd, *dnd12, *JT, *KT, *tmp, *hc, *E, *grad, *cj, *ck;
+};
+
+static int occ_bufs(pnjk_engine* e, int64_t n5, int64_t nv, OccBufs& b) {
+  const int64_t mm = n5 * n5, mv = n5 * nv;
+  const int64_t sizes[17] = {nv, n5, mv, n5, n5, n5, mv, mv, mv, mm, mm, mm, n5, 1, nv, mm, mm};
+  int64_t total = 0;
+  for (int64_t z : sizes) total += (z + 31) / 32 * 32;  // 256-byte aligned pieces
+  int rc = e->occ.ensure((size_t)std::max<int64_t>(total, 32) * 8);
+  if (rc) return rc;
+  double* base = e->occ.as<double>();
+  double** slots[17] = {&b.x, &b.n, &b.dn, &b.fi, &b.nd, &b.nd12, &b.dfi, &b.dnd, &b.dnd12,
+                        &b.JT, &b.KT, &b.tmp, &b.hc, &b.E, &b.grad, &b.cj, &b.ck};
+  int64_t off = 0;
+  for (int i = 0; i < 17; ++i) {
+    *slots[i] = base + off;
+    off += (sizes[i] + 31) / 32 * 32;
+  }
+  return PNJK_OK;
+}
+
+// Stages M (n x n, host or device) transposed into dst.
+static int stage_transposed(const double* M, int64_t n, double* tmp, double* dst, hipStream_t s) {
+  const double* src = M;
+  if (!is_device_ptr(M)) {
+    PNJK_HIP_CHECK(hipMemcpyAsync(tmp, M, (size_t)(n * n) * 8, hipMemcpyHostToDevice, s));
+    src = tmp;
+  }
+  PNJK_HIP_CHECK(launch_occ_transpose(src, (int)n, dst, s));
+  return PNJK_OK;
+}
+
+// Copies n doubles from device memory to host or device memory.
+static int from_device(const double* src, double* dst, int64_t n, hipStream_t s, bool* host) {
+  const bool h = !is_device_ptr(dst);
+  PNJK_HIP_CHECK(hipMemcpyAsync(dst, src, (size_t)n * 8, h ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+  *host = *host || h;
+  return PNJK_OK;
+}
+
+int pnjk_occupation_step(pnjk_engine* e, const pnjk_pnof* p, const double* gamma, const double* J_MO,
+                         const double* K_MO, const double* H_core, double* E, double* grad, double* n,
+                         double* dn_dgamma, double* cj12, double* ck12, void* stream) {
+  if (!e) return set_error(PNJK_EINVAL, "pnjk_occupation_step: engine is NULL");
+  std::lock_guard<std::mutex> lk(e->mu);
+  PNJK_HIP_CHECK(hipSetDevice(e->device));
+  hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
+  PnofDev d;
+  int rc;
+  if ((rc = pnof_from(p, &d, "pnjk_occupation_step"))) return rc;
+  if (!E && !grad && !n && !dn_dgamma && !cj12 && !ck12)
+    return set_error(PNJK_EINVAL, "pnjk_occupation_step: no output requested");
+  if (d.nv > 0 && !gamma) return set_error(PNJK_EINVAL, "pnjk_occupation_step: gamma is NULL");
+  if ((E || grad) && (!J_MO || !K_MO || !H_core))
+    return set_error(PNJK_EINVAL, "pnjk_occupation_step: E and grad need J_MO, K_MO and H_core");
+  const int64_t n5 = d.nbf5, nv = d.nv;
+  OccBufs b;
+  if ((rc = occ_bufs(e, n5, nv, b))) return rc;
+  if (nv > 0 && (rc = to_device(gamma, b.x, nv, s))) return rc;
+  PNJK_HIP_CHECK(launch_occupation(d, b.x, b.n, b.dn, s));
+  const bool want_der = grad != nullptr;
+  PNJK_HIP_CHECK(launch_pnof_orbitals(d, b.n, want_der ? b.dn : nullptr, b.fi, b.nd, b.nd12, b.dfi, b.dnd, b.dnd12, s));
+  if (E || grad) {
+    if ((rc = stage_transposed(J_MO, n5, b.tmp, b.JT, s)) || (rc = stage_transposed(K_MO, n5, b.tmp, b.KT, s)) ||
+        (rc = to_device(H_core, b.hc, n5, s)))
+      return rc;
+  }
+  if (E) PNJK_HIP_CHECK(launch_occ_energy(d, b.n, b.fi, b.nd, b.nd12, nullptr, nullptr, b.JT, b.KT, b.hc, b.E, s));
+  if (grad)
+    PNJK_HIP_CHECK(launch_occ_grad(d, b.n, b.fi, b.nd, b.nd12, b.dn, b.dfi, b.dnd, b.dnd12, b.JT, b.KT, b.hc, b.grad,
+                                   s));
+  if (cj12 || ck12) PNJK_HIP_CHECK(launch_pnof_coef(d, b.n, b.fi, b.nd, b.nd12, b.cj, b.ck, s));
+  bool host = false;
+  if ((E && (rc = from_device(b.E, E, 1, s, &host))) || (grad && nv > 0 && (rc = from_device(b.grad, grad, nv, s, &host))) ||
+      (n && (rc = from_device(b.n, n, n5, s, &host))) ||
+      (dn_dgamma && nv > 0 && (rc = from_device(b.dn, dn_dgamma, n5 * nv, s, &host))) ||
+      (cj12 && (rc = from_device(b.cj, cj12, n5 * n5, s, &host))) ||
+      (ck12 && (rc = from_device(b.ck, ck12, n5 * n5, s, &host))))
+    return rc;
+  if (host || (gamma && !is_device_ptr(gamma))) PNJK_HIP_CHECK(hipStreamSynchronize(s));
+  return PNJK_OK;
+}
+
+int pnjk_pnof_coefficients(pnjk_engine* e, const pnjk_pnof* p, const double* n, double* cj12, double* ck12,
+                           void* stream) {
+  if (!e) return set_error(PNJK_EINVAL, "pnjk_pnof_coefficients: engine is NULL");
+  std::lock_guard<std::mutex> lk(e->mu);
+  PNJK_HIP_CHECK(hipSetDevice(e->device));
+  hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
+  PnofDev d;
+  int rc;
+  if ((rc = pnof_from(p, &d, "pnjk_pnof_coefficients"))) return rc;
+  if (!n || (!cj12 && !ck12)) return set_error(PNJK_EINVAL, "pnjk_pnof_coefficients: n and an output are required");
+  const int64_t n5 = d.nbf5;
+  OccBufs b;
+  if ((rc = occ_bufs(e, n5, d.nv, b)) || (rc = to_device(n, b.n, n5, s))) return rc;
+  PNJK_HIP_CHECK(launch_pnof_orbitals(d, b.n, nullptr, b.fi, b.nd, b.nd12, nullptr, nullptr, nullptr, s));
+  PNJK_HIP_CHECK(launch_pnof_coef(d, b.n, b.fi, b.nd, b.nd12, b.cj, b.ck, s));
+  bool host = false;
+  if ((cj12 && (rc = from_device(b.cj, cj12, n5 * n5, s, &host))) ||
+      (ck12 && (rc = from_device(b.ck, ck12, n5 * n5, s, &host))))
+    return rc;
+  if (host || !is_device_ptr(n)) PNJK_HIP_CHECK(hipStreamSynchronize(s));
+  return PNJK_OK;
+}
+
+int pnjk_energy(pnjk_engine* e, int64_t nbf5, int64_t nbeta, int64_t nalpha, const double* n, const double* cj12,
+                const double* ck12, const double* J_MO, const double* K_MO, const double* H_core, double* E,
+                void* stream) {
+  if (!e) return set_error(PNJK_EINVAL, "pnjk_energy: engine is NULL");
+  std::lock_guard<std::mutex> lk(e->mu);
+  PNJK_HIP_CHECK(hipSetDevice(e->device));
+  hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
+  if (!n || !cj12 || !ck12 || !J_MO || !K_MO || !H_core || !E)
+    return set_error(PNJK_EINVAL, "pnjk_energy: every array is required");
+  if (nbf5 < 1 || nbf5 > 16384 || nbeta < 0 || nalpha < nbeta || nalpha > nbf5)
+    return set_error(PNJK_EINVAL, "pnjk_energy: need 1 <= nbf5 <= 16384 and 0 <= nbeta <= nalpha <= nbf5");
+  PnofDev d{};
+  d.nbf5 = (int)nbf5;
+  d.nbeta = (int)nbeta;
+  d.nalpha = (int)nalpha;
+  OccBufs b;
+  int rc;
+  if ((rc = occ_bufs(e, nbf5, 0, b))) return rc;
+  if ((rc = to_device(n, b.n, nbf5, s)) || (rc = to_device(cj12, b.cj, nbf5 * nbf5, s)) ||
+      (rc = to_device(ck12, b.ck, nbf5 * nbf5, s)) || (rc = stage_transposed(J_MO, nbf5, b.tmp, b.JT, s)) ||
+      (rc = stage_transposed(K_MO, nbf5, b.tmp, b.KT, s)) || (rc = to_device(H_core, b.hc, nbf5, s)))
+    return rc;
+  PNJK_HIP_CHECK(launch_occ_energy(d, b.n, nullptr, nullptr, nullptr, b.cj, b.ck, b.JT, b.KT, b.hc, b.E, s));
+  bool host = false;
+  if ((rc = from_device(b.E, E, 1, s, &host))) return rc;
+  if (host || !is_device_ptr(n) || !is_device_ptr(cj12) || !is_device_ptr(ck12) || !is_device_ptr(J_MO))
+    PNJK_HIP_CHECK(hipStreamSynchronize(s));
   return PNJK_OK;
 }
 

@@ -170,4 +170,33 @@ hipError_t launch_jn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* d_hi, co
 hipError_t launch_kn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* x_hi, const uint16_t* x_lo, double* K,
                         hipStream_t s, GroupCollector* col = nullptr);
 
+// ---------------------------------------------------------------- occupation step
+// PNOF functional and occupation parametrisation (the param fields of parameters.py:46-110).
+struct PnofDev {
+  int ipnof;      // 5, 7 or 8 (GNOF)
+  int ista;       // PNOF7 Phi: 0 sqrt(n(1-n)), else 2n(1-n)
+  int method;     // 0 softmax, 1 trigonometric
+  int high_spin;  // singly occupied n = 1 instead of 0.5
+  int no1, ndoc, nsoc, ncwo, nbf5, nv, nbeta, nalpha, ndns;
+};
+
+// n (nbf5) and dn (nbf5 x nv, orbital-major) from the parameters x (nv).
+hipError_t launch_occupation(const PnofDev& p, const double* x, double* n, double* dn, hipStream_t s);
+// Per-orbital Phi, n_d, sqrt(n_d); with dn, also their derivatives (nbf5 x nv each).
+hipError_t launch_pnof_orbitals(const PnofDev& p, const double* n, const double* dn, double* fi, double* nd,
+                                double* nd12, double* dfi, double* dnd, double* dnd12, hipStream_t s);
+// C^J, C^K (nbf5 x nbf5); either may be null.
+hipError_t launch_pnof_coef(const PnofDev& p, const double* n, const double* fi, const double* nd, const double* nd12,
+                            double* cj, double* ck, hipStream_t s);
+// dst[i][j] = src[j][i], n x n.
+hipError_t launch_occ_transpose(const double* src, int n, double* dst, hipStream_t s);
+// Closed-shell energy into E[0]; C^J, C^K from cjm/ckm when given, else from the PNOF.
+hipError_t launch_occ_energy(const PnofDev& p, const double* n, const double* fi, const double* nd, const double* nd12,
+                             const double* cjm, const double* ckm, const double* JT, const double* KT,
+                             const double* hc, double* E, hipStream_t s);
+// Occupation gradient (nv).
+hipError_t launch_occ_grad(const PnofDev& p, const double* n, const double* fi, const double* nd, const double* nd12,
+                           const double* dn, const double* dfi, const double* dnd, const double* dnd12,
+                           const double* JT, const double* KT, const double* hc, double* grad, hipStream_t s);
+
 }  // namespace pnjk

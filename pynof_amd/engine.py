@@ -273,6 +273,89 @@ class JKEngine:
             int(nbeta), int(nalpha), _ptr(F), stream))
         return F
 
+    # ---------------------------------------------------------------- occupation step
+    @staticmethod
+    def pnof(no1: int, ndoc: int, nsoc: int, ncwo: int, ipnof: int = 8, ista: int = 0,
+             occ_method: str = "Softmax", HighSpin: bool = False, MSpin: int = 0):
+        """struct pnjk_pnof from PyNOF's param fields (parameters.py:46-110)."""
+        nbf5 = int(no1) + int(ndoc) * (1 + int(ncwo)) + int(nsoc)
+        if occ_method not in _lib.OCC_METHODS:
+            raise ValueError(f"occ_method must be one of {sorted(_lib.OCC_METHODS)}; got {occ_method!r}")
+        return _lib.Pnof(int(ipnof), int(ista), _lib.OCC_METHODS[occ_method], int(bool(HighSpin)), int(MSpin),
+                         int(no1), int(ndoc), int(nsoc), int(ncwo), nbf5)
+
+    def occupation_step(self, gamma, pnof, J_MO=None, K_MO=None, H_core=None,
+                        outputs=("E", "grad"), out=None, stream=None):
+        """One occupation step (pnof.py:692-1025): any of E (calcocce), grad (calcoccg),
+        n and dn_dgamma (ocupacion), cj12 and ck12 (PNOFi_selector) from the parameters
+        gamma (nv,). E and grad need J_MO, K_MO (nbf5, nbf5) and H_core (nbf5,). Device
+        (torch) inputs give device outputs; E is then a one-element tensor."""
+        nbf5, nv = pnof.nbf5, pnof.ndoc * pnof.ncwo
+        shapes = {"E": (1,), "grad": (nv,), "n": (nbf5,), "dn_dgamma": (nbf5, nv),
+                  "cj12": (nbf5, nbf5), "ck12": (nbf5, nbf5)}
+        for name in outputs:
+            if name not in shapes:
+                raise ValueError(f"unknown output {name!r}; choose from {list(shapes)}")
+        gi = _as_input(gamma, "gamma") if nv > 0 else None
+        if gi is not None and tuple(gi.shape) != (nv,):
+            raise ValueError(f"gamma must have shape ({nv},); got {tuple(gi.shape)}")
+        mats = [None, None, None]
+        if "E" in outputs or "grad" in outputs:
+            if J_MO is None or K_MO is None or H_core is None:
+                raise ValueError("E and grad need J_MO, K_MO and H_core")
+            mats = [_as_input(J_MO, "J_MO"), _as_input(K_MO, "K_MO"), _as_input(H_core, "H_core")]
+            if tuple(mats[0].shape) != (nbf5, nbf5) or tuple(mats[1].shape) != (nbf5, nbf5) or \
+                    tuple(mats[2].shape) != (nbf5,):
+                raise ValueError(f"J_MO, K_MO must have shape ({nbf5}, {nbf5}) and H_core ({nbf5},)")
+        ref = next((x for x in [gi] + mats if x is not None), None)
+        on_device = ref is not None and _is_torch(ref) and ref.is_cuda
+        res = {}
+        for name in outputs:
+            if out is not None and name in out:
+                res[name] = out[name]
+            elif on_device:
+                res[name] = _torch().empty(shapes[name], dtype=_torch().float64, device=ref.device)
+            else:
+                res[name] = np.empty(shapes[name], dtype=np.float64)
+        if stream is None and on_device:
+            stream = _torch().cuda.current_stream(ref.device).cuda_stream
+        ptr = lambda x: None if x is None else _ptr(x)  # noqa: E731
+        _lib.check(_lib.lib().pnjk_occupation_step(
+            self._h, ctypes.addressof(pnof), ptr(gi), ptr(mats[0]), ptr(mats[1]), ptr(mats[2]),
+            *(ptr(res.get(k)) for k in ("E", "grad", "n", "dn_dgamma", "cj12", "ck12")), stream))
+        return res
+
+    def pnof_coefficients(self, n, pnof, stream=None):
+        """(cj12, ck12) for occupations n (PNOFi_selector, pnof.py:657-667)."""
+        ni = _as_input(n, "n")
+        if tuple(ni.shape) != (pnof.nbf5,):
+            raise ValueError(f"n must have shape ({pnof.nbf5},); got {tuple(ni.shape)}")
+        shape = (pnof.nbf5, pnof.nbf5)
+        if _is_torch(ni) and ni.is_cuda:
+            cj = _torch().empty(shape, dtype=_torch().float64, device=ni.device)
+            ck = _torch().empty_like(cj)
+            stream = stream if stream is not None else _torch().cuda.current_stream(ni.device).cuda_stream
+        else:
+            cj, ck = np.empty(shape), np.empty(shape)
+        _lib.check(_lib.lib().pnjk_pnof_coefficients(self._h, ctypes.addressof(pnof), _ptr(ni), _ptr(cj), _ptr(ck),
+                                                      stream))
+        return cj, ck
+
+    def energy(self, n, cj12, ck12, J_MO, K_MO, H_core, nbeta: int, nalpha: int, stream=None) -> float:
+        """Closed-shell energy for given coefficients (calce with MSpin = 0, pnof.py:905-921)."""
+        arrs = [_as_input(x, nm) for x, nm in ((n, "n"), (cj12, "cj12"), (ck12, "ck12"), (J_MO, "J_MO"),
+                                               (K_MO, "K_MO"), (H_core, "H_core"))]
+        nbf5 = int(arrs[0].shape[0])
+        for a, shp in zip(arrs, [(nbf5,)] + [(nbf5, nbf5)] * 4 + [(nbf5,)]):
+            if tuple(a.shape) != shp:
+                raise ValueError(f"shapes must be n ({nbf5},), matrices ({nbf5}, {nbf5}), H_core ({nbf5},)")
+        E = np.empty(1)
+        if stream is None and _is_torch(arrs[0]) and arrs[0].is_cuda:
+            stream = _torch().cuda.current_stream(arrs[0].device).cuda_stream
+        _lib.check(_lib.lib().pnjk_energy(self._h, nbf5, int(nbeta), int(nalpha), *(_ptr(a) for a in arrs),
+                                          E.ctypes.data, stream))
+        return float(E[0])
+
     def lagrange_fock(self, F, C, noptorb: int, out=None, stream=None):
         """Lagrange multipliers from the Fock matrices (computeLagrange, utils.py:249-259):
         elag (nbf, nbf) with elag[i, j] = sum_mn C[m, i] F[j, m, n] C[n, j] for
