@@ -338,6 +338,34 @@ def main():
     del lag_in, lag_ref, lo, F, elag_f
     torch.cuda.empty_cache()
 
+    # ---- occupation step (calcocce + calcoccg, GNOF, softmax) on this build's J_MO, K_MO, H_core
+    log("occupation step")
+    occ_ndoc, occ_ncwo = 60, (nbf5 - 60) // 60  # nbf5 = ndoc (1 + ncwo) = 300
+    occ = {}
+    if occ_ndoc * (1 + occ_ncwo) == nbf5:
+        pn = JKEngine.pnof(0, occ_ndoc, 0, occ_ncwo, ipnof=8, occ_method="Softmax")
+        gam = torch.from_numpy(np.random.default_rng(7100).uniform(-2.0, 0.5, occ_ndoc * occ_ncwo)).to(dev)
+        oo = eng.occupation_step(gam, pn, out["J_MO"], out["K_MO"], out["H_core"], stream=stream)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        reps = 20
+        ev[0].record()
+        for _ in range(reps):
+            eng.occupation_step(gam, pn, out["J_MO"], out["K_MO"], out["H_core"], out=oo, stream=stream)
+        ev[1].record()
+        torch.cuda.synchronize()
+        occ = {"functional": "GNOF", "occ_method": "Softmax", "ndoc": occ_ndoc, "ncwo": occ_ncwo,
+               "nv": occ_ndoc * occ_ncwo, "energy_and_gradient_ms": round(ev[0].elapsed_time(ev[1]) / reps, 4)}
+        if not args.no_cpu_baseline:  # the oracle's numpy restatement of the same step, for scale
+            from oracle import pnof_oracle as PO
+
+            po = PO.Pnof(0, occ_ndoc, 0, occ_ncwo, ipnof=8, occ_method="Softmax")
+            Jh, Kh, hh = (out[k].cpu().numpy() for k in ("J_MO", "K_MO", "H_core"))
+            gh = gam.cpu().numpy()
+            t0 = time.perf_counter()
+            PO.calcocce(gh, Jh, Kh, hh, po)
+            PO.calcoccg(gh, Jh, Kh, hh, po)
+            occ["oracle_numpy_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+
     line = {
         "metric": METRIC,
         "value": round(1e3 / ms_per_step, 4),
@@ -366,6 +394,7 @@ def main():
         "mo_integrals": mo,
         "lagrangian": lag,
         "fock": fock,
+        "occupation_step": occ,
     }
     eng.close()
     del out, Cd, Hd

@@ -96,6 +96,8 @@ class Server:
         self.step_sizes: list[int] = []
 
     def submit(self, req: Request) -> None:
+        if req.builds_done or req.energies or req.done is not None:
+            raise ValueError(f"request {req.rid} was already served; submit a fresh Request")
         req.footprint = int(self.ex.footprint(req))
         if req.footprint > self.budget:
             req.error = f"footprint {req.footprint} B exceeds the device budget {int(self.budget)} B"
@@ -143,7 +145,7 @@ class Server:
             t = self.clock()
             for r in batch:
                 r.builds_done += 1
-                if r.builds_done == r.n_builds:
+                if r.builds_done >= r.n_builds:
                     r.done = t
                     self.active.remove(r)
                     self.ex.release(r)
@@ -185,9 +187,17 @@ def poisson_requests(count: int, rate: float, seed: int = 5000, median_nbf: int 
 class DeviceExecutor(Executor):
     """Runs requests on one GPU: one JKEngine per request (its b resident), grouped
     builds per step. C is rotated on the device by exp(kappa_i) with a seeded
-    antisymmetric kappa_i, ||kappa_i||_F = 1e-2."""
+    antisymmetric kappa_i, ||kappa_i||_F = 1e-2.
 
-    def __init__(self, device: int = 0, with_jk: bool = True, keep_final_C: bool = False):
+    With ``occupation=True`` each build is followed by an occupation step on the
+    device (GNOF, softmax parameters; pnjk_occupation_step): the request's energy
+    is calcocce at its current parameters gamma (pnof.py:994-1001), and gamma takes
+    one gradient step, gamma -= occ_lr * calcoccg, before the next build. The
+    occupation block is the first ndoc (1 + ncwo) orbitals, ncwo = (nbf5 - ndoc) // ndoc.
+    J_MO, K_MO, H_core and gamma never leave the GPU."""
+
+    def __init__(self, device: int = 0, with_jk: bool = True, keep_final_C: bool = False,
+                 occupation: bool = False, occ_lr: float = 0.05):
         import torch
 
         self.torch = torch
@@ -195,6 +205,8 @@ class DeviceExecutor(Executor):
         self.device = device
         self.with_jk = with_jk
         self.keep_final_C = keep_final_C
+        self.occupation = occupation
+        self.occ_lr = occ_lr
         self.state: dict[int, dict] = {}
         self._groups: dict = {}
 
@@ -231,6 +243,11 @@ class DeviceExecutor(Executor):
             outs["K"] = torch.empty((n5, nbf, nbf), dtype=torch.float64, device=self.dev)
         self.state[req.rid] = {"eng": eng, "C": C, "H": H, "n": torch.from_numpy(n).to(self.dev), "outs": outs,
                                "gen": g}
+        if self.occupation:
+            ncwo = max(0, (n5 - req.ndoc) // req.ndoc)
+            pn = JKEngine.pnof(0, req.ndoc, 0, ncwo, ipnof=8, occ_method="Softmax")
+            gamma = torch.full((req.ndoc * ncwo,), -2.0, dtype=torch.float64, device=self.dev)
+            self.state[req.rid].update(pnof=pn, gamma=gamma, occ_out=None)
 
     def _group(self, reqs):
         from .batch import JKGroup
@@ -258,7 +275,16 @@ class DeviceExecutor(Executor):
         for r in reqs:  # energies (FP64, on the device), then the orbital rotation for the next build
             s = self.state[r.rid]
             o, n = s["outs"], s["n"]
-            e = 2.0 * torch.dot(n, o["H_core"]) + n @ (2.0 * o["J_MO"] - o["K_MO"]) @ n
+            if self.occupation:
+                m = s["pnof"].nbf5
+                s["occ_out"] = s["eng"].occupation_step(
+                    s["gamma"], s["pnof"], o["J_MO"][:m, :m].contiguous(), o["K_MO"][:m, :m].contiguous(),
+                    o["H_core"][:m].contiguous(), out=s["occ_out"])
+                e = s["occ_out"]["E"][0].clone()
+                if r.builds_done + 1 < r.n_builds:
+                    s["gamma"] = s["gamma"] - self.occ_lr * s["occ_out"]["grad"]
+            else:
+                e = 2.0 * torch.dot(n, o["H_core"]) + n @ (2.0 * o["J_MO"] - o["K_MO"]) @ n
             r.energies.append(e)
             if r.builds_done + 1 < r.n_builds:
                 a = torch.randn((r.nbf, r.nbf), dtype=torch.float64, device=self.dev, generator=s["gen"])
@@ -273,6 +299,8 @@ class DeviceExecutor(Executor):
         if self.keep_final_C:
             req.final_C = s["C"].cpu().numpy()
             req.final_outs = {k: v.cpu().numpy() for k, v in s["outs"].items()}
+            if self.occupation:
+                req.final_gamma = s["gamma"].cpu().numpy()
         for k in list(self._groups):
             if req.rid in k:
                 self._groups.pop(k).close()

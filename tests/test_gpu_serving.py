@@ -60,3 +60,33 @@ def test_tight_request_final_energy_is_fp64(gpu_required):
     e1, e2 = synth.energy_terms(n, o["J_MO"], o["K_MO"], o["H_core"])
     assert abs((e1 + e2) - r.energies[-1]) <= 1e-10 * abs(e1 + e2)
     eng.close()
+
+
+def test_serving_with_occupation_step(gpu_required):
+    """Each build followed by a device occupation step (GNOF, softmax): the final energy
+    equals the oracle's calcocce at the request's final gamma and final MO integrals,
+    and the mix equals a request alone."""
+    from oracle import pnof_oracle as PO
+
+    print("case: serving with the occupation step", flush=True)
+    rs = reqs()[:3]
+    srv = Server(DeviceExecutor(0, keep_final_C=True, occupation=True), hbm_budget=200e9)
+    for r in rs:
+        srv.submit(r)
+    srv.run(timeout=300)
+    assert not srv.rejected
+    mix = {r.rid: r for r in srv.finished}
+    for r in mix.values():
+        ncwo = (r.nbf5 - r.ndoc) // r.ndoc
+        m = r.ndoc * (1 + ncwo)
+        p = PO.Pnof(0, r.ndoc, 0, ncwo, ipnof=8, occ_method="Softmax")
+        o = r.final_outs
+        E = PO.calcocce(r.final_gamma, o["J_MO"][:m, :m], o["K_MO"][:m, :m], o["H_core"][:m], p)
+        assert abs(E - r.energies[-1]) <= 1e-12 * max(1.0, abs(E)), r.rid
+        assert np.all(r.final_gamma != -2.0)  # the parameters moved on the device
+    r0 = reqs()[1]  # a fresh request object: a served one is not resubmitted
+    r0.arrival = 0.0
+    srv1 = Server(DeviceExecutor(0, keep_final_C=True, occupation=True), hbm_budget=200e9)
+    srv1.submit(r0)
+    srv1.run(timeout=300)
+    assert srv1.finished[0].energies == mix[r0.rid].energies

@@ -140,3 +140,17 @@ def test_poisson_requests_same_requests_at_every_rate():
     a, b = poisson_requests(50, rate=0.0, seed=5000), poisson_requests(50, rate=7.5, seed=5000)
     assert [(r.nbf, r.tol, r.seed) for r in a] == [(r.nbf, r.tol, r.seed) for r in b]
     assert all(r.arrival == 0.0 for r in a) and b[-1].arrival > 0.0
+
+
+def test_served_request_is_not_resubmitted():
+    """A served Request object carries its finished build count; submitting it again
+    raises instead of looping (the count would never meet n_builds again)."""
+    clock = Clock()
+    ex = FakeExecutor(clock)
+    srv = Server(ex, hbm_budget=1e12, clock=clock, sleep=clock.sleep)
+    r = make(0, 40)
+    srv.submit(r)
+    srv.run(t0=0.0)
+    assert r.builds_done == r.n_builds
+    with pytest.raises(ValueError):
+        Server(FakeExecutor(clock), hbm_budget=1e12, clock=clock, sleep=clock.sleep).submit(r)
