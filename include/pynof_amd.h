@@ -66,6 +66,26 @@ int pnjk_destroy(pnjk_engine* e);
 int pnjk_upload_b(pnjk_engine* e, const double* b_mnl, int64_t nbf, int64_t naux, double sym_tol,
                   uint64_t* footprint, double* asym);
 
+/* Forms b on the device from the 3-centre integrals and the metric, as the
+ * reference's b producer does (integrals.py:35-49): mnk[mu][nu][P] = (mu nu|P)
+ * (nbf, nbf, naux), C-order, symmetric in (mu, nu), host or device;
+ * metric[P][Q] = (P|Q) (naux, naux), host or device. The metric's inverse square
+ * root is taken as psi4's Matrix::power(-0.5, cutoff) does (integrals.py:41-42;
+ * psi4 is not pinned by the reference): eigenpairs (rocSOLVER dsyevd), eigenvalues
+ * with |a| < cutoff * max|a| dropped, V diag(a^-1/2) V^T. Then
+ * b[mu][nu][P] = sum_Q (mu nu|Q) [(P|Q)^-1/2][Q][P] (integrals.py:49) as one FP64
+ * GEMM over the packed mu >= nu rows, left resident as pnjk_upload_b leaves b.
+ * mnk is checked like b in pnjk_upload_b (finiteness, symmetry within sym_tol,
+ * *asym). The reference passes cutoff = 1e-14. */
+int pnjk_upload_b_3c(pnjk_engine* e, const double* mnk, int64_t nbf, int64_t naux, const double* metric,
+                     double cutoff, double sym_tol, uint64_t* footprint, double* asym);
+
+/* Writes the resident b back in the reference layout (nbf, nbf, naux), C-order,
+ * to host or device memory (host memory, e.g. a memory-mapped .npy, is filled in
+ * mu-row chunks). The inverse of pnjk_upload_b; no counterpart in the reference,
+ * which keeps b as a numpy/cupy array (integrals.py:49-53). */
+int pnjk_download_b(pnjk_engine* e, double* b_mnl);
+
 /* Aux stride of the packed layout the engine uses for (nbf, naux). */
 int64_t pnjk_packed_ld(int64_t nbf, int64_t naux);
 

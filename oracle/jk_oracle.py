@@ -316,6 +316,29 @@ def rotate_orbital(y, C, nbf5):
     return C @ expm(kap)
 
 
+def metric_power(metric, alpha=-0.5, cutoff=1e-14):
+    """psi4 Matrix::power(alpha, cutoff) as integrals.py:42 uses it (psi4 is a dependency the
+    reference does not pin): eigenpairs of the symmetric metric; for alpha < 0, eigenvalues
+    with |a| < cutoff * max|a| are dropped; the rest are raised to alpha, non-finite results
+    dropped; V diag(a') V^T."""
+    a, V = np.linalg.eigh(np.asarray(metric, dtype=np.float64))
+    amax = max(abs(a[0]), abs(a[-1]))
+    f = np.zeros_like(a)
+    for i, x in enumerate(a):
+        if alpha < 0.0 and abs(x) < cutoff * amax:
+            continue
+        with np.errstate(all="ignore"):
+            v = x ** alpha
+        f[i] = v if np.isfinite(v) else 0.0
+    return (V * f) @ V.T
+
+
+def b_from_3c(mnk, metric, cutoff=1e-14):
+    """The reference's b producer (integrals.py:35-49): b[m, n, l] = sum_k (mn|k) [(P|Q)^-1/2][k, l]."""
+    W = metric_power(metric, -0.5, cutoff)
+    return np.einsum("mnk,kl->mnl", mnk, W, optimize=True)
+
+
 def energy(n, J_MO, K_MO, H_core):
     """E_1 + E_2 with E_1 = 2 sum n H_core and E_2 = sum_pq n_p n_q (2J_pq - K_pq)."""
     e1 = 2.0 * float(np.dot(n, H_core))

@@ -37,6 +37,10 @@ _SIGS = {
     "pnjk_upload_b": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
                                      ctypes.c_double, ctypes.POINTER(ctypes.c_uint64),
                                      ctypes.POINTER(ctypes.c_double)]),
+    "pnjk_upload_b_3c": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                        ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                        ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]),
+    "pnjk_download_b": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "pnjk_packed_ld": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int64]),
     "pnjk_upload_b_packed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
                                             ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64)]),

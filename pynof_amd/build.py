@@ -59,7 +59,9 @@ def build(verbose: bool = False) -> pathlib.Path:
     with cf.ThreadPoolExecutor(workers) as ex:
         objs = list(ex.map(_compile, sources))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    # rocSOLVER (and its rocBLAS handle) for the metric eigensolver of the b producer
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-L/opt/rocm/lib",
+           "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -6,6 +6,9 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "jk_common.h"
 #include "jk_kernels.h"
 #include "pynof_amd.h"
@@ -133,6 +136,7 @@ struct pnjk_engine {
   DevBuf bm16h, bm16l, w16h, w16l;               // split form of the Lagrangian's exchange operands
   DevBuf fjk, fh, fco, ff;                       // Fock: staged J/K, H, coefficients, F
   DevBuf occ;                                    // occupation step: every array of the step
+  rocblas_handle blas = nullptr;                 // for the metric eigensolver (b from 3-centre integrals)
   bool split_ready = false;
   // Tile override per arithmetic (0: FP64 kernels, 1: BF16x3 kernels) and stage;
   // {0, 0} selects the built-in choice.
@@ -176,6 +180,7 @@ int pnjk_destroy(pnjk_engine* e) {
   if (!e) return PNJK_OK;
   (void)hipSetDevice(e->device);
   (void)hipStreamSynchronize(e->own);
+  if (e->blas) (void)rocblas_destroy_handle(e->blas);
   for (int i = 0; i < e->nall; ++i) e->all[i]->release();
   for (hipEvent_t& ev : e->ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -245,10 +250,11 @@ static int upload_common(pnjk_engine* e, int64_t nbf, int64_t naux) {
   return e->btri.ensure((size_t)(e->ntri * e->naux_p) * sizeof(double));
 }
 
-int pnjk_upload_b(pnjk_engine* e, const double* b, int64_t nbf, int64_t naux, double sym_tol, uint64_t* footprint,
-                  double* asym) {
-  if (!e || !b) return set_error(PNJK_EINVAL, "pnjk_upload_b: null argument");
-  std::lock_guard<std::mutex> lk(e->mu);
+}  // extern "C"
+
+// pnjk_upload_b with the engine's mutex held.
+static int upload_b_locked(pnjk_engine* e, const double* b, int64_t nbf, int64_t naux, double sym_tol,
+                           double* asym) {
   PNJK_HIP_CHECK(hipSetDevice(e->device));
   e->nbf = 0;
   int rc = upload_common(e, nbf, naux);
@@ -290,6 +296,109 @@ int pnjk_upload_b(pnjk_engine* e, const double* b, int64_t nbf, int64_t naux, do
     return set_error(PNJK_EINVAL, "upload_b: b is not symmetric in (mu, nu): max |b - b^T| = " +
                                       std::to_string(host[0]) + " > tolerance " + std::to_string(sym_tol));
   }
+  return PNJK_OK;
+}
+
+extern "C" {
+
+int pnjk_upload_b(pnjk_engine* e, const double* b, int64_t nbf, int64_t naux, double sym_tol, uint64_t* footprint,
+                  double* asym) {
+  if (!e || !b) return set_error(PNJK_EINVAL, "pnjk_upload_b: null argument");
+  std::lock_guard<std::mutex> lk(e->mu);
+  const int rc = upload_b_locked(e, b, nbf, naux, sym_tol, asym);
+  if (rc) return rc;
+  if (footprint) *footprint = pnjk_footprint(e);
+  return PNJK_OK;
+}
+
+int pnjk_download_b(pnjk_engine* e, double* b_mnl) {
+  if (!e || !b_mnl) return set_error(PNJK_EINVAL, "pnjk_download_b: null argument");
+  std::lock_guard<std::mutex> lk(e->mu);
+  PNJK_HIP_CHECK(hipSetDevice(e->device));
+  if (e->nbf == 0) return set_error(PNJK_ESTATE, "pnjk_download_b: no b uploaded");
+  hipStream_t s = e->own;
+  const int64_t nbf = e->nbf, naux = e->naux, row_bytes = nbf * naux * 8;
+  if (is_device_ptr(b_mnl)) {
+    PNJK_HIP_CHECK(launch_unpack_b((int)nbf, (int)naux, e->naux_p, 0, (int)nbf, e->btri.as<double>(), b_mnl, s));
+  } else {  // stream mu-row chunks (~512 MiB) through a staging buffer
+    const int64_t rows = std::max<int64_t>(1, std::min<int64_t>(nbf, ((int64_t)512 << 20) / row_bytes));
+    int rc;
+    if ((rc = e->stage.ensure((size_t)(rows * row_bytes)))) return rc;
+    for (int64_t mu0 = 0; mu0 < nbf; mu0 += rows) {
+      const int64_t mu1 = std::min(nbf, mu0 + rows);
+      PNJK_HIP_CHECK(launch_unpack_b((int)nbf, (int)naux, e->naux_p, (int)mu0, (int)mu1, e->btri.as<double>(),
+                                     e->stage.as<double>(), s));
+      PNJK_HIP_CHECK(hipMemcpyAsync(b_mnl + mu0 * nbf * naux, e->stage.p, (size_t)((mu1 - mu0) * row_bytes),
+                                    hipMemcpyDeviceToHost, s));
+      PNJK_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    e->stage.release();
+  }
+  PNJK_HIP_CHECK(hipStreamSynchronize(s));
+  return PNJK_OK;
+}
+
+int pnjk_upload_b_3c(pnjk_engine* e, const double* mnk, int64_t nbf, int64_t naux, const double* metric,
+                     double cutoff, double sym_tol, uint64_t* footprint, double* asym) {
+  if (!e || !mnk || !metric) return set_error(PNJK_EINVAL, "pnjk_upload_b_3c: null argument");
+  if (!(cutoff >= 0.0)) return set_error(PNJK_EINVAL, "pnjk_upload_b_3c: cutoff must be >= 0");
+  std::lock_guard<std::mutex> lk(e->mu);
+  // The packed 3-centre integrals (mu nu|P), checked like b, land where b will live.
+  int rc = upload_b_locked(e, mnk, nbf, naux, sym_tol, asym);
+  if (rc) return rc;
+  hipStream_t s = e->own;
+  const int n = (int)naux, ld = e->naux_p;
+  const int64_t ldld = (int64_t)ld * ld;
+  DevBuf M, D, E, info, Vp, Vs, W, out;
+  struct Release {  // frees the local buffers on every return
+    DevBuf* b[8];
+    ~Release() {
+      for (DevBuf* x : b) x->release();
+    }
+  } release_all{{&M, &D, &E, &info, &Vp, &Vs, &W, &out}};
+  if ((rc = M.ensure((size_t)naux * naux * 8)) || (rc = D.ensure((size_t)naux * 8)) ||
+      (rc = E.ensure((size_t)naux * 8)) || (rc = info.ensure(64)) || (rc = Vp.ensure((size_t)ldld * 8)) ||
+      (rc = Vs.ensure((size_t)ldld * 8)) || (rc = W.ensure((size_t)(ldld + 128) * 8)) ||
+      (rc = out.ensure((size_t)(e->ntri * ld) * 8))) {
+    e->nbf = 0;
+    return rc;
+  }
+  auto fail = [&](int code, const std::string& msg) {
+    e->nbf = 0;
+    return set_error(code, msg);
+  };
+  if (hipMemcpyAsync(M.p, metric, (size_t)naux * naux * 8,
+                     is_device_ptr(metric) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s) != hipSuccess)
+    return fail(PNJK_EHIP, "pnjk_upload_b_3c: metric copy failed");
+  // (P|Q)^-1/2 as psi4's Matrix::power(-0.5, cutoff) forms it (integrals.py:41-42):
+  // eigenpairs, the relative eigenvalue cut, V diag(a^-1/2) V^T.
+  if (!e->blas) {
+    if (rocblas_create_handle(&e->blas) != rocblas_status_success) {
+      e->blas = nullptr;
+      return fail(PNJK_EHIP, "pnjk_upload_b_3c: rocblas_create_handle failed");
+    }
+  }
+  if (rocblas_set_stream(e->blas, s) != rocblas_status_success)
+    return fail(PNJK_EHIP, "pnjk_upload_b_3c: rocblas_set_stream failed");
+  if (rocsolver_dsyevd(e->blas, rocblas_evect_original, rocblas_fill_lower, n, M.as<double>(), n, D.as<double>(),
+                       E.as<double>(), reinterpret_cast<rocblas_int*>(info.p)) != rocblas_status_success)
+    return fail(PNJK_EHIP, "pnjk_upload_b_3c: rocsolver_dsyevd failed");
+  rocblas_int hinfo = 0;
+  PNJK_HIP_CHECK(hipMemcpyAsync(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost, s));
+  PNJK_HIP_CHECK(hipStreamSynchronize(s));
+  if (hinfo != 0)
+    return fail(PNJK_EINVAL, "pnjk_upload_b_3c: the metric eigensolver did not converge (info " +
+                                 std::to_string(hinfo) + ")");
+  PNJK_HIP_CHECK(launch_metric_scale(D.as<double>(), M.as<double>(), n, ld, cutoff, Vp.as<double>(), Vs.as<double>(),
+                                     s));
+  PNJK_HIP_CHECK(fill(W.as<double>() + ldld, 128, 0.0, s));
+  PNJK_HIP_CHECK(launch_atb_f64(Vp.as<double>(), Vs.as<double>(), ld, W.as<double>(), s));
+  // b[t][P] = sum_Q (mu nu|Q) W[Q][P] over the packed rows (integrals.py:49)
+  PNJK_HIP_CHECK(launch_rowgemm_f64(e->btri.as<double>(), e->ntri, W.as<double>(), ld, out.as<double>(), s));
+  PNJK_HIP_CHECK(hipStreamSynchronize(s));
+  std::swap(e->btri.p, out.p);  // `out` now holds the 3-centre integrals and is freed on return
+  std::swap(e->btri.bytes, out.bytes);
+  e->split_ready = false;
   if (footprint) *footprint = pnjk_footprint(e);
   return PNJK_OK;
 }

@@ -694,6 +694,79 @@ struct ProbFock {
   }
 };
 
+// out[t][n] = sum_k A[t][k] W[k][n] for rows t < rows, n < K: the packed b from the packed
+// 3-centre integrals (mu nu|P) and the metric's inverse square root (integrals.py:49). A has
+// row stride K; W is K x K (row stride K) with at least 128 doubles of slack at its end.
+struct ProbRowGemm {
+  static constexpr int kA = MK, kB = KM;
+  const double* A;
+  const double* W;
+  double* out;
+  int rows, K, mtiles;
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.batch = 0;
+    t.mt = id % mtiles;
+    t.nt = id / mtiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ bool block_active(const Tile&, int r0, int) const { return r0 < rows; }
+  __device__ int ksteps(int BK) const { return K / BK; }
+  __device__ SrcMKDense a_src(const Tile&) const { return {A, K, rows}; }
+  __device__ SrcKMDense b_src(const Tile&) const { return {W, K}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4d (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int m = t.m0 + dm, n = t.n0 + dn;
+          if (m < rows && n < K) gmem(out)[(int64_t)m * K + n] = acc[i][j][r];
+        }
+  }
+};
+
+// out[m][n] = sum_k A[k][m] B[k][n], all K x K with row stride K (K a multiple of 64):
+// A^T B, for V diag(f) V^T from the eigenvectors.
+struct ProbAtB {
+  static constexpr int kA = KM, kB = KM;
+  const double* A;
+  const double* B;
+  double* out;
+  int K, mtiles;
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.batch = 0;
+    t.mt = id % mtiles;
+    t.nt = id / mtiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ bool block_active(const Tile&, int r0, int) const { return r0 < K; }
+  __device__ int ksteps(int BK) const { return K / BK; }
+  __device__ SrcKMDense a_src(const Tile&) const { return {A, K}; }
+  __device__ SrcKMDense b_src(const Tile&) const { return {B, K}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4d (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int m = t.m0 + dm, n = t.n0 + dn;
+          if (m < K && n < K) gmem(out)[(int64_t)m * K + n] = acc[i][j][r];
+        }
+  }
+};
+
 // ---------------------------------------------------------------- small kernels
 __global__ void k_kmo_reduce(int nbf5, int nt, const double* __restrict__ kpart, double* __restrict__ kmo) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1169,6 +1242,63 @@ hipError_t launch_lagrange_fock_f64(const BuildGeom& g, const double* F, const d
   return hipGetLastError();
 }
 
+// Eigenvectors of the metric (column-major from the eigensolver: vector i at M[i*n ..]) and
+// eigenvalues D (ascending) -> Vp[i][r] = V[r][i], Vs[i][r] = f(D_i) V[r][i], both ld x ld,
+// zero-padded. f is the inverse square root with the cut of the psi4 Matrix::power used
+// at integrals.py:42: |D_i| < cutoff * max|D| -> 0; a non-finite power -> 0.
+__global__ void k_metric_scale(const double* __restrict__ D, const double* __restrict__ M, int n, int ld,
+                               double cutoff, double* __restrict__ Vp, double* __restrict__ Vs) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)ld * ld) return;
+  const int i = (int)(idx / ld), r = (int)(idx % ld);
+  double v = 0.0, f = 0.0;
+  if (i < n && r < n) {
+    v = M[(int64_t)i * n + r];
+    const double lo = fabs(D[0]), hi = fabs(D[n - 1]);
+    const double amax = hi > lo ? hi : lo;
+    const double a = D[i];
+    if (!(fabs(a) < cutoff * amax)) {
+      f = pow(a, -0.5);
+      if (!isfinite(f)) f = 0.0;
+    }
+  }
+  Vp[idx] = v;
+  Vs[idx] = f * v;
+}
+
+hipError_t launch_metric_scale(const double* D, const double* M, int n, int ld, double cutoff, double* Vp, double* Vs,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_metric_scale, dim3(cdiv((int64_t)ld * ld, 256)), dim3(256), 0, s, D, M, n, ld, cutoff, Vp, Vs);
+  return hipGetLastError();
+}
+
+hipError_t launch_atb_f64(const double* A, const double* B, int K, double* out, hipStream_t s) {
+  if (K % 64 != 0) return hipErrorInvalidValue;
+  ProbAtB pr;
+  pr.A = A;
+  pr.B = B;
+  pr.out = out;
+  pr.K = K;
+  pr.mtiles = K / 64;
+  return launch_prob(pr, TileF64{2, 2}, pr.mtiles * pr.mtiles, s);
+}
+
+hipError_t launch_rowgemm_f64(const double* A, int64_t rows, const double* W, int K, double* out, hipStream_t s) {
+  if (K % 64 != 0 || rows <= 0 || rows > 2000000000LL) return hipErrorInvalidValue;
+  const bool big = rows >= 4096 && K >= 256;
+  const TileF64 tc = big ? TileF64{5, 4} : TileF64{2, 2};
+  ProbRowGemm pr;
+  pr.A = A;
+  pr.W = W;
+  pr.out = out;
+  pr.rows = (int)rows;
+  pr.K = K;
+  pr.mtiles = cdiv(rows, 32 * tc.fm);
+  const int64_t tiles = (int64_t)pr.mtiles * cdiv(K, 32 * tc.fn);
+  if (tiles > 2147483647LL) return hipErrorInvalidValue;
+  return launch_prob(pr, tc, (int)tiles, s);
+}
+
 hipError_t launch_hcore_f64(const BuildGeom& g, const double* Hp, int ldh, const double* Cp, double* T,
                             double* hcore, hipStream_t s) {
   const TileF64 tc{2, 2};
@@ -1184,6 +1314,25 @@ hipError_t launch_hcore_f64(const BuildGeom& g, const double* Hp, int ldh, const
   hipError_t e = launch_prob(pr, tc, pr.mtiles * cdiv(g.ldcp, 64), s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_hcore, dim3(cdiv(g.nbf5, 256)), dim3(256), 0, s, g.nbf, g.nbf5, g.ldcp, Cp, T, hcore);
+  return hipGetLastError();
+}
+
+// Inverse of k_pack_b for rows mu0 <= mu < mu1: out[(mu - mu0) * nbf + nu][P] = btri[tri(max, min)][P].
+__global__ void k_unpack_b(int nbf, int naux, int naux_p, int mu0, const double* __restrict__ btri,
+                           double* __restrict__ out) {
+  const int64_t row = blockIdx.x;
+  const int mu = mu0 + (int)(row / nbf), nu = (int)(row % nbf);
+  const double* src = btri + (mu >= nu ? tri_index(mu, nu) : tri_index(nu, mu)) * naux_p;
+  double* dst = out + row * naux;
+  for (int p = threadIdx.x; p < naux; p += blockDim.x) dst[p] = src[p];
+}
+
+hipError_t launch_unpack_b(int nbf, int naux, int naux_p, int mu0, int mu1, const double* btri, double* out,
+                           hipStream_t s) {
+  const int64_t rows = (int64_t)(mu1 - mu0) * nbf;
+  if (rows <= 0) return hipSuccess;
+  if (rows > 2147483647LL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_unpack_b, dim3((unsigned)rows), dim3(256), 0, s, nbf, naux, naux_p, mu0, btri, out);
   return hipGetLastError();
 }
 

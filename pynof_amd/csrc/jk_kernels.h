@@ -131,6 +131,9 @@ hipError_t launch_hcore_f64(const BuildGeom& g, const double* Hp, int ldh, const
 // a zero tail. Chunks must be packed in descending mu order on one stream.
 // Folds max |b[mu,nu,P] - b[nu,mu,P]| into *asym_max and counts non-finite
 // values into *nonfinite (both device scalars).
+// Rows mu0 <= mu < mu1 of b in the (nbf, nbf, naux) layout from the packed rows.
+hipError_t launch_unpack_b(int nbf, int naux, int naux_p, int mu0, int mu1, const double* btri, double* out,
+                           hipStream_t s);
 hipError_t launch_pack_b(int nbf, int naux, int naux_p, int mu0, int mu1, const double* b_chunk, double* btri,
                          double* asym_max, unsigned long long* nonfinite, hipStream_t s);
 
@@ -169,6 +172,16 @@ hipError_t launch_jn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* d_hi, co
 // K^n from split X16 (FP64 output, lower triangle of tiles only).
 hipError_t launch_kn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* x_hi, const uint16_t* x_lo, double* K,
                         hipStream_t s, GroupCollector* col = nullptr);
+
+// ---------------------------------------------------------------- b from 3-centre integrals
+// Vp, Vs (ld x ld, zero-padded) from the metric's eigenpairs; Vs rows scaled by D^-1/2 with
+// the relative eigenvalue cut.
+hipError_t launch_metric_scale(const double* D, const double* M, int n, int ld, double cutoff, double* Vp, double* Vs,
+                               hipStream_t s);
+// out = A^T B, K x K (K a multiple of 64).
+hipError_t launch_atb_f64(const double* A, const double* B, int K, double* out, hipStream_t s);
+// out (rows x K) = A (rows x K) W (K x K, + 128 doubles of slack); K a multiple of 64.
+hipError_t launch_rowgemm_f64(const double* A, int64_t rows, const double* W, int K, double* out, hipStream_t s);
 
 // ---------------------------------------------------------------- occupation step
 // PNOF functional and occupation parametrisation (the param fields of parameters.py:46-110).

@@ -104,6 +104,47 @@ class JKEngine:
         self.nbf, self.naux = nbf, naux
         return int(fp.value)
 
+    def upload_b_3c(self, mnk, metric, cutoff: float = 1e-14, sym_tol: float = 1e-10) -> int:
+        """Forms b on the device from the 3-centre integrals mnk[mu, nu, P] = (mu nu|P)
+        (nbf, nbf, naux) and the metric (P|Q) (naux, naux) (integrals.py:35-49): b = mnk
+        (P|Q)^-1/2 with the inverse square root cut as psi4's Matrix::power(-0.5, cutoff).
+        Host (numpy) or device (torch) arrays. Returns the resident footprint in bytes."""
+        m3 = _as_input(mnk, "mnk")
+        mt = _as_input(metric, "metric")
+        if m3.ndim != 3 or m3.shape[0] != m3.shape[1]:
+            raise ValueError(f"mnk must have shape (nbf, nbf, naux); got {tuple(m3.shape)}")
+        nbf, _, naux = (int(x) for x in m3.shape)
+        if tuple(mt.shape) != (naux, naux):
+            raise ValueError(f"metric must have shape ({naux}, {naux}); got {tuple(mt.shape)}")
+        for x in (m3, mt):
+            if _is_torch(x):
+                _torch().cuda.current_stream(x.device).synchronize()
+        fp = ctypes.c_uint64()
+        asym = ctypes.c_double()
+        _lib.check(_lib.lib().pnjk_upload_b_3c(self._h, _ptr(m3), nbf, naux, _ptr(mt), float(cutoff),
+                                               float(sym_tol), ctypes.byref(fp), ctypes.byref(asym)))
+        self.nbf, self.naux, self.asym = nbf, naux, asym.value
+        return int(fp.value)
+
+    def download_b(self, out=None):
+        """The resident b in the reference layout (nbf, nbf, naux): into `out` (numpy, a
+        numpy memmap or a device tensor) or a new host array."""
+        if self.nbf == 0:
+            raise _lib.PnjkError(_lib.ESTATE, "no b uploaded")
+        shape = (self.nbf, self.nbf, self.naux)
+        if out is None:
+            out = np.empty(shape, dtype=np.float64)
+        if tuple(out.shape) != shape:
+            raise ValueError(f"out must have shape {shape}; got {tuple(out.shape)}")
+        if _is_torch(out):
+            if out.dtype != _torch().float64 or not out.is_contiguous():
+                raise ValueError("out must be a contiguous float64 tensor")
+            _torch().cuda.current_stream(out.device).synchronize()
+        elif out.dtype != np.float64 or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous float64 array")
+        _lib.check(_lib.lib().pnjk_download_b(self._h, _ptr(out)))
+        return out
+
     def build(self, C, nbf5: int, H=None, precision: str = "fp64",
               outputs=("J", "K", "J_MO", "K_MO", "H_core"), out=None, stream=None):
         """One fused JK build.

@@ -328,3 +328,51 @@ def test_fock_planted_mistake_fails():
     # C^K applied to J instead of K
     wrong = F + np.einsum("ij,jmn->imn", ck - np.diag(np.diag(ck)), K - J)
     assert rel(O.lagrange_fock(wrong, C, 6), el2) > 1e-3
+
+
+# ---------------------------------------------------------------- b producer (integrals.py:35-49)
+def _three_center(nbf, naux, seed):
+    rng = np.random.default_rng(seed)
+    m = rng.standard_normal((nbf, nbf, naux))
+    return 0.5 * (m + m.transpose(1, 0, 2))
+
+
+def test_b_from_3c_closed_forms():
+    mnk = _three_center(5, 7, 900)
+    assert np.allclose(O.b_from_3c(mnk, np.eye(7)), mnk, rtol=0, atol=1e-14)
+    d = np.linspace(0.3, 4.0, 7)
+    assert np.allclose(O.b_from_3c(mnk, np.diag(d)), mnk / np.sqrt(d), rtol=1e-14, atol=1e-14)
+
+
+def test_b_from_3c_reproduces_ri_integrals():
+    """sum_P b[mn,P] b[ls,P] = sum_PQ (mn|P) [(P|Q)^-1]_PQ (Q|ls), by a linear solve (no eigenpairs)."""
+    nbf, naux = 6, 11
+    mnk = _three_center(nbf, naux, 901)
+    rng = np.random.default_rng(902)
+    A = rng.standard_normal((naux, naux))
+    metric = A @ A.T + naux * np.eye(naux)
+    b = O.b_from_3c(mnk, metric)
+    lhs = np.einsum("mnP,lsP->mnls", b, b)
+    x = np.linalg.solve(metric, mnk.reshape(-1, naux).T)  # (naux, nbf^2)
+    rhs = (mnk.reshape(-1, naux) @ x).reshape(nbf, nbf, nbf, nbf)
+    assert np.max(np.abs(lhs - rhs)) < 1e-12 * np.max(np.abs(rhs))
+    assert np.array_equal(b, b.transpose(1, 0, 2)) or np.max(np.abs(b - b.transpose(1, 0, 2))) < 1e-14
+    bad = np.einsum("mnk,kl->mnl", mnk, O.metric_power(metric, +0.5))  # planted: wrong power
+    assert np.max(np.abs(np.einsum("mnP,lsP->mnls", bad, bad) - rhs)) > 1e-3
+
+
+def test_b_from_3c_drops_tiny_eigenvalues():
+    """A metric built as V diag(a) V^T with one eigenvalue below 1e-14 max|a|: that direction
+    is dropped, b = mnk V diag(a'^-1/2) V^T with a'^-1/2 = 0 there."""
+    naux = 8
+    rng = np.random.default_rng(903)
+    V, _ = np.linalg.qr(rng.standard_normal((naux, naux)))
+    a = np.linspace(1.0, 5.0, naux)
+    a[2] = 1e-16 * 5.0
+    metric = (V * a) @ V.T
+    f = 1.0 / np.sqrt(a)
+    f[2] = 0.0
+    mnk = _three_center(4, naux, 904)
+    expect = np.einsum("mnk,kl->mnl", mnk, (V * f) @ V.T)
+    got = O.b_from_3c(mnk, metric)
+    assert np.max(np.abs(got - expect)) < 1e-12 * np.max(np.abs(expect))
