@@ -117,6 +117,8 @@ def main():
     ap.add_argument("--form-steps", type=int, default=5, help="builds timed per single form")
     ap.add_argument("--cpu-sample-naux", type=int, default=72)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip", nargs="*", default=[], choices=["b_producer"],
+                    help="report sections to leave out (the headline stream is always run)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every box here has one MI355X, so --gpus must be 1 "
@@ -366,6 +368,11 @@ def main():
             PO.calcoccg(gh, Jh, Kh, hh, po)
             occ["oracle_numpy_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
 
+    # ---- b from 3-centre integrals (integrals.py:35-49): pack + metric eigenpairs + FP64 GEMM
+    bprod = {}
+    if "b_producer" not in args.skip:
+        bprod = bench_b_producer(JKEngine, nbf, naux, dev)
+
     line = {
         "metric": METRIC,
         "value": round(1e3 / ms_per_step, 4),
@@ -395,6 +402,7 @@ def main():
         "lagrangian": lag,
         "fock": fock,
         "occupation_step": occ,
+        "b_producer": bprod,
     }
     eng.close()
     del out, Cd, Hd
@@ -412,6 +420,36 @@ def main():
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
+
+
+def bench_b_producer(JKEngine, nbf, naux, dev):
+    """b from 3-centre integrals (integrals.py:35-49) at full size: pack + metric eigenpairs +
+    FP64 GEMM, with (mu nu|P) and the metric drawn on the device."""
+    import torch
+
+    log("b from 3-centre integrals")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(7200)
+    g3 = torch.randn((nbf, nbf, naux), dtype=torch.float64, device=dev, generator=gen)
+    mnk = g3 + g3.transpose(0, 1)
+    del g3
+    A = torch.randn((naux, naux), dtype=torch.float64, device=dev, generator=gen) / np.sqrt(naux)
+    metric = A @ A.T + torch.eye(naux, dtype=torch.float64, device=dev)
+    del A
+    torch.cuda.synchronize()
+    eng3 = JKEngine(0)
+    eng3.upload_b_3c(mnk, metric)  # first call: rocBLAS/rocSOLVER set-up
+    t0 = time.perf_counter()
+    eng3.upload_b_3c(mnk, metric)
+    bp_s = time.perf_counter() - t0
+    ld = JKEngine.packed_ld(nbf, naux)
+    bp_flops = 2.0 * (nbf * (nbf + 1) // 2) * ld * ld
+    bprod = {"s": round(bp_s, 3), "gemm_flops": bp_flops, "gemm_tflops_if_all_time": round(bp_flops / bp_s / 1e12, 2),
+             "input": "device (mu nu|P) 28.8 GB dense + metric", "eigensolver": "rocsolver_dsyevd"}
+    eng3.close()
+    del mnk, metric
+    torch.cuda.empty_cache()
+    return bprod
 
 
 def bench_batch(args):

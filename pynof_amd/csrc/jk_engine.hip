@@ -1051,21 +1051,21 @@ static int pnof_from(const pnjk_pnof* p, PnofDev* d, const char* who) {
 
 // Device arrays of one occupation step, carved from the engine's `occ` buffer.
 struct OccBufs {
-  double *x, *n, *dn, *fi, *nd, *nd12, *dfi, *dnd, *dnd12, *JT, *KT, *tmp, *hc, *E, *grad, *cj, *ck;
+  double *x, *n, *dn, *fi, *nd, *nd12, *dfi, *dnd, *dnd12, *JT, *KT, *tmp, *hc, *E, *grad, *cj, *ck, *part;
 };
 
 static int occ_bufs(pnjk_engine* e, int64_t n5, int64_t nv, OccBufs& b) {
   const int64_t mm = n5 * n5, mv = n5 * nv;
-  const int64_t sizes[17] = {nv, n5, mv, n5, n5, n5, mv, mv, mv, mm, mm, mm, n5, 1, nv, mm, mm};
+  const int64_t sizes[18] = {nv, n5, mv, n5, n5, n5, mv, mv, mv, mm, mm, mm, n5, 1, nv, mm, mm, (n5 + 3) / 4};
   int64_t total = 0;
   for (int64_t z : sizes) total += (z + 31) / 32 * 32;  // 256-byte aligned pieces
   int rc = e->occ.ensure((size_t)std::max<int64_t>(total, 32) * 8);
   if (rc) return rc;
   double* base = e->occ.as<double>();
-  double** slots[17] = {&b.x, &b.n, &b.dn, &b.fi, &b.nd, &b.nd12, &b.dfi, &b.dnd, &b.dnd12,
-                        &b.JT, &b.KT, &b.tmp, &b.hc, &b.E, &b.grad, &b.cj, &b.ck};
+  double** slots[18] = {&b.x, &b.n, &b.dn, &b.fi, &b.nd, &b.nd12, &b.dfi, &b.dnd, &b.dnd12,
+                        &b.JT, &b.KT, &b.tmp, &b.hc, &b.E, &b.grad, &b.cj, &b.ck, &b.part};
   int64_t off = 0;
-  for (int i = 0; i < 17; ++i) {
+  for (int i = 0; i < 18; ++i) {
     *slots[i] = base + off;
     off += (sizes[i] + 31) / 32 * 32;
   }
@@ -1118,7 +1118,7 @@ int pnjk_occupation_step(pnjk_engine* e, const pnjk_pnof* p, const double* gamma
         (rc = to_device(H_core, b.hc, n5, s)))
       return rc;
   }
-  if (E) PNJK_HIP_CHECK(launch_occ_energy(d, b.n, b.fi, b.nd, b.nd12, nullptr, nullptr, b.JT, b.KT, b.hc, b.E, s));
+  if (E) PNJK_HIP_CHECK(launch_occ_energy(d, b.n, b.fi, b.nd, b.nd12, nullptr, nullptr, b.JT, b.KT, b.hc, b.part, b.E, s));
   if (grad)
     PNJK_HIP_CHECK(launch_occ_grad(d, b.n, b.fi, b.nd, b.nd12, b.dn, b.dfi, b.dnd, b.dnd12, b.JT, b.KT, b.hc, b.grad,
                                    s));
@@ -1179,7 +1179,7 @@ int pnjk_energy(pnjk_engine* e, int64_t nbf5, int64_t nbeta, int64_t nalpha, con
       (rc = to_device(ck12, b.ck, nbf5 * nbf5, s)) || (rc = stage_transposed(J_MO, nbf5, b.tmp, b.JT, s)) ||
       (rc = stage_transposed(K_MO, nbf5, b.tmp, b.KT, s)) || (rc = to_device(H_core, b.hc, nbf5, s)))
     return rc;
-  PNJK_HIP_CHECK(launch_occ_energy(d, b.n, nullptr, nullptr, nullptr, b.cj, b.ck, b.JT, b.KT, b.hc, b.E, s));
+  PNJK_HIP_CHECK(launch_occ_energy(d, b.n, nullptr, nullptr, nullptr, b.cj, b.ck, b.JT, b.KT, b.hc, b.part, b.E, s));
   bool host = false;
   if ((rc = from_device(b.E, E, 1, s, &host))) return rc;
   if (host || !is_device_ptr(n) || !is_device_ptr(cj12) || !is_device_ptr(ck12) || !is_device_ptr(J_MO))

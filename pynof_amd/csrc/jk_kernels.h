@@ -204,9 +204,10 @@ hipError_t launch_pnof_coef(const PnofDev& p, const double* n, const double* fi,
 // dst[i][j] = src[j][i], n x n.
 hipError_t launch_occ_transpose(const double* src, int n, double* dst, hipStream_t s);
 // Closed-shell energy into E[0]; C^J, C^K from cjm/ckm when given, else from the PNOF.
+// part: ceil(nbf5 / 4) doubles of workspace.
 hipError_t launch_occ_energy(const PnofDev& p, const double* n, const double* fi, const double* nd, const double* nd12,
                              const double* cjm, const double* ckm, const double* JT, const double* KT,
-                             const double* hc, double* E, hipStream_t s);
+                             const double* hc, double* part, double* E, hipStream_t s);
 // Occupation gradient (nv).
 hipError_t launch_occ_grad(const PnofDev& p, const double* n, const double* fi, const double* nd, const double* nd12,
                            const double* dn, const double* dfi, const double* dnd, const double* dnd12,

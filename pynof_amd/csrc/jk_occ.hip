@@ -279,33 +279,48 @@ __global__ void k_occ_transpose(const double* __restrict__ s, int n, double* __r
   t[idx] = s[(int64_t)j * n + i];
 }
 
-// Closed-shell energy (pnof.py:905-921), one workgroup. JT, KT are J_MO, K_MO transposed
+// Closed-shell energy (pnof.py:905-921) in two passes. Pass 1: workgroup b takes rows
+// i = 4b .. 4b + 3, one wave per row; lane sums j = lane + 64 t of C^J[i][j] J[j][i] -
+// C^K[i][j] K[j][i] (j != i) plus, on lane 0, the row's one-body terms; the workgroup's sum
+// goes to part[b]. Pass 2 sums the parts in order. JT, KT are J_MO, K_MO transposed
 // (JT[i][j] = J_MO[j][i]); with cjm / ckm null, C^J, C^K come from coef().
-__global__ void __launch_bounds__(256) k_occ_energy(PnofDev p, const double* __restrict__ n,
-                                                    const double* __restrict__ fi, const double* __restrict__ nd,
-                                                    const double* __restrict__ nd12, const double* __restrict__ cjm,
-                                                    const double* __restrict__ ckm, const double* __restrict__ JT,
-                                                    const double* __restrict__ KT, const double* __restrict__ hc,
-                                                    double* __restrict__ E) {
+__global__ void __launch_bounds__(256) k_occ_energy_rows(PnofDev p, const double* __restrict__ n,
+                                                         const double* __restrict__ fi, const double* __restrict__ nd,
+                                                         const double* __restrict__ nd12,
+                                                         const double* __restrict__ cjm, const double* __restrict__ ckm,
+                                                         const double* __restrict__ JT, const double* __restrict__ KT,
+                                                         const double* __restrict__ hc, double* __restrict__ part) {
   __shared__ double red[4];
-  const int n5 = p.nbf5;
+  const int n5 = p.nbf5, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   double acc = 0.0;
-  for (int i = threadIdx.x; i < n5; i += blockDim.x) {
-    acc += 2.0 * n[i] * hc[i];
-    if (i < p.nbeta || i >= p.nalpha) acc += n[i] * JT[(int64_t)i * n5 + i];
-  }
-  for (int64_t idx = threadIdx.x; idx < (int64_t)n5 * n5; idx += blockDim.x) {
-    const int i = (int)(idx / n5), j = (int)(idx % n5);
-    if (i == j) continue;
-    double a, b;
-    if (cjm) {
-      a = cjm[idx];
-      b = ckm[idx];
-    } else {
-      coef(p, n, fi, nd, nd12, i, j, a, b);
+  if (i < n5) {
+    if (lane == 0) {
+      acc = 2.0 * n[i] * hc[i];
+      if (i < p.nbeta || i >= p.nalpha) acc += n[i] * JT[(int64_t)i * n5 + i];
     }
-    acc += a * JT[idx] - b * KT[idx];
+    for (int j = lane; j < n5; j += 64) {
+      if (j == i) continue;
+      const int64_t idx = (int64_t)i * n5 + j;
+      double a, b;
+      if (cjm) {
+        a = cjm[idx];
+        b = ckm[idx];
+      } else {
+        coef(p, n, fi, nd, nd12, i, j, a, b);
+      }
+      acc += a * JT[idx] - b * KT[idx];
+    }
   }
+  const double tot = block_sum256(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256) k_occ_energy_sum(const double* __restrict__ part, int nparts,
+                                                        double* __restrict__ E) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) acc += part[b];
   const double tot = block_sum256(acc, red);
   if (threadIdx.x == 0) E[0] = tot;
 }
@@ -371,8 +386,10 @@ hipError_t launch_occ_transpose(const double* src, int n, double* dst, hipStream
 
 hipError_t launch_occ_energy(const PnofDev& p, const double* n, const double* fi, const double* nd, const double* nd12,
                              const double* cjm, const double* ckm, const double* JT, const double* KT,
-                             const double* hc, double* E, hipStream_t s) {
-  hipLaunchKernelGGL(k_occ_energy, dim3(1), dim3(256), 0, s, p, n, fi, nd, nd12, cjm, ckm, JT, KT, hc, E);
+                             const double* hc, double* part, double* E, hipStream_t s) {
+  const int nparts = (p.nbf5 + 3) / 4;
+  hipLaunchKernelGGL(k_occ_energy_rows, dim3(nparts), dim3(256), 0, s, p, n, fi, nd, nd12, cjm, ckm, JT, KT, hc, part);
+  hipLaunchKernelGGL(k_occ_energy_sum, dim3(1), dim3(256), 0, s, part, nparts, E);
   return hipGetLastError();
 }
 
