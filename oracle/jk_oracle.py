@@ -15,6 +15,12 @@ J_MO/K_MO, orbital invariance of sum_q J^q and sum_q K^q, the MO integrals'
 nbf5 x nbf5 block reproducing J_MO, K_MO and H_core), and (5) the
 committed golden fixtures of the water-sized workload. The reference ships
 no tests or stored outputs, so no reference-held vector exists to pin against.
+
+The b producer (``metric_power``, ``b_from_3c``; integrals.py:35-49) is pinned by
+closed forms (identity and diagonal metrics), by the RI identity through a
+linear solve, and by a metric with a known dropped eigenvector. The eigenvalue
+cut restates psi4's Matrix::power (a dependency the reference does not pin);
+with no psi4 output available, the cut rule at its margin is parity unpinned.
 """
 from __future__ import annotations
 
