@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 METRIC = "JK builds/s at nbf=1000,naux=3600 split vs FP64 plain @1e-9 Eh; p50/p99 latency"
 PEAK_FP64 = 78.6      # TFLOP/s, MI355X dense FP64 MFMA
 PEAK_BF16 = 2500.0    # TFLOP/s, MI355X dense BF16 MFMA
+PEAK_HBM = 8000.0     # GB/s, MI355X HBM3E
 REQUEST = 20          # builds per orbital-optimisation request
 OUTS = ("J", "K", "J_MO", "K_MO", "H_core")
 # rocprofv3 summary of this workload (kernel times and --pmc HBM traffic per dispatch)
@@ -286,6 +287,12 @@ def main():
                                                      torch.linalg.norm(mo_ref))
     mo["flops"] = mo_flops
     mo["speedup_split_vs_fp64"] = round(mo["fp64"]["ms"] / mo["split"]["ms"], 3)
+    mo["roofline"] = {"bound": "mfma", "unit": "TFLOP/s", "scope": "whole call (HT + b_MO + H_mat)",
+                      "fp64": {"achieved": mo["fp64"]["useful_tflops"], "peak": PEAK_FP64,
+                               "frac": round(mo["fp64"]["useful_tflops"] / PEAK_FP64, 4)},
+                      "split": {"achieved": mo["split"]["useful_tflops"], "peak": PEAK_BF16,
+                                "frac": round(mo["split"]["useful_tflops"] / PEAK_BF16, 4),
+                                "issued_frac": round(3 * mo["split"]["useful_tflops"] / PEAK_BF16, 4)}}
     del mo_out, mo_ref
     torch.cuda.empty_cache()
 
@@ -337,6 +344,11 @@ def main():
             "fock_bytes_min": 8.0 * 3 * nbf5 * nbf * nbf,
             "lagrange_from_F_ms": round(l_ms, 3),
             "lagrange_from_F_GBps": round(8.0 * nbf5 * nbf * nbf / (l_ms * 1e-3) / 1e9, 1)}
+    fock["roofline"] = {"fock_gemm": {"bound": "mfma", "unit": "TFLOP/s", "achieved": fock["fock_tflops"],
+                                      "peak": PEAK_FP64, "frac": round(fock["fock_tflops"] / PEAK_FP64, 4)},
+                        "lagrange_from_F": {"bound": "hbm", "unit": "GB/s", "achieved": fock["lagrange_from_F_GBps"],
+                                            "peak": PEAK_HBM, "frac": round(fock["lagrange_from_F_GBps"] / PEAK_HBM, 4),
+                                            "scope": "whole call; algorithmic bytes = F read once"}}
     del lag_in, lag_ref, lo, F, elag_f
     torch.cuda.empty_cache()
 

@@ -29,7 +29,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kPad = 2;  // LDS row padding (doubles); keeps 16-byte alignment
 
-enum Kind { KM = 0, MK = 1 };
+enum Kind { KM = 0, MK = 1, MKLANE = 2 };
 
 // Global -> register -> LDS loader for one operand tile of ROWS x BK.
 // LDS layout is k-major: lds[k * (ROWS + kPad) + r].
@@ -65,9 +65,13 @@ struct Loader<KM, ROWS, BK> {
   }
 };
 
-// Source element (r, k) at src.row(r)[k]; k is contiguous in memory.
-template <int ROWS, int BK>
-struct Loader<MK, ROWS, BK> {
+// Source element (r, k) at src.row(r)[k]; k is contiguous in memory. MK gives
+// consecutive lanes consecutive rows; MKLANE gives them consecutive k pairs of one
+// row (BK/2 lanes per row), which coalesces the global loads of long k-contiguous
+// rows: measured 5% faster on the whole FP64 Lagrangian (its split-K exchange
+// GEMM) and 4% slower on the Coulomb GEMM, so each problem picks its kind.
+template <int KIND, int ROWS, int BK>
+struct LoaderMK {
   static constexpr int kChunks = ROWS * (BK / 2);
   static constexpr int kPer = (kChunks + kThreads - 1) / kThreads;
   v2d v[kPer];
@@ -77,7 +81,7 @@ struct Loader<MK, ROWS, BK> {
     for (int i = 0; i < kPer; ++i) {
       const int c = threadIdx.x + i * kThreads;
       if (kChunks % kThreads == 0 || c < kChunks) {
-        const int r = c % ROWS, kp = (c / ROWS) * 2;
+        const int r = KIND == MK ? c % ROWS : c / (BK / 2), kp = KIND == MK ? (c / ROWS) * 2 : (c % (BK / 2)) * 2;
         v[i] = *gmem(reinterpret_cast<const v2d*>(s.row(r0 + r) + k0 + kp));
       }
     }
@@ -87,13 +91,17 @@ struct Loader<MK, ROWS, BK> {
     for (int i = 0; i < kPer; ++i) {
       const int c = threadIdx.x + i * kThreads;
       if (kChunks % kThreads == 0 || c < kChunks) {
-        const int r = c % ROWS, kp = (c / ROWS) * 2;
+        const int r = KIND == MK ? c % ROWS : c / (BK / 2), kp = KIND == MK ? (c / ROWS) * 2 : (c % (BK / 2)) * 2;
         lds[kp * (ROWS + kPad) + r] = v[i].x;
         lds[(kp + 1) * (ROWS + kPad) + r] = v[i].y;
       }
     }
   }
 };
+template <int ROWS, int BK>
+struct Loader<MK, ROWS, BK> : LoaderMK<MK, ROWS, BK> {};
+template <int ROWS, int BK>
+struct Loader<MKLANE, ROWS, BK> : LoaderMK<MKLANE, ROWS, BK> {};
 
 // The GEMM core. Accumulator element (fm, fn, r) of a lane is output element
 //   m = m0 + wm + 16*fm + lane/16 + 4*r,  n = n0 + wn + 16*fn + lane%16
@@ -603,7 +611,7 @@ struct ProbHM {
 //   part[c][a][b] = sum_{k' in chunk c} B[a][k'] W[b][k'],   W[b][q naux + P] = ck[b][q] B[b][q][P],
 // a < nbf, b < nbf5, with B = b_MO rows of stride ldrow (zero past nbf5 * naux).
 struct ProbLagX {
-  static constexpr int kA = MK, kB = MK;
+  static constexpr int kA = MKLANE, kB = MKLANE;
   const double* B;
   const double* W;
   double* part;
