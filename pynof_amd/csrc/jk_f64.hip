@@ -788,13 +788,18 @@ __global__ void k_kmo_reduce(int nbf5, int nt, const double* __restrict__ kpart,
   kmo[(int64_t)q * nbf5 + p] = s;
 }
 
-__global__ void k_hcore(int nbf, int nbf5, int ldcp, const double* __restrict__ Cp, const double* __restrict__ T,
-                        double* __restrict__ hcore) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// H_core[i] = sum_mu Cp[mu][i] T[mu][i]: one wave per orbital i; lane l sums mu = l + 64 j
+// in order, then an xor-shuffle reduction (fixed order).
+__global__ void __launch_bounds__(256) k_hcore(int nbf, int nbf5, int ldcp, const double* __restrict__ Cp,
+                                               const double* __restrict__ T, double* __restrict__ hcore) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= nbf5) return;
   double s = 0.0;
-  for (int mu = 0; mu < nbf; ++mu) s += Cp[(int64_t)mu * ldcp + i] * T[(int64_t)mu * ldcp + i];
-  hcore[i] = s;
+  for (int mu = lane; mu < nbf; mu += 64) s += Cp[(int64_t)mu * ldcp + i] * T[(int64_t)mu * ldcp + i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) hcore[i] = s;
 }
 
 __device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
@@ -1321,7 +1326,7 @@ hipError_t launch_hcore_f64(const BuildGeom& g, const double* Hp, int ldh, const
   pr.mtiles = cdiv(g.nbf, 64);
   hipError_t e = launch_prob(pr, tc, pr.mtiles * cdiv(g.ldcp, 64), s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_hcore, dim3(cdiv(g.nbf5, 256)), dim3(256), 0, s, g.nbf, g.nbf5, g.ldcp, Cp, T, hcore);
+  hipLaunchKernelGGL(k_hcore, dim3(cdiv(g.nbf5, 4)), dim3(256), 0, s, g.nbf, g.nbf5, g.ldcp, Cp, T, hcore);
   return hipGetLastError();
 }
 
