@@ -121,6 +121,10 @@ static const char* kStageNames[ST_COUNT] = {"copy_in", "half_transform", "quarte
 struct pnjk_engine {
   int device = 0;
   hipStream_t own = nullptr;
+  // Side stream of a single build (default priority) and its fork/join events: the
+  // latency-bound J_MO and H_core steps overlap the MFMA-bound J^n GEMM.
+  hipStream_t aux = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::mutex mu;
   bool prof = false;
   bool prof_recorded = false;
@@ -168,7 +172,14 @@ int pnjk_create(int device, pnjk_engine** out) {
   pnjk_engine* e = new pnjk_engine();
   e->device = device;
   hipError_t he = hipStreamCreateWithFlags(&e->own, hipStreamNonBlocking);
+  if (he == hipSuccess) he = hipStreamCreateWithFlags(&e->aux, hipStreamNonBlocking);
+  for (hipEvent_t* ev : {&e->fork_ev, &e->join_ev})
+    if (he == hipSuccess) he = hipEventCreateWithFlags(ev, hipEventDisableTiming);
   if (he != hipSuccess) {
+    for (hipEvent_t ev : {e->fork_ev, e->join_ev})
+      if (ev) (void)hipEventDestroy(ev);
+    if (e->aux) (void)hipStreamDestroy(e->aux);
+    if (e->own) (void)hipStreamDestroy(e->own);
     delete e;
     return set_error(PNJK_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
   }
@@ -184,6 +195,10 @@ int pnjk_destroy(pnjk_engine* e) {
   for (int i = 0; i < e->nall; ++i) e->all[i]->release();
   for (hipEvent_t& ev : e->ev)
     if (ev) (void)hipEventDestroy(ev);
+  (void)hipStreamSynchronize(e->aux);
+  for (hipEvent_t ev : {e->fork_ev, e->join_ev})
+    if (ev) (void)hipEventDestroy(ev);
+  (void)hipStreamDestroy(e->aux);
   (void)hipStreamDestroy(e->own);
   delete e;
   return PNJK_OK;
@@ -714,7 +729,17 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
   mark(ST_KMO);
   if ((rc = stage_kmo(e, st, s))) return rc;
   mark(ST_JMO);
-  if ((rc = stage_jmo(e, st, s, nullptr)) || (rc = stage_jmo_post(e, st, s))) return rc;
+  // split_refine converts d and X for J^n, K^n in the J_MO stage: keep it in line. Otherwise
+  // J_MO and H_core run on the side stream beside J^n (about 0.3 ms hidden per build at
+  // nbf=1000). J^n's mirror stays in line: beside K^n it slowed K^n by more than it saved.
+  const bool side = st.a.precision != PNJK_SPLIT_REFINE;
+  hipStream_t sa = side ? e->aux : s;
+  if (side) {
+    PNJK_HIP_CHECK(hipEventRecord(e->fork_ev, s));
+    PNJK_HIP_CHECK(hipStreamWaitEvent(sa, e->fork_ev, 0));
+  }
+  if ((rc = stage_jmo(e, st, sa, nullptr)) || (rc = stage_jmo_post(e, st, sa))) return rc;
+  if (side && (rc = stage_hcore(e, st, sa))) return rc;
   mark(ST_JN);
   if ((rc = stage_jn(e, st, s, nullptr))) return rc;
   if (st.want_j && (rc = stage_sym(st, st.dJ, s))) return rc;
@@ -722,7 +747,12 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
   if ((rc = stage_k(e, st, s, nullptr))) return rc;
   if (st.want_k && (rc = stage_sym(st, st.dK, s))) return rc;
   mark(ST_HC);
-  if ((rc = stage_hcore(e, st, s))) return rc;
+  if (side) {  // join the side stream
+    PNJK_HIP_CHECK(hipEventRecord(e->join_ev, sa));
+    PNJK_HIP_CHECK(hipStreamWaitEvent(s, e->join_ev, 0));
+  } else if ((rc = stage_hcore(e, st, s))) {
+    return rc;
+  }
   mark(ST_OUT);
   bool sync = false;
   if ((rc = stage_out(st, s, &sync))) return rc;
