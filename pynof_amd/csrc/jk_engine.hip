@@ -790,7 +790,8 @@ static int load_cf(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf, in
 
 // The MO integrals into device buffers: b_MO rows of stride ldrow (dB, may be
 // NULL) and H_mat (dHm, may be NULL). `st` comes from build_prepare with no outputs.
-static int mo_compute(pnjk_engine* e, BuildState& st, double* dB, int64_t ldrow, double* dHm, hipStream_t s) {
+static int mo_compute(pnjk_engine* e, BuildState& st, double* dB, int64_t ldrow, double* dHm, hipStream_t s,
+                      uint16_t* bh = nullptr, uint16_t* bl = nullptr) {
   const BuildGeom& g = st.g;
   const int64_t nbf = g.nbf, n5 = g.nbf5;
   const int precision = st.a.precision;
@@ -817,7 +818,7 @@ static int mo_compute(pnjk_engine* e, BuildState& st, double* dB, int64_t ldrow,
                                      e->ctf16l.as<uint16_t>(), s));
       PNJK_HIP_CHECK(launch_mo_x3(st.sg, g.naux, big ? TileF64{5, 4} : TileF64{2, 2}, e->ctf16h.as<uint16_t>(),
                                   e->ctf16l.as<uint16_t>(), e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), dB, ldrow,
-                                  s));
+                                  s, bh, bl));
     } else {
       PNJK_HIP_CHECK(
           launch_mo_f64(g, big ? TileF64{5, 2} : TileF64{2, 2}, Cf, ldcf, e->X.as<double>(), dB, ldrow, s));
@@ -913,24 +914,30 @@ int pnjk_lagrangian(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, 
   double* cjd = cf;
   double* ckd = cf + n5 * n5;
   double* nd = cf + 2 * n5 * n5;
-  if ((rc = mo_compute(e, st, B, ldrow, Hm, s))) return rc;
+  const bool split = precision == PNJK_SPLIT;
+  if (split) {  // exchange in BF16x3: b_MO's split planes come from the MO GEMM's epilogue
+    const size_t bb = (size_t)(nbf * ldrow) * 2, wb = (size_t)(n5 * ldrow) * 2;
+    if ((rc = e->bm16h.ensure(bb)) || (rc = e->bm16l.ensure(bb)) || (rc = e->w16h.ensure(wb)) ||
+        (rc = e->w16l.ensure(wb)))
+      return rc;
+  }
+  if ((rc = mo_compute(e, st, B, ldrow, Hm, s, split ? e->bm16h.as<uint16_t>() : nullptr,
+                       split ? e->bm16l.as<uint16_t>() : nullptr)))
+    return rc;
   PNJK_HIP_CHECK(launch_fill_cols(B, (int)nbf, ldrow, kfull, ldrow, s));
+  if (split)
+    PNJK_HIP_CHECK(launch_fill_cols16(e->bm16h.as<uint16_t>(), e->bm16l.as<uint16_t>(), (int)nbf, ldrow, kfull, ldrow,
+                                      s));
   // Coefficients: copy, then clear the diagonal (utils.py:269-270) on the device.
   if ((rc = to_device(cj12, e->lagw.as<double>(), n5 * n5, s))) return rc;
   PNJK_HIP_CHECK(launch_lag_coef(e->lagw.as<double>(), cjd, (int)n5, s));
   if ((rc = to_device(ck12, e->lagw.as<double>(), n5 * n5, s))) return rc;
   PNJK_HIP_CHECK(launch_lag_coef(e->lagw.as<double>(), ckd, (int)n5, s));
   if ((rc = to_device(n, nd, n5, s))) return rc;
-  const bool split = precision == PNJK_SPLIT;
-  if (split) {  // exchange in BF16x3
-    const size_t bb = (size_t)(nbf * ldrow) * 2, wb = (size_t)(n5 * ldrow) * 2;
-    if ((rc = e->bm16h.ensure(bb)) || (rc = e->bm16l.ensure(bb)) || (rc = e->w16h.ensure(wb)) ||
-        (rc = e->w16l.ensure(wb)))
-      return rc;
+  if (split)
     PNJK_HIP_CHECK(launch_lag_x3((int)nbf, (int)n5, (int)naux, B, ldrow, kc, (int)nchunks, ckd,
                                  e->bm16h.as<uint16_t>(), e->bm16l.as<uint16_t>(), e->w16h.as<uint16_t>(),
-                                 e->w16l.as<uint16_t>(), e->lagp.as<double>(), s));
-  }
+                                 e->w16l.as<uint16_t>(), e->lagp.as<double>(), true, s));
   PNJK_HIP_CHECK(launch_lag_f64(g, B, ldrow, kc, (int)nchunks, nd, cjd, ckd, (int)nbeta, (int)nalpha, Hm,
                                 e->lagg.as<double>(), e->lagw.as<double>(), e->lagp.as<double>(), dE, !split, s));
   if (host_e) PNJK_HIP_CHECK(hipMemcpyAsync(elag, dE, (size_t)(nbf * nbf) * 8, hipMemcpyDeviceToHost, s));

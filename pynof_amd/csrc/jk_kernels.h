@@ -91,7 +91,7 @@ hipError_t launch_lag_f64(const BuildGeom& g, const double* B, int64_t ldrow, in
 // nbf x ldrow) and forms W16 (wh, wl: nbf5 x ldrow); part as in launch_lag_f64.
 hipError_t launch_lag_x3(int nbf, int nbf5, int naux, const double* B, int64_t ldrow, int64_t kc, int nchunks,
                          const double* ck, uint16_t* bh, uint16_t* bl, uint16_t* wh, uint16_t* wl, double* part,
-                         hipStream_t s);
+                         bool planes_ready, hipStream_t s);
 hipError_t launch_lag_coef(const double* src, double* dst, int n, hipStream_t s);
 // Closed-shell generalised Fock matrices F (nbf5 x nbf x nbf) from J^n, K^n rows of stride
 // ld (even, >= nbf^2), device H (nbf x nbf), occupations and coefficients; A (nbf5 x
@@ -160,8 +160,12 @@ hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
 // Quarter transform from X16: K_MO row-sum partials, d (FP64 and split planes).
 // BF16x3 orbital-gradient MO integrals from X16 and the split planes of C^T over
 // all nbf columns (rows clamped at nbf); FP64 output b_MO[p][q][P].
+// With bmo_hi/bmo_lo, also writes the bf16 split planes of b_MO (same layout, valid columns).
 hipError_t launch_mo_x3(const SplitGeom& g, int naux, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
-                        const uint16_t* x_hi, const uint16_t* x_lo, double* b_mo, int64_t ldrow, hipStream_t s);
+                        const uint16_t* x_hi, const uint16_t* x_lo, double* b_mo, int64_t ldrow, hipStream_t s,
+                        uint16_t* bmo_hi = nullptr, uint16_t* bmo_lo = nullptr);
+// Zeroes columns [c0, c1) of two bf16 planes (rows of stride ld).
+hipError_t launch_fill_cols16(uint16_t* h, uint16_t* l, int rows, int64_t ld, int64_t c0, int64_t c1, hipStream_t s);
 hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
                         const uint16_t* x_hi, const uint16_t* x_lo, double* kpart, double* d, uint16_t* d_hi,
                         uint16_t* d_lo, int* ntiles, hipStream_t s, GroupCollector* col = nullptr);

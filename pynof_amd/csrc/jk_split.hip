@@ -421,10 +421,15 @@ struct ProbQT {
 // Orbital-gradient MO integrals in BF16x3 (integrals.py:703-716): batch q,
 //   b_MO[p][q][P] = sum_nu Ctf16[p][nu] X16[q][nu][P]   for p < nbf, q < nbf5,
 // with Ctf16 the split planes of C^T over all nbf columns; FP64 output.
-struct ProbMO {
+// kPlanes: the epilogue also writes the bf16 split planes of b_MO at the same index (the
+// Lagrangian's exchange operand), bit for bit what a conversion pass over b_MO would give.
+template <bool kPlanes>
+struct ProbMOT {
   static constexpr int kA = RK, kB = KR;
   Planes ct, x;
   double* out;
+  uint16_t* oh;
+  uint16_t* ol;
   int64_t ldrow;  // output row stride (nbf5 * naux for the reference layout)
   int nbf, naux, naux_p, nbf5, ldk, mtiles, ntiles;
   __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
@@ -450,7 +455,17 @@ struct ProbMO {
           int dm, dn;
           coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int m = t.m0 + dm, n = t.n0 + dn;
-          if (m < nbf && n < naux) gmem(out)[(int64_t)m * ldrow + (int64_t)t.batch * naux + n] = acc[i][j][r];
+          if (m < nbf && n < naux) {
+            const int64_t idx = (int64_t)m * ldrow + (int64_t)t.batch * naux + n;
+            const double v = acc[i][j][r];
+            gmem(out)[idx] = v;
+            if (kPlanes) {
+              uint16_t h, l;
+              split2(v, h, l);
+              gmem(oh)[idx] = h;
+              gmem(ol)[idx] = l;
+            }
+          }
         }
   }
 };
@@ -691,10 +706,10 @@ __global__ void k_lag_w16(const double* __restrict__ B, int64_t ldrow, int naux,
 
 hipError_t launch_lag_x3(int nbf, int nbf5, int naux, const double* B, int64_t ldrow, int64_t kc, int nchunks,
                          const double* ck, uint16_t* bh, uint16_t* bl, uint16_t* wh, uint16_t* wl, double* part,
-                         hipStream_t s) {
+                         bool planes_ready, hipStream_t s) {
   if (kc % kBK != 0 || (int64_t)nchunks * kc != ldrow) return hipErrorInvalidValue;
   const int64_t nb = (int64_t)nbf * ldrow, nw = (int64_t)nbf5 * ldrow;
-  hipLaunchKernelGGL(k_split_packed, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, s, B, nb, bh, bl);
+  if (!planes_ready) hipLaunchKernelGGL(k_split_packed, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, s, B, nb, bh, bl);
   hipLaunchKernelGGL(k_lag_w16, dim3((unsigned)cdiv(nw, 256)), dim3(256), 0, s, B, ldrow, naux, nbf5, ck, wh, wl);
   const TileF64 tc{5, 2};
   ProbLagX pr;
@@ -767,15 +782,19 @@ hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
   return launch_x3(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s, col);
 }
 
-hipError_t launch_mo_x3(const SplitGeom& g, int naux, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
-                        const uint16_t* x_hi, const uint16_t* x_lo, double* b_mo, int64_t ldrow, hipStream_t s) {
+template <bool kPlanes>
+hipError_t launch_mo_x3_t(const SplitGeom& g, int naux, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
+                          const uint16_t* x_hi, const uint16_t* x_lo, double* b_mo, int64_t ldrow, hipStream_t s,
+                          uint16_t* bmo_hi, uint16_t* bmo_lo) {
   // A 128-column tile may read up to 64 columns past naux_p (X16 slack) and drops them.
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
   if ((tc.fn != 2 && tc.fn != 4) || ldrow < (int64_t)g.nbf5 * naux) return hipErrorInvalidValue;
-  ProbMO pr;
+  ProbMOT<kPlanes> pr;
   pr.ct = {ct_hi, ct_lo};
   pr.x = {x_hi, x_lo};
   pr.out = b_mo;
+  pr.oh = bmo_hi;
+  pr.ol = bmo_lo;
   pr.ldrow = ldrow;
   pr.nbf = g.nbf;
   pr.naux = naux;
@@ -785,6 +804,32 @@ hipError_t launch_mo_x3(const SplitGeom& g, int naux, TileF64 tc, const uint16_t
   pr.mtiles = cdiv(g.nbf, BM);
   pr.ntiles = cdiv(g.naux_p, BN);
   return launch_x3(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s);
+}
+
+hipError_t launch_mo_x3(const SplitGeom& g, int naux, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
+                        const uint16_t* x_hi, const uint16_t* x_lo, double* b_mo, int64_t ldrow, hipStream_t s,
+                        uint16_t* bmo_hi, uint16_t* bmo_lo) {
+  if (bmo_hi && bmo_lo)
+    return launch_mo_x3_t<true>(g, naux, tc, ct_hi, ct_lo, x_hi, x_lo, b_mo, ldrow, s, bmo_hi, bmo_lo);
+  return launch_mo_x3_t<false>(g, naux, tc, ct_hi, ct_lo, x_hi, x_lo, b_mo, ldrow, s, nullptr, nullptr);
+}
+
+// Zeroes columns [c0, c1) of `rows` rows of stride ld (the pad of the b_MO planes).
+__global__ void k_fill_cols16(uint16_t* __restrict__ h, uint16_t* __restrict__ l, int rows, int64_t ld, int64_t c0,
+                              int64_t c1) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t w = c1 - c0;
+  if (idx >= (int64_t)rows * w) return;
+  const int64_t off = (idx / w) * ld + c0 + idx % w;
+  h[off] = 0;
+  l[off] = 0;
+}
+
+hipError_t launch_fill_cols16(uint16_t* h, uint16_t* l, int rows, int64_t ld, int64_t c0, int64_t c1, hipStream_t s) {
+  if (c1 <= c0 || rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_cols16, dim3((unsigned)cdiv((int64_t)rows * (c1 - c0), 256)), dim3(256), 0, s, h, l, rows,
+                     ld, c0, c1);
+  return hipGetLastError();
 }
 
 hipError_t launch_jn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* d_hi, const uint16_t* d_lo,
