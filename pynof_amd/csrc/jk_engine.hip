@@ -361,6 +361,9 @@ int pnjk_upload_b_3c(pnjk_engine* e, const double* mnk, int64_t nbf, int64_t nau
   // The packed 3-centre integrals (mu nu|P), checked like b, land where b will live.
   int rc = upload_b_locked(e, mnk, nbf, naux, sym_tol, asym);
   if (rc) return rc;
+  // Not a b until the transform below completes: any early return leaves no b uploaded.
+  const int nbf_ready = e->nbf;
+  e->nbf = 0;
   hipStream_t s = e->own;
   const int n = (int)naux, ld = e->naux_p;
   const int64_t ldld = (int64_t)ld * ld;
@@ -414,6 +417,7 @@ int pnjk_upload_b_3c(pnjk_engine* e, const double* mnk, int64_t nbf, int64_t nau
   std::swap(e->btri.p, out.p);  // `out` now holds the 3-centre integrals and is freed on return
   std::swap(e->btri.bytes, out.bytes);
   e->split_ready = false;
+  e->nbf = nbf_ready;
   if (footprint) *footprint = pnjk_footprint(e);
   return PNJK_OK;
 }

@@ -88,6 +88,7 @@ class JKEngine:
         asym = ctypes.c_double()
         if _is_torch(b):
             _torch().cuda.current_stream(b.device).synchronize()
+        self.nbf = 0  # the library holds no b until the call succeeds
         _lib.check(_lib.lib().pnjk_upload_b(self._h, _ptr(b), nbf, naux, float(sym_tol), ctypes.byref(fp),
                                             ctypes.byref(asym)))
         self.nbf, self.naux, self.asym = nbf, naux, asym.value
@@ -100,6 +101,7 @@ class JKEngine:
             raise ValueError(f"packed b must have shape {(nbf * (nbf + 1) // 2, ld)}; got {tuple(btri.shape)}")
         _torch().cuda.current_stream(btri.device).synchronize()
         fp = ctypes.c_uint64()
+        self.nbf = 0  # the library holds no b until the call succeeds
         _lib.check(_lib.lib().pnjk_upload_b_packed(self._h, btri.data_ptr(), nbf, naux, ld, ctypes.byref(fp)))
         self.nbf, self.naux = nbf, naux
         return int(fp.value)
@@ -121,6 +123,7 @@ class JKEngine:
                 _torch().cuda.current_stream(x.device).synchronize()
         fp = ctypes.c_uint64()
         asym = ctypes.c_double()
+        self.nbf = 0  # the library holds no b until the call succeeds
         _lib.check(_lib.lib().pnjk_upload_b_3c(self._h, _ptr(m3), nbf, naux, _ptr(mt), float(cutoff),
                                                float(sym_tol), ctypes.byref(fp), ctypes.byref(asym)))
         self.nbf, self.naux, self.asym = nbf, naux, asym.value
