@@ -181,14 +181,14 @@ typedef struct pnjk_pnof {
   int32_t ista;       /* p.ista: PNOF7 Phi = sqrt(n(1-n)) for 0, else 2n(1-n) */
   int32_t occ_method; /* PNJK_OCC_SOFTMAX or PNJK_OCC_TRIGONOMETRIC */
   int32_t high_spin;  /* p.HighSpin: singly occupied n = 1 instead of 0.5 */
-  int32_t mspin;      /* p.MSpin: only 0 (closed shell) is supported */
+  int32_t mspin;      /* p.MSpin: 0 closed shell; otherwise the open-shell energy and gradient */
   int32_t no1, ndoc, nsoc, ncwo, nbf5;
 } pnjk_pnof;
 
 /* One occupation step from the parameters gamma (nv): replaces ocupacion
  * (pnof.py:692-810), PNOFi_selector (:657-667, CJCKD5/7/8 :155-422), calcocce
- * (:994-1001, calce :905-921 with MSpin = 0), der_PNOFi_selector (:670-689,
- * der_CJCKD5/7/8 :206-525) and calcoccg (:1004-1025, MSpin = 0), which the
+ * (:994-1001, calce :905-990), der_PNOFi_selector (:670-689, der_CJCKD5/7/8
+ * :206-525) and calcoccg (:1004-1098), closed (MSpin = 0) and open shell, which the
  * occupation optimiser (minimization.py:86-88) calls on every iteration.
  * Outputs, each written when not NULL: E (1) the energy of calcocce; grad (nv)
  * of calcoccg; n (nbf5) and dn_dgamma (nbf5, nv) of ocupacion; cj12, ck12
@@ -206,14 +206,16 @@ int pnjk_occupation_step(pnjk_engine* e, const pnjk_pnof* p, const double* gamma
 int pnjk_pnof_coefficients(pnjk_engine* e, const pnjk_pnof* p, const double* n, double* cj12, double* ck12,
                            void* stream);
 
-/* Closed-shell energy for given n, cj12, ck12: replaces calce with MSpin = 0
- * (pnof.py:905-921): 2 sum n H_core + sum n_i J_ii over i < nbeta and
- * nalpha <= i < nbf5 + sum' cj12 J_MO - sum' ck12 K_MO, diagonals excluded (the
- * reference zeroes them in the caller's arrays, :913, :917; this call leaves its
- * inputs unchanged). E is one double, host or device. */
-int pnjk_energy(pnjk_engine* e, int64_t nbf5, int64_t nbeta, int64_t nalpha, const double* n, const double* cj12,
-                const double* ck12, const double* J_MO, const double* K_MO, const double* H_core, double* E,
-                void* stream);
+/* Energy for given n, cj12, ck12: replaces calce (pnof.py:905-990). MSpin = 0:
+ * 2 sum n H_core + sum n_i J_ii over i < nbeta and nalpha <= i < nbf5
+ * + sum' cj12 J_MO - sum' ck12 K_MO, diagonals excluded (the reference zeroes them
+ * in the caller's arrays, :913, :917; this call leaves its inputs unchanged).
+ * MSpin != 0: the cj12/ck12 sums skip the singly occupied block
+ * [nbeta, nalpha), which enters through the n-weighted J_MO/K_MO terms of
+ * :924-988. E is one double, host or device. */
+int pnjk_energy(pnjk_engine* e, int64_t nbf5, int64_t nbeta, int64_t nalpha, int32_t mspin, const double* n,
+                const double* cj12, const double* ck12, const double* J_MO, const double* K_MO, const double* H_core,
+                double* E, void* stream);
 
 /* Grouped builds: many engines (one b each, different sizes allowed) built in
  * one pass. Each GEMM stage of all instances runs as one persistent grouped

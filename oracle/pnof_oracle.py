@@ -3,8 +3,8 @@
 TEST INFRASTRUCTURE ONLY. Nothing in the product imports this module: it is
 used by ``tests/`` as the checker. It restates, in numpy, the reference's
 occupation parametrisations, the PNOF5 / PNOF7 / GNOF coefficient matrices
-C^J, C^K and their derivatives, the closed-shell energy and the occupation
-gradient; every function cites the reference lines it follows (paths
+C^J, C^K and their derivatives, the energy and the occupation gradient (closed
+and open shell); every function cites the reference lines it follows (paths
 relative to the reference root, ``src/pynof/pnof.py`` unless named).
 
 Where the reference's code has quirks, they are restated as they are:
@@ -20,7 +20,12 @@ Pinning (tests/test_pnof_oracle.py):
   two-electron Slater rules; PNOF7 minus PNOF5 against -sum'' Phi_p Phi_q K_pq;
   symmetry;
 - derivatives: the occupation gradient against central finite differences
-  of the energy, for every PNOF and both parametrisations.
+  of the energy, for every PNOF and both parametrisations, closed and open
+  shell (MSpin != 0);
+- the open-shell energy (pnof.py:924-988) against explicit loops over the
+  reference's blocks; it has no independent closed form here (its
+  singly-occupied terms are not the ROHF expression), so beyond the loops and
+  the gradient consistency its terms follow the code.
 The GNOF inter-pair static and dynamic terms (pnof.py:378-396) have no
 independent closed form here: they are pinned by the HF limit (where they
 vanish), by the single-pair limit, by symmetry and by the finite-difference
@@ -318,18 +323,33 @@ def der_coefficients(n, dn, p: Pnof):
 
 # ---------------------------------------------------------------- energy and gradient
 def calce(n, cj12, ck12, J_MO, K_MO, H_core, p: Pnof) -> float:
-    """calce, MSpin = 0 (pnof.py:905-921): 2 sum n H + sum n_i J_ii over [:nbeta] and
-    [nalpha:nbf5] + sum' C^J J - sum' C^K K, diagonals of C^J, C^K removed."""
-    if p.MSpin != 0:
-        raise ValueError("only MSpin = 0 is restated")
+    """calce (pnof.py:905-990). MSpin = 0: 2 sum n H + sum n_i J_ii over [:nbeta] and
+    [nalpha:nbf5] + sum' C^J J - sum' C^K K, diagonals of C^J, C^K removed. MSpin != 0:
+    the C^J/C^K sums skip the singly occupied block S = [nbeta, nalpha), whose orbitals
+    enter through n-weighted J/K terms instead (pnof.py:924-988)."""
     cj = np.array(cj12, dtype=np.float64, copy=True)
     ck = np.array(ck12, dtype=np.float64, copy=True)
     np.fill_diagonal(cj, 0.0)
     np.fill_diagonal(ck, 0.0)
+    b, a, n5 = p.nbeta, p.nalpha, p.nbf5
     dj = np.diagonal(J_MO)
-    E = 2.0 * np.dot(n, H_core)
-    E += np.dot(n[:p.nbeta], dj[:p.nbeta]) + np.dot(n[p.nalpha:p.nbf5], dj[p.nalpha:p.nbf5])
-    E += np.sum(cj * J_MO.T) - np.sum(ck * K_MO.T)
+    if p.MSpin == 0:
+        E = 2.0 * np.dot(n, H_core)
+        E += np.dot(n[:b], dj[:b]) + np.dot(n[a:n5], dj[a:n5])
+        E += np.sum(cj * J_MO.T) - np.sum(ck * K_MO.T)
+        return float(E)
+    keep = np.ones(n5, dtype=bool)
+    keep[b:a] = False  # rows and columns outside S
+    E = np.dot(n[:b], 2.0 * H_core[:b] + dj[:b]) + np.dot(n[b:a], 2.0 * H_core[b:a])
+    E += np.dot(n[a:n5], 2.0 * H_core[a:n5] + dj[a:n5])
+    blk = np.outer(keep, keep)
+    E += np.sum((cj * J_MO.T)[blk]) - np.sum((ck * K_MO.T)[blk])
+    sj = J_MO[b:a, :].sum(axis=0)  # sum_{s in S} J[s, i]
+    sk = K_MO[b:a, :].sum(axis=0)
+    E += 2.0 * np.dot(n[:b], sj[:b]) + 2.0 * np.dot(n[a:n5], sj[a:n5])
+    E += 0.5 * (np.dot(n[b:a], sj[b:a]) - np.dot(n[b:a], dj[b:a]))
+    E -= np.dot(n[:b], sk[:b]) + np.dot(n[a:n5], sk[a:n5])
+    E -= 0.5 * (np.dot(n[b:a], sk[b:a]) + np.dot(n[b:a], np.diagonal(K_MO)[b:a]))
     return float(E)
 
 
@@ -341,15 +361,32 @@ def calcocce(gamma, J_MO, K_MO, H_core, p: Pnof) -> float:
 
 
 def calcoccg(gamma, J_MO, K_MO, H_core, p: Pnof) -> np.ndarray:
-    """calcoccg, MSpin = 0 (pnof.py:1004-1025): dn^T (2 H + diag J) + 2 sum' Dcj J - 2 sum' Dck K."""
-    if p.MSpin != 0:
-        raise ValueError("only MSpin = 0 is restated")
+    """calcoccg (pnof.py:1004-1100). MSpin = 0: dn^T (2 H + diag J) + 2 sum' Dcj J - 2 sum' Dck K.
+    MSpin != 0: rows [no1, nbeta) and [nalpha, nbf5) only, columns outside S for the
+    derivative sums, plus 2 dn sum_S J - dn sum_S K (pnof.py:1027-1098)."""
     n, dn = occupation(gamma, p)
     Dcj, Dck = der_coefficients(n, dn, p)
     idx = np.arange(p.nbf5)
     Dcj[idx, idx, :] = 0.0
     Dck[idx, idx, :] = 0.0
-    grad = dn.T @ (2.0 * H_core + np.diagonal(J_MO))
-    grad += 2.0 * np.einsum("ijk,ji->k", Dcj, J_MO)
-    grad -= 2.0 * np.einsum("ijk,ji->k", Dck, K_MO)
+    if p.MSpin == 0:
+        grad = dn.T @ (2.0 * H_core + np.diagonal(J_MO))
+        grad += 2.0 * np.einsum("ijk,ji->k", Dcj, J_MO)
+        grad -= 2.0 * np.einsum("ijk,ji->k", Dck, K_MO)
+        return grad
+    o0, b, a, n5 = p.no1, p.nbeta, p.nalpha, p.nbf5
+    rows = np.zeros(n5, dtype=bool)
+    rows[o0:b] = True
+    rows[a:n5] = True
+    cols = np.ones(n5, dtype=bool)
+    cols[b:a] = False
+    w = 2.0 * H_core + np.diagonal(J_MO)
+    grad = dn[rows].T @ w[rows]
+    mask = np.outer(rows, cols)[:, :, None]
+    grad += 2.0 * np.einsum("ijk,ji->k", Dcj * mask, J_MO)
+    grad -= 2.0 * np.einsum("ijk,ji->k", Dck * mask, K_MO)
+    sj = J_MO[b:a, :].sum(axis=0)
+    sk = K_MO[b:a, :].sum(axis=0)
+    grad += 2.0 * dn[rows].T @ sj[rows]
+    grad -= dn[rows].T @ sk[rows]
     return grad

@@ -76,7 +76,7 @@ _SIGS = {
                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "pnjk_occupation_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_void_p] * 11),
     "pnjk_pnof_coefficients": (ctypes.c_int, [ctypes.c_void_p] * 6),
-    "pnjk_energy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]
+    "pnjk_energy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32]
                     + [ctypes.c_void_p] * 8),
     "pnjk_footprint": (ctypes.c_uint64, [ctypes.c_void_p]),
     "pnjk_workspace_bytes": (ctypes.c_uint64, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int]),

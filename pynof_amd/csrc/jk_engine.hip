@@ -1064,7 +1064,6 @@ int pnjk_lagrange_fock(pnjk_engine* e, const double* F, int64_t nbf, int64_t nbf
 static int pnof_from(const pnjk_pnof* p, PnofDev* d, const char* who) {
   const std::string w(who);
   if (!p) return set_error(PNJK_EINVAL, w + ": pnof parameters are NULL");
-  if (p->mspin != 0) return set_error(PNJK_EUNSUPPORTED, w + ": only MSpin = 0 (closed shell) is supported");
   if (p->ipnof != 5 && p->ipnof != 7 && p->ipnof != 8)
     return set_error(PNJK_EUNSUPPORTED, w + ": ipnof must be 5, 7 or 8");
   if (p->occ_method != PNJK_OCC_SOFTMAX && p->occ_method != PNJK_OCC_TRIGONOMETRIC)
@@ -1078,6 +1077,7 @@ static int pnof_from(const pnjk_pnof* p, PnofDev* d, const char* who) {
   d->ista = p->ista;
   d->method = p->occ_method;
   d->high_spin = p->high_spin ? 1 : 0;
+  d->mspin = p->mspin;
   d->no1 = p->no1;
   d->ndoc = p->ndoc;
   d->nsoc = p->nsoc;
@@ -1198,9 +1198,9 @@ int pnjk_pnof_coefficients(pnjk_engine* e, const pnjk_pnof* p, const double* n, 
   return PNJK_OK;
 }
 
-int pnjk_energy(pnjk_engine* e, int64_t nbf5, int64_t nbeta, int64_t nalpha, const double* n, const double* cj12,
-                const double* ck12, const double* J_MO, const double* K_MO, const double* H_core, double* E,
-                void* stream) {
+int pnjk_energy(pnjk_engine* e, int64_t nbf5, int64_t nbeta, int64_t nalpha, int32_t mspin, const double* n,
+                const double* cj12, const double* ck12, const double* J_MO, const double* K_MO, const double* H_core,
+                double* E, void* stream) {
   if (!e) return set_error(PNJK_EINVAL, "pnjk_energy: engine is NULL");
   std::lock_guard<std::mutex> lk(e->mu);
   PNJK_HIP_CHECK(hipSetDevice(e->device));
@@ -1213,6 +1213,8 @@ int pnjk_energy(pnjk_engine* e, int64_t nbf5, int64_t nbeta, int64_t nalpha, con
   d.nbf5 = (int)nbf5;
   d.nbeta = (int)nbeta;
   d.nalpha = (int)nalpha;
+  d.nsoc = (int)(nalpha - nbeta);
+  d.mspin = mspin;
   OccBufs b;
   int rc;
   if ((rc = occ_bufs(e, nbf5, 0, b))) return rc;

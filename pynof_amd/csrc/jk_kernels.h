@@ -194,6 +194,7 @@ struct PnofDev {
   int ista;       // PNOF7 Phi: 0 sqrt(n(1-n)), else 2n(1-n)
   int method;     // 0 softmax, 1 trigonometric
   int high_spin;  // singly occupied n = 1 instead of 0.5
+  int mspin;      // 0 closed shell; otherwise the open-shell energy and gradient
   int no1, ndoc, nsoc, ncwo, nbf5, nv, nbeta, nalpha, ndns;
 };
 

@@ -45,17 +45,17 @@ __device__ void coef(const PnofDev& p, const double* __restrict__ n, const doubl
   const bool ss = in_s(p, i) && in_s(p, j);
   if (p.ipnof == 7) ck = ni * nj + fi[i] * fi[j];
   if (p.ipnof != 8) {
-    if (p.nsoc > 1 && ss) ck = 2.0 * ni * nj;
+    if (p.mspin == 0 && p.nsoc > 1 && ss) ck = 2.0 * ni * nj;
   } else {
     const bool oi = in_o(p, i), vi = in_v(p, i), oj = in_o(p, j), vj = in_v(p, j);
     if ((oi && vj) || (vi && oj) || (vi && vj)) ck += fi[i] * fi[j];
-    if (p.nsoc > 0) {
+    if (p.mspin == 0 && p.nsoc > 0) {
       if (oi && in_s(p, j)) ck += 0.5 * (fi[i] * 0.5);
       if (in_s(p, i) && oj) ck += 0.5 * (0.5 * fi[j]);
       if (in_s(p, i) && vj) ck += 0.5 * fi[j];
       if (vi && in_s(p, j)) ck += fi[i] * 0.5;
     }
-    if (p.nsoc > 1 && ss) ck = 0.5;
+    if (p.mspin == 0 && p.nsoc > 1 && ss) ck = 0.5;
     if ((oi && vj) || (vi && oj)) ck += nd12[i] * nd12[j] - nd[i] * nd[j];
     if (vi && vj) ck += -nd12[i] * nd12[j] - nd[i] * nd[j];
   }
@@ -94,11 +94,11 @@ __device__ void dcoef(const PnofDev& p, const Der& d, int i, int j, int k, doubl
     const double dfi = d.dfi[ik];
     if (oi && vj) dck += dfi * d.fi[j];
     if (vi && (oj || vj)) dck += d.fi[j] * dfi;
-    if (p.nsoc > 0 && in_s(p, j)) {
+    if (p.mspin == 0 && p.nsoc > 0 && in_s(p, j)) {
       if (oi) dck += 0.5 * dfi * 0.5;
       if (vi) dck += dfi * 0.5;
     }
-    if (p.nsoc > 1 && in_s(p, i) && in_s(p, j)) dck = 0.0;
+    if (p.mspin == 0 && p.nsoc > 1 && in_s(p, i) && in_s(p, j)) dck = 0.0;
     if (oi && vj) dck += d.dnd12[ik] * d.nd12[j] - d.dnd[ik] * d.nd[j];
     if (vi && oj) dck += d.nd12[j] * d.dnd12[ik] - d.nd[j] * d.dnd[ik];
     if (vi && vj) dck += -d.nd12[j] * d.dnd12[ik] - d.nd[j] * d.dnd[ik];
@@ -279,11 +279,13 @@ __global__ void k_occ_transpose(const double* __restrict__ s, int n, double* __r
   t[idx] = s[(int64_t)j * n + i];
 }
 
-// Closed-shell energy (pnof.py:905-921) in two passes. Pass 1: workgroup b takes rows
+// Energy (calce, pnof.py:905-988) in two passes. Pass 1: workgroup b takes rows
 // i = 4b .. 4b + 3, one wave per row; lane sums j = lane + 64 t of C^J[i][j] J[j][i] -
-// C^K[i][j] K[j][i] (j != i) plus, on lane 0, the row's one-body terms; the workgroup's sum
-// goes to part[b]. Pass 2 sums the parts in order. JT, KT are J_MO, K_MO transposed
-// (JT[i][j] = J_MO[j][i]); with cjm / ckm null, C^J, C^K come from coef().
+// C^K[i][j] K[j][i] (j != i; with MSpin != 0 only i, j outside the singly occupied block
+// S) plus, on lane 0, the row's one-body terms and, with MSpin != 0, its n-weighted sums
+// over S; the workgroup's sum goes to part[b]. Pass 2 sums the parts in order. JT, KT are
+// J_MO, K_MO transposed (JT[i][j] = J_MO[j][i]); with cjm / ckm null, C^J, C^K come from
+// coef().
 __global__ void __launch_bounds__(256) k_occ_energy_rows(PnofDev p, const double* __restrict__ n,
                                                          const double* __restrict__ fi, const double* __restrict__ nd,
                                                          const double* __restrict__ nd12,
@@ -295,12 +297,24 @@ __global__ void __launch_bounds__(256) k_occ_energy_rows(PnofDev p, const double
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   double acc = 0.0;
   if (i < n5) {
+    const bool si = in_s(p, i), open = p.mspin != 0;
     if (lane == 0) {
       acc = 2.0 * n[i] * hc[i];
-      if (i < p.nbeta || i >= p.nalpha) acc += n[i] * JT[(int64_t)i * n5 + i];
+      if (!si) acc += n[i] * JT[(int64_t)i * n5 + i];
+      if (open) {  // pnof.py:966-988
+        double sj = 0.0, sk = 0.0;
+        for (int t = p.nbeta; t < p.nalpha; ++t) {
+          sj += JT[(int64_t)i * n5 + t];
+          sk += KT[(int64_t)i * n5 + t];
+        }
+        if (!si)
+          acc += 2.0 * n[i] * sj - n[i] * sk;
+        else
+          acc += 0.5 * (n[i] * sj - n[i] * JT[(int64_t)i * n5 + i]) - 0.5 * (n[i] * sk + n[i] * KT[(int64_t)i * n5 + i]);
+      }
     }
     for (int j = lane; j < n5; j += 64) {
-      if (j == i) continue;
+      if (j == i || (open && (si || in_s(p, j)))) continue;
       const int64_t idx = (int64_t)i * n5 + j;
       double a, b;
       if (cjm) {
@@ -325,7 +339,7 @@ __global__ void __launch_bounds__(256) k_occ_energy_sum(const double* __restrict
   if (threadIdx.x == 0) E[0] = tot;
 }
 
-// Occupation gradient, closed shell (pnof.py:1004-1025): one workgroup per parameter k.
+// Occupation gradient (calcoccg, pnof.py:1004-1098): one workgroup per parameter k.
 // Row i contributes only where one of its derivative factors at k is nonzero; every
 // term of dC[i][j] carries one of them, so skipped rows add exact zeros.
 __global__ void __launch_bounds__(256) k_occ_grad(PnofDev p, Der d, const double* __restrict__ JT,
@@ -334,17 +348,31 @@ __global__ void __launch_bounds__(256) k_occ_grad(PnofDev p, Der d, const double
   __shared__ double red[4];
   const int n5 = p.nbf5, k = blockIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool open = p.mspin != 0;
   double acc = 0.0;
-  for (int i = threadIdx.x; i < n5; i += blockDim.x)
-    acc += d.dn[(int64_t)i * p.nv + k] * (2.0 * hc[i] + JT[(int64_t)i * n5 + i]);
+  for (int i = threadIdx.x; i < n5; i += blockDim.x) {
+    // MSpin != 0 keeps rows [no1, nbeta) and [nalpha, nbf5) only (pnof.py:1029-1098)
+    if (open && (i < p.no1 || in_s(p, i))) continue;
+    const double dni = d.dn[(int64_t)i * p.nv + k];
+    acc += dni * (2.0 * hc[i] + JT[(int64_t)i * n5 + i]);
+    if (open && dni != 0.0) {
+      double sj = 0.0, sk = 0.0;
+      for (int t = p.nbeta; t < p.nalpha; ++t) {
+        sj += JT[(int64_t)i * n5 + t];
+        sk += KT[(int64_t)i * n5 + t];
+      }
+      acc += 2.0 * dni * sj - dni * sk;
+    }
+  }
   for (int i = wave; i < n5; i += 4) {
+    if (open && (i < p.no1 || in_s(p, i))) continue;
     const int64_t ik = (int64_t)i * p.nv + k;
     bool live = d.dn[ik] != 0.0;
     if (p.ipnof >= 7) live = live || d.dfi[ik] != 0.0;
     if (p.ipnof == 8) live = live || d.dnd[ik] != 0.0 || d.dnd12[ik] != 0.0;
     if (!live) continue;
     for (int j = lane; j < n5; j += 64) {
-      if (j == i) continue;
+      if (j == i || (open && in_s(p, j))) continue;
       double a, b;
       dcoef(p, d, i, j, k, a, b);
       acc += 2.0 * a * JT[(int64_t)i * n5 + j] - 2.0 * b * KT[(int64_t)i * n5 + j];

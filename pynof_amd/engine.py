@@ -385,8 +385,9 @@ class JKEngine:
                                                       stream))
         return cj, ck
 
-    def energy(self, n, cj12, ck12, J_MO, K_MO, H_core, nbeta: int, nalpha: int, stream=None) -> float:
-        """Closed-shell energy for given coefficients (calce with MSpin = 0, pnof.py:905-921)."""
+    def energy(self, n, cj12, ck12, J_MO, K_MO, H_core, nbeta: int, nalpha: int, MSpin: int = 0,
+               stream=None) -> float:
+        """Energy for given coefficients (calce, pnof.py:905-990; MSpin != 0 the open-shell form)."""
         arrs = [_as_input(x, nm) for x, nm in ((n, "n"), (cj12, "cj12"), (ck12, "ck12"), (J_MO, "J_MO"),
                                                (K_MO, "K_MO"), (H_core, "H_core"))]
         nbf5 = int(arrs[0].shape[0])
@@ -396,8 +397,8 @@ class JKEngine:
         E = np.empty(1)
         if stream is None and _is_torch(arrs[0]) and arrs[0].is_cuda:
             stream = _torch().cuda.current_stream(arrs[0].device).cuda_stream
-        _lib.check(_lib.lib().pnjk_energy(self._h, nbf5, int(nbeta), int(nalpha), *(_ptr(a) for a in arrs),
-                                          E.ctypes.data, stream))
+        _lib.check(_lib.lib().pnjk_energy(self._h, nbf5, int(nbeta), int(nalpha), int(MSpin),
+                                          *(_ptr(a) for a in arrs), E.ctypes.data, stream))
         return float(E[0])
 
     def lagrange_fock(self, F, C, noptorb: int, out=None, stream=None):

@@ -4,8 +4,8 @@
 reference's signatures, argument meaning and return values (pnof.py:692, :657,
 :905, :994, :1004) and run on the MI355X through the HIP library; there is no CPU
 path. Supported: PNOF5, PNOF7 and GNOF (p.ipnof 5, 7, 8), the Softmax and
-Trigonometric parametrisations, MSpin = 0. Other settings raise
-NotImplementedError. Arrays may be numpy (host results) or torch device tensors
+Trigonometric parametrisations, closed (MSpin = 0) and open shell. Other
+settings raise NotImplementedError. Arrays may be numpy (host results) or torch device tensors
 (device results).
 """
 from __future__ import annotations
@@ -17,8 +17,6 @@ from .integrals import _plain_engine
 
 
 def _pnof_of(p, occ_method=None):
-    if getattr(p, "MSpin", 0) != 0:
-        raise NotImplementedError("pynof_amd implements the closed-shell (MSpin = 0) occupation step only")
     ipnof = int(getattr(p, "ipnof", 8))
     if ipnof not in (5, 7, 8):
         raise NotImplementedError(f"PNOF{ipnof} is not implemented (PNOF5, PNOF7 and GNOF are)")
@@ -26,7 +24,7 @@ def _pnof_of(p, occ_method=None):
     if method not in ("Softmax", "Trigonometric"):
         raise NotImplementedError(f"occupation method {method!r} is not implemented")
     return JKEngine.pnof(p.no1, p.ndoc, p.nsoc, p.ncwo, ipnof=ipnof, ista=getattr(p, "ista", 0),
-                         occ_method=method, HighSpin=getattr(p, "HighSpin", False))
+                         occ_method=method, HighSpin=getattr(p, "HighSpin", False), MSpin=getattr(p, "MSpin", 0))
 
 
 class _Sizes:
@@ -52,16 +50,14 @@ def PNOFi_selector(n, p):
 
 
 def calce(n, cj12, ck12, J_MO, K_MO, H_core, p):
-    """Closed-shell energy (pnof.py:905-921). As in the reference, the diagonals of
-    the caller's cj12 and ck12 are set to zero in place (:913, :917)."""
-    if getattr(p, "MSpin", 0) != 0:
-        raise NotImplementedError("pynof_amd implements the closed-shell (MSpin = 0) energy only")
+    """Energy (pnof.py:905-990, closed and open shell). As in the reference, the
+    diagonals of the caller's cj12 and ck12 are set to zero in place (:913, :917)."""
     for c in (cj12, ck12):
         if isinstance(c, np.ndarray):
             np.fill_diagonal(c, 0)
         else:
             c.fill_diagonal_(0)
-    return _plain_engine(p).energy(n, cj12, ck12, J_MO, K_MO, H_core, p.nbeta, p.nalpha)
+    return _plain_engine(p).energy(n, cj12, ck12, J_MO, K_MO, H_core, p.nbeta, p.nalpha, getattr(p, "MSpin", 0))
 
 
 def calcocce(gamma, J_MO, K_MO, H_core, p):

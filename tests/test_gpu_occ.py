@@ -156,7 +156,7 @@ def test_dropins(gpu_required):
     assert abs(dropin.calcocce(g, J, K, hc, prm) - Eo) <= 1e-12 * max(1.0, abs(Eo))
     go = O.calcoccg(g, J, K, hc, p)
     assert np.max(np.abs(dropin.calcoccg(g, J, K, hc, prm) - go)) <= 1e-11 * max(1.0, np.max(np.abs(go)))
-    prm.MSpin = 1
+    prm.ipnof = 4
     with pytest.raises(NotImplementedError):
         dropin.calcocce(g, J, K, hc, prm)
     release_cached_engines()
@@ -173,7 +173,7 @@ def test_errors(eng):
                                       None, None, None, None, None, None)
 
     assert call(JKEngine.pnof(0, 2, 1, 3)) == _lib.OK
-    assert call(JKEngine.pnof(0, 2, 1, 3, MSpin=1)) == _lib.EUNSUPPORTED
+    assert call(JKEngine.pnof(0, 2, 1, 3, MSpin=1)) == _lib.OK  # open shell
     assert call(JKEngine.pnof(0, 2, 1, 3, ipnof=4)) == _lib.EUNSUPPORTED
     bad = JKEngine.pnof(0, 2, 1, 3)
     bad.nbf5 = 10
@@ -183,7 +183,33 @@ def test_errors(eng):
     assert call(JKEngine.pnof(0, 2, 1, 3), Ep=None) == _lib.EINVAL  # nothing requested
     assert L.pnjk_occupation_step(eng._h, None, None, None, None, None, E.ctypes.data, None, None, None, None, None,
                                   None) == _lib.EINVAL
-    assert L.pnjk_energy(eng._h, 9, 4, 3, hc.ctypes.data, J.ctypes.data, K.ctypes.data, J.ctypes.data,
+    assert L.pnjk_energy(eng._h, 9, 4, 3, 0, hc.ctypes.data, J.ctypes.data, K.ctypes.data, J.ctypes.data,
                          K.ctypes.data, hc.ctypes.data, E.ctypes.data, None) == _lib.EINVAL  # nbeta > nalpha
     with pytest.raises(ValueError):
         JKEngine.pnof(0, 2, 1, 3, occ_method="EBI")
+
+
+OPEN = [(0, 2, 1, 2), (1, 2, 2, 2), (2, 3, 1, 2), (0, 3, 3, 1), (0, 4, 2, 3)]
+
+
+@pytest.mark.parametrize("method", ["Softmax", "Trigonometric"])
+@pytest.mark.parametrize("ipnof,ista", FUNCS)
+@pytest.mark.parametrize("no1,ndoc,nsoc,ncwo", OPEN)
+def test_open_shell_vs_oracle(eng, method, ipnof, ista, no1, ndoc, nsoc, ncwo):
+    """MSpin != 0: coefficients without the closed-shell singly-occupied blocks, the
+    open-shell energy (pnof.py:924-988) and gradient (:1027-1098)."""
+    print(f"case: occ open shell {method} ipnof={ipnof} ista={ista} cfg={(no1, ndoc, nsoc, ncwo)}", flush=True)
+    p = O.Pnof(no1, ndoc, nsoc, ncwo, ipnof=ipnof, ista=ista, occ_method=method, MSpin=nsoc)
+    J, K, hc = mo_mats(p.nbf5, 400 + 7 * ndoc + ncwo)
+    g = np.random.default_rng(401 + ncwo).uniform(-0.8, 1.2, p.nv)
+    pn = JKEngine.pnof(no1, ndoc, nsoc, ncwo, ipnof=ipnof, ista=ista, occ_method=method, MSpin=nsoc)
+    r = eng.occupation_step(g, pn, J, K, hc, outputs=("E", "grad", "cj12", "ck12"))
+    n, _ = O.occupation(g, p)
+    cj, ck = O.coefficients(n, p)
+    assert np.max(np.abs(r["cj12"] - cj)) <= 1e-14 and np.max(np.abs(r["ck12"] - ck)) <= 1e-14
+    E = O.calce(n, cj, ck, J, K, hc, p)
+    assert abs(r["E"][0] - E) <= 1e-12 * max(1.0, abs(E))
+    grad = O.calcoccg(g, J, K, hc, p)
+    assert np.max(np.abs(r["grad"] - grad)) <= 1e-11 * max(1.0, np.max(np.abs(grad)))
+    Eg = eng.energy(n, cj, ck, J, K, hc, p.nbeta, p.nalpha, MSpin=nsoc)
+    assert abs(Eg - E) <= 1e-12 * max(1.0, abs(E))

@@ -240,3 +240,59 @@ def test_planted_mistakes_fail(monkeypatch):
 
     monkeypatch.setattr(O, "_dphi", doubled)
     assert np.max(np.abs(O.calcoccg(g, J, K, hc, p8) - fd_grad(g, J, K, hc, p8))) > 1e-6
+
+
+def calce_mspin_loops(n, cj, ck, J, K, hc, p):
+    """calce with MSpin != 0 by explicit loops over the reference's blocks (pnof.py:924-988)."""
+    b, a, n5 = p.nbeta, p.nalpha, p.nbf5
+    E = 0.0
+    for i in range(n5):
+        E += n[i] * 2.0 * hc[i]
+        if i < b or i >= a:
+            E += n[i] * J[i, i]
+    for i in range(n5):
+        for j in range(n5):
+            if i != j and not (b <= i < a) and not (b <= j < a):
+                E += cj[i, j] * J[j, i] - ck[i, j] * K[j, i]
+    for i in list(range(b)) + list(range(a, n5)):
+        for s in range(b, a):
+            E += 2.0 * n[i] * J[s, i] - n[i] * K[s, i]
+    for i in range(b, a):
+        for s in range(b, a):
+            E += 0.5 * n[i] * J[s, i] - 0.5 * n[i] * K[s, i]
+        E += -0.5 * n[i] * J[i, i] - 0.5 * n[i] * K[i, i]
+    return E
+
+
+@pytest.mark.parametrize("ipnof", [5, 7, 8])
+@pytest.mark.parametrize("no1,ndoc,nsoc,ncwo", [(0, 2, 1, 2), (1, 2, 2, 2), (0, 3, 3, 1)])
+def test_open_shell_energy_loops(ipnof, no1, ndoc, nsoc, ncwo):
+    p = O.Pnof(no1, ndoc, nsoc, ncwo, ipnof=ipnof, MSpin=nsoc)
+    J, K, hc, _ = mo_integrals(p.nbf5, 8, 90 + ipnof)
+    n = random_n(p, 91)
+    cj, ck = O.coefficients(n, p)
+    E = O.calce(n, cj, ck, J, K, hc, p)
+    assert abs(E - calce_mspin_loops(n, cj, ck, J, K, hc, p)) < 1e-12 * max(1.0, abs(E))
+    # the singly occupied block's override (MSpin == 0 only, pnof.py:179-180) is absent:
+    # PNOF5 keeps n_s n_s' there (PNOF7's n n + Phi Phi equals 2 n n at n = 1/2)
+    p0 = O.Pnof(no1, ndoc, nsoc, ncwo, ipnof=ipnof, MSpin=0)
+    cj0, ck0 = O.coefficients(n, p0)
+    s = slice(p.nbeta, p.nalpha)
+    if nsoc > 1 and ipnof == 5:
+        assert np.allclose(ck[s, s], np.outer(n[s], n[s])) and np.allclose(ck0[s, s], 2 * np.outer(n[s], n[s]))
+
+
+OPEN_CASES = [(ip, st, m, c) for ip, st in [(5, 0), (7, 0), (7, 1), (8, 0)] for m in ["Softmax", "Trigonometric"]
+              for c in [(0, 2, 1, 2), (0, 2, 2, 2), (2, 3, 1, 2), (0, 3, 3, 1)]
+              if not (m == "Softmax" and c[0] > 0)]  # softmax with no1 > 0: see test_softmax_no1_quirk
+
+
+@pytest.mark.parametrize("ipnof,ista,method,cfg", OPEN_CASES)
+def test_open_shell_gradient_matches_energy_differences(ipnof, ista, method, cfg):
+    no1, ndoc, nsoc, ncwo = cfg
+    p = O.Pnof(no1, ndoc, nsoc, ncwo, ipnof=ipnof, ista=ista, occ_method=method, MSpin=nsoc)
+    J, K, hc, _ = mo_integrals(p.nbf5, 8, 95 + ipnof + ndoc)
+    g = np.random.default_rng(96 + ncwo).uniform(-0.8, 1.2, p.nv)
+    grad = O.calcoccg(g, J, K, hc, p)
+    fd = fd_grad(g, J, K, hc, p)
+    assert np.max(np.abs(grad - fd)) < 1e-7 * max(1.0, np.max(np.abs(fd)))
