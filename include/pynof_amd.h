@@ -169,6 +169,13 @@ int pnjk_fock_rc(pnjk_engine* e, const double* J, const double* K, int64_t nbf, 
 int pnjk_lagrange_fock(pnjk_engine* e, const double* F, int64_t nbf, int64_t nbf5, const double* C, int64_t ldc,
                        int64_t noptorb, double* elag, void* stream);
 
+/* Convergence measures of the Lagrange multipliers: replaces
+ * computeLagrangeConvergency (utils.py:405-411): *sumdiff = sum |elag - elag^T|,
+ * *maxdiff = max |elag - elag^T| over the (nbf, nbf) matrix elag (host or
+ * device). Waits for the result. */
+int pnjk_lagrange_convergence(pnjk_engine* e, const double* elag, int64_t nbf, double* sumdiff, double* maxdiff,
+                              void* stream);
+
 /* ---- occupation step (pnof.py) ----
  * The PNOF functional and occupation parametrisation, the fields of PyNOF's param
  * (parameters.py:46-110) that the occupation step reads. nbf5 must equal

@@ -9,7 +9,9 @@ Entry points:
 from ._lib import PnjkError  # noqa: F401
 from .engine import JKEngine  # noqa: F401
 from .integrals import (  # noqa: F401
+    ENERGY1r,
     JKH_MO_tmp,
+    computeLagrangeConvergency,
     computeF_RC_driver,
     computeJKH_MO,
     computeJKj,

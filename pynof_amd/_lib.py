@@ -74,6 +74,9 @@ _SIGS = {
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "pnjk_lagrange_convergence": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.c_void_p]),
     "pnjk_occupation_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_void_p] * 11),
     "pnjk_pnof_coefficients": (ctypes.c_int, [ctypes.c_void_p] * 6),
     "pnjk_energy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32]

@@ -1058,6 +1058,32 @@ int pnjk_lagrange_fock(pnjk_engine* e, const double* F, int64_t nbf, int64_t nbf
   return PNJK_OK;
 }
 
+int pnjk_lagrange_convergence(pnjk_engine* e, const double* elag, int64_t nbf, double* sumdiff, double* maxdiff,
+                              void* stream) {
+  if (!e) return set_error(PNJK_EINVAL, "pnjk_lagrange_convergence: engine is NULL");
+  std::lock_guard<std::mutex> lk(e->mu);
+  PNJK_HIP_CHECK(hipSetDevice(e->device));
+  hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
+  if (!elag || !sumdiff || !maxdiff) return set_error(PNJK_EINVAL, "pnjk_lagrange_convergence: null argument");
+  if (nbf < 1 || nbf * nbf > (int64_t)1 << 31) return set_error(PNJK_EINVAL, "pnjk_lagrange_convergence: bad nbf");
+  int rc;
+  const bool host_e = !is_device_ptr(elag);
+  if ((rc = e->scal.ensure(256)) || (host_e && (rc = e->outs.ensure((size_t)(nbf * nbf) * 8)))) return rc;
+  const double* dE = elag;
+  if (host_e) {
+    if ((rc = to_device(elag, e->outs.as<double>(), nbf * nbf, s))) return rc;
+    dE = e->outs.as<double>();
+  }
+  double* out = e->scal.as<double>() + 2;
+  PNJK_HIP_CHECK(launch_lag_conv(dE, (int)nbf, out, s));
+  double h[2];
+  PNJK_HIP_CHECK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, s));
+  PNJK_HIP_CHECK(hipStreamSynchronize(s));
+  *sumdiff = h[0];
+  *maxdiff = h[1];
+  return PNJK_OK;
+}
+
 // ---------------------------------------------------------------- occupation step
 // Checks a pnjk_pnof against the relations of parameters.py:46-65, 94 and fills the
 // kernel-side copy.

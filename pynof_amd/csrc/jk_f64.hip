@@ -1312,6 +1312,47 @@ hipError_t launch_rowgemm_f64(const double* A, int64_t rows, const double* W, in
   return launch_prob(pr, tc, (int)tiles, s);
 }
 
+// Convergence measures of the Lagrange multipliers (utils.py:405-411): out[0] = sum |e - e^T|,
+// out[1] = max |e - e^T| over the nbf x nbf matrix. One workgroup: thread t visits i = t,
+// t + 1024, ... in order, so the sum's order is fixed.
+__global__ void __launch_bounds__(1024) k_lag_conv(const double* __restrict__ e, int n, double* __restrict__ out) {
+  __shared__ double ssum[16], smax[16];
+  double sum = 0.0, mx = 0.0;
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t idx = threadIdx.x; idx < nn; idx += blockDim.x) {
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    const double d = fabs(e[idx] - e[(int64_t)j * n + i]);
+    sum += d;
+    mx = d > mx ? d : mx;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sum += __shfl_xor(sum, off, 64);
+    const double o = __shfl_xor(mx, off, 64);
+    mx = o > mx ? o : mx;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    ssum[wave] = sum;
+    smax[wave] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0, m = 0.0;
+    for (int w = 0; w < 16; ++w) {
+      s += ssum[w];
+      m = smax[w] > m ? smax[w] : m;
+    }
+    out[0] = s;
+    out[1] = m;
+  }
+}
+
+hipError_t launch_lag_conv(const double* elag, int n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_lag_conv, dim3(1), dim3(1024), 0, s, elag, n, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_hcore_f64(const BuildGeom& g, const double* Hp, int ldh, const double* Cp, double* T,
                             double* hcore, hipStream_t s) {
   const TileF64 tc{2, 2};

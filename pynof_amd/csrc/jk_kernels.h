@@ -177,6 +177,9 @@ hipError_t launch_jn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* d_hi, co
 hipError_t launch_kn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* x_hi, const uint16_t* x_lo, double* K,
                         hipStream_t s, GroupCollector* col = nullptr);
 
+// out[0] = sum |e - e^T|, out[1] = max |e - e^T| of an n x n matrix (device out[2]).
+hipError_t launch_lag_conv(const double* elag, int n, double* out, hipStream_t s);
+
 // ---------------------------------------------------------------- b from 3-centre integrals
 // Vp, Vs (ld x ld, zero-padded) from the metric's eigenpairs; Vs rows scaled by D^-1/2 with
 // the relative eigenvalue cut.

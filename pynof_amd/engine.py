@@ -401,6 +401,19 @@ class JKEngine:
                                           *(_ptr(a) for a in arrs), E.ctypes.data, stream))
         return float(E[0])
 
+    def lagrange_convergence(self, elag, stream=None):
+        """(sumdiff, maxdiff) = (sum |elag - elag^T|, max |elag - elag^T|), computeLagrangeConvergency
+        (utils.py:405-411); elag host or device (nbf, nbf)."""
+        ei = _as_input(elag, "elag")
+        if ei.ndim != 2 or ei.shape[0] != ei.shape[1]:
+            raise ValueError(f"elag must be square; got {tuple(ei.shape)}")
+        if stream is None and _is_torch(ei) and ei.is_cuda:
+            stream = _torch().cuda.current_stream(ei.device).cuda_stream
+        sd, md = ctypes.c_double(), ctypes.c_double()
+        _lib.check(_lib.lib().pnjk_lagrange_convergence(self._h, _ptr(ei), int(ei.shape[0]), ctypes.byref(sd),
+                                                         ctypes.byref(md), stream))
+        return sd.value, md.value
+
     def lagrange_fock(self, F, C, noptorb: int, out=None, stream=None):
         """Lagrange multipliers from the Fock matrices (computeLagrange, utils.py:249-259):
         elag (nbf, nbf) with elag[i, j] = sum_mn C[m, i] F[j, m, n] C[n, j] for

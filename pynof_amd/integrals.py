@@ -142,6 +142,62 @@ def computeLagrange(F, C, p):
     return _plain_engine(p).lagrange_fock(F, C, p.noptorb)
 
 
+def computeLagrangeConvergency(elag):
+    """(sumdiff, maxdiff) of the Lagrange multipliers (utils.py:405-411), on the device."""
+    import numpy as np
+
+    if isinstance(elag, np.ndarray):
+        return _plain_engine(_OnDevice(0)).lagrange_convergence(np.ascontiguousarray(elag))
+    return _plain_engine(_OnDevice(elag.device.index or 0)).lagrange_convergence(elag)
+
+
+class _OnDevice:
+    """The device field of a param, for the engine cache."""
+
+    def __init__(self, device: int):
+        self.device = int(device)
+
+
+def ENERGY1r(C, n, H, I, b_mnl, cj12, ck12, p):
+    """(E, elag, sumdiff, maxdiff) of one orbital step (utils.py:415-432), host outputs as the
+    reference returns them. no1 == 0: computeLagrange2 on the device (b_MO never leaves it),
+    E = tr elag[:nbf5, :nbf5] + sum n_i Hmat[i, i]. no1 > 0 (MSpin = 0): J^n, K^n, the Fock
+    matrices and elag stay on the device (the reference moves J^n, K^n and F across the bus),
+    E as computeE_elec (utils.py:385-402) from the build's H_core = diag(C^T H C). As in the
+    reference, the diagonals of the caller's cj12 and ck12 are zeroed in place."""
+    import numpy as np
+    import torch
+
+    _check_p(p)
+    n5 = int(p.nbf5)
+    n = np.asarray(n, dtype=np.float64)
+    if p.no1 == 0:
+        elag, Hmat = computeLagrange2(n, cj12, ck12, C, H, I, b_mnl, p)
+        E = float(np.trace(elag[:n5, :n5]) + np.dot(n[:n5], np.diagonal(Hmat[:n5, :n5])))
+        sumdiff, maxdiff = _engine_for(b_mnl, p).lagrange_convergence(elag)
+        return E, elag, sumdiff, maxdiff
+    if getattr(p, "MSpin", 0) != 0:
+        raise NotImplementedError("the open-shell Fock matrices (computeF_RO, utils.py:87-246) are not implemented")
+    ini = p.no1 if p.no1 > 1 else 0
+    for c in (cj12, ck12):
+        np.fill_diagonal(c[ini:, ini:], 0)
+    eng = _engine_for(b_mnl, p)
+    dev = torch.device("cuda", int(getattr(p, "device", 0)))
+    td = lambda x: torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(dev)  # noqa: E731
+    Cd, Hd = td(C), td(H)
+    prec = getattr(p, "precision", "fp64")
+    out = eng.build(Cd, n5, H=Hd, precision=prec, outputs=("J", "K", "H_core"))
+    F = eng.fock_rc(out["J"], out["K"], td(n[:n5]), Hd, td(cj12), td(ck12), p.no1, p.nbeta, p.nalpha)
+    elag_d = eng.lagrange_fock(F, Cd, p.noptorb)
+    sumdiff, maxdiff = eng.lagrange_convergence(elag_d)
+    hc = out["H_core"].cpu().numpy()
+    b, a = p.nbeta, p.nalpha
+    elag = elag_d.cpu().numpy()
+    E = float(np.trace(elag[:n5, :n5]) + np.dot(n[:b], hc[:b]) + np.dot(n[a:n5], hc[a:n5]))
+    E += 0.5 * float(np.sum(hc[b:a])) if getattr(p, "HighSpin", False) else float(np.dot(n[b:a], hc[b:a]))
+    return E, elag, sumdiff, maxdiff
+
+
 def release_cached_engines() -> None:
     """Drops every cached device copy of b and the per-device engines."""
     with _cache_lock:

@@ -1,0 +1,79 @@
+"""ENERGY1r (utils.py:415-432) as a drop-in: both branches against the oracle's composition
+of the same reference functions, and computeLagrangeConvergency (utils.py:405-411)."""
+import numpy as np
+import pytest
+
+from oracle import jk_oracle as O
+from pynof_amd import param, synth
+from pynof_amd.integrals import ENERGY1r, computeLagrangeConvergency, release_cached_engines
+
+pytestmark = pytest.mark.gpu
+
+
+def inputs(nbf, naux, nbf5, seed):
+    b = synth.make_b(nbf, naux, seed, 0.05)
+    C = synth.make_C(nbf, seed + 1)
+    H = synth.make_H(nbf, seed + 2)
+    rng = np.random.default_rng(seed + 3)
+    n = rng.uniform(0.02, 0.98, nbf5)
+    cj = rng.standard_normal((nbf5, nbf5))
+    ck = rng.standard_normal((nbf5, nbf5))
+    return b, C, H, n, cj + cj.T, ck + ck.T
+
+
+def conv(e):
+    d = np.abs(e - e.T)
+    return float(d.sum()), float(d.max())
+
+
+def test_energy1r_no1_zero(gpu_required):
+    nbf, naux, nbf5 = 36, 130, 28
+    print(f"case: ENERGY1r no1=0 nbf={nbf}", flush=True)
+    b, C, H, n, cj, ck = inputs(nbf, naux, nbf5, 1100)
+    p = param(nbf=nbf, nbf5=nbf5, nbfaux=naux, ndoc=9, RI=True)
+    el, Hm = O.lagrange2_ri(n, cj, ck, C, H, b, nbf5, p.nbeta, p.nalpha)
+    E_ref = np.trace(el[:nbf5, :nbf5]) + np.dot(n, np.diagonal(Hm[:nbf5, :nbf5]))
+    E, elag, sd, md = ENERGY1r(C, n, H, [], b, cj.copy(), ck.copy(), p)
+    assert abs(E - E_ref) <= 1e-10 * abs(E_ref)
+    assert np.linalg.norm(elag - el) <= 1e-10 * np.linalg.norm(el)
+    sd0, md0 = conv(el)
+    assert abs(sd - sd0) <= 1e-9 * sd0 and abs(md - md0) <= 1e-9 * md0
+    release_cached_engines()
+
+
+@pytest.mark.parametrize("high_spin", [False, True])
+def test_energy1r_no1_positive(gpu_required, high_spin):
+    nbf, naux, nbf5, no1 = 34, 120, 26, 2
+    print(f"case: ENERGY1r no1={no1} high_spin={high_spin}", flush=True)
+    b, C, H, n, cj, ck = inputs(nbf, naux, nbf5, 1110)
+    p = param(nbf=nbf, nbf5=nbf5, nbfaux=naux, ndoc=6, RI=True, no1=no1, nsoc=2, noptorb=nbf - 1,
+              HighSpin=high_spin)
+    J, K, _, _, hc = O.jk_build(C, H, b, nbf5)
+    F = O.fock_rc(J, K, n, H, cj, ck, p.nbeta, p.nalpha, no1)
+    el = O.lagrange_fock(F, C, p.noptorb)
+    bt, al = p.nbeta, p.nalpha
+    E_ref = np.trace(el[:nbf5, :nbf5]) + np.dot(n[:bt], hc[:bt]) + np.dot(n[al:], hc[al:])
+    E_ref += 0.5 * hc[bt:al].sum() if high_spin else np.dot(n[bt:al], hc[bt:al])  # utils.py:391-396
+    cj_in, ck_in = cj.copy(), ck.copy()
+    E, elag, sd, md = ENERGY1r(C, n, H, [], b, cj_in, ck_in, p)
+    assert abs(E - E_ref) <= 1e-10 * abs(E_ref)
+    assert np.linalg.norm(elag - el) <= 1e-10 * np.linalg.norm(el)
+    assert np.all(np.diag(cj_in)[no1:] == 0) and np.all(np.diag(ck_in)[no1:] == 0)
+    sd0, md0 = conv(el)
+    assert abs(sd - sd0) <= 1e-9 * sd0 and abs(md - md0) <= 1e-9 * md0
+    p.MSpin = 1
+    with pytest.raises(NotImplementedError):
+        ENERGY1r(C, n, H, [], b, cj.copy(), ck.copy(), p)
+    release_cached_engines()
+
+
+def test_lagrange_convergence(gpu_required):
+    import torch
+
+    e = np.random.default_rng(1120).standard_normal((157, 157))
+    sd, md = computeLagrangeConvergency(e)
+    sd0, md0 = conv(e)
+    assert abs(sd - sd0) <= 1e-12 * sd0 and md == md0
+    sd2, md2 = computeLagrangeConvergency(torch.from_numpy(e).cuda())
+    assert sd2 == sd and md2 == md
+    release_cached_engines()
