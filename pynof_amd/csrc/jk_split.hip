@@ -9,8 +9,9 @@
 //   Ct16[q][mu]        C^T, per build                 (A of HT, QT)
 //   bht16[nu][P][mu]   b with mu contiguous, at upload (B of HT)
 //   btri16[t][P]       packed lower triangle, at upload (B of JN)
-//   X16[q][nu][P]      half-transformed b, per build  (A/B of K^n; B of QT via
-//                      a transposing LDS loader)
+//   X16[q][nu][P]      half-transformed b, per build  (A/B of K^n; B of QT with
+//                      P contiguous: copied to LDS as is and read with the
+//                      LDS transpose read)
 // Contractions (reference lines as in the FP64 kernels): HT integrals.py:211,
 // QT :354 (-> d :212, K_MO :358), JN :214, K^n :216.
 #include <type_traits>
@@ -22,6 +23,7 @@ namespace pnjk {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kThreads = 256;
@@ -43,11 +45,10 @@ struct Planes {
   const uint16_t* lo;
 };
 
-enum LdKind { RK = 0, KR = 1 };
-
-// Loader of a ROWS x 32 tile (both planes) into LDS rows of kLDK elements.
-template <int KIND, int ROWS>
-struct Ld;
+// Operand layouts: RK = rows with the contraction index k contiguous (copied to
+// LDS row by row); KN = k-major with the tile's columns contiguous (copied to LDS
+// in that orientation and transposed by the fragment read).
+enum LdKind { RK = 0, KN = 2 };
 
 // Direct HBM -> LDS copy of both planes of a ROWS x 32 tile whose contraction
 // index k is contiguous in memory (global element (r, k) at off(r) + k), with
@@ -80,43 +81,66 @@ struct Dma {
   }
 };
 
-// Global element (k, r) at off(k) + r, r contiguous: 16-byte loads along r,
-// transposed into LDS with 16-bit stores.
-template <int ROWS>
-struct Ld<KR, ROWS> {
-  static constexpr int kChunks = kBK * (ROWS / 8);
-  static constexpr int kPer = (kChunks + kThreads - 1) / kThreads;
-  // Every lane loads and stores (no branches in the k-loop); when kChunks is not
-  // a multiple of kThreads the surplus lanes repeat chunk c % kChunks and store
-  // identical bytes to the same LDS address.
-  static __device__ __forceinline__ int chunk(int i) {
-    const int c = threadIdx.x + i * kThreads;
-    return kChunks % kThreads == 0 ? c : c % kChunks;
-  }
-  u16x8 h[kPer], l[kPer];
+// 16-column group ("quad", 32 bytes) swizzle of row k of a KN tile of COLS
+// columns: quad Q of row k sits at Q ^ qswz(k). One transpose read (below) of a
+// half wave touches rows {r, r+1, r+2, r+3, r+8, .., r+11} (r a multiple of 4) at one
+// logical quad; the swizzle puts those eight 32-byte pieces on distinct banks.
+template <int COLS>
+__device__ __forceinline__ int qswz(int k) {
+  static_assert(COLS == 64 || COLS == 128, "KN tiles are 64 or 128 columns wide");
+  if constexpr (COLS == 128) return (k & 3) | ((k >> 1) & 4);
+  else return ((k >> 1) & 1) | ((k >> 2) & 2);  // rows of 128 bytes: odd rows sit 32 banks on
+}
+
+// Direct HBM -> LDS copy of both planes of a 32 (k) x COLS tile whose columns are
+// contiguous in memory (global element (k, n) at off(k) + n). LDS keeps the
+// orientation: row k holds the COLS elements, quads swizzled by qswz. Lane L of
+// instruction i writes the 16 bytes at LDS chunk 64 i + L, i.e. row k =
+// (64 i + L) / (COLS / 8); it fetches the global chunk that belongs there.
+template <int COLS, int NW>
+struct DmaKN {
+  static constexpr int kInst = COLS / 16;  // 32 rows x 2 COLS bytes per plane, 1 KB per instruction
+  static constexpr int kRowChunks = COLS / 8;
   template <class S>
-  __device__ __forceinline__ void load(const S& s, int r0, int k0) {
+  static __device__ __forceinline__ void issue(const S& s, int n0, int k0, uint16_t* sh, uint16_t* sl, int wave,
+                                               int lane) {
 #pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-      const int c = chunk(i);
-      const int k = c % kBK, rc = (c / kBK) * 8;
-      const int64_t o = s.off(k0 + k) + r0 + rc;
-      h[i] = *gmem(reinterpret_cast<const u16x8*>(s.p.hi + o));
-      l[i] = *gmem(reinterpret_cast<const u16x8*>(s.p.lo + o));
-    }
-  }
-  __device__ __forceinline__ void store(uint16_t* sh, uint16_t* sl) const {
-#pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-      const int c = chunk(i);
-      const int k = c % kBK, rc = (c / kBK) * 8;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int o = swz(rc + e, k >> 3) + (k & 7);
-        sh[o] = h[i][e];
-        sl[o] = l[i][e];
+    for (int j = 0; j < (kInst + NW - 1) / NW; ++j) {
+      const int i = NW * j + wave;  // wave-uniform
+      if (kInst % NW == 0 || i < kInst) {
+        const int o = 64 * i + lane;
+        const int k = o / kRowChunks, pc = o % kRowChunks;
+        const int64_t g = s.off(k0 + k) + n0 + 16 * ((pc >> 1) ^ qswz<COLS>(k)) + 8 * (pc & 1);
+        __builtin_amdgcn_global_load_lds((const PNJK_GLOBAL void*)(s.p.hi + g),
+                                         (__attribute__((address_space(3))) void*)(sh + 512 * i), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const PNJK_GLOBAL void*)(s.p.lo + g),
+                                         (__attribute__((address_space(3))) void*)(sl + 512 * i), 16, 0, 0);
       }
     }
+  }
+};
+
+// The MFMA B fragment of columns [n, n + 16) from a KN tile in LDS: lane l needs
+// elements k = 8 (l / 16) + t, t < 8, of column n + l % 16. ds_read_b64_tr_b16 over
+// each group of 16 lanes hands lane i element e of the 8 bytes that lane
+// 4 e + i / 4 of the group addressed; so lane 4 j + c of group g addresses row
+// r = 8 g + j (then r + 4 for t >= 4), columns n + 4 c .. n + 4 c + 3. Rows r and
+// r + 4 share their swizzle, so the second read is the first one's address plus a
+// constant.
+template <int COLS>
+struct TrRead {
+  int base, sw;  // this lane's element offset (row r, column 4 c) and its quad swizzle
+  __device__ __forceinline__ explicit TrRead(int lane) {
+    const int r = 8 * (lane >> 4) + ((lane >> 2) & 3);
+    base = r * COLS + 4 * (lane & 3);
+    sw = qswz<COLS>(r);
+  }
+  __device__ __forceinline__ bf16x8 read(const uint16_t* tile, int n) const {
+    typedef __attribute__((address_space(3))) bf16x4 lds4;
+    const uint16_t* p = tile + base + 16 * ((n >> 4) ^ sw);
+    const bf16x4 x = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds4*)p);
+    const bf16x4 y = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds4*)(p + 4 * COLS));
+    return __builtin_shufflevector(x, y, 0, 1, 2, 3, 4, 5, 6, 7);
   }
 };
 
@@ -130,8 +154,7 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
                                              uint16_t* sB0, uint16_t* sB1) {
   constexpr int BM = 16 * FM * WGM, BN = 16 * FN * WGN, NW = WGM * WGN;
   static_assert(Prob::kA == RK, "the A operand is k-contiguous");
-  constexpr bool kBdma = Prob::kB == RK;
-  static_assert(kBdma || NW == 4, "the transposing register loader is written for four waves");
+  constexpr bool kBkn = Prob::kB == KN;
   const auto as = prob.a_src(t);
   const auto bs = prob.b_src(t);
   const int nk = prob.ksteps();
@@ -141,16 +164,14 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wm = (wave / WGN) * (16 * FM), wn = (wave % WGN) * (16 * FN);
 
-  Ld<KR, BN> lb;  // register path of a transposed B operand (the quarter transform)
   auto fetch = [&](int k0, uint16_t* a, uint16_t* b) {
     Dma<BM, NW>::issue(as, t.m0, k0, a, a + BM * kLDK, wave, lane);
-    if constexpr (kBdma)
-      Dma<BN, NW>::issue(bs, t.n0, k0, b, b + BN * kLDK, wave, lane);
+    if constexpr (kBkn)
+      DmaKN<BN, NW>::issue(bs, t.n0, k0, b, b + BN * kLDK, wave, lane);
     else
-      lb.load(bs, t.n0, k0);
+      Dma<BN, NW>::issue(bs, t.n0, k0, b, b + BN * kLDK, wave, lane);
   };
   fetch(0, sA0, sB0);
-  if constexpr (!kBdma) lb.store(sB0, sB0 + BN * kLDK);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
 
@@ -161,6 +182,7 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
     for (int j = 0; j < FN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
   const int frow = lane & 15, fk = 8 * (lane >> 4);
+  const TrRead<kBkn ? BN : 64> tr(lane);
   auto step = [&](const int kt, auto stage, auto more) {
     constexpr int cur = decltype(stage)::value;
     constexpr bool kMore = decltype(more)::value;
@@ -178,15 +200,19 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
     }
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
-      const int o = swz(wn + 16 * j + frow, fk >> 3);
-      bh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(b_c + o));
-      bl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(b_c + BN * kLDK + o));
+      if constexpr (kBkn) {
+        bh[j] = tr.read(b_c, wn + 16 * j);
+        bl[j] = tr.read(b_c + BN * kLDK, wn + 16 * j);
+      } else {
+        const int o = swz(wn + 16 * j + frow, fk >> 3);
+        bh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(b_c + o));
+        bl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(b_c + BN * kLDK + o));
+      }
     }
     // While this wave issues its MFMA block it runs at raised priority, so the SIMD's other
     // wave (the second workgroup, usually in its copy/wait phase) does not take the issue
-    // slots: 3-5% on the HT and K^n cores. Not with the transposing register loader, whose
-    // stores after the block it delayed (QT 40% slower).
-    if constexpr (kBdma) __builtin_amdgcn_s_setprio(1);
+    // slots: 3-5% on the HT and K^n cores.
+    __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -195,8 +221,7 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
       }
-    if constexpr (kBdma) __builtin_amdgcn_s_setprio(0);
-    if constexpr (kMore && !kBdma) lb.store(b_n, b_n + BN * kLDK);
+    __builtin_amdgcn_s_setprio(0);
     __builtin_amdgcn_s_waitcnt(0);  // this wave's copies into the next buffers have landed
     __syncthreads();
   };
@@ -321,7 +346,7 @@ struct ProbHT {
 // Tiles are 64 or 128 aux columns wide (FN = 2 or 4); both give the same kpart
 // bits: a 64-column partial is always S(first 32 columns) + S(last 32 columns),
 // S being one wave's sum over two 16-column blocks reduced across 16 lanes.
-struct SrcXT {  // element (k = nu, r = P) at p*nbf*naux_p + min(nu, nbf-1)*naux_p + P
+struct SrcXT {  // element (k = nu, n = P) at p*nbf*naux_p + min(nu, nbf-1)*naux_p + P
   Planes p;
   int64_t base, ld;
   int kmax;
@@ -329,7 +354,7 @@ struct SrcXT {  // element (k = nu, r = P) at p*nbf*naux_p + min(nu, nbf-1)*naux
 };
 
 struct ProbQT {
-  static constexpr int kA = RK, kB = KR;
+  static constexpr int kA = RK, kB = KN;
   Planes ct, x;
   double* kpart;
   double* d;
@@ -431,7 +456,7 @@ struct ProbQT {
 // Lagrangian's exchange operand), bit for bit what a conversion pass over b_MO would give.
 template <bool kPlanes>
 struct ProbMOT {
-  static constexpr int kA = RK, kB = KR;
+  static constexpr int kA = RK, kB = KN;
   Planes ct, x;
   double* out;
   uint16_t* oh;
