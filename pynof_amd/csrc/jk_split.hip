@@ -149,6 +149,8 @@ struct TrRead {
 // arrays, so the compiler sees that fragment reads of one never alias copies
 // into the other), each holding the hi plane then the lo plane. k-step kt+1 is
 // fetched into the other buffer while k-step kt computes; one barrier per k-step.
+// (Three stages in eight-wave workgroups, and in four-wave ones at one workgroup
+// per CU, measured no faster at nbf = 1000.)
 template <int FM, int FN, int WGM, int WGN, class Prob>
 __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, uint16_t* sA0, uint16_t* sA1,
                                              uint16_t* sB0, uint16_t* sB1) {
@@ -171,6 +173,8 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
     else
       Dma<BN, NW>::issue(bs, t.n0, k0, b, b + BN * kLDK, wave, lane);
   };
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
   fetch(0, sA0, sB0);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
@@ -183,20 +187,23 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
 
   const int frow = lane & 15, fk = 8 * (lane >> 4);
   const TrRead<kBkn ? BN : 64> tr(lane);
-  auto step = [&](const int kt, auto stage, auto more) {
+  // MFMA row blocks of this wave that hold a needed output row: with Prob::kRowSkip,
+  // rows from the problem's row limit on (past the diagonal q <= p of the quarter
+  // transform, padding past nbf5) get no fragment reads and no MFMAs. Wave-uniform.
+  const int nact = Prob::kRowSkip ? min(FM, max(0, (prob.row_limit(t) - (t.m0 + wm) + 15) >> 4)) : FM;
+  auto step = [&](const int kt, auto stage) {
     constexpr int cur = decltype(stage)::value;
-    constexpr bool kMore = decltype(more)::value;
     uint16_t* const a_c = cur ? sA1 : sA0;
     uint16_t* const b_c = cur ? sB1 : sB0;
-    uint16_t* const a_n = cur ? sA0 : sA1;
-    uint16_t* const b_n = cur ? sB0 : sB1;
-    if constexpr (kMore) fetch((kt + 1) * kBK, a_n, b_n);
+    if (kt + 1 < nk) fetch((kt + 1) * kBK, cur ? sA0 : sA1, cur ? sB0 : sB1);  // wave-uniform
     bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
-      const int o = swz(wm + 16 * i + frow, fk >> 3);
-      ah[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(a_c + o));
-      al[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(a_c + BM * kLDK + o));
+      if (i < nact) {
+        const int o = swz(wm + 16 * i + frow, fk >> 3);
+        ah[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(a_c + o));
+        al[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(a_c + BM * kLDK + o));
+      }
     }
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
@@ -210,35 +217,26 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
       }
     }
     // While this wave issues its MFMA block it runs at raised priority, so the SIMD's other
-    // wave (the second workgroup, usually in its copy/wait phase) does not take the issue
-    // slots: 3-5% on the HT and K^n cores.
+    // wave (usually in its copy/wait phase) does not take the issue slots: 3-5% on the HT
+    // and K^n cores.
     __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int i = 0; i < FM; ++i)
+      if (i < nact) {
 #pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < FN; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        }
       }
     __builtin_amdgcn_s_setprio(0);
     __builtin_amdgcn_s_waitcnt(0);  // this wave's copies into the next buffers have landed
     __syncthreads();
   };
-  using S0 = std::integral_constant<int, 0>;
-  using S1 = std::integral_constant<int, 1>;
-  using Y = std::true_type;
-  using N = std::false_type;
-  int kt = 0;
-  for (; kt + 2 < nk; kt += 2) {
-    step(kt, S0{}, Y{});
-    step(kt + 1, S1{}, Y{});
-  }
-  if (nk - kt == 2) {  // kt is even: one or two k-steps remain
-    step(kt, S0{}, Y{});
-    step(kt + 1, S1{}, N{});
-  } else {
-    step(kt, S0{}, N{});
+  for (int kt = 0; kt < nk; kt += 2) {
+    step(kt, S0{});
+    if (kt + 1 < nk) step(kt + 1, S1{});
   }
   prob.template epilogue<FM, FN>(t, acc, wm, wn, lane, reinterpret_cast<double*>(sA0));
 }
@@ -303,6 +301,7 @@ __device__ __forceinline__ void coords(int lane, int wm, int wn, int i, int j, i
 // X16[q][n] = split(sum_mu Ct[q][mu] bht[n][mu]),  n = nu*naux_p + P
 struct ProbHT {
   static constexpr int kA = RK, kB = RK;
+  static constexpr bool kRowSkip = false;
   Planes ct, bht;
   uint16_t* xhi;
   uint16_t* xlo;
@@ -317,6 +316,7 @@ struct ProbHT {
     return true;
   }
   __device__ int ksteps() const { return ldk / kBK; }
+  __device__ int row_limit(const Tile&) const { return nbf5; }
   __device__ SrcRows a_src(const Tile&) const { return {ct, 0, ldk, nbf5}; }
   __device__ SrcRows b_src(const Tile&) const { return {bht, 0, ldk, (int)N}; }
   template <int FM, int FN>
@@ -355,6 +355,7 @@ struct SrcXT {  // element (k = nu, n = P) at p*nbf*naux_p + min(nu, nbf-1)*naux
 
 struct ProbQT {
   static constexpr int kA = RK, kB = KN;
+  static constexpr bool kRowSkip = true;
   Planes ct, x;
   double* kpart;
   double* d;
@@ -371,6 +372,7 @@ struct ProbQT {
     return t.m0 <= t.batch;
   }
   __device__ int ksteps() const { return ldk / kBK; }
+  __device__ int row_limit(const Tile& t) const { return t.batch + 1; }  // q <= p
   __device__ SrcRows a_src(const Tile&) const { return {ct, 0, ldk, nbf5}; }
   __device__ SrcXT b_src(const Tile& t) const { return {x, (int64_t)t.batch * nbf * naux_p, naux_p, nbf}; }
   template <int FM, int FN>
@@ -457,6 +459,7 @@ struct ProbQT {
 template <bool kPlanes>
 struct ProbMOT {
   static constexpr int kA = RK, kB = KN;
+  static constexpr bool kRowSkip = false;
   Planes ct, x;
   double* out;
   uint16_t* oh;
@@ -473,6 +476,7 @@ struct ProbMOT {
     return true;
   }
   __device__ int ksteps() const { return ldk / kBK; }
+  __device__ int row_limit(const Tile&) const { return nbf; }
   __device__ SrcRows a_src(const Tile&) const { return {ct, 0, ldk, nbf}; }
   __device__ SrcXT b_src(const Tile& t) const { return {x, (int64_t)t.batch * nbf * naux_p, naux_p, nbf}; }
   template <int FM, int FN>
@@ -506,6 +510,7 @@ struct ProbMOT {
 // W = ck[b][q] B[b][k'], both operands rows of stride ldrow (zero past nbf5 * naux).
 struct ProbLagX {
   static constexpr int kA = RK, kB = RK;
+  static constexpr bool kRowSkip = false;
   Planes bm, w;
   double* part;
   int64_t ldrow, kc;
@@ -520,6 +525,7 @@ struct ProbLagX {
     return true;
   }
   __device__ int ksteps() const { return (int)(kc / kBK); }
+  __device__ int row_limit(const Tile&) const { return nbf; }
   __device__ SrcRows a_src(const Tile& t) const { return {bm, (int64_t)t.batch * kc, ldrow, nbf}; }
   __device__ SrcRows b_src(const Tile& t) const { return {w, (int64_t)t.batch * kc, ldrow, nbf5}; }
   template <int FM, int FN>
@@ -542,6 +548,7 @@ struct ProbLagX {
 // upper triangle is filled afterwards by launch_symmetrize)
 struct ProbJN {
   static constexpr int kA = RK, kB = RK;
+  static constexpr bool kRowSkip = true;
   Planes dd, bt;
   double* J;
   int nbf, naux_p, nbf5, mtiles, m_base;  // rows [m_base, m_base + mtiles*BM) of this launch
@@ -555,6 +562,7 @@ struct ProbJN {
     return true;
   }
   __device__ int ksteps() const { return naux_p / kBK; }
+  __device__ int row_limit(const Tile&) const { return nbf5; }
   __device__ SrcRows a_src(const Tile&) const { return {dd, 0, naux_p, nbf5}; }
   __device__ SrcRows b_src(const Tile&) const { return {bt, 0, naux_p, (int)ntri}; }
   template <int FM, int FN>
@@ -586,6 +594,7 @@ struct ProbJN {
 // output; the strict upper triangle is filled afterwards by launch_symmetrize)
 struct ProbSYRK {
   static constexpr int kA = RK, kB = RK;
+  static constexpr bool kRowSkip = false;
   Planes x;
   double* O;
   int64_t lda, sa, ldo, so;
@@ -603,6 +612,7 @@ struct ProbSYRK {
     return true;
   }
   __device__ int ksteps() const { return K / kBK; }
+  __device__ int row_limit(const Tile&) const { return M; }
   __device__ SrcRows a_src(const Tile& t) const { return {x, (int64_t)t.batch * sa, lda, M}; }
   __device__ SrcRows b_src(const Tile& t) const { return {x, (int64_t)t.batch * sa, lda, M}; }
   template <int FM, int FN>
