@@ -1285,10 +1285,9 @@ enum { GS_HT = 0, GS_QT, GS_JMO, GS_JN, GS_K, GS_COUNT };
 int launch_plan(pnjk_group* g, int stage, hipStream_t s) {
   const char* base = static_cast<const char*>(g->plan_dev.p);
   for (const GroupBatch& b : g->plan[stage].batches) {
-    const int nt = (int)b.tiles.size();
-    if (nt == 0) continue;
-    const int grid = std::min(nt, 1024);
-    PNJK_HIP_CHECK(b.launch(base + b.probs_off, reinterpret_cast<const int2*>(base + b.tiles_off), nt, grid, s));
+    if (b.tiles.empty()) continue;
+    PNJK_HIP_CHECK(b.launch(base + b.probs_off, reinterpret_cast<const int2*>(base + b.tiles_off),
+                            reinterpret_cast<const int*>(base + b.bounds_off), b.grid, s));
   }
   return PNJK_OK;
 }
@@ -1420,6 +1419,8 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
         bytes = roundup((int64_t)(bytes + b.probs.size()), 256);
         b.tiles_off = bytes;
         bytes = roundup((int64_t)(bytes + b.tiles.size() * sizeof(int2)), 256);
+        b.bounds_off = bytes;
+        bytes = roundup((int64_t)(bytes + (kXcds + 1) * sizeof(int)), 256);
       }
     PNJK_HIP_CHECK(hipStreamSynchronize(s));  // the previous plan may still be in use
     if ((rc = g->plan_dev.ensure(std::max<size_t>(bytes, 256)))) return rc;
@@ -1429,6 +1430,11 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
         PNJK_HIP_CHECK(hipMemcpy(base + b.probs_off, b.probs.data(), b.probs.size(), hipMemcpyHostToDevice));
         PNJK_HIP_CHECK(hipMemcpy(base + b.tiles_off, b.tiles.data(), b.tiles.size() * sizeof(int2),
                                  hipMemcpyHostToDevice));
+        const std::vector<int> xb = b.xcd_bounds();
+        int longest = 0;
+        for (int x = 0; x < kXcds; ++x) longest = std::max(longest, xb[x + 1] - xb[x]);
+        b.grid = kXcds * longest;
+        PNJK_HIP_CHECK(hipMemcpy(base + b.bounds_off, xb.data(), xb.size() * sizeof(int), hipMemcpyHostToDevice));
       }
     g->sig = sig;
     g->graph_valid = false;

@@ -186,37 +186,28 @@ __global__ void __launch_bounds__(kThreads) k_gemm_f64(const Prob prob) {
   gemm_tile_f64<FM, FN, BK>(prob, t, sA, sB);
 }
 
-// Grouped persistent form: one launch over the tiles of many instances of the
-// same problem type. tiles[i] = (slot of the instance's problem in probs, local
-// tile id); each workgroup walks the table with stride gridDim.x.
+// Grouped form: one launch over the tiles of many instances of the same problem
+// type, one workgroup per tile; tiles[i] = (slot of the instance's problem in
+// probs, local tile id). Workgroup b runs on XCD x = b % 8 and takes entry b / 8 of
+// that XCD's run [xcd_bounds[x], xcd_bounds[x + 1]) of the table; the grid covers
+// the longest run and the surplus workgroups of shorter runs exit at once.
 template <int FM, int FN, int BK, class Prob>
 __global__ void __launch_bounds__(kThreads)
-    k_gemm_f64_grouped(const Prob* __restrict__ probs, const int2* __restrict__ tiles, int ntiles) {
+    k_gemm_f64_grouped(const Prob* __restrict__ probs, const int2* __restrict__ tiles,
+                       const int* __restrict__ xcd_bounds) {
   constexpr int BM = 32 * FM, BN = 32 * FN;
   __shared__ __align__(16) double sA[2 * BK * (BM + kPad)];
   __shared__ __align__(16) double sB[2 * BK * (BN + kPad)];
-  // Workgroup b runs on XCD b % 8: give each XCD one contiguous run of the
-  // table, so tiles that share an operand (adjacent ids) stay in one L2.
-  // (With fewer than 8 workgroups, every workgroup strides the whole table.)
-  constexpr int kXcd = 8;
-  const bool split = gridDim.x >= kXcd;
-  const int x = blockIdx.x % kXcd;
-  const int wi = split ? blockIdx.x / kXcd : blockIdx.x;
-  const int nw = split ? (gridDim.x - x + kXcd - 1) / kXcd : gridDim.x;  // workgroups sharing the range
-  const int per = ntiles / kXcd, rem = ntiles % kXcd;
-  const int lo = split ? x * per + (x < rem ? x : rem) : 0;
-  const int hi = split ? lo + per + (x < rem ? 1 : 0) : ntiles;
-  for (int id = lo + wi; id < hi; id += nw) {
-    // The entry is uniform over the workgroup: readfirstlane lets the problem
-    // struct come in through scalar loads.
-    const int2 e = tiles[id];
-    const int slot = __builtin_amdgcn_readfirstlane(e.x), local = __builtin_amdgcn_readfirstlane(e.y);
-    const Prob prob = probs[slot];
-    const int2 tr = make_int2(slot, local);
-    Tile t;
-    if (prob.tile(tr.y, BM, BN, t)) gemm_tile_f64<FM, FN, BK>(prob, t, sA, sB);
-    __syncthreads();
-  }
+  const int x = blockIdx.x % kXcds;
+  const int id = xcd_bounds[x] + (int)(blockIdx.x / kXcds);
+  if (id >= xcd_bounds[x + 1]) return;
+  // The entry is uniform over the workgroup: readfirstlane lets the problem
+  // struct come in through scalar loads.
+  const int2 e = tiles[id];
+  const int slot = __builtin_amdgcn_readfirstlane(e.x), local = __builtin_amdgcn_readfirstlane(e.y);
+  const Prob prob = probs[slot];
+  Tile t;
+  if (prob.tile(local, BM, BN, t)) gemm_tile_f64<FM, FN, BK>(prob, t, sA, sB);
 }
 
 // ---------------------------------------------------------------- sources
@@ -286,7 +277,7 @@ struct ProbHT {
     return true;
   }
   __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf5; }
-  __device__ int ksteps(int BK) const { return nbf_p / BK; }
+  __host__ __device__ int ksteps(int BK) const { return nbf_p / BK; }
   __device__ SrcKMDense a_src(const Tile&) const { return {Cp, ldcp}; }
   __device__ SrcKMBtri b_src(const Tile&) const { return {btri, nbf, naux_p}; }
   template <int FM, int FN>
@@ -332,7 +323,7 @@ struct ProbQT {
   }
   // only q <= p is needed (Y_p[q] = Y_q[p])
   __device__ bool block_active(const Tile& t, int r0, int) const { return r0 <= t.batch && r0 < nbf5; }
-  __device__ int ksteps(int BK) const { return nbf_p / BK; }
+  __host__ __device__ int ksteps(int BK) const { return nbf_p / BK; }
   __device__ SrcKMDense a_src(const Tile&) const { return {Cp, ldcp}; }
   __device__ SrcKMRows b_src(const Tile& t) const {
     return {X + (int64_t)t.batch * nbf * naux_p, naux_p, nbf};
@@ -434,7 +425,7 @@ struct ProbSYRK {
   __device__ bool block_active(const Tile& t, int r0, int c0) const {
     return r0 < M && c0 < M && !(t.mt == t.nt && r0 + 16 <= c0);
   }
-  __device__ int ksteps(int BK) const { return K / BK; }
+  __host__ __device__ int ksteps(int BK) const { return K / BK; }
   __device__ SrcMKDense a_src(const Tile& t) const { return {A + (int64_t)t.batch * sa, lda, M}; }
   __device__ SrcMKDense b_src(const Tile& t) const { return {A + (int64_t)t.batch * sa, lda, M}; }
   template <int FM, int FN>
@@ -470,7 +461,7 @@ struct ProbJN {
     return true;
   }
   __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf5; }
-  __device__ int ksteps(int BK) const { return naux_p / BK; }
+  __host__ __device__ int ksteps(int BK) const { return naux_p / BK; }
   __device__ SrcMKDense a_src(const Tile&) const { return {d, naux_p, nbf5}; }
   __device__ SrcMKDense b_src(const Tile&) const { return {btri, naux_p, (int)(ntri < (1 << 30) ? ntri : (1 << 30))}; }
   template <int FM, int FN>
@@ -511,7 +502,7 @@ struct ProbGemmT {  // T[mu][i] = sum_nu Hp[mu][nu] Cp[nu][i]
     return true;
   }
   __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf; }
-  __device__ int ksteps(int BK) const { return nbf_p / BK; }
+  __host__ __device__ int ksteps(int BK) const { return nbf_p / BK; }
   __device__ SrcMKDense a_src(const Tile&) const { return {Hp, ldh, nbf}; }
   __device__ SrcKMDense b_src(const Tile&) const { return {Cp, ldcp}; }
   template <int FM, int FN>
@@ -551,7 +542,7 @@ struct ProbMO {
     return true;
   }
   __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf; }
-  __device__ int ksteps(int BK) const { return nbf_p / BK; }
+  __host__ __device__ int ksteps(int BK) const { return nbf_p / BK; }
   __device__ SrcKMDense a_src(const Tile&) const { return {Cf, ldcf}; }
   __device__ SrcKMRows b_src(const Tile& t) const { return {X + (int64_t)t.batch * nbf * naux_p, naux_p, nbf}; }
   template <int FM, int FN>
@@ -587,7 +578,7 @@ struct ProbHM {
     return true;
   }
   __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf; }
-  __device__ int ksteps(int BK) const { return nbf_p / BK; }
+  __host__ __device__ int ksteps(int BK) const { return nbf_p / BK; }
   __device__ SrcKMDense a_src(const Tile&) const { return {Cf, ldcf}; }
   __device__ SrcKMRows b_src(const Tile&) const { return {T, ldcp, nbf}; }
   template <int FM, int FN>
@@ -627,7 +618,7 @@ struct ProbLagX {
     return true;
   }
   __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf; }
-  __device__ int ksteps(int BK) const { return (int)(kc / BK); }
+  __host__ __device__ int ksteps(int BK) const { return (int)(kc / BK); }
   __device__ SrcMKDense a_src(const Tile& t) const { return {B + (int64_t)t.batch * kc, ldrow, nbf}; }
   __device__ SrcMKDense b_src(const Tile& t) const { return {W + (int64_t)t.batch * kc, ldrow, nbf5}; }
   template <int FM, int FN>
@@ -682,7 +673,7 @@ struct ProbFock {
     return true;
   }
   __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf5; }
-  __device__ int ksteps(int BK) const { return kp / BK; }
+  __host__ __device__ int ksteps(int BK) const { return kp / BK; }
   __device__ SrcMKDense a_src(const Tile&) const { return {A, kp, nbf5}; }
   __device__ SrcKMJK b_src(const Tile&) const { return {J, K, ld, nbf5}; }
   template <int FM, int FN>
@@ -720,7 +711,7 @@ struct ProbRowGemm {
     return true;
   }
   __device__ bool block_active(const Tile&, int r0, int) const { return r0 < rows; }
-  __device__ int ksteps(int BK) const { return K / BK; }
+  __host__ __device__ int ksteps(int BK) const { return K / BK; }
   __device__ SrcMKDense a_src(const Tile&) const { return {A, K, rows}; }
   __device__ SrcKMDense b_src(const Tile&) const { return {W, K}; }
   template <int FM, int FN>
@@ -756,7 +747,7 @@ struct ProbAtB {
     return true;
   }
   __device__ bool block_active(const Tile&, int r0, int) const { return r0 < K; }
-  __device__ int ksteps(int BK) const { return K / BK; }
+  __host__ __device__ int ksteps(int BK) const { return K / BK; }
   __device__ SrcKMDense a_src(const Tile&) const { return {A, K}; }
   __device__ SrcKMDense b_src(const Tile&) const { return {B, K}; }
   template <int FM, int FN>
@@ -864,9 +855,9 @@ __global__ void k_symmetrize(double* __restrict__ O, int n, int64_t ld, int64_t 
 }
 
 template <int FM, int FN, int BK, class Prob>
-hipError_t grouped_launch(const void* probs, const int2* tiles, int ntiles, int grid, hipStream_t s) {
+hipError_t grouped_launch(const void* probs, const int2* tiles, const int* xcd_bounds, int grid, hipStream_t s) {
   hipLaunchKernelGGL((k_gemm_f64_grouped<FM, FN, BK, Prob>), dim3(grid), dim3(kThreads), 0, s,
-                     static_cast<const Prob*>(probs), tiles, ntiles);
+                     static_cast<const Prob*>(probs), tiles, xcd_bounds);
   return hipGetLastError();
 }
 
@@ -879,7 +870,7 @@ hipError_t launch_prob(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s,
 #define PNJK_CASE(FMV, FNV, BKV)                                                                            \
   if (tc.fm == FMV && tc.fn == FNV) {                                                                       \
     if (col) {                                                                                              \
-      col->add(prob, tc, nblocks, 32 * FMV, 32 * FNV, &grouped_launch<FMV, FNV, BKV, Prob>,                 \
+      col->add(prob, tc, nblocks, 32 * FMV, 32 * FNV, prob.ksteps(BKV), &grouped_launch<FMV, FNV, BKV, Prob>, \
                __PRETTY_FUNCTION__);                                                                        \
       return hipSuccess;                                                                                    \
     }                                                                                                       \

@@ -13,23 +13,48 @@ namespace pnjk {
 
 // ---------------------------------------------------------------- grouped launches
 // A grouped launch runs the tiles of many instances of one problem type and
-// tile shape in one persistent kernel. Launch functions given a collector do
-// not launch: they append the instance's problem struct and its valid tile ids.
-typedef hipError_t (*GroupedFn)(const void* probs, const int2* tiles, int ntiles, int grid, hipStream_t s);
+// tile shape in one kernel, one workgroup per tile. Launch functions given a
+// collector do not launch: they append the instance's problem struct and its valid
+// tile ids. Workgroup b of a grouped launch runs on XCD b % 8 and takes entry b / 8
+// of that XCD's range [xcd_bounds[b % 8], xcd_bounds[b % 8 + 1]) of the table: one
+// contiguous run per XCD (tiles that share an operand stay in one L2), the runs cut
+// at equal k-step totals.
+constexpr int kXcds = 8;
+typedef hipError_t (*GroupedFn)(const void* probs, const int2* tiles, const int* xcd_bounds, int grid,
+                                hipStream_t s);
 
 struct GroupBatch {
   std::string key;        // problem type and tile shape
   size_t prob_size = 0;
   std::vector<char> probs;  // packed problem structs, one per instance slot
   std::vector<int2> tiles;  // (slot, local tile id)
+  std::vector<int> ksteps;  // k-steps of each tile: its cost
   GroupedFn launch = nullptr;
-  size_t probs_off = 0, tiles_off = 0;  // offsets of the device copies in the plan buffer
+  size_t probs_off = 0, tiles_off = 0, bounds_off = 0;  // offsets of the device copies in the plan buffer
+  int grid = 0;                                         // workgroups of the launch
+
+  // kXcds + 1 cut points of the table with equal k-step totals between them; the
+  // launch grid is kXcds times the longest run.
+  std::vector<int> xcd_bounds() const {
+    std::vector<int> b(kXcds + 1, 0);
+    int64_t total = 0;
+    for (int k : ksteps) total += k;
+    int64_t run = 0;
+    size_t i = 0;
+    for (int x = 1; x < kXcds; ++x) {
+      const int64_t target = total * x / kXcds;
+      while (i < ksteps.size() && run + ksteps[i] <= target) run += ksteps[i++];
+      b[x] = (int)i;
+    }
+    b[kXcds] = (int)tiles.size();
+    return b;
+  }
 };
 
 struct GroupCollector {
   std::vector<GroupBatch> batches;
   template <class Prob>
-  void add(const Prob& pr, TileF64 tc, int nblocks, int BM, int BN, GroupedFn fn, const char* type) {
+  void add(const Prob& pr, TileF64 tc, int nblocks, int BM, int BN, int ksteps, GroupedFn fn, const char* type) {
     const std::string key = std::string(type) + "/" + std::to_string(tc.fm) + "x" + std::to_string(tc.fn);
     GroupBatch* b = nullptr;
     for (GroupBatch& x : batches)
@@ -46,7 +71,10 @@ struct GroupCollector {
     b->probs.insert(b->probs.end(), raw, raw + sizeof(Prob));
     for (int id = 0; id < nblocks; ++id) {
       Tile t;
-      if (pr.tile(id, BM, BN, t)) b->tiles.push_back(make_int2(slot, id));
+      if (pr.tile(id, BM, BN, t)) {
+        b->tiles.push_back(make_int2(slot, id));
+        b->ksteps.push_back(ksteps);
+      }
     }
   }
 };

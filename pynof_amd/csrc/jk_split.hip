@@ -253,35 +253,27 @@ __global__ void __launch_bounds__(64 * WGM * WGN, 2) k_gemm_x3(const Prob prob) 
   gemm_tile_x3<FM, FN, WGM, WGN>(prob, t, sA0, sA1, sB0, sB1);
 }
 
-// Grouped persistent form over a (slot, local tile) table; see jk_kernels.h.
+// Grouped form over a (slot, local tile) table (see jk_kernels.h): one workgroup
+// per tile. Workgroup b runs on XCD x = b % 8 and takes entry b / 8 of that XCD's
+// run [xcd_bounds[x], xcd_bounds[x + 1]) of the table; the grid covers the longest
+// run and the surplus workgroups of shorter runs exit at once.
 template <int FM, int FN, int WGM, int WGN, class Prob>
 __global__ void __launch_bounds__(64 * WGM * WGN, 2)
-    k_gemm_x3_grouped(const Prob* __restrict__ probs, const int2* __restrict__ tiles, int ntiles) {
+    k_gemm_x3_grouped(const Prob* __restrict__ probs, const int2* __restrict__ tiles,
+                      const int* __restrict__ xcd_bounds) {
   constexpr int BM = 16 * FM * WGM, BN = 16 * FN * WGN;
   __shared__ __align__(16) uint16_t sA0[2 * BM * kLDK], sA1[2 * BM * kLDK];
   __shared__ __align__(16) uint16_t sB0[2 * BN * kLDK], sB1[2 * BN * kLDK];
-  // Workgroup b runs on XCD b % 8: give each XCD one contiguous run of the
-  // table, so tiles that share an operand (adjacent ids) stay in one L2.
-  // (With fewer than 8 workgroups, every workgroup strides the whole table.)
-  constexpr int kXcd = 8;
-  const bool split = gridDim.x >= kXcd;
-  const int x = blockIdx.x % kXcd;
-  const int wi = split ? blockIdx.x / kXcd : blockIdx.x;
-  const int nw = split ? (gridDim.x - x + kXcd - 1) / kXcd : gridDim.x;  // workgroups sharing the range
-  const int per = ntiles / kXcd, rem = ntiles % kXcd;
-  const int lo = split ? x * per + (x < rem ? x : rem) : 0;
-  const int hi = split ? lo + per + (x < rem ? 1 : 0) : ntiles;
-  for (int id = lo + wi; id < hi; id += nw) {
-    // The entry is uniform over the workgroup: readfirstlane lets the problem
-    // struct come in through scalar loads.
-    const int2 e = tiles[id];
-    const int slot = __builtin_amdgcn_readfirstlane(e.x), local = __builtin_amdgcn_readfirstlane(e.y);
-    const Prob prob = probs[slot];
-    const int2 tr = make_int2(slot, local);
-    Tile t;
-    if (prob.tile(tr.y, BM, BN, t)) gemm_tile_x3<FM, FN, WGM, WGN>(prob, t, sA0, sA1, sB0, sB1);
-    __syncthreads();
-  }
+  const int x = blockIdx.x % kXcds;
+  const int id = xcd_bounds[x] + (int)(blockIdx.x / kXcds);
+  if (id >= xcd_bounds[x + 1]) return;
+  // The entry is uniform over the workgroup: readfirstlane lets the problem
+  // struct come in through scalar loads.
+  const int2 e = tiles[id];
+  const int slot = __builtin_amdgcn_readfirstlane(e.x), local = __builtin_amdgcn_readfirstlane(e.y);
+  const Prob prob = probs[slot];
+  Tile t;
+  if (prob.tile(local, BM, BN, t)) gemm_tile_x3<FM, FN, WGM, WGN>(prob, t, sA0, sA1, sB0, sB1);
 }
 
 // ---------------------------------------------------------------- sources
@@ -315,7 +307,7 @@ struct ProbHT {
     t.n0 = t.nt * BN;
     return true;
   }
-  __device__ int ksteps() const { return ldk / kBK; }
+  __host__ __device__ int ksteps() const { return ldk / kBK; }
   __device__ int row_limit(const Tile&) const { return nbf5; }
   __device__ SrcRows a_src(const Tile&) const { return {ct, 0, ldk, nbf5}; }
   __device__ SrcRows b_src(const Tile&) const { return {bht, 0, ldk, (int)N}; }
@@ -371,7 +363,7 @@ struct ProbQT {
     t.n0 = t.nt * BN;
     return t.m0 <= t.batch;
   }
-  __device__ int ksteps() const { return ldk / kBK; }
+  __host__ __device__ int ksteps() const { return ldk / kBK; }
   __device__ int row_limit(const Tile& t) const { return t.batch + 1; }  // q <= p
   __device__ SrcRows a_src(const Tile&) const { return {ct, 0, ldk, nbf5}; }
   __device__ SrcXT b_src(const Tile& t) const { return {x, (int64_t)t.batch * nbf * naux_p, naux_p, nbf}; }
@@ -475,7 +467,7 @@ struct ProbMOT {
     t.n0 = t.nt * BN;
     return true;
   }
-  __device__ int ksteps() const { return ldk / kBK; }
+  __host__ __device__ int ksteps() const { return ldk / kBK; }
   __device__ int row_limit(const Tile&) const { return nbf; }
   __device__ SrcRows a_src(const Tile&) const { return {ct, 0, ldk, nbf}; }
   __device__ SrcXT b_src(const Tile& t) const { return {x, (int64_t)t.batch * nbf * naux_p, naux_p, nbf}; }
@@ -524,7 +516,7 @@ struct ProbLagX {
     t.n0 = t.nt * BN;
     return true;
   }
-  __device__ int ksteps() const { return (int)(kc / kBK); }
+  __host__ __device__ int ksteps() const { return (int)(kc / kBK); }
   __device__ int row_limit(const Tile&) const { return nbf; }
   __device__ SrcRows a_src(const Tile& t) const { return {bm, (int64_t)t.batch * kc, ldrow, nbf}; }
   __device__ SrcRows b_src(const Tile& t) const { return {w, (int64_t)t.batch * kc, ldrow, nbf5}; }
@@ -561,7 +553,7 @@ struct ProbJN {
     t.n0 = t.nt * BN;
     return true;
   }
-  __device__ int ksteps() const { return naux_p / kBK; }
+  __host__ __device__ int ksteps() const { return naux_p / kBK; }
   __device__ int row_limit(const Tile&) const { return nbf5; }
   __device__ SrcRows a_src(const Tile&) const { return {dd, 0, naux_p, nbf5}; }
   __device__ SrcRows b_src(const Tile&) const { return {bt, 0, naux_p, (int)ntri}; }
@@ -611,7 +603,7 @@ struct ProbSYRK {
     t.n0 = t.nt * BN;
     return true;
   }
-  __device__ int ksteps() const { return K / kBK; }
+  __host__ __device__ int ksteps() const { return K / kBK; }
   __device__ int row_limit(const Tile&) const { return M; }
   __device__ SrcRows a_src(const Tile& t) const { return {x, (int64_t)t.batch * sa, lda, M}; }
   __device__ SrcRows b_src(const Tile& t) const { return {x, (int64_t)t.batch * sa, lda, M}; }
@@ -686,9 +678,9 @@ __global__ void k_split_ct(const double* __restrict__ Cp, int ldcp, int nbf, int
 }
 
 template <int FM, int FN, int WGM, int WGN, class Prob>
-hipError_t grouped_launch(const void* probs, const int2* tiles, int ntiles, int grid, hipStream_t s) {
+hipError_t grouped_launch(const void* probs, const int2* tiles, const int* xcd_bounds, int grid, hipStream_t s) {
   hipLaunchKernelGGL((k_gemm_x3_grouped<FM, FN, WGM, WGN, Prob>), dim3(grid), dim3(64 * WGM * WGN), 0, s,
-                     static_cast<const Prob*>(probs), tiles, ntiles);
+                     static_cast<const Prob*>(probs), tiles, xcd_bounds);
   return hipGetLastError();
 }
 
@@ -703,7 +695,7 @@ hipError_t launch_x3(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s, G
 #define PNJK_CASE(FMV, FNV, WFM, WFN, WGMV, WGNV)                                                          \
   if (tc.fm == FMV && tc.fn == FNV) {                                                                      \
     if (col) {                                                                                             \
-      col->add(prob, tc, nblocks, 32 * FMV, 32 * FNV, &grouped_launch<WFM, WFN, WGMV, WGNV, Prob>,        \
+      col->add(prob, tc, nblocks, 32 * FMV, 32 * FNV, prob.ksteps(), &grouped_launch<WFM, WFN, WGMV, WGNV, Prob>, \
                __PRETTY_FUNCTION__);                                                                       \
       return hipSuccess;                                                                                   \
     }                                                                                                      \
