@@ -1266,6 +1266,9 @@ struct pnjk_group {
   std::vector<BuildState> st;
   std::vector<char> sig;                // signature of the cached plan
   std::vector<GroupCollector> plan;     // grouped stages: HT, QT, J_MO, J^n, K^n
+  std::vector<SymJob> sym[2];           // symmetrize jobs of the J^n and K^n outputs
+  size_t sym_off[2] = {0, 0};           // their offsets in the plan buffer
+  int64_t sym_grid[2] = {0, 0};
   DevBuf plan_dev;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1304,12 +1307,13 @@ int group_enqueue(pnjk_group* g, hipStream_t s) {
   if ((rc = launch_plan(g, GS_JMO, s))) return rc;
   for (size_t i = 0; i < n; ++i)
     if ((rc = stage_jmo_post(g->eng[i], g->st[i], s))) return rc;
+  const char* base = static_cast<const char*>(g->plan_dev.p);
   if ((rc = launch_plan(g, GS_JN, s))) return rc;
-  for (size_t i = 0; i < n; ++i)
-    if (g->st[i].want_j && (rc = stage_sym(g->st[i], g->st[i].dJ, s))) return rc;
+  PNJK_HIP_CHECK(launch_symmetrize_grouped(reinterpret_cast<const SymJob*>(base + g->sym_off[0]),
+                                           (int)g->sym[0].size(), g->sym_grid[0], s));
   if ((rc = launch_plan(g, GS_K, s))) return rc;
-  for (size_t i = 0; i < n; ++i)
-    if (g->st[i].want_k && (rc = stage_sym(g->st[i], g->st[i].dK, s))) return rc;
+  PNJK_HIP_CHECK(launch_symmetrize_grouped(reinterpret_cast<const SymJob*>(base + g->sym_off[1]),
+                                           (int)g->sym[1].size(), g->sym_grid[1], s));
   for (size_t i = 0; i < n; ++i)
     if ((rc = stage_hcore(g->eng[i], g->st[i], s))) return rc;
   return PNJK_OK;
@@ -1396,7 +1400,7 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
   if (sig != g->sig) {
     // New plan: collect every instance's tiles per grouped stage, upload once.
     for (GroupCollector& c : g->plan) c.batches.clear();
-    // Largest instances first: the persistent walk then deals big tiles before
+    // Largest instances first: each XCD's run then dispatches big tiles before
     // small ones (longest-first balance); an instance's tiles stay adjacent.
     std::vector<size_t> order(n);
     for (size_t i = 0; i < n; ++i) order[i] = i;
@@ -1412,7 +1416,24 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
           (rc = stage_k(e, st, s, &g->plan[GS_K])))
         return rc;
     }
+    // One symmetrize launch per output kind over every instance's matrices.
+    for (int k = 0; k < 2; ++k) {
+      g->sym[k].clear();
+      int64_t first = 0;
+      for (size_t i = 0; i < n; ++i) {
+        const BuildState& st = g->st[i];
+        if (!(k == 0 ? st.want_j : st.want_k) || st.g.nbf <= 1) continue;
+        const int64_t nbf = st.g.nbf;
+        g->sym[k].push_back(SymJob{k == 0 ? st.dJ : st.dK, nbf, nbf * nbf, first, (int)nbf, st.g.nbf5});
+        first += (int64_t)st.g.nbf5 * sym_blocks((int)nbf);
+      }
+      g->sym_grid[k] = first;
+    }
     size_t bytes = 0;
+    for (int k = 0; k < 2; ++k) {
+      g->sym_off[k] = bytes;
+      bytes = roundup((int64_t)(bytes + g->sym[k].size() * sizeof(SymJob)), 256);
+    }
     for (GroupCollector& c : g->plan)
       for (GroupBatch& b : c.batches) {
         b.probs_off = bytes;
@@ -1425,6 +1446,10 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
     PNJK_HIP_CHECK(hipStreamSynchronize(s));  // the previous plan may still be in use
     if ((rc = g->plan_dev.ensure(std::max<size_t>(bytes, 256)))) return rc;
     char* base = static_cast<char*>(g->plan_dev.p);
+    for (int k = 0; k < 2; ++k)
+      if (!g->sym[k].empty())
+        PNJK_HIP_CHECK(hipMemcpy(base + g->sym_off[k], g->sym[k].data(), g->sym[k].size() * sizeof(SymJob),
+                                 hipMemcpyHostToDevice));
     for (GroupCollector& c : g->plan)
       for (GroupBatch& b : c.batches) {
         PNJK_HIP_CHECK(hipMemcpy(base + b.probs_off, b.probs.data(), b.probs.size(), hipMemcpyHostToDevice));

@@ -854,6 +854,39 @@ __global__ void k_symmetrize(double* __restrict__ O, int n, int64_t ld, int64_t 
   }
 }
 
+// k_symmetrize over a table of jobs (one batch of n x n matrices each): workgroup b
+// finds its job by binary search on the jobs' first workgroups.
+__global__ void k_symmetrize_grouped(const SymJob* __restrict__ jobs, int njobs) {
+  __shared__ double tile[32][33];
+  const int64_t b = blockIdx.x;
+  int lo = 0, hi = njobs - 1;  // last job with first <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].first <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const SymJob jb = jobs[lo];
+  const int64_t pairs = sym_blocks(jb.n);
+  const int64_t local = b - jb.first;
+  const int64_t m = local / pairs, pair = local - m * pairs;
+  if (m >= jb.count) return;
+  const int ti = tri_row(pair);
+  const int tj = (int)(pair - (int64_t)ti * (ti + 1) / 2);
+  double* o = jb.O + m * jb.stride;
+  const int n = jb.n;
+  for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
+    const int r = e >> 5, c = e & 31;
+    const int i = ti * 32 + r, j = tj * 32 + c;
+    tile[r][c] = (i < n && j < n) ? o[(int64_t)i * jb.ld + j] : 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
+    const int r = e >> 5, c = e & 31;
+    const int j = tj * 32 + r, i = ti * 32 + c;
+    if (i < n && j < n && i > j) o[(int64_t)j * jb.ld + i] = tile[c][r];
+  }
+}
+
 template <int FM, int FN, int BK, class Prob>
 hipError_t grouped_launch(const void* probs, const int2* tiles, const int* xcd_bounds, int grid, hipStream_t s) {
   hipLaunchKernelGGL((k_gemm_f64_grouped<FM, FN, BK, Prob>), dim3(grid), dim3(kThreads), 0, s,
@@ -946,6 +979,13 @@ hipError_t launch_symmetrize(int batch, int n, double* O, int64_t ld, int64_t st
   const int64_t pairs = t * (t + 1) / 2;
   if (batch > 65535 || pairs > 2147483647LL) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_symmetrize, dim3((unsigned)pairs, (unsigned)batch), dim3(256), 0, s, O, n, ld, stride);
+  return hipGetLastError();
+}
+
+hipError_t launch_symmetrize_grouped(const SymJob* jobs, int njobs, int64_t blocks, hipStream_t s) {
+  if (njobs <= 0 || blocks <= 0) return hipSuccess;
+  if (blocks > 2147483647LL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_symmetrize_grouped, dim3((unsigned)blocks), dim3(256), 0, s, jobs, njobs);
   return hipGetLastError();
 }
 

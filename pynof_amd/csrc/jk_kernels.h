@@ -148,6 +148,18 @@ hipError_t launch_jn_f64(const BuildGeom& g, TileF64 tc, const double* d, const 
 
 // O[b][j][i] = O[b][i][j] for i > j, b < batch (n x n matrices, row stride ld).
 hipError_t launch_symmetrize(int batch, int n, double* O, int64_t ld, int64_t stride, hipStream_t s);
+// launch_symmetrize over many batches of matrices in one launch: job j covers workgroups
+// [first, first + count * pairs(n)) of the grid, jobs in increasing `first`.
+struct SymJob {
+  double* O;
+  int64_t ld, stride, first;
+  int n, count;
+};
+hipError_t launch_symmetrize_grouped(const SymJob* jobs, int njobs, int64_t blocks, hipStream_t s);
+__host__ __device__ inline int64_t sym_blocks(int n) {  // 32 x 32 lower tile pairs of an n x n matrix
+  const int64_t t = (n + 31) / 32;
+  return t * (t + 1) / 2;
+}
 
 // T[mu][i] = sum_nu Hp[mu][nu] Cp[nu][i] (mu < nbf, i < ldcp), then
 // hcore[i] = sum_mu Cp[mu][i] T[mu][i] for i < nbf5.
