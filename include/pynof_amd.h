@@ -225,8 +225,8 @@ int pnjk_energy(pnjk_engine* e, int64_t nbf5, int64_t nbeta, int64_t nalpha, int
                 double* E, void* stream);
 
 /* Grouped builds: many engines (one b each, different sizes allowed) built in
- * one pass. Each GEMM stage of all instances runs as one persistent grouped
- * launch per problem type and tile shape (tile shapes per instance from the
+ * one pass. Each GEMM stage of all instances runs as one grouped launch (one
+ * workgroup per tile) per problem type and tile shape (tile shapes per instance from the
  * engines' tile settings), so small instances share the device instead of
  * each underfilling it. The reference has no batched form; it builds one
  * system per call (integrals.py:172, :307). Arrays hold one entry per engine,

@@ -1,7 +1,7 @@
 """Grouped JK builds: many device-resident instances in one pass.
 
 ``JKGroup`` wraps ``pnjk_group``: each GEMM stage of every instance runs in one
-persistent grouped launch per problem type and tile shape, optionally replayed
+grouped launch (one workgroup per tile) per problem type and tile shape, optionally replayed
 from a captured hipGraph. Tile shapes come from the autotuner's table
 (``autotune.lookup``) per instance size; results are bitwise those of building
 each instance alone (tile shapes do not change results).
