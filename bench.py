@@ -83,6 +83,42 @@ def cpu_baseline(w, sample_naux: int):
     }
 
 
+def cpu_baseline_batch(ws, flops_total: float, every: int = 8, naux_div: int = 8):
+    """The oracle (reference call sequence) on every `every`-th instance of the batch by
+    size, each on an naux-slice of 1/naux_div: the sample time is scaled by naux (every
+    contraction is linear in it) and then by the batch's total over the sample's useful
+    flops. Returns builds/s of the whole batch."""
+    from oracle import jk_oracle as O
+    from pynof_amd import synth, workcount
+
+    sample = sorted(ws, key=lambda w: w.nbf)[every // 2::every]
+    t_full, f_sample, t_meas = 0.0, 0.0, 0.0
+    for w in sample:
+        na = max(8, w.naux // naux_div)
+        b = synth.make_b(w.nbf, na, w.seed_b, w.lam)
+        C = synth.make_C(w.nbf, w.seed_c)
+        H = synth.make_H(w.nbf, w.seed_h)
+        t0 = time.perf_counter()
+        O.jkj_ri(C, b, w.nbf5)
+        O.jkh_mo_ri(C, H, b, w.nbf5)
+        dt = time.perf_counter() - t0
+        t_meas += dt
+        t_full += dt * w.naux / na
+        f_sample += workcount.build_flops(w.nbf, w.naux, w.nbf5)
+    t_batch = t_full * flops_total / f_sample
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": len(ws) / t_batch,
+        "unit": "builds/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"numpy FP64 oracle (reference call sequence) on {len(sample)} of {len(ws)} instances "
+                   f"(every {every}th by nbf: {[w.nbf for w in sample]}), each at naux/{naux_div}: "
+                   f"{t_meas:.1f} s measured, scaled by naux per instance and by total/sample useful flops "
+                   f"({flops_total / f_sample:.2f}); BLAS threads={cores}; CPU: {cpu_model()}"),
+    }
+
+
 def profiled_traffic(kernel: str):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc summary:
     FETCH_SIZE scaled by the copy-kernel calibration plus WRITE_SIZE. None when the
@@ -529,6 +565,9 @@ def bench_batch(args):
             f: round(res[f"{f}_sequential"]["ms_per_pass"] / res[f"{f}_grouped_graph"]["ms_per_pass"], 3)
             for f in ("fp64", "split")},
     }
+    if not getattr(args, "no_cpu_baseline", False):
+        log("batch64: CPU baseline (oracle on a sample of instances)")
+        line["cpu_baseline"] = cpu_baseline_batch(ws, flops)
     print(json.dumps(line), flush=True)
 
 
