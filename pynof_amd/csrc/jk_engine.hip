@@ -634,7 +634,6 @@ int stage_jmo(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col
 }
 
 int stage_jmo_post(pnjk_engine* e, BuildState& st, hipStream_t s) {
-  PNJK_HIP_CHECK(launch_symmetrize(1, st.g.nbf5, st.dJMO, st.g.nbf5, 0, s));
   if (st.a.precision == PNJK_SPLIT_REFINE) {  // J^n, K^n in split form from the exact d and X
     if (st.want_j)
       PNJK_HIP_CHECK(launch_split_btri(e->d.as<double>(), (int64_t)st.g.nbf5 * st.g.naux_p, e->d16h.as<uint16_t>(),
@@ -748,8 +747,7 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
   if ((rc = stage_jn(e, st, s, nullptr))) return rc;
   if (st.want_j && (rc = stage_sym(st, st.dJ, s))) return rc;
   mark(ST_K);
-  if ((rc = stage_k(e, st, s, nullptr))) return rc;
-  if (st.want_k && (rc = stage_sym(st, st.dK, s))) return rc;
+  if ((rc = stage_k(e, st, s, nullptr))) return rc;  // K^n comes out symmetric
   mark(ST_HC);
   if (side) {  // join the side stream
     PNJK_HIP_CHECK(hipEventRecord(e->join_ev, sa));
@@ -1266,9 +1264,9 @@ struct pnjk_group {
   std::vector<BuildState> st;
   std::vector<char> sig;                // signature of the cached plan
   std::vector<GroupCollector> plan;     // grouped stages: HT, QT, J_MO, J^n, K^n
-  std::vector<SymJob> sym[2];           // symmetrize jobs of the J^n and K^n outputs
-  size_t sym_off[2] = {0, 0};           // their offsets in the plan buffer
-  int64_t sym_grid[2] = {0, 0};
+  std::vector<SymJob> sym;              // symmetrize jobs of the J^n outputs
+  size_t sym_off = 0;                   // their offset in the plan buffer
+  int64_t sym_grid = 0;
   DevBuf plan_dev;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1309,11 +1307,9 @@ int group_enqueue(pnjk_group* g, hipStream_t s) {
     if ((rc = stage_jmo_post(g->eng[i], g->st[i], s))) return rc;
   const char* base = static_cast<const char*>(g->plan_dev.p);
   if ((rc = launch_plan(g, GS_JN, s))) return rc;
-  PNJK_HIP_CHECK(launch_symmetrize_grouped(reinterpret_cast<const SymJob*>(base + g->sym_off[0]),
-                                           (int)g->sym[0].size(), g->sym_grid[0], s));
-  if ((rc = launch_plan(g, GS_K, s))) return rc;
-  PNJK_HIP_CHECK(launch_symmetrize_grouped(reinterpret_cast<const SymJob*>(base + g->sym_off[1]),
-                                           (int)g->sym[1].size(), g->sym_grid[1], s));
+  PNJK_HIP_CHECK(launch_symmetrize_grouped(reinterpret_cast<const SymJob*>(base + g->sym_off),
+                                           (int)g->sym.size(), g->sym_grid, s));
+  if ((rc = launch_plan(g, GS_K, s))) return rc;  // K^n comes out symmetric
   for (size_t i = 0; i < n; ++i)
     if ((rc = stage_hcore(g->eng[i], g->st[i], s))) return rc;
   return PNJK_OK;
@@ -1416,24 +1412,20 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
           (rc = stage_k(e, st, s, &g->plan[GS_K])))
         return rc;
     }
-    // One symmetrize launch per output kind over every instance's matrices.
-    for (int k = 0; k < 2; ++k) {
-      g->sym[k].clear();
-      int64_t first = 0;
-      for (size_t i = 0; i < n; ++i) {
-        const BuildState& st = g->st[i];
-        if (!(k == 0 ? st.want_j : st.want_k) || st.g.nbf <= 1) continue;
-        const int64_t nbf = st.g.nbf;
-        g->sym[k].push_back(SymJob{k == 0 ? st.dJ : st.dK, nbf, nbf * nbf, first, (int)nbf, st.g.nbf5});
-        first += (int64_t)st.g.nbf5 * sym_blocks((int)nbf);
-      }
-      g->sym_grid[k] = first;
+    // One symmetrize launch over every instance's J^n matrices.
+    g->sym.clear();
+    int64_t first = 0;
+    for (size_t i = 0; i < n; ++i) {
+      const BuildState& st = g->st[i];
+      if (!st.want_j || st.g.nbf <= 1) continue;
+      const int64_t nbf = st.g.nbf;
+      g->sym.push_back(SymJob{st.dJ, nbf, nbf * nbf, first, (int)nbf, st.g.nbf5});
+      first += (int64_t)st.g.nbf5 * sym_blocks((int)nbf);
     }
+    g->sym_grid = first;
     size_t bytes = 0;
-    for (int k = 0; k < 2; ++k) {
-      g->sym_off[k] = bytes;
-      bytes = roundup((int64_t)(bytes + g->sym[k].size() * sizeof(SymJob)), 256);
-    }
+    g->sym_off = bytes;
+    bytes = roundup((int64_t)(bytes + g->sym.size() * sizeof(SymJob)), 256);
     for (GroupCollector& c : g->plan)
       for (GroupBatch& b : c.batches) {
         b.probs_off = bytes;
@@ -1446,10 +1438,9 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
     PNJK_HIP_CHECK(hipStreamSynchronize(s));  // the previous plan may still be in use
     if ((rc = g->plan_dev.ensure(std::max<size_t>(bytes, 256)))) return rc;
     char* base = static_cast<char*>(g->plan_dev.p);
-    for (int k = 0; k < 2; ++k)
-      if (!g->sym[k].empty())
-        PNJK_HIP_CHECK(hipMemcpy(base + g->sym_off[k], g->sym[k].data(), g->sym[k].size() * sizeof(SymJob),
-                                 hipMemcpyHostToDevice));
+    if (!g->sym.empty())
+      PNJK_HIP_CHECK(hipMemcpy(base + g->sym_off, g->sym.data(), g->sym.size() * sizeof(SymJob),
+                               hipMemcpyHostToDevice));
     for (GroupCollector& c : g->plan)
       for (GroupBatch& b : c.batches) {
         PNJK_HIP_CHECK(hipMemcpy(base + b.probs_off, b.probs.data(), b.probs.size(), hipMemcpyHostToDevice));

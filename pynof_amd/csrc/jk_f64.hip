@@ -421,7 +421,7 @@ struct ProbSYRK {
     t.n0 = t.nt * BN;
     return true;
   }
-  // blocks strictly above the diagonal of a diagonal tile are filled by symmetrize
+  // blocks strictly above the diagonal of a diagonal tile are filled by the mirror writes
   __device__ bool block_active(const Tile& t, int r0, int c0) const {
     return r0 < M && c0 < M && !(t.mt == t.nt && r0 + 16 <= c0);
   }
@@ -440,7 +440,11 @@ struct ProbSYRK {
           int dm, dn;
           elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int m = t.m0 + dm, n = t.n0 + dn;
-          if (m < M && n < M) gmem(o)[(int64_t)m * ldo + n] = acc[i][j][r];
+          if (m < M && n <= m) {  // (i, j) and its mirror (j, i): O exactly symmetric
+            const double v = acc[i][j][r];
+            gmem(o)[(int64_t)m * ldo + n] = v;
+            if (n < m) gmem(o)[(int64_t)n * ldo + m] = v;
+          }
         }
   }
 };

@@ -137,8 +137,9 @@ hipError_t launch_hmat_f64(const BuildGeom& g, const double* Cf, int ldcf, const
                            hipStream_t s);
 hipError_t launch_kmo_reduce(int nbf5, int nt, const double* kpart, double* kmo, hipStream_t s);
 
-// O[b][i][j] = sum_k A[b][i][k] A[b][j][k] for i, j < M on the lower triangle of
-// tiles; the strict upper triangle is left for launch_symmetrize.
+// O[b][i][j] = sum_k A[b][i][k] A[b][j][k] for i, j < M, computed on the lower
+// triangle of tiles: each value with i >= j is written at (i, j) and, for i > j,
+// also at (j, i), so O is exactly symmetric with no separate pass.
 hipError_t launch_syrk_f64(int batch, int M, int K, const double* A, int64_t lda, int64_t sa, double* O,
                            int64_t ldo, int64_t so, TileF64 tc, hipStream_t s, GroupCollector* col = nullptr);
 

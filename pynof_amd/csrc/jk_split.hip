@@ -583,7 +583,9 @@ struct ProbJN {
 };
 
 // K[b][i][j] = sum_P X16[b][i][P] X16[b][j][P] on the lower triangle of tiles (FP64
-// output; the strict upper triangle is filled afterwards by launch_symmetrize)
+// output). Each value with i >= j goes to (i, j) and, for i > j, also to (j, i): the
+// output is exactly symmetric without a separate pass (the upper half of a diagonal
+// tile, whose own products pair the planes the other way round, is not stored).
 struct ProbSYRK {
   static constexpr int kA = RK, kB = RK;
   static constexpr bool kRowSkip = false;
@@ -619,8 +621,10 @@ struct ProbSYRK {
           int dm, dn;
           coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int m = t.m0 + dm, n = t.n0 + dn;
-          if (m < M && n < M) {
-            gmem(o)[(int64_t)m * ldo + n] = acc[i][j][r];
+          if (m < M && n <= m) {
+            const double v = acc[i][j][r];
+            gmem(o)[(int64_t)m * ldo + n] = v;
+            if (n < m) gmem(o)[(int64_t)n * ldo + m] = v;
           }
         }
   }
