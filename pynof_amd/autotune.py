@@ -26,9 +26,9 @@ from . import _lib
 
 STAGES = {"half_transform": 1, "quarter_transform": 2, "coulomb_jn": 5, "exchange_kn": 6}
 CANDIDATES = {
-    "half_transform": [(2, 2), (4, 4), (5, 4), (4, 2), (2, 4), (5, 2), (10, 4)],  # (10, 4): BF16x3 only
+    "half_transform": [(2, 2), (4, 4), (5, 4), (4, 2), (2, 4), (5, 2), (10, 4), (5, 5)],  # last two: BF16x3 only
     "quarter_transform": [(2, 2), (4, 2), (5, 2), (2, 4), (4, 4), (5, 4)],
-    "coulomb_jn": [(2, 2), (4, 4), (5, 4), (4, 2), (2, 4), (5, 2), (10, 4)],
+    "coulomb_jn": [(2, 2), (4, 4), (5, 4), (4, 2), (2, 4), (5, 2), (10, 4), (5, 5)],
     "exchange_kn": [(2, 2), (4, 4)],
 }
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_tiles.json")

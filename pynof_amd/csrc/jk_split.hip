@@ -714,6 +714,7 @@ hipError_t launch_x3(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s, G
   PNJK_CASE(2, 4, 2, 4, 2, 2)
   if constexpr (std::is_same<Prob, ProbHT>::value || std::is_same<Prob, ProbJN>::value) {
     PNJK_CASE(10, 4, 5, 4, 4, 2)
+    PNJK_CASE(5, 5, 5, 5, 2, 2)
   }
 #undef PNJK_CASE
   return hipErrorInvalidValue;

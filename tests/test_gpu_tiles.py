@@ -22,7 +22,7 @@ def test_tiles_bitwise(gpu_required, precision):
     a = autotune.ARITH[precision]
     for st, idx in autotune.STAGES.items():
         for fm, fn in autotune.CANDIDATES[st]:
-            if precision == "fp64" and (fm, fn) == (10, 4):
+            if precision == "fp64" and (fm, fn) in ((10, 4), (5, 5)):
                 assert L.pnjk_set_tile(eng._h, a, idx, fm, fn) == _lib.EINVAL  # BF16x3-only shapes
                 continue
             _lib.check(L.pnjk_set_tile(eng._h, a, idx, fm, fn))
@@ -43,6 +43,9 @@ def test_set_tile_rejects(gpu_required):
     assert L.pnjk_set_tile(eng._h, 1, 1, 10, 4) == _lib.OK      # eight-wave tile: BF16x3 HT and J^n
     assert L.pnjk_set_tile(eng._h, 1, 6, 10, 4) == _lib.EINVAL  # not the exchange SYRK
     assert L.pnjk_set_tile(eng._h, 1, 2, 10, 4) == _lib.EINVAL  # nor the quarter transform
+    assert L.pnjk_set_tile(eng._h, 1, 1, 5, 5) == _lib.OK       # 160 x 160 tile: BF16x3 HT and J^n
+    assert L.pnjk_set_tile(eng._h, 0, 1, 5, 5) == _lib.EINVAL
+    assert L.pnjk_set_tile(eng._h, 1, 6, 5, 5) == _lib.EINVAL
     assert L.pnjk_set_tile(eng._h, 1, 6, 5, 4) == _lib.EINVAL   # exchange SYRK needs square tiles
     assert L.pnjk_set_tile(eng._h, 0, 0, 2, 2) == _lib.EINVAL   # copy_in takes no tile
     assert L.pnjk_set_tile(eng._h, 1, 1, 3, 3) == _lib.EINVAL   # unknown shape
