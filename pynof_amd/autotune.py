@@ -29,7 +29,7 @@ CANDIDATES = {
     "half_transform": [(2, 2), (4, 4), (5, 4), (4, 2), (2, 4), (5, 2), (10, 4), (5, 5)],  # last two: BF16x3 only
     "quarter_transform": [(2, 2), (4, 2), (5, 2), (2, 4), (4, 4), (5, 4)],
     "coulomb_jn": [(2, 2), (4, 4), (5, 4), (4, 2), (2, 4), (5, 2), (10, 4), (5, 5)],
-    "exchange_kn": [(2, 2), (4, 4)],
+    "exchange_kn": [(2, 2), (4, 4), (5, 5)],  # (5, 5): BF16x3 only
 }
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_tiles.json")
 

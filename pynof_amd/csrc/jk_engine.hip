@@ -1508,10 +1508,10 @@ int pnjk_set_tile(pnjk_engine* e, int arith, int stage, int fm, int fn) {
   bool known = false;
   for (const TileF64 t : {TileF64{2, 2}, TileF64{4, 4}, TileF64{5, 4}, TileF64{4, 2}, TileF64{2, 4}, TileF64{5, 2}})
     known = known || (t.fm == fm && t.fn == fn);
-  // The eight-wave 320 x 128 tile and the 160 x 160 one exist for the BF16x3 half-transform
-  // and J^n only.
-  if (arith == 1 && (stage == ST_HT || stage == ST_JN) && ((fm == 10 && fn == 4) || (fm == 5 && fn == 5)))
-    known = true;
+  // The eight-wave 320 x 128 tile exists for the BF16x3 half-transform and J^n only, the
+  // 160 x 160 one for those and the BF16x3 exchange SYRK.
+  if (arith == 1 && (stage == ST_HT || stage == ST_JN) && fm == 10 && fn == 4) known = true;
+  if (arith == 1 && (stage == ST_HT || stage == ST_JN || stage == ST_K) && fm == 5 && fn == 5) known = true;
   if (!reset && !known) return set_error(PNJK_EINVAL, "pnjk_set_tile: unsupported tile shape");
   if (!reset && stage == ST_QT && fn != 2 && fn != 4)
     return set_error(PNJK_EINVAL, "pnjk_set_tile: the quarter transform needs fn = 2 or 4 (64 or 128 aux columns)");

@@ -187,10 +187,13 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
 
   const int frow = lane & 15, fk = 8 * (lane >> 4);
   const TrRead<kBkn ? BN : 64> tr(lane);
-  // MFMA row blocks of this wave that hold a needed output row: with Prob::kRowSkip,
+  // MFMA row blocks [i0, nact) of this wave hold a needed output row: with Prob::kRowSkip,
   // rows from the problem's row limit on (past the diagonal q <= p of the quarter
-  // transform, padding past nbf5) get no fragment reads and no MFMAs. Wave-uniform.
+  // transform, padding past nbf5 or M) get no fragment reads and no MFMAs; with
+  // Prob::kLower, neither do row blocks wholly above the diagonal of the output (their
+  // last row before this wave's first column). Wave-uniform.
   const int nact = Prob::kRowSkip ? min(FM, max(0, (prob.row_limit(t) - (t.m0 + wm) + 15) >> 4)) : FM;
+  const int i0 = Prob::kLower ? min(FM, max(0, (t.n0 + wn - (t.m0 + wm)) >> 4)) : 0;
   auto step = [&](const int kt, auto stage) {
     constexpr int cur = decltype(stage)::value;
     uint16_t* const a_c = cur ? sA1 : sA0;
@@ -199,7 +202,7 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
     bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
-      if (i < nact) {
+      if (i >= i0 && i < nact) {
         const int o = swz(wm + 16 * i + frow, fk >> 3);
         ah[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(a_c + o));
         al[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(a_c + BM * kLDK + o));
@@ -222,7 +225,7 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
     __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int i = 0; i < FM; ++i)
-      if (i < nact) {
+      if (i >= i0 && i < nact) {
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
@@ -294,6 +297,7 @@ __device__ __forceinline__ void coords(int lane, int wm, int wn, int i, int j, i
 struct ProbHT {
   static constexpr int kA = RK, kB = RK;
   static constexpr bool kRowSkip = false;
+  static constexpr bool kLower = false;
   Planes ct, bht;
   uint16_t* xhi;
   uint16_t* xlo;
@@ -348,6 +352,7 @@ struct SrcXT {  // element (k = nu, n = P) at p*nbf*naux_p + min(nu, nbf-1)*naux
 struct ProbQT {
   static constexpr int kA = RK, kB = KN;
   static constexpr bool kRowSkip = true;
+  static constexpr bool kLower = false;
   Planes ct, x;
   double* kpart;
   double* d;
@@ -452,6 +457,7 @@ template <bool kPlanes>
 struct ProbMOT {
   static constexpr int kA = RK, kB = KN;
   static constexpr bool kRowSkip = false;
+  static constexpr bool kLower = false;
   Planes ct, x;
   double* out;
   uint16_t* oh;
@@ -503,6 +509,7 @@ struct ProbMOT {
 struct ProbLagX {
   static constexpr int kA = RK, kB = RK;
   static constexpr bool kRowSkip = false;
+  static constexpr bool kLower = false;
   Planes bm, w;
   double* part;
   int64_t ldrow, kc;
@@ -541,6 +548,7 @@ struct ProbLagX {
 struct ProbJN {
   static constexpr int kA = RK, kB = RK;
   static constexpr bool kRowSkip = true;
+  static constexpr bool kLower = false;
   Planes dd, bt;
   double* J;
   int nbf, naux_p, nbf5, mtiles, m_base;  // rows [m_base, m_base + mtiles*BM) of this launch
@@ -588,7 +596,8 @@ struct ProbJN {
 // tile, whose own products pair the planes the other way round, is not stored).
 struct ProbSYRK {
   static constexpr int kA = RK, kB = RK;
-  static constexpr bool kRowSkip = false;
+  static constexpr bool kRowSkip = true;
+  static constexpr bool kLower = true;  // rows above the diagonal are never stored
   Planes x;
   double* O;
   int64_t lda, sa, ldo, so;
@@ -714,6 +723,9 @@ hipError_t launch_x3(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s, G
   PNJK_CASE(2, 4, 2, 4, 2, 2)
   if constexpr (std::is_same<Prob, ProbHT>::value || std::is_same<Prob, ProbJN>::value) {
     PNJK_CASE(10, 4, 5, 4, 4, 2)
+  }
+  if constexpr (std::is_same<Prob, ProbHT>::value || std::is_same<Prob, ProbJN>::value ||
+                std::is_same<Prob, ProbSYRK>::value) {
     PNJK_CASE(5, 5, 5, 5, 2, 2)
   }
 #undef PNJK_CASE

@@ -45,7 +45,8 @@ def test_set_tile_rejects(gpu_required):
     assert L.pnjk_set_tile(eng._h, 1, 2, 10, 4) == _lib.EINVAL  # nor the quarter transform
     assert L.pnjk_set_tile(eng._h, 1, 1, 5, 5) == _lib.OK       # 160 x 160 tile: BF16x3 HT and J^n
     assert L.pnjk_set_tile(eng._h, 0, 1, 5, 5) == _lib.EINVAL
-    assert L.pnjk_set_tile(eng._h, 1, 6, 5, 5) == _lib.EINVAL
+    assert L.pnjk_set_tile(eng._h, 1, 6, 5, 5) == _lib.OK       # and the BF16x3 exchange SYRK
+    assert L.pnjk_set_tile(eng._h, 0, 6, 5, 5) == _lib.EINVAL
     assert L.pnjk_set_tile(eng._h, 1, 6, 5, 4) == _lib.EINVAL   # exchange SYRK needs square tiles
     assert L.pnjk_set_tile(eng._h, 0, 0, 2, 2) == _lib.EINVAL   # copy_in takes no tile
     assert L.pnjk_set_tile(eng._h, 1, 1, 3, 3) == _lib.EINVAL   # unknown shape
