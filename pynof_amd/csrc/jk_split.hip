@@ -725,7 +725,7 @@ hipError_t launch_x3(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s, G
     PNJK_CASE(10, 4, 5, 4, 4, 2)
   }
   if constexpr (std::is_same<Prob, ProbHT>::value || std::is_same<Prob, ProbJN>::value ||
-                std::is_same<Prob, ProbSYRK>::value) {
+                std::is_same<Prob, ProbSYRK>::value || std::is_same<Prob, ProbLagX>::value) {
     PNJK_CASE(5, 5, 5, 5, 2, 2)
   }
 #undef PNJK_CASE
@@ -761,7 +761,9 @@ hipError_t launch_lag_x3(int nbf, int nbf5, int naux, const double* B, int64_t l
   const int64_t nb = (int64_t)nbf * ldrow, nw = (int64_t)nbf5 * ldrow;
   if (!planes_ready) hipLaunchKernelGGL(k_split_packed, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, s, B, nb, bh, bl);
   hipLaunchKernelGGL(k_lag_w16, dim3((unsigned)cdiv(nw, 256)), dim3(256), 0, s, B, ldrow, naux, nbf5, ck, wh, wl);
-  const TileF64 tc{5, 2};
+  // 160 x 160 tiles: each b_MO row block is read by 2 column tiles over nbf5 = 300, not by 5
+  // as with 160 x 64 (the whole split Lagrangian 24.9 -> 22.9 ms at nbf = 1000).
+  const TileF64 tc{5, 5};
   ProbLagX pr;
   pr.bm = {bh, bl};
   pr.w = {wh, wl};
