@@ -456,7 +456,7 @@ struct ProbQT {
 template <bool kPlanes>
 struct ProbMOT {
   static constexpr int kA = RK, kB = KN;
-  static constexpr bool kRowSkip = false;
+  static constexpr bool kRowSkip = true;  // rows p >= nbf of the last row tile
   static constexpr bool kLower = false;
   Planes ct, x;
   double* out;
