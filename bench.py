@@ -152,7 +152,8 @@ def main():
     ap.add_argument("--workload", default="large", choices=["water", "mid", "large", "batch64", "serving", "gemmref"])
     ap.add_argument("--requests", type=int, default=120, help="serving: requests per run")
     ap.add_argument("--form-steps", type=int, default=5, help="builds timed per single form")
-    ap.add_argument("--cpu-sample-naux", type=int, default=72)
+    ap.add_argument("--cpu-sample-naux", type=int, default=72,
+                    help="naux slice of the CPU baseline for workloads with nbf > 400 (smaller ones run in full)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip", nargs="*", default=[], choices=["b_producer"],
                     help="report sections to leave out (the headline stream is always run)")
@@ -464,7 +465,8 @@ def main():
     del src, dst
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        line["cpu_baseline"] = cpu_baseline(w, min(args.cpu_sample_naux, w.naux))
+        # The water and mid-size workloads are timed in full; the large one on an naux slice.
+        line["cpu_baseline"] = cpu_baseline(w, w.naux if w.nbf <= 400 else min(args.cpu_sample_naux, w.naux))
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
