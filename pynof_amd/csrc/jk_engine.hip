@@ -633,14 +633,24 @@ int stage_jmo(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col
   return PNJK_OK;
 }
 
-int stage_jmo_post(pnjk_engine* e, BuildState& st, hipStream_t s) {
-  if (st.a.precision == PNJK_SPLIT_REFINE) {  // J^n, K^n in split form from the exact d and X
+// split_refine: bf16 planes of the exact X, for K^n in split form.
+int stage_convert_x(pnjk_engine* e, BuildState& st, hipStream_t s) {
+  PNJK_HIP_CHECK(launch_split_btri(e->X.as<double>(), (int64_t)st.g.nbf5 * st.g.nbf * st.g.naux_p,
+                                   e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), s));
+  return PNJK_OK;
+}
+
+// split_refine: J^n, K^n in split form from the exact d and X (X's planes here unless the
+// caller made them already).
+int stage_jmo_post(pnjk_engine* e, BuildState& st, hipStream_t s, bool convert_x = true) {
+  if (st.a.precision == PNJK_SPLIT_REFINE) {
     if (st.want_j)
       PNJK_HIP_CHECK(launch_split_btri(e->d.as<double>(), (int64_t)st.g.nbf5 * st.g.naux_p, e->d16h.as<uint16_t>(),
                                        e->d16l.as<uint16_t>(), s));
-    if (st.want_k)
-      PNJK_HIP_CHECK(launch_split_btri(e->X.as<double>(), (int64_t)st.g.nbf5 * st.g.nbf * st.g.naux_p,
-                                       e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), s));
+    if (st.want_k && convert_x) {
+      int rc = stage_convert_x(e, st, s);
+      if (rc) return rc;
+    }
   }
   return PNJK_OK;
 }
@@ -727,33 +737,40 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
   if ((rc = stage_in(e, st, s))) return rc;
   mark(ST_HT);
   if ((rc = stage_ht(e, st, s, nullptr))) return rc;
+  // split_refine: the split K^n needs only X, so its planes and K^n run on the side stream
+  // beside the FP64 quarter transform (which keeps the MFMA pipes 40% busy). The other forms
+  // run J_MO and H_core on the side stream beside J^n (about 0.3 ms hidden per build at
+  // nbf=1000); J^n's mirror stays in line: beside K^n it slowed K^n by more than it saved.
+  const bool refine = st.a.precision == PNJK_SPLIT_REFINE;
+  const bool k_side = refine && st.want_k, side = !refine;
+  hipStream_t sa = (side || k_side) ? e->aux : s;
+  if (k_side) {
+    PNJK_HIP_CHECK(hipEventRecord(e->fork_ev, s));
+    PNJK_HIP_CHECK(hipStreamWaitEvent(sa, e->fork_ev, 0));
+    if ((rc = stage_convert_x(e, st, sa)) || (rc = stage_k(e, st, sa, nullptr))) return rc;
+  }
   mark(ST_QT);
   if ((rc = stage_qt(e, st, s, nullptr))) return rc;
   mark(ST_KMO);
   if ((rc = stage_kmo(e, st, s))) return rc;
   mark(ST_JMO);
-  // split_refine converts d and X for J^n, K^n in the J_MO stage: keep it in line. Otherwise
-  // J_MO and H_core run on the side stream beside J^n (about 0.3 ms hidden per build at
-  // nbf=1000). J^n's mirror stays in line: beside K^n it slowed K^n by more than it saved.
-  const bool side = st.a.precision != PNJK_SPLIT_REFINE;
-  hipStream_t sa = side ? e->aux : s;
   if (side) {
     PNJK_HIP_CHECK(hipEventRecord(e->fork_ev, s));
     PNJK_HIP_CHECK(hipStreamWaitEvent(sa, e->fork_ev, 0));
   }
-  if ((rc = stage_jmo(e, st, sa, nullptr)) || (rc = stage_jmo_post(e, st, sa))) return rc;
+  hipStream_t sj = side ? sa : s;  // J_MO (and in split_refine the planes of d) stream
+  if ((rc = stage_jmo(e, st, sj, nullptr)) || (rc = stage_jmo_post(e, st, sj, !k_side))) return rc;
   if (side && (rc = stage_hcore(e, st, sa))) return rc;
   mark(ST_JN);
   if ((rc = stage_jn(e, st, s, nullptr))) return rc;
   if (st.want_j && (rc = stage_sym(st, st.dJ, s))) return rc;
   mark(ST_K);
-  if ((rc = stage_k(e, st, s, nullptr))) return rc;  // K^n comes out symmetric
+  if (!k_side && (rc = stage_k(e, st, s, nullptr))) return rc;  // K^n comes out symmetric
   mark(ST_HC);
-  if (side) {  // join the side stream
+  if (!side && (rc = stage_hcore(e, st, s))) return rc;
+  if (side || k_side) {  // join the side stream
     PNJK_HIP_CHECK(hipEventRecord(e->join_ev, sa));
     PNJK_HIP_CHECK(hipStreamWaitEvent(s, e->join_ev, 0));
-  } else if ((rc = stage_hcore(e, st, s))) {
-    return rc;
   }
   mark(ST_OUT);
   bool sync = false;
