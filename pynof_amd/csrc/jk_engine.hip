@@ -741,9 +741,10 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
   // beside the FP64 quarter transform (which keeps the MFMA pipes 40% busy). The other forms
   // run J_MO and H_core on the side stream beside J^n (about 0.3 ms hidden per build at
   // nbf=1000); J^n's mirror stays in line: beside K^n it slowed K^n by more than it saved.
-  // With the stage profiler on, K^n stays in line, so every stage's time is its own.
+  // (In split_refine's stage profile, K^n's time therefore shows inside the quarter
+  // transform's.)
   const bool refine = st.a.precision == PNJK_SPLIT_REFINE;
-  const bool k_side = refine && st.want_k && !e->prof, side = !refine;
+  const bool k_side = refine && st.want_k, side = !refine;
   hipStream_t sa = (side || k_side) ? e->aux : s;
   if (k_side) {
     PNJK_HIP_CHECK(hipEventRecord(e->fork_ev, s));
