@@ -837,7 +837,7 @@ static int mo_compute(pnjk_engine* e, BuildState& st, double* dB, int64_t ldrow,
       if ((rc = e->ctf16h.ensure(ct_bytes)) || (rc = e->ctf16l.ensure(ct_bytes))) return rc;
       PNJK_HIP_CHECK(launch_split_ct(Cf, ldcf, (int)nbf, (int)nbf, (int)nbf, st.sg.ldk, e->ctf16h.as<uint16_t>(),
                                      e->ctf16l.as<uint16_t>(), s));
-      PNJK_HIP_CHECK(launch_mo_x3(st.sg, g.naux, big ? TileF64{5, 4} : TileF64{2, 2}, e->ctf16h.as<uint16_t>(),
+      PNJK_HIP_CHECK(launch_mo_x3(st.sg, g.naux, big ? TileF64{6, 4} : TileF64{2, 2}, e->ctf16h.as<uint16_t>(),
                                   e->ctf16l.as<uint16_t>(), e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), dB, ldrow,
                                   s, bh, bl));
     } else {

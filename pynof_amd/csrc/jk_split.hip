@@ -724,6 +724,9 @@ hipError_t launch_x3(const Prob& prob, TileF64 tc, int nblocks, hipStream_t s, G
   if constexpr (std::is_same<Prob, ProbHT>::value || std::is_same<Prob, ProbJN>::value) {
     PNJK_CASE(10, 4, 5, 4, 4, 2)
   }
+  if constexpr (std::is_same<Prob, ProbMOT<true>>::value || std::is_same<Prob, ProbMOT<false>>::value) {
+    PNJK_CASE(6, 4, 6, 4, 2, 2)
+  }
   if constexpr (std::is_same<Prob, ProbHT>::value || std::is_same<Prob, ProbJN>::value ||
                 std::is_same<Prob, ProbSYRK>::value || std::is_same<Prob, ProbLagX>::value) {
     PNJK_CASE(5, 5, 5, 5, 2, 2)
