@@ -840,9 +840,9 @@ static int mo_compute(pnjk_engine* e, BuildState& st, double* dB, int64_t ldrow,
       PNJK_HIP_CHECK(launch_mo_x3(st.sg, g.naux, big ? TileF64{6, 4} : TileF64{2, 2}, e->ctf16h.as<uint16_t>(),
                                   e->ctf16l.as<uint16_t>(), e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), dB, ldrow,
                                   s, bh, bl));
-    } else {
+    } else {  // 128 x 64 tiles (160 x 64: the FP64 Lagrangian 3.5 ms slower at nbf = 1000)
       PNJK_HIP_CHECK(
-          launch_mo_f64(g, big ? TileF64{5, 2} : TileF64{2, 2}, Cf, ldcf, e->X.as<double>(), dB, ldrow, s));
+          launch_mo_f64(g, big ? TileF64{4, 2} : TileF64{2, 2}, Cf, ldcf, e->X.as<double>(), dB, ldrow, s));
     }
   }
   if (dHm) {
