@@ -746,34 +746,45 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
   const bool refine = st.a.precision == PNJK_SPLIT_REFINE;
   const bool k_side = refine && st.want_k, side = !refine;
   hipStream_t sa = (side || k_side) ? e->aux : s;
-  if (k_side) {
+  bool forked = false;
+  auto fork = [&]() -> int {
     PNJK_HIP_CHECK(hipEventRecord(e->fork_ev, s));
     PNJK_HIP_CHECK(hipStreamWaitEvent(sa, e->fork_ev, 0));
-    if ((rc = stage_convert_x(e, st, sa)) || (rc = stage_k(e, st, sa, nullptr))) return rc;
+    forked = true;
+    return PNJK_OK;
+  };
+  // Everything from the fork to the join; an error part-way still joins the side stream
+  // below, so no work of this build is left running past the call on the side stream.
+  auto body = [&]() -> int {
+    int r;
+    if (k_side) {
+      if ((r = fork()) || (r = stage_convert_x(e, st, sa)) || (r = stage_k(e, st, sa, nullptr))) return r;
+    }
+    mark(ST_QT);
+    if ((r = stage_qt(e, st, s, nullptr))) return r;
+    mark(ST_KMO);
+    if ((r = stage_kmo(e, st, s))) return r;
+    mark(ST_JMO);
+    if (side && (r = fork())) return r;
+    hipStream_t sj = side ? sa : s;  // J_MO (and in split_refine the planes of d) stream
+    if ((r = stage_jmo(e, st, sj, nullptr)) || (r = stage_jmo_post(e, st, sj, !k_side))) return r;
+    if (side && (r = stage_hcore(e, st, sa))) return r;
+    mark(ST_JN);
+    if ((r = stage_jn(e, st, s, nullptr))) return r;
+    if (st.want_j && (r = stage_sym(st, st.dJ, s))) return r;
+    mark(ST_K);
+    if (!k_side && (r = stage_k(e, st, s, nullptr))) return r;  // K^n comes out symmetric
+    mark(ST_HC);
+    if (!side && (r = stage_hcore(e, st, s))) return r;
+    return PNJK_OK;
+  };
+  rc = body();
+  if (forked) {  // join the side stream
+    const hipError_t j1 = hipEventRecord(e->join_ev, sa);
+    const hipError_t j2 = j1 == hipSuccess ? hipStreamWaitEvent(s, e->join_ev, 0) : j1;
+    if (j2 != hipSuccess && !rc) PNJK_HIP_CHECK(j2);
   }
-  mark(ST_QT);
-  if ((rc = stage_qt(e, st, s, nullptr))) return rc;
-  mark(ST_KMO);
-  if ((rc = stage_kmo(e, st, s))) return rc;
-  mark(ST_JMO);
-  if (side) {
-    PNJK_HIP_CHECK(hipEventRecord(e->fork_ev, s));
-    PNJK_HIP_CHECK(hipStreamWaitEvent(sa, e->fork_ev, 0));
-  }
-  hipStream_t sj = side ? sa : s;  // J_MO (and in split_refine the planes of d) stream
-  if ((rc = stage_jmo(e, st, sj, nullptr)) || (rc = stage_jmo_post(e, st, sj, !k_side))) return rc;
-  if (side && (rc = stage_hcore(e, st, sa))) return rc;
-  mark(ST_JN);
-  if ((rc = stage_jn(e, st, s, nullptr))) return rc;
-  if (st.want_j && (rc = stage_sym(st, st.dJ, s))) return rc;
-  mark(ST_K);
-  if (!k_side && (rc = stage_k(e, st, s, nullptr))) return rc;  // K^n comes out symmetric
-  mark(ST_HC);
-  if (!side && (rc = stage_hcore(e, st, s))) return rc;
-  if (side || k_side) {  // join the side stream
-    PNJK_HIP_CHECK(hipEventRecord(e->join_ev, sa));
-    PNJK_HIP_CHECK(hipStreamWaitEvent(s, e->join_ev, 0));
-  }
+  if (rc) return rc;
   mark(ST_OUT);
   bool sync = false;
   if ((rc = stage_out(st, s, &sync))) return rc;
