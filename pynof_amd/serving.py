@@ -8,7 +8,11 @@ server:
 * admits requests first-come-first-served while the device-memory budget
   holds their b tensor, its split planes and their workspace (footprint from
   the C ABI's pnjk_workspace_bytes); a request that could never fit is
-  rejected, one that does not fit now waits until memory frees up;
+  rejected, one that does not fit now waits until memory frees up; with an
+  executor that allows it (``concurrent_admission``), admissions (upload of b,
+  set-up of C, H, n) run on a worker thread and their own stream while the
+  active requests keep stepping, and a request joins the batch at the first
+  step after its admission completes;
 * keeps each admitted request's b, C, H and occupations resident on the device
   until it completes;
 * runs continuous batching: every scheduler step advances every active request
@@ -25,8 +29,10 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import threading
 import time
 from collections import deque
+from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
 
 import numpy as np
 
@@ -62,7 +68,11 @@ class Request:
 
 
 class Executor:
-    """Device side of the server."""
+    """Device side of the server. An executor whose ``admit`` may run on another thread
+    than ``step`` and ``release`` (at most one admission at a time) sets
+    ``concurrent_admission = True``."""
+
+    concurrent_admission = False
 
     def footprint(self, req: Request) -> int:
         raise NotImplementedError
@@ -88,6 +98,8 @@ class Server:
         self.clock = clock
         self.sleep = sleep
         self.pending: deque[Request] = deque()
+        self.admitting: deque = deque()  # (request, future) of admissions in flight, FIFO
+        self._pool: ThreadPoolExecutor | None = None
         self.active: list[Request] = []
         self.finished: list[Request] = []
         self.rejected: list[Request] = []
@@ -106,21 +118,39 @@ class Server:
         self.pending.append(req)
 
     def _admit(self, now: float) -> None:
-        while self.pending and self.pending[0].arrival <= now and len(self.active) < self.max_active:
+        while (self.pending and self.pending[0].arrival <= now
+               and len(self.active) + len(self.admitting) < self.max_active):
             r = self.pending[0]
             if self.in_use + r.footprint > self.budget:
                 break  # first-come-first-served: later requests wait behind it
             self.pending.popleft()
             r.started = self.clock()
+            self.in_use += r.footprint
+            if self._pool is not None:
+                self.admitting.append((r, self._pool.submit(self.ex.admit, r)))
+                continue
             try:
                 self.ex.admit(r)
             except Exception as exc:  # allocation or upload failure: fail this request only
-                r.error = f"admission failed: {exc}"
-                self.rejected.append(r)
+                self._fail(r, exc)
                 continue
-            r.admitted = self.clock()
-            self.in_use += r.footprint
-            self.active.append(r)
+            self._activate(r)
+        while self.admitting and self.admitting[0][1].done():
+            r, fut = self.admitting.popleft()
+            exc = fut.exception()
+            if exc is not None:
+                self._fail(r, exc)
+            else:
+                self._activate(r)
+
+    def _activate(self, r: Request) -> None:
+        r.admitted = self.clock()
+        self.active.append(r)
+
+    def _fail(self, r: Request, exc: BaseException) -> None:
+        r.error = f"admission failed: {exc}"
+        self.in_use -= r.footprint
+        self.rejected.append(r)
 
     def run(self, t0: float | None = None, timeout: float = 1e9) -> None:
         """Serves until every submitted request is done or rejected. Arrival times
@@ -129,13 +159,25 @@ class Server:
         for r in self.pending:
             r.arrival += base
         end = base + timeout
-        while self.pending or self.active:
+        if getattr(self.ex, "concurrent_admission", False):
+            self._pool = ThreadPoolExecutor(1, thread_name_prefix="admit")
+        try:
+            self._loop(end)
+        finally:
+            if self._pool is not None:
+                self._pool.shutdown(wait=True)
+                self._pool = None
+
+    def _loop(self, end: float) -> None:
+        while self.pending or self.active or self.admitting:
             now = self.clock()
             if now > end:
                 raise TimeoutError("serving loop exceeded its time limit")
             self._admit(now)
             if not self.active:
-                if self.pending:
+                if self.admitting:
+                    wait([self.admitting[0][1]], timeout=0.05, return_when=FIRST_COMPLETED)
+                elif self.pending:
                     self.sleep(max(0.0, min(self.pending[0].arrival - now, 0.05)))
                 continue
             batch = list(self.active)
@@ -194,7 +236,13 @@ class DeviceExecutor(Executor):
     is calcocce at its current parameters gamma (pnof.py:994-1001), and gamma takes
     one gradient step, gamma -= occ_lr * calcoccg, before the next build. The
     occupation block is the first ndoc (1 + ncwo) orbitals, ncwo = (nbf5 - ndoc) // ndoc.
-    J_MO, K_MO, H_core and gamma never leave the GPU."""
+    J_MO, K_MO, H_core and gamma never leave the GPU.
+
+    Admission may run on the server's worker thread: it works on a stream of its own and
+    synchronises it before returning, so the request's tensors are ready for the build
+    stream; the per-request state table is guarded by a lock."""
+
+    concurrent_admission = True
 
     def __init__(self, device: int = 0, with_jk: bool = True, keep_final_C: bool = False,
                  occupation: bool = False, occ_lr: float = 0.05):
@@ -208,6 +256,8 @@ class DeviceExecutor(Executor):
         self.occupation = occupation
         self.occ_lr = occ_lr
         self.state: dict[int, dict] = {}
+        self._state_lock = threading.Lock()
+        self._admit_stream = None
         self._groups: dict = {}
 
     def footprint(self, req: Request) -> int:
@@ -222,6 +272,16 @@ class DeviceExecutor(Executor):
         return int(b_bytes + ws + outs + 3 * req.nbf * req.nbf * 8)
 
     def admit(self, req: Request) -> None:
+        torch = self.torch
+        if self._admit_stream is None:
+            self._admit_stream = torch.cuda.Stream(self.dev)
+        with torch.cuda.device(self.dev), torch.cuda.stream(self._admit_stream):
+            st = self._admit_on_stream(req)
+            self._admit_stream.synchronize()
+        with self._state_lock:
+            self.state[req.rid] = st
+
+    def _admit_on_stream(self, req: Request) -> dict:
         from .engine import JKEngine
         from .synth import device_b_packed, make_H, make_n
 
@@ -241,13 +301,13 @@ class DeviceExecutor(Executor):
         if self.with_jk:
             outs["J"] = torch.empty((n5, nbf, nbf), dtype=torch.float64, device=self.dev)
             outs["K"] = torch.empty((n5, nbf, nbf), dtype=torch.float64, device=self.dev)
-        self.state[req.rid] = {"eng": eng, "C": C, "H": H, "n": torch.from_numpy(n).to(self.dev), "outs": outs,
-                               "gen": g}
+        st = {"eng": eng, "C": C, "H": H, "n": torch.from_numpy(n).to(self.dev), "outs": outs, "gen": g}
         if self.occupation:
             ncwo = max(0, (n5 - req.ndoc) // req.ndoc)
             pn = JKEngine.pnof(0, req.ndoc, 0, ncwo, ipnof=8, occ_method="Softmax")
             gamma = torch.full((req.ndoc * ncwo,), -2.0, dtype=torch.float64, device=self.dev)
-            self.state[req.rid].update(pnof=pn, gamma=gamma, occ_out=None)
+            st.update(pnof=pn, gamma=gamma, occ_out=None)
+        return st
 
     def _group(self, reqs):
         from .batch import JKGroup
@@ -255,8 +315,10 @@ class DeviceExecutor(Executor):
         key = tuple(r.rid for r in reqs)
         grp = self._groups.get(key)
         if grp is None:
+            with self._state_lock:
+                live = set(self.state)
             for k in list(self._groups):
-                if not set(k) <= set(self.state):
+                if not set(k) <= live:
                     self._groups.pop(k).close()
             grp = JKGroup([self.state[r.rid]["eng"] for r in reqs])
             self._groups[key] = grp
@@ -294,7 +356,8 @@ class DeviceExecutor(Executor):
         torch.cuda.current_stream(self.dev).synchronize()
 
     def release(self, req: Request) -> None:
-        s = self.state.pop(req.rid)
+        with self._state_lock:
+            s = self.state.pop(req.rid)
         req.energies = [float(e) for e in req.energies]
         if self.keep_final_C:
             req.final_C = s["C"].cpu().numpy()

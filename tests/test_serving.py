@@ -154,3 +154,67 @@ def test_served_request_is_not_resubmitted():
     assert r.builds_done == r.n_builds
     with pytest.raises(ValueError):
         Server(FakeExecutor(clock), hbm_budget=1e12, clock=clock, sleep=clock.sleep).submit(r)
+
+
+class ThreadedExecutor(Executor):
+    """Admissions on the server's worker thread (real clock): request 1's admission is slow."""
+
+    concurrent_admission = True
+
+    def __init__(self, fail_rids=()):
+        import threading
+
+        self.fail = set(fail_rids)
+        self.live = set()
+        self.admit_threads = set()
+        self.in_admission = threading.Event()
+        self.steps_during_admission = 0
+        self.order = []
+
+    def footprint(self, req):
+        return 1000
+
+    def admit(self, req):
+        import threading
+        import time
+
+        self.admit_threads.add(threading.current_thread().name)
+        if req.rid in self.fail:
+            raise MemoryError("injected allocation failure")
+        if req.rid == 1:
+            self.in_admission.set()
+            time.sleep(0.05)
+            self.in_admission.clear()
+        self.live.add(req.rid)
+
+    def step(self, reqs):
+        import time
+
+        assert all(r.rid in self.live for r in reqs)
+        if self.in_admission.is_set():
+            self.steps_during_admission += 1
+        for r in reqs:
+            if r.builds_done == 0:
+                self.order.append(r.rid)
+            r.energies.append(float(r.rid))
+        time.sleep(0.002)
+
+    def release(self, req):
+        self.live.discard(req.rid)
+
+
+def test_concurrent_admission_overlaps_steps():
+    ex = ThreadedExecutor(fail_rids=(2,))
+    srv = Server(ex, hbm_budget=1e6)
+    reqs = [make(0, 10, arrival=0.0, n=40), make(1, 10, arrival=0.005, n=5), make(2, 10, arrival=0.006, n=5),
+            make(3, 10, arrival=0.007, n=5)]
+    for r in reqs:
+        srv.submit(r)
+    srv.run(timeout=30)
+    assert sorted(r.rid for r in srv.finished) == [0, 1, 3]
+    assert [r.rid for r in srv.rejected] == [2] and "injected" in srv.rejected[0].error
+    assert srv.in_use == 0 and not srv.admitting
+    assert ex.steps_during_admission > 0  # request 0 kept stepping while request 1 was admitted
+    assert all(name.startswith("admit") for name in ex.admit_threads)
+    assert ex.order == [0, 1, 3]  # requests join in arrival order
+    assert all(len(r.energies) == r.n_builds for r in srv.finished)
