@@ -358,7 +358,8 @@ class DeviceExecutor(Executor):
     def release(self, req: Request) -> None:
         with self._state_lock:
             s = self.state.pop(req.rid)
-        req.energies = [float(e) for e in req.energies]
+        if req.energies:  # one device-to-host copy for the request's energies
+            req.energies = self.torch.stack(req.energies).cpu().tolist()
         if self.keep_final_C:
             req.final_C = s["C"].cpu().numpy()
             req.final_outs = {k: v.cpu().numpy() for k, v in s["outs"].items()}
@@ -367,4 +368,6 @@ class DeviceExecutor(Executor):
         for k in list(self._groups):
             if req.rid in k:
                 self._groups.pop(k).close()
+        # In line: freeing the engine on a background thread measured no faster (its frees
+        # slowed the loop's launches and group set-up by as much as they saved).
         s["eng"].close()
