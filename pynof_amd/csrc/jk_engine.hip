@@ -1293,6 +1293,7 @@ struct pnjk_group {
   hipStream_t cap = nullptr;            // capture stream
   std::vector<BuildState> st;
   std::vector<char> sig;                // signature of the cached plan
+  std::vector<char> in_sig;             // C and H pointers the captured graph reads
   std::vector<GroupCollector> plan;     // grouped stages: HT, QT, J_MO, J^n, K^n
   std::vector<SymJob> sym;              // symmetrize jobs of the J^n outputs
   size_t sym_off = 0;                   // their offset in the plan buffer
@@ -1403,7 +1404,10 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
   auto at = [](double* const* a, size_t i) { return a ? a[i] : nullptr; };
   int rc;
   g->st.resize(n);
-  std::vector<char> sig;
+  // The plan (problem structs, tile tables) refers to the engines' buffers and the outputs;
+  // C and H are only read by the per-call staging, so new C or H tensors (an optimiser's
+  // next orbitals) reuse the plan and only invalidate a captured graph.
+  std::vector<char> sig, in_sig;
   sig_put(sig, precision);
   sig_put(sig, outputs);
   for (size_t i = 0; i < n; ++i) {
@@ -1416,7 +1420,14 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
     if (use_graph && (!st.c_dev || (st.want_h && !st.h_dev)))
       return set_error(PNJK_EINVAL, "pnjk_group_build: graph mode needs device-resident C and H");
     if (precision != PNJK_FP64 && (rc = ensure_split_planes(g->eng[i], s))) return rc;
-    sig_put(sig, st.a);
+    for (const void* ptr : {(const void*)st.a.J, (const void*)st.a.K, (const void*)st.a.J_MO,
+                            (const void*)st.a.K_MO, (const void*)st.a.H_core})
+      sig_put(sig, ptr);
+    sig_put(sig, st.a.nbf5);
+    sig_put(in_sig, st.a.C);
+    sig_put(in_sig, st.a.ldc);
+    sig_put(in_sig, st.a.H);
+    sig_put(in_sig, st.a.ldh);
     sig_put(sig, st.ht);
     sig_put(sig, st.qt);
     sig_put(sig, st.jt);
@@ -1485,10 +1496,12 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
     g->sig = sig;
     g->graph_valid = false;
   } else {
-    for (size_t i = 0; i < n; ++i) {  // quarter-transform tile counts for the K_MO reduction
-      BuildState& st = g->st[i];
-      st.nt = st.g.naux_p / (32 * st.qt.fn);
-    }
+    for (size_t i = 0; i < n; ++i)  // K_MO partials per row pair: one per 64 aux columns, as
+      g->st[i].nt = g->st[i].g.naux_p / 64;  // the quarter transform's launch sets it (any tile width)
+  }
+  if (in_sig != g->in_sig) {
+    g->in_sig = in_sig;
+    g->graph_valid = false;
   }
   if (!use_graph) return group_enqueue(g, s);
   if (!g->graph_valid) {

@@ -3,7 +3,7 @@ fp64, split and split_refine forms, eagerly and from a replayed hipGraph."""
 import numpy as np
 import pytest
 
-from pynof_amd import JKEngine, synth
+from pynof_amd import JKEngine, autotune, synth
 from pynof_amd.batch import JKGroup
 
 pytestmark = pytest.mark.gpu
@@ -95,3 +95,39 @@ def test_group_rejects_host_outputs(setup):
                                      None, None, None, 0, None)
     assert rc == _lib.EINVAL
     g.close()
+
+
+# Quarter-transform tiles 128 columns wide (fn = 4) as well as the small defaults: the
+# reused plan's K_MO reduction must count 64-column partials for either width.
+WIDE_QT = {"split:100:360:100": {"quarter_transform": [5, 4]}, "fp64:100:360:100": {"quarter_transform": [2, 4]}}
+
+
+@pytest.mark.parametrize("tuned", [{}, WIDE_QT], ids=["default-tiles", "wide-qt"])
+def test_group_new_C_tensors_reuse_plan(setup, tuned):
+    """An optimiser hands a new C tensor each iteration: the group keeps its plan
+    (eager) and re-captures its graph (use_graph), and every build matches eager
+    single builds of the same C, bitwise."""
+    import torch
+
+    engs, Cs, Hs, n5s = setup
+    print(f"case: group with new C tensors, eager then graph, split and fp64, tiles {tuned}", flush=True)
+    g = JKGroup(engs, tuned=tuned)
+    outs = g.alloc_outputs(n5s)
+    keep = []  # earlier C tensors stay alive, so every iteration's C has new addresses
+    for rep, (prec, graph) in enumerate((("split", False), ("split", False), ("fp64", False), ("fp64", False),
+                                         ("split_refine", False), ("split", True), ("split", True))):
+        Cnew = [Cs[i] @ torch.from_numpy(synth.rotation(c.shape[0], 7500 + 10 * rep + i)).cuda()
+                for i, c in enumerate(Cs)]
+        keep.append(Cnew)
+        torch.cuda.synchronize()
+        g.build(Cnew, n5s, Hs, precision=prec, outs=outs, use_graph=graph)
+        torch.cuda.synchronize()
+        got = [{k: v.cpu().numpy() for k, v in o.items()} for o in outs]
+        ref = alone(engs, Cnew, Hs, n5s, prec)
+        for i in range(len(engs)):
+            for k in OUTS:
+                assert np.array_equal(got[i][k], ref[i][k]), (rep, prec, graph, i, k)
+    g.close()
+    for e in engs:  # the group set these engines' tiles; back to the built-in choices
+        autotune.apply(e, "split", None)
+        autotune.apply(e, "fp64", None)
