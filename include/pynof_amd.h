@@ -234,8 +234,11 @@ int pnjk_energy(pnjk_engine* e, int64_t nbf5, int64_t nbeta, int64_t nalpha, int
  * Outputs must be device memory. use_graph = 1 captures the launch sequence
  * in a hipGraph on the first call (and again whenever any pointer, size, tile
  * or precision changes) and replays it afterwards; it needs device-resident C
- * and H. Results are bitwise those of pnjk_jk_build on each engine alone. An
- * engine must not be used elsewhere while its group builds. */
+ * and H. The launch plan (problem structs and tile tables on the device) is
+ * kept across calls whose engines, outputs, sizes, precision and tiles are
+ * unchanged; new C or H arrays reuse it. Results are bitwise those of
+ * pnjk_jk_build on each engine alone. An engine must not be used elsewhere
+ * while its group builds. */
 typedef struct pnjk_group pnjk_group;
 int pnjk_group_create(pnjk_engine* const* engines, int count, pnjk_group** out);
 int pnjk_group_destroy(pnjk_group* g);
