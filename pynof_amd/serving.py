@@ -167,6 +167,23 @@ class Server:
             if self._pool is not None:
                 self._pool.shutdown(wait=True)
                 self._pool = None
+            self._abandon()
+
+    def _abandon(self) -> None:
+        """After the loop stopped on an error (a time-out, a failed step): releases the
+        requests it admitted but did not finish, so their device memory is freed, and lists
+        them with the rejected ones (done stays None); a normal end leaves nothing to release."""
+        left = [r for r, fut in self.admitting if fut.done() and fut.exception() is None] + self.active
+        self.admitting.clear()
+        self.active = []
+        for r in left:
+            r.error = r.error or "not finished: the serving loop stopped"
+            try:
+                self.ex.release(r)
+            except Exception:  # best effort: keep releasing the others
+                pass
+            self.in_use -= r.footprint
+            self.rejected.append(r)
 
     def _loop(self, end: float) -> None:
         while self.pending or self.active or self.admitting:

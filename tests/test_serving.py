@@ -122,6 +122,9 @@ def test_timeout():
     srv.submit(make(0, 10, n=100))
     with pytest.raises(TimeoutError):
         srv.run(t0=0.0, timeout=5.0)
+    # the unfinished request is released (its device memory freed) and marked
+    assert not ex.live and ex.released == [0] and srv.in_use == 0 and not srv.active
+    assert [r.rid for r in srv.rejected] == [0] and "not finished" in srv.rejected[0].error
 
 
 def test_poisson_requests_shape():
