@@ -83,16 +83,27 @@ def tune(eng, build, arith: str, reps: int = 2, stages=None, log=None):
     return chosen
 
 
+_loaded: dict = {}  # path -> (modification time, table): groups are created often (serving)
+
+
 def load(path: str = DEFAULT_TABLE) -> dict:
-    if not os.path.exists(path):
+    """The table at `path` ({} if absent); parsed again only when the file changes.
+    Returns a fresh top-level dict; treat the entries as read-only."""
+    try:
+        mtime = os.stat(path).st_mtime_ns
+    except FileNotFoundError:
         return {}
-    with open(path) as f:
-        return json.load(f)
+    hit = _loaded.get(path)
+    if hit is None or hit[0] != mtime:
+        with open(path) as f:
+            hit = _loaded[path] = (mtime, json.load(f))
+    return dict(hit[1])
 
 
 def save(table: dict, path: str = DEFAULT_TABLE) -> None:
     with open(path, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
+    _loaded.pop(path, None)
 
 
 def lookup(table: dict, precision: str, nbf: int, naux: int, nbf5: int):
