@@ -221,3 +221,21 @@ def test_concurrent_admission_overlaps_steps():
     assert all(name.startswith("admit") for name in ex.admit_threads)
     assert ex.order == [0, 1, 3]  # requests join in arrival order
     assert all(len(r.energies) == r.n_builds for r in srv.finished)
+
+
+def test_small_rotation_exponential_matches_expm():
+    import numpy as np
+    import scipy.linalg
+    import torch
+
+    from pynof_amd.serving import _expm_small
+
+    g = torch.Generator().manual_seed(3)
+    for n in (1, 24, 150):
+        a = torch.randn((n, n), dtype=torch.float64, generator=g)
+        k = a - a.T
+        if n > 1:
+            k *= 1e-2 / torch.linalg.norm(k)
+        e = _expm_small(k).numpy()
+        assert np.max(np.abs(e - scipy.linalg.expm(k.numpy()))) < 1e-16
+        assert np.max(np.abs(e.T @ e - np.eye(n))) < 1e-14
