@@ -73,6 +73,23 @@ __device__ __forceinline__ void split2(double x, uint16_t& hi, uint16_t& lo) {
   lo = bf16_rne((float)(x - (double)bf16_to_f(hi)));
 }
 
+// Cyclic-window schedule of the quarter transform Y_p[q][P] = sum_nu C[nu][q] X[p][nu][P].
+// Y_p[q] = Y_q[p] (b is symmetric in its two orbital indices), so each unordered pair
+// {p, q} of the n = nbf5 orbitals is needed once. Batch p computes the window rows
+// r < qt_window(n), q = (p + r) mod n: every batch has the same n / 2 + 1 rows (the lower
+// triangle would give batch p its p + 1 rows q <= p). Distances (q - p) mod n below the
+// window are owned by batch p and the others by batch q; for even n the antipodal pair
+// (r = n / 2) is reached from both ends and only the batch p < n / 2 keeps it.
+__host__ __device__ inline int qt_window(int n) { return n / 2 + 1; }
+__host__ __device__ inline int qt_row(int p, int r, int n) {
+  const int q = p + r;
+  return q >= n ? q - n : q;
+}
+__host__ __device__ inline bool qt_owns(int p, int r, int n) { return !((n & 1) == 0 && 2 * r == n && 2 * p >= n); }
+// Columns of the FP64 quarter transform's window copy of C (jk_f64.hip): batch offsets below
+// n plus whole tiles of at most 160 rows over the window.
+__host__ __device__ inline int qt_window_cols(int n) { return (n + qt_window(n) + 160 + 31) / 32 * 32; }
+
 // Row index of the packed lower triangle: t = mu*(mu+1)/2 + nu for mu >= nu.
 __device__ __forceinline__ int64_t tri_index(int a, int b) {
   const int64_t hi = a > b ? a : b, lo = a > b ? b : a;

@@ -95,7 +95,7 @@ TileF64 default_tile(int arith, int stage_kind, int nbf, int nbf5) {
   if (arith == 0) {
     switch (stage_kind) {
       case 0: return big ? TileF64{5, 4} : TileF64{2, 2};
-      case 1: return TileF64{2, 2};
+      case 1: return big ? TileF64{5, 2} : TileF64{2, 2};
       case 2: return big ? TileF64{5, 4} : TileF64{2, 2};
       default: return big ? TileF64{4, 4} : TileF64{2, 2};
     }
@@ -134,6 +134,7 @@ struct pnjk_engine {
   DevBuf btri;
   DevBuf bt16h, bt16l, bht16h, bht16l;  // split planes of b, built on the first split build
   DevBuf Cp, Hp, X, d, kpart, T, outs, stage, scal;
+  DevBuf Cw;                                     // window copies of C for the FP64 quarter transform
   DevBuf ct16h, ct16l, x16h, x16l, d16h, d16l;  // split workspace
   DevBuf Cf, ctf16h, ctf16l, hc;                 // MO integrals: all columns of C; H_core scratch
   DevBuf bmo, hm, lagw, lagg, lagp, lagc;        // Lagrangian workspace: b_MO, H_mat, W, G, partials, n/C^J/C^K
@@ -145,12 +146,12 @@ struct pnjk_engine {
   // Tile override per arithmetic (0: FP64 kernels, 1: BF16x3 kernels) and stage;
   // {0, 0} selects the built-in choice.
   TileF64 tiles[2][ST_COUNT] = {};
-  DevBuf* all[41];
+  DevBuf* all[42];
   int nall = 0;
   pnjk_engine() {
     for (DevBuf* b : {&btri, &bt16h, &bt16l, &bht16h, &bht16l, &Cp, &Hp, &X, &d, &kpart, &T, &outs, &stage, &scal,
                       &ct16h, &ct16l, &x16h, &x16l, &d16h, &d16l, &Cf, &ctf16h, &ctf16l, &hc,
-                      &bmo, &hm, &lagw, &lagg, &lagp, &lagc, &bm16h, &bm16l, &w16h, &w16l, &fjk, &fh, &fco, &ff, &occ})
+                      &bmo, &hm, &lagw, &lagg, &lagp, &lagc, &bm16h, &bm16l, &w16h, &w16l, &fjk, &fh, &fco, &ff, &occ, &Cw})
       all[nall++] = b;
   }
 };
@@ -226,6 +227,7 @@ uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int preci
   s += (uint64_t)(roundup(nbf5, 160) * naux_p) * 8;          // d
   s += (uint64_t)(nbf5 * nbf5 * (naux_p / 64)) * 8;          // kpart
   s += (uint64_t)(nbf * ldcp) * 8;                           // T
+  if (precision != PNJK_SPLIT) s += (uint64_t)(2 * nbf_p * qt_window_cols((int)nbf5)) * 8;  // Cw
   if (precision != PNJK_FP64) {
     const int64_t ntri = nbf * (nbf + 1) / 2;
     s += (uint64_t)(ntri * naux_p) * 4;                      // btri16 hi/lo
@@ -512,6 +514,8 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st) {
   if (precision != PNJK_SPLIT && (rc = e->X.ensure((size_t)(n5 * nbf * g.naux_p + 64) * 8))) return rc;
   if ((rc = e->d.ensure((size_t)(roundup(n5, 160) * g.naux_p) * 8))) return rc;
   if ((rc = e->kpart.ensure((size_t)(n5 * n5 * (g.naux_p / 64)) * 8))) return rc;
+  if (precision != PNJK_SPLIT && (rc = e->Cw.ensure((size_t)(2 * g.nbf_p * (int64_t)qt_window_cols(g.nbf5)) * 8)))
+    return rc;
 
   st.want_j = a.outputs & PNJK_OUT_J;
   st.want_k = a.outputs & PNJK_OUT_K;
@@ -594,6 +598,8 @@ int stage_in(pnjk_engine* e, BuildState& st, hipStream_t s) {
   if (st.a.precision == PNJK_SPLIT)
     PNJK_HIP_CHECK(launch_split_ct(Cp, g.ldcp, g.nbf, g.nbf5, g.nbf5, st.sg.ldk, e->ct16h.as<uint16_t>(),
                                    e->ct16l.as<uint16_t>(), s));
+  else
+    PNJK_HIP_CHECK(launch_window_c(Cp, g.ldcp, g.nbf_p, g.nbf5, qt_window_cols(g.nbf5), e->Cw.as<double>(), s));
   return PNJK_OK;
 }
 
@@ -616,8 +622,8 @@ int stage_qt(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col)
                                 e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), e->kpart.as<double>(),
                                 e->d.as<double>(), e->d16h.as<uint16_t>(), e->d16l.as<uint16_t>(), &st.nt, s, col));
   } else {
-    PNJK_HIP_CHECK(launch_qt_f64(st.g, st.qt, e->Cp.as<double>(), e->X.as<double>(), e->kpart.as<double>(),
-                                 e->d.as<double>(), &st.nt, s, col));
+    PNJK_HIP_CHECK(launch_qt_f64(st.g, st.qt, e->Cw.as<double>(), qt_window_cols(st.g.nbf5), e->X.as<double>(),
+                                 e->kpart.as<double>(), e->d.as<double>(), &st.nt, s, col));
   }
   return PNJK_OK;
 }

@@ -305,13 +305,20 @@ struct ProbHT {
   }
 };
 
+// Batch p: Y_p[q][P] = sum_nu C[nu][q] X[p][nu][P] for the cyclic window of rows
+// q = (p + r) mod nbf5, r < h = qt_window(nbf5) (jk_common.h), so every batch has the same
+// nbf5 / 2 + 1 rows. The A operand is the window copy of C: Cw0[nu][j] = C[nu][j mod nbf5]
+// and Cw1[nu][j] = C[nu][(j + 1) mod nbf5] (rows nu >= nbf zero), so window row r of batch p
+// is column p + r of Cw0, or column p - 1 + r of Cw1 for odd p: a contiguous column range at
+// an even offset, as the pair loads need. Writes d[p][P] = Y_p[p][P] (window row 0) and the
+// K_MO row-sum partials of the pairs the batch owns at (max(p, q), min(p, q)).
 struct ProbQT {
   static constexpr int kA = KM, kB = KM;
-  const double* Cp;
+  const double* Cw;  // Cw0, then Cw1 at Cw + nbf_p * ldw
   const double* X;
   double* kpart;
   double* d;
-  int nbf, naux_p, nbf5, ldcp, nbf_p, mtiles, ntiles, nt64;  // nt64 = naux_p / 64 (kpart stride)
+  int nbf, naux_p, nbf5, ldw, nbf_p, mtiles, ntiles, nt64, h;  // nt64 = naux_p / 64 (kpart stride)
   __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.mt = id % mtiles;
     const int rest = id / mtiles;
@@ -319,30 +326,34 @@ struct ProbQT {
     t.batch = rest / ntiles;
     t.m0 = t.mt * BM;
     t.n0 = t.nt * BN;
-    return t.m0 <= t.batch;  // only q <= p is needed (Y_p[q] = Y_q[p])
+    return true;
   }
-  // only q <= p is needed (Y_p[q] = Y_q[p])
-  __device__ bool block_active(const Tile& t, int r0, int) const { return r0 <= t.batch && r0 < nbf5; }
+  __device__ bool block_active(const Tile&, int r0, int) const { return r0 < h; }
   __host__ __device__ int ksteps(int BK) const { return nbf_p / BK; }
-  __device__ SrcKMDense a_src(const Tile&) const { return {Cp, ldcp}; }
+  __device__ SrcKMDense a_src(const Tile& t) const {
+    const int p = t.batch;
+    return {(p & 1) ? Cw + (int64_t)nbf_p * ldw + (p - 1) : Cw + p, ldw};
+  }
   __device__ SrcKMRows b_src(const Tile& t) const {
     return {X + (int64_t)t.batch * nbf * naux_p, naux_p, nbf};
+  }
+  // kpart slot of window row w of batch p, or -1 when the row is past the window or its pair
+  // belongs to the other batch.
+  __device__ __forceinline__ int64_t kslot(int p, int w) const {
+    if (w >= h || !qt_owns(p, w, nbf5)) return -1;
+    const int q = qt_row(p, w, nbf5);
+    const int hi = p > q ? p : q, lo = p > q ? q : p;
+    return ((int64_t)hi * nbf5 + lo) * nt64;
   }
   template <int FM, int FN>
   __device__ void epilogue(const Tile& t, v4d (&acc)[FM][FN], int wm, int wn, int lane, double* lds) const {
     const int p = t.batch;
-    // d[p][P] = Y_p[p][P]
-    if (p >= t.m0 && p < t.m0 + 32 * FM) {
+    if (t.m0 == 0 && wm == 0 && lane < 16) {  // window row 0 is q = p: d[p][P] = Y_p[p][P]
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            int dm, dn;
-            elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
-            if (t.m0 + dm == p && t.n0 + dn < naux_p) gmem(d)[(int64_t)p * naux_p + t.n0 + dn] = acc[i][j][r];
-          }
+      for (int j = 0; j < FN; ++j) {
+        const int n = t.n0 + wn + 16 * j + lane;
+        if (n < naux_p) gmem(d)[(int64_t)p * naux_p + n] = acc[0][j][0];
+      }
     }
     // Row sums of squares per 64 aux columns: a 64-column partial is always
     // S(first 32 columns) + S(last 32 columns), S being one wave's sum over two
@@ -356,13 +367,13 @@ struct ProbQT {
       for (int r = 0; r < 4; ++r) {
         double tot = 0.0;
 #pragma unroll
-        for (int h = 0; h < FN / 2; ++h) {
+        for (int hh = 0; hh < FN / 2; ++hh) {
           double v = 0.0;
 #pragma unroll
-          for (int j = 2 * h; j < 2 * h + 2; ++j) v += acc[i][j][r] * acc[i][j][r];
+          for (int j = 2 * hh; j < 2 * hh + 2; ++j) v += acc[i][j][r] * acc[i][j][r];
 #pragma unroll
           for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
-          tot = h == 0 ? v : tot + v;
+          tot = hh == 0 ? v : tot + v;
         }
         s[i][r] = tot;
       }
@@ -374,8 +385,8 @@ struct ProbQT {
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int q = t.m0 + wm + 16 * i + (lane >> 4) + 4 * r;
-            if (q <= p && q < nbf5) gmem(kpart)[((int64_t)p * nbf5 + q) * nt64 + c0 / 64] = s[i][r];
+            const int64_t k = kslot(p, t.m0 + wm + 16 * i + (lane >> 4) + 4 * r);
+            if (k >= 0) gmem(kpart)[k + c0 / 64] = s[i][r];
           }
       }
       (void)wave;
@@ -394,9 +405,8 @@ struct ProbQT {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int dm = wm + 16 * i + (lane >> 4) + 4 * r;
-            const int q = t.m0 + dm;
-            if (q <= p && q < nbf5)
-              gmem(kpart)[((int64_t)p * nbf5 + q) * nt64 + t.n0 / 64] = s[i][r] + lds[dm];
+            const int64_t k = kslot(p, t.m0 + dm);
+            if (k >= 0) gmem(kpart)[k + t.n0 / 64] = s[i][r] + lds[dm];
           }
       }
     }
@@ -954,27 +964,49 @@ hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const
   return launch_ht_f64_t<false>(g, tc, Cp, btri, X, s, nullptr, nullptr, col);
 }
 
-hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* X, double* kpart,
+hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cw, int ldw, const double* X, double* kpart,
                          double* d, int* ntiles, hipStream_t s, GroupCollector* col) {
   // naux_p is a multiple of 64; a 128-column tile may cover 64 columns past it,
   // which the B loader reads from the workspace slack of X and the epilogue drops.
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
   if (tc.fn != 2 && tc.fn != 4) return hipErrorInvalidValue;
   ProbQT pr;
-  pr.Cp = Cp;
+  pr.Cw = Cw;
   pr.X = X;
   pr.kpart = kpart;
   pr.d = d;
   pr.nbf = g.nbf;
   pr.naux_p = g.naux_p;
   pr.nbf5 = g.nbf5;
-  pr.ldcp = g.ldcp;
+  pr.ldw = ldw;
   pr.nbf_p = g.nbf_p;
-  pr.mtiles = cdiv(g.nbf5, BM);
+  pr.h = qt_window(g.nbf5);
+  pr.mtiles = cdiv(pr.h, BM);
+  // The window copy must cover every column a tile reads, (p or p - 1) + mtiles * BM - 1 <=
+  // nbf5 - 2 + mtiles * BM, at an even row stride.
+  if ((int64_t)pr.mtiles * BM + g.nbf5 - 1 > ldw || (ldw & 1)) return hipErrorInvalidValue;
   pr.ntiles = cdiv(g.naux_p, BN);
   pr.nt64 = g.naux_p / 64;
   *ntiles = pr.nt64;
   return launch_prob(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s, col);
+}
+
+// Cw0[nu][j] = Cp[nu][j mod n], Cw1[nu][j] = Cp[nu][(j + 1) mod n] for nu < rows (Cp rows
+// past nbf are zero).
+__global__ void k_window_c(const double* __restrict__ Cp, int ldcp, int rows, int n, int ldw, double* __restrict__ Cw) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)rows * ldw;
+  if (idx >= 2 * per) return;
+  const int which = (int)(idx / per);
+  const int64_t e = idx - which * per;
+  const int nu = (int)(e / ldw), j = (int)(e % ldw);
+  gmem(Cw)[idx] = Cp[(int64_t)nu * ldcp + (j + which) % n];
+}
+
+hipError_t launch_window_c(const double* Cp, int ldcp, int rows, int n, int ldw, double* Cw, hipStream_t s) {
+  const int64_t total = 2 * (int64_t)rows * ldw;
+  hipLaunchKernelGGL(k_window_c, dim3((unsigned)cdiv(total, 256)), dim3(256), 0, s, Cp, ldcp, rows, n, ldw, Cw);
+  return hipGetLastError();
 }
 
 hipError_t launch_symmetrize(int batch, int n, double* O, int64_t ld, int64_t stride, hipStream_t s) {

@@ -97,11 +97,15 @@ hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const
                          hipStream_t s, uint16_t* x_hi = nullptr, uint16_t* x_lo = nullptr,
                          GroupCollector* col = nullptr);
 
-// Y_p[q][P] = sum_nu Cp[nu][q] X[p][nu][P] for q <= p; writes row sums of Y_p^2
-// per P-tile into kpart[(p*nbf5+q)*nt + tile] and d[p][P] = Y_p[p][P].
-// Returns the number of P-tiles in *ntiles.
-hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* X, double* kpart,
+// Y_p[q][P] = sum_nu C[nu][q] X[p][nu][P] over the cyclic window of rows q (qt_window);
+// writes the row sums of Y_p[q]^2 per 64 aux columns of the pairs batch p owns into
+// kpart[(max(p,q)*nbf5 + min(p,q))*nt + tile] and d[p][P] = Y_p[p][P]. Cw: the two window
+// copies of C (launch_window_c), ldw >= qt_window_cols(nbf5). Returns the number of
+// 64-column partials in *ntiles.
+hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cw, int ldw, const double* X, double* kpart,
                          double* d, int* ntiles, hipStream_t s, GroupCollector* col = nullptr);
+// Cw (2 x rows x ldw): Cw0[nu][j] = Cp[nu][j mod n], Cw1[nu][j] = Cp[nu][(j + 1) mod n].
+hipError_t launch_window_c(const double* Cp, int ldcp, int rows, int n, int ldw, double* Cw, hipStream_t s);
 
 // K_MO[p][q] = K_MO[q][p] = sum_t kpart[(p*nbf5+q)*nt + t]  (q <= p)
 // Orbital-gradient MO integrals: b_MO[p][q][P] (p < nbf, q < nbf5, P < naux) from X

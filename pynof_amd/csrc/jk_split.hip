@@ -337,16 +337,26 @@ struct ProbHT {
   }
 };
 
-// Batch p: Y_p[q][P] = sum_nu Ct[q][nu] X16[p][nu][P] for q <= p; writes d[p][P]
-// (FP64 and split planes) and per-64-column row sums of Y_p^2 (FP64) into kpart.
-// Tiles are 64 or 128 aux columns wide (FN = 2 or 4); both give the same kpart
-// bits: a 64-column partial is always S(first 32 columns) + S(last 32 columns),
+// Batch p: Y_p[q][P] = sum_nu Ct[q][nu] X16[p][nu][P] for the cyclic window of rows
+// q = (p + r) mod nbf5, r < qt_window(nbf5) (jk_common.h): every batch carries nbf5 / 2 + 1
+// live rows, so a 160-row tile holds a whole window for nbf5 <= 318 and copies only the C^T
+// rows it uses. Writes d[p][P] = Y_p[p][P] (window row 0; FP64 and split planes) and the
+// per-64-column sums of Y_p[q]^2 (FP64) of the pairs the batch owns into kpart at
+// (max(p, q), min(p, q)). Tiles are 64 or 128 aux columns wide (FN = 2 or 4); both give the
+// same kpart bits: a 64-column partial is always S(first 32 columns) + S(last 32 columns),
 // S being one wave's sum over two 16-column blocks reduced across 16 lanes.
 struct SrcXT {  // element (k = nu, n = P) at p*nbf*naux_p + min(nu, nbf-1)*naux_p + P
   Planes p;
   int64_t base, ld;
   int kmax;
   __device__ __forceinline__ int64_t off(int k) const { return base + (int64_t)(k < kmax ? k : kmax - 1) * ld; }
+};
+
+struct SrcWindow {  // window row r of batch b: C^T row qt_row(b, min(r, h - 1), n), k contiguous
+  Planes p;
+  int64_t ld;
+  int b, h, n;
+  __device__ __forceinline__ int64_t off(int r) const { return (int64_t)qt_row(b, r < h ? r : h - 1, n) * ld; }
 };
 
 struct ProbQT {
@@ -358,7 +368,7 @@ struct ProbQT {
   double* d;
   uint16_t* dhi;
   uint16_t* dlo;
-  int nbf, naux_p, nbf5, ldk, mtiles, ntiles, nt64;  // nt64 = naux_p / 64 (kpart stride)
+  int nbf, naux_p, nbf5, ldk, mtiles, ntiles, nt64, h;  // nt64 = naux_p / 64 (kpart stride); h window rows
   __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.mt = id % mtiles;
     const int rest = id / mtiles;
@@ -366,34 +376,40 @@ struct ProbQT {
     t.batch = rest / ntiles;
     t.m0 = t.mt * BM;
     t.n0 = t.nt * BN;
-    return t.m0 <= t.batch;
+    return true;
   }
   __host__ __device__ int ksteps() const { return ldk / kBK; }
-  __device__ int row_limit(const Tile& t) const { return t.batch + 1; }  // q <= p
-  __device__ SrcRows a_src(const Tile&) const { return {ct, 0, ldk, nbf5}; }
+  __device__ int row_limit(const Tile&) const { return h; }
+  __device__ SrcWindow a_src(const Tile& t) const { return {ct, ldk, t.batch, h, nbf5}; }
   __device__ SrcXT b_src(const Tile& t) const { return {x, (int64_t)t.batch * nbf * naux_p, naux_p, nbf}; }
+  // kpart slot of window row w of batch p, or -1 when the row is past the window or its pair
+  // belongs to the other batch.
+  __device__ __forceinline__ int64_t kslot(int p, int w) const {
+    if (w >= h || !qt_owns(p, w, nbf5)) return -1;
+    const int q = qt_row(p, w, nbf5);
+    const int hi = p > q ? p : q, lo = p > q ? q : p;
+    return ((int64_t)hi * nbf5 + lo) * nt64;
+  }
   template <int FM, int FN>
   __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double* lds) const {
     static_assert(FN == 2 || FN == 4, "quarter-transform tiles are 64 or 128 aux columns wide");
     const int p = t.batch;
-    if (p >= t.m0 && p < t.m0 + 32 * FM) {
+    if (t.m0 == 0 && wm == 0) {  // window row 0 is q = p: d[p][P] = Y_p[p][P]
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+      for (int j = 0; j < FN; ++j)
 #pragma unroll
-        for (int j = 0; j < FN; ++j)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            int dm, dn;
-            coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
-            if (t.m0 + dm == p && t.n0 + dn < naux_p) {
-              const int64_t o = (int64_t)p * naux_p + t.n0 + dn;
-              const float v = acc[i][j][r];
-              gmem(d)[o] = v;
-              const uint16_t h = bf16_rne(v);
-              gmem(dhi)[o] = h;
-              gmem(dlo)[o] = bf16_rne(v - bf16_to_f(h));
-            }
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          coords<FM, FN>(lane, wm, wn, 0, j, r, dm, dn);
+          if (dm == 0 && t.n0 + dn < naux_p) {
+            const int64_t o = (int64_t)p * naux_p + t.n0 + dn;
+            const float v = acc[0][j][r];
+            gmem(d)[o] = v;
+            const uint16_t hb = bf16_rne(v);
+            gmem(dhi)[o] = hb;
+            gmem(dlo)[o] = bf16_rne(v - bf16_to_f(hb));
           }
+        }
     }
     double s[FM][4];
 #pragma unroll
@@ -402,13 +418,13 @@ struct ProbQT {
       for (int r = 0; r < 4; ++r) {
         double tot = 0.0;
 #pragma unroll
-        for (int h = 0; h < FN / 2; ++h) {
+        for (int hh = 0; hh < FN / 2; ++hh) {
           double v = 0.0;
 #pragma unroll
-          for (int j = 2 * h; j < 2 * h + 2; ++j) v += (double)acc[i][j][r] * (double)acc[i][j][r];
+          for (int j = 2 * hh; j < 2 * hh + 2; ++j) v += (double)acc[i][j][r] * (double)acc[i][j][r];
 #pragma unroll
           for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
-          tot = h == 0 ? v : tot + v;
+          tot = hh == 0 ? v : tot + v;
         }
         s[i][r] = tot;
       }
@@ -420,8 +436,8 @@ struct ProbQT {
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int q = t.m0 + wm + 16 * i + out_row(lane, r);
-            if (q <= p && q < nbf5) gmem(kpart)[((int64_t)p * nbf5 + q) * nt64 + c0 / 64] = s[i][r];
+            const int64_t k = kslot(p, t.m0 + wm + 16 * i + out_row(lane, r));
+            if (k >= 0) gmem(kpart)[k + c0 / 64] = s[i][r];
           }
       }
       (void)wave;
@@ -440,8 +456,8 @@ struct ProbQT {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int dm = wm + 16 * i + out_row(lane, r);
-            const int q = t.m0 + dm;
-            if (q <= p && q < nbf5) gmem(kpart)[((int64_t)p * nbf5 + q) * nt64 + t.n0 / 64] = s[i][r] + lds[dm];
+            const int64_t k = kslot(p, t.m0 + dm);
+            if (k >= 0) gmem(kpart)[k + t.n0 / 64] = s[i][r] + lds[dm];
           }
       }
     }
@@ -830,7 +846,8 @@ hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
   pr.naux_p = g.naux_p;
   pr.nbf5 = g.nbf5;
   pr.ldk = g.ldk;
-  pr.mtiles = cdiv(g.nbf5, BM);
+  pr.h = qt_window(g.nbf5);
+  pr.mtiles = cdiv(pr.h, BM);
   pr.ntiles = cdiv(g.naux_p, BN);
   pr.nt64 = g.naux_p / 64;
   *ntiles = pr.nt64;
