@@ -30,11 +30,8 @@ class JKGroup:
         if not engines:
             raise ValueError("a group needs at least one engine")
         self.engines = list(engines)
-        table = autotune.load() if tuned is None else tuned
-        for e in self.engines:
-            for arith in ("fp64", "split"):
-                entry = autotune.lookup(table, arith, e.nbf, e.naux, e.nbf) if table else None
-                autotune.apply(e, arith, entry)
+        self._table = autotune.load() if tuned is None else tuned
+        self._tiled_for = None  # (nbf, naux, nbf5) per engine of the tiles last applied
         L = _lib.lib()
         handles = _ptr_array([e._h.value for e in self.engines])
         h = ctypes.c_void_p()
@@ -52,6 +49,18 @@ class JKGroup:
             self.close()
         except Exception:
             pass
+
+    def _apply_tiles(self, nbf5s):
+        """Tile shapes of each engine from the tuned table entry nearest its (nbf, naux,
+        nbf5), looked up with the nbf5 of this build (tiles never change results)."""
+        shapes = tuple((e.nbf, e.naux, int(n5)) for e, n5 in zip(self.engines, nbf5s))
+        if shapes == self._tiled_for:
+            return
+        for e, (nbf, naux, n5) in zip(self.engines, shapes):
+            for arith in ("fp64", "split"):
+                entry = autotune.lookup(self._table, arith, nbf, naux, n5) if self._table else None
+                autotune.apply(e, arith, entry)
+        self._tiled_for = shapes
 
     def alloc_outputs(self, nbf5s, outputs=_ORDER, device=None):
         torch = _torch()
@@ -82,6 +91,7 @@ class JKGroup:
         for e, c, n5 in zip(self.engines, Cs, nbf5s):
             if c.dtype != torch.float64 or not c.is_cuda or c.shape[0] != e.nbf or c.shape[1] < n5:
                 raise ValueError("each C must be a float64 device tensor of shape (nbf, >= nbf5)")
+        self._apply_tiles(nbf5s)
         if outs is None:
             outs = self.alloc_outputs(nbf5s, outputs)
         prec = _lib.PRECISIONS[precision]

@@ -1430,6 +1430,11 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
                             (const void*)st.a.K_MO, (const void*)st.a.H_core})
       sig_put(sig, ptr);
     sig_put(sig, st.a.nbf5);
+    // The problem structs carry the instance's geometry: a b re-uploaded with another naux
+    // into the same (grow-only) buffers keeps every pointer but needs a new plan.
+    for (int64_t v : {(int64_t)st.g.nbf, (int64_t)st.g.naux, (int64_t)st.g.naux_p, (int64_t)st.g.nbf5,
+                      st.g.ntri, (int64_t)st.g.ldcp, (int64_t)st.g.nbf_p, (int64_t)st.sg.ldk})
+      sig_put(sig, v);
     sig_put(in_sig, st.a.C);
     sig_put(in_sig, st.a.ldc);
     sig_put(in_sig, st.a.H);

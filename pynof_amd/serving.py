@@ -18,9 +18,14 @@ server:
 * runs continuous batching: every scheduler step advances every active request
   by one build, all of them together in grouped launches (one per form), and
   requests that arrive while a step runs join at the next step;
-* picks each request's form from its energy tolerance: tol >= 1e-6 Eh runs
-  every build split; a tighter tolerance runs the final build, whose energy
-  the request returns, as split_refine (FP64 MO path).
+* picks each request's form from its energy tolerance: every build but the last
+  runs split; the final build, whose energy the request returns, runs split too
+  only when the split form's energy error at the request's size is below the
+  tolerance (``split_error_bound``, from errors measured on the MI355X), and
+  otherwise as split_refine, whose FP64 MO path gives the FP64 energy. For the
+  synthetic systems served here the split error (1e-4 to 1e-2 Eh over nbf 24 to
+  800) is above both the 1e-6 and the 1e-9 Eh tolerance, so every request
+  refines its final build.
 
 The scheduler is independent of the device: `Executor` is the interface, with
 `DeviceExecutor` driving pynof_amd on the GPU.
@@ -35,6 +40,23 @@ from collections import deque
 from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
 
 import numpy as np
+
+# Split-form (BF16x3, FP32 accumulation) energy error against the FP64 form in Eh, measured on
+# one MI355X with the serving generators (b decay 0.05, naux = ceil8(3.6 nbf), nbf5 = nbf,
+# seeded C, H, n): the largest of three seeds per size. |E| grows from 37-54 Eh at nbf = 24 to
+# 1.5e5 Eh at nbf = 800, and the error with it.
+SPLIT_ENERGY_ERROR_EH = ((24, 1.44e-4), (40, 3.58e-4), (60, 3.86e-4), (96, 1.31e-3), (150, 1.63e-3),
+                         (240, 1.88e-3), (400, 3.85e-3), (600, 4.30e-3), (800, 1.02e-2))
+SPLIT_ERROR_MARGIN = 10.0
+
+
+def split_error_bound(nbf: int) -> float:
+    """Bound on the split form's energy error (Eh) for an instance of nbf basis functions:
+    the measured errors interpolated log-log in nbf (held at the ends), times a margin of 10
+    (three seeds per size are few)."""
+    xs = np.log([n for n, _ in SPLIT_ENERGY_ERROR_EH])
+    ys = np.log([e for _, e in SPLIT_ENERGY_ERROR_EH])
+    return SPLIT_ERROR_MARGIN * float(np.exp(np.interp(math.log(max(nbf, 1)), xs, ys)))
 
 
 @dataclasses.dataclass
@@ -58,7 +80,7 @@ class Request:
     footprint: int = 0
 
     def form_for_build(self, i: int) -> str:
-        if self.tol < 1e-6 and i == self.n_builds - 1:
+        if i == self.n_builds - 1 and self.tol < split_error_bound(self.nbf):
             return "split_refine"
         return "split"
 
@@ -173,7 +195,14 @@ class Server:
         """After the loop stopped on an error (a time-out, a failed step): releases the
         requests it admitted but did not finish, so their device memory is freed, and lists
         them with the rejected ones (done stays None); a normal end leaves nothing to release."""
-        left = [r for r, fut in self.admitting if fut.done() and fut.exception() is None] + self.active
+        left = []
+        for r, fut in self.admitting:  # the pool has shut down: every admission has ended
+            exc = fut.exception()
+            if exc is not None:
+                self._fail(r, exc)
+            else:
+                left.append(r)
+        left += self.active
         self.admitting.clear()
         self.active = []
         for r in left:
@@ -207,9 +236,11 @@ class Server:
                 if r.builds_done >= r.n_builds:
                     r.done = t
                     self.active.remove(r)
-                    self.ex.release(r)
-                    self.in_use -= r.footprint
-                    self.finished.append(r)
+                    try:
+                        self.ex.release(r)
+                    finally:  # the request has left the device's books even if release failed
+                        self.in_use -= r.footprint
+                        self.finished.append(r)
 
     def stats(self) -> dict:
         lat = np.array([r.latency for r in self.finished]) if self.finished else np.zeros(0)
@@ -272,16 +303,24 @@ class DeviceExecutor(Executor):
 
     Admission may run on the server's worker thread: it works on a stream of its own and
     synchronises it before returning, so the request's tensors are ready for the build
-    stream; the per-request state table is guarded by a lock."""
+    stream; those tensors are then marked as used on the build stream, so that when a step
+    replaces one (the rotated C, the updated gamma) the caching allocator does not hand its
+    memory back to the admission stream before the build stream's queued reads of it have
+    run. The per-request state table is guarded by a lock.
+
+    ``keep_final_C`` keeps each request's final C and the outputs named in ``keep_outs``
+    (default: all) on the host at release."""
 
     concurrent_admission = True
 
     def __init__(self, device: int = 0, with_jk: bool = True, keep_final_C: bool = False,
-                 occupation: bool = False, occ_lr: float = 0.05):
+                 occupation: bool = False, occ_lr: float = 0.05, keep_outs=None):
         import torch
 
         self.torch = torch
         self.dev = torch.device("cuda", device)
+        self.keep_outs = keep_outs
+        self._build_stream = torch.cuda.current_stream(self.dev)  # the stream step() runs on
         self.device = device
         self.with_jk = with_jk
         self.keep_final_C = keep_final_C
@@ -310,6 +349,9 @@ class DeviceExecutor(Executor):
         with torch.cuda.device(self.dev), torch.cuda.stream(self._admit_stream):
             st = self._admit_on_stream(req)
             self._admit_stream.synchronize()
+        for t in [st["C"], st["H"], st["n"], st.get("gamma"), *st["outs"].values()]:
+            if t is not None:
+                t.record_stream(self._build_stream)
         with self._state_lock:
             self.state[req.rid] = st
 
@@ -394,7 +436,8 @@ class DeviceExecutor(Executor):
             req.energies = self.torch.stack(req.energies).cpu().tolist()
         if self.keep_final_C:
             req.final_C = s["C"].cpu().numpy()
-            req.final_outs = {k: v.cpu().numpy() for k, v in s["outs"].items()}
+            req.final_outs = {k: v.cpu().numpy() for k, v in s["outs"].items()
+                              if self.keep_outs is None or k in self.keep_outs}
             if self.occupation:
                 req.final_gamma = s["gamma"].cpu().numpy()
         for k in list(self._groups):

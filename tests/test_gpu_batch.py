@@ -131,3 +131,36 @@ def test_group_new_C_tensors_reuse_plan(setup, tuned):
     for e in engs:  # the group set these engines' tiles; back to the built-in choices
         autotune.apply(e, "split", None)
         autotune.apply(e, "fp64", None)
+
+
+def test_group_replans_after_reupload_with_smaller_naux(gpu_required):
+    """An engine of a group gets a new b with the same nbf and a smaller naux: its grow-only
+    buffers (and so every pointer) stay, but the geometry differs, so the group must build a
+    new plan; the grouped result equals a single build of the new b."""
+    import torch
+
+    print("case: group re-plan after re-upload with smaller naux", flush=True)
+    sizes = [(64, 232, 64), (48, 176, 40)]
+    engs = [JKEngine(0) for _ in sizes]
+    for i, (nbf, naux, _) in enumerate(sizes):
+        engs[i].upload_b(synth.make_b(nbf, naux, 1300 + i, 0.05))
+    Cs = [torch.from_numpy(synth.make_C(n, 1310 + i)).cuda() for i, (n, _, _) in enumerate(sizes)]
+    Hs = [torch.from_numpy(synth.make_H(n, 1320 + i)).cuda() for i, (n, _, _) in enumerate(sizes)]
+    n5s = [s[2] for s in sizes]
+    g = JKGroup(engs, tuned={})
+    outs = g.alloc_outputs(n5s)
+    for prec in ("fp64", "split"):
+        g.build(Cs, n5s, Hs, precision=prec, outs=outs)
+        torch.cuda.synchronize()
+        engs[0].upload_b(synth.make_b(64, 104, 1330, 0.05))  # same nbf, naux 232 -> 104
+        g.build(Cs, n5s, Hs, precision=prec, outs=outs)
+        torch.cuda.synchronize()
+        for i, e in enumerate(engs):
+            r = e.build(Cs[i], n5s[i], H=Hs[i], precision=prec)
+            torch.cuda.synchronize()
+            for k in OUTS:
+                assert torch.equal(outs[i][k], r[k]), (prec, i, k)
+        engs[0].upload_b(synth.make_b(64, 232, 1300, 0.05))
+    g.close()
+    for e in engs:
+        e.close()

@@ -1,7 +1,9 @@
 """Serving loop on the GPU: every request's energies and final MO integrals are
 the same alone and in the mix (bitwise: grouped builds equal single builds and
-each request's rotations are seeded per request), and a 1e-9 Eh request's final
-energy equals the FP64 form's energy at its final orbitals."""
+each request's rotations are seeded per request), a 1e-9 Eh request's final
+energy equals the FP64 form's energy at its final orbitals, and on the Poisson
+request mix every request's final energy is within its own tolerance (Eh) of
+the FP64 form's."""
 import copy
 
 import numpy as np
@@ -90,3 +92,54 @@ def test_serving_with_occupation_step(gpu_required):
     srv1.submit(r0)
     srv1.run(timeout=300)
     assert srv1.finished[0].energies == mix[r0.rid].energies
+
+
+def test_poisson_mix_tolerances_and_alone(gpu_required):
+    """The Poisson request mix (seed 5000: lognormal nbf, median 150; 20 builds with C
+    rotated by exp(kappa) between builds, minimization.py:200-253, utils.py:550-565),
+    capped at its first 24 requests, arriving at 200 req/s so that they overlap. Every request's
+    final energy is within its own tolerance (Eh, absolute) of the FP64 form's energy at
+    its final orbitals, and every request's energies are the same in the mix as alone."""
+    import torch
+
+    from pynof_amd.serving import poisson_requests, split_error_bound
+
+    print("case: serving poisson seed 5000, 24 requests at 200 req/s", flush=True)
+    keep = ("J_MO", "K_MO", "H_core")
+    rs = poisson_requests(24, rate=200.0, seed=5000)
+    assert {r.tol for r in rs} == {1e-6, 1e-9}
+    srv = Server(DeviceExecutor(0, keep_final_C=True, keep_outs=keep), hbm_budget=200e9)
+    for r in rs:
+        srv.submit(r)
+    srv.run(timeout=600)
+    assert not srv.rejected and len(srv.finished) == 24
+    print(f"steps {srv.steps}, largest step {max(srv.step_sizes)} requests", flush=True)
+    assert max(srv.step_sizes) >= 2  # requests of different sizes shared steps
+    mix = {r.rid: r for r in srv.finished}
+    for r in sorted(mix.values(), key=lambda x: x.rid):
+        assert len(r.energies) == r.n_builds
+        refined = r.form_for_build(r.n_builds - 1) == "split_refine"
+        assert refined == (r.tol < split_error_bound(r.nbf))
+        eng = JKEngine(0)
+        btri = synth.device_b_packed(r.nbf, r.naux, r.seed, 0.05, JKEngine.packed_ld(r.nbf, r.naux))
+        eng.upload_b_packed(btri, r.nbf, r.naux)
+        del btri
+        H = synth.make_H(r.nbf, r.seed + 1)
+        n = synth.make_n(r.nbf5, r.ndoc, r.seed + 2)
+        o = eng.build(r.final_C, r.nbf5, H=H, precision="fp64", outputs=keep)
+        torch.cuda.synchronize()
+        e64 = sum(synth.energy_terms(n, o["J_MO"], o["K_MO"], o["H_core"]))
+        err = abs(r.energies[-1] - e64)
+        print(f"rid {r.rid} nbf {r.nbf} tol {r.tol:g} refined {refined} final |E - E_fp64| = {err:.3e} Eh", flush=True)
+        assert err <= r.tol, (r.rid, r.nbf, r.tol, err)
+        eng.close()
+    for r0 in poisson_requests(24, rate=200.0, seed=5000):
+        r0.arrival = 0.0
+        s1 = Server(DeviceExecutor(0, keep_final_C=True, keep_outs=keep), hbm_budget=200e9)
+        s1.submit(r0)
+        s1.run(timeout=300)
+        a = s1.finished[0]
+        assert a.energies == mix[r0.rid].energies, r0.rid
+        for k in keep:
+            assert np.array_equal(a.final_outs[k], mix[r0.rid].final_outs[k]), (r0.rid, k)
+    torch.cuda.empty_cache()

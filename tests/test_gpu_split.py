@@ -3,8 +3,10 @@
 split:        every contraction in BF16x3 (hi*hi + hi*lo + lo*hi, FP32 accumulation).
               Bar: normwise relative error <= 3e-5 for J, K, J_MO, K_MO (a BF16x3
               product carries ~2^-16 relative error); H_core is FP64 (1e-10).
-split_refine: J_MO, K_MO, H_core through the FP64 path (bar 1e-10, energy 1e-10
-              relative); J, K in BF16x3 from the exact half-transform (3e-5).
+split_refine: J_MO, K_MO, H_core through the FP64 path (bar 1e-10 against the oracle,
+              bitwise equal to the FP64 form's, so the energy is the FP64 energy);
+              J, K in BF16x3 from the exact half-transform (3e-5).
+Energies are compared in Eh (FP64_E_EH, SPLIT_E_EH_MID below).
 """
 import numpy as np
 import pytest
@@ -15,6 +17,11 @@ from pynof_amd import JKEngine, synth
 pytestmark = pytest.mark.gpu
 SPLIT_TOL = 3e-5
 FP64_TOL = 1e-10
+# Energy bars in Eh at nbf=280 (|E| ~ 1.9e4 Eh): the FP64 form against the oracle (FP64 sums
+# in another order), and the split form against the FP64 form (measured 1.5e-4..1.2e-3 Eh on
+# the MI355X over the serving sizes 24..800; BF16x3 products carry ~2^-16 relative error).
+FP64_E_EH = 1e-9
+SPLIT_E_EH_MID = 5e-3
 NAMES = ("J", "K", "J_MO", "K_MO", "H_core")
 
 
@@ -53,21 +60,27 @@ def test_split_shapes(eng, nbf, naux, nbf5):
 
 
 def test_split_energies_mid(eng):
+    """Energies at nbf=280, naux=1100 in Eh: the FP64 form against the oracle within
+    FP64_E_EH; split_refine's MO outputs bitwise those of the FP64 form (so its energy
+    is the FP64 energy); the split form within SPLIT_E_EH_MID of the FP64 energy."""
     w = synth.workload(1)
     print(f"case: split energies {w.name}", flush=True)
     b, C, H, n = synth.make_inputs(w)
     eng.upload_b(b)
     ref = O.jk_build(C, H, b, w.nbf5)
     e_ref = O.energy(n, *ref[2:])
-    out = {}
+    out, res = {}, {}
     for prec in ("fp64", "split", "split_refine"):
         r = eng.build(C, w.nbf5, H=H, precision=prec)
+        res[prec] = r
         e = O.energy(n, r["J_MO"], r["K_MO"], r["H_core"])
-        out[prec] = (e - e_ref, {k: rel(r[k], a) for k, a in zip(NAMES, ref)})
-        print(prec, "dE", e - e_ref, out[prec][1], flush=True)
-    assert abs(out["fp64"][0]) <= FP64_TOL * abs(e_ref)
-    assert abs(out["split_refine"][0]) <= FP64_TOL * abs(e_ref)
-    assert abs(out["split"][0]) <= 1e-6 * abs(e_ref)  # BF16x3 energy: ~1e-7 relative expected
+        out[prec] = (e, {k: rel(r[k], a) for k, a in zip(NAMES, ref)})
+        print(prec, "E", e, "E - E_oracle (Eh)", e - e_ref, out[prec][1], flush=True)
+    assert abs(out["fp64"][0] - e_ref) <= FP64_E_EH
+    for k in ("J_MO", "K_MO", "H_core"):
+        assert np.array_equal(res["split_refine"][k], res["fp64"][k]), k
+    assert out["split_refine"][0] == out["fp64"][0]
+    assert abs(out["split"][0] - out["fp64"][0]) <= SPLIT_E_EH_MID
     for k in ("J", "K", "J_MO", "K_MO"):
         assert out["split"][1][k] <= SPLIT_TOL
 

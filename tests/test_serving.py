@@ -111,8 +111,22 @@ def test_continuous_batching_joins_mid_flight():
 
 
 def test_form_per_tolerance():
+    from pynof_amd.serving import SPLIT_ENERGY_ERROR_EH, split_error_bound
+
+    # the final build is refined when the split error bound at the size exceeds the tolerance
     assert [make(0, 10, tol=1e-9, n=3).form_for_build(i) for i in range(3)] == ["split", "split", "split_refine"]
-    assert [make(0, 10, tol=1e-6, n=3).form_for_build(i) for i in range(3)] == ["split"] * 3
+    assert [make(0, 10, tol=1e-6, n=3).form_for_build(i) for i in range(3)] == ["split", "split", "split_refine"]
+    assert [make(0, 10, tol=1.0, n=3).form_for_build(i) for i in range(3)] == ["split"] * 3
+    big = split_error_bound(800)
+    assert make(0, 800, tol=0.5 * big, n=2).form_for_build(1) == "split_refine"
+    assert make(0, 800, tol=2.0 * big, n=2).form_for_build(1) == "split"
+    # the bound is the measured error times the margin at the tabulated sizes, monotone in between
+    for nbf, e in SPLIT_ENERGY_ERROR_EH:
+        assert abs(split_error_bound(nbf) - 10.0 * e) <= 1e-12 * e
+    assert split_error_bound(24) <= split_error_bound(100) <= split_error_bound(800)
+    assert split_error_bound(10) == split_error_bound(24) and split_error_bound(2000) == split_error_bound(800)
+    # every size of the serving mix refines both tolerance classes
+    assert min(split_error_bound(n) for n in range(24, 801)) > 1e-6
 
 
 def test_timeout():
@@ -239,3 +253,44 @@ def test_small_rotation_exponential_matches_expm():
         e = _expm_small(k).numpy()
         assert np.max(np.abs(e - scipy.linalg.expm(k.numpy()))) < 1e-16
         assert np.max(np.abs(e.T @ e - np.eye(n))) < 1e-14
+
+
+class SlowFailExecutor(ThreadedExecutor):
+    """Request 1's admission is still running when the loop stops, and then fails."""
+
+    def admit(self, req):
+        import time
+
+        if req.rid == 1:
+            time.sleep(0.1)
+            raise MemoryError("injected late allocation failure")
+        self.live.add(req.rid)
+
+
+def test_loop_stop_settles_failed_admission_in_flight():
+    ex = SlowFailExecutor()
+    srv = Server(ex, hbm_budget=1e6)
+    srv.submit(make(0, 10, arrival=0.0, n=100000))
+    srv.submit(make(1, 10, arrival=0.001, n=5))
+    with pytest.raises(TimeoutError):
+        srv.run(timeout=0.03)
+    assert srv.in_use == 0 and not srv.admitting and not srv.active
+    errs = {r.rid: r.error for r in srv.rejected}
+    assert set(errs) == {0, 1}
+    assert "injected late" in errs[1] and "not finished" in errs[0]
+    assert ex.live == set()  # request 0 was released
+
+
+class ReleaseFailExecutor(FakeExecutor):
+    def release(self, req):
+        super().release(req)
+        raise RuntimeError("injected release failure")
+
+
+def test_failed_release_keeps_the_books():
+    clock = Clock()
+    srv = Server(ReleaseFailExecutor(clock), hbm_budget=1e12, clock=clock, sleep=clock.sleep)
+    srv.submit(make(0, 40, n=2))
+    with pytest.raises(RuntimeError):
+        srv.run(t0=0.0)
+    assert [r.rid for r in srv.finished] == [0] and srv.in_use == 0 and not srv.active
