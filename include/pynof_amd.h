@@ -111,6 +111,22 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
                   int precision, uint32_t outputs, double* J, double* K, double* J_MO, double* K_MO,
                   double* H_core, void* stream);
 
+/* pnjk_jk_build without allocation: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, outputs and bits as pnjk_jk_build,
+ * always asynchronous on `stream`, and it calls no hipMalloc, hipFree or allocating
+ * library routine: the build's workspace (C staging, the half-transform X, d, the K_MO
+ * partials, ...) lives in the library's static device array of 64 MiB per GPU (held until
+ * the process ends). C, H and every output must be device memory; for the split forms the
+ * split planes of b must exist already (the first pnjk_jk_build in a split form after an
+ * upload makes them). A build whose workspace exceeds 64 MiB (FP64 form, nbf5 = nbf: about
+ * nbf = 120 with naux = 3.6 nbf) fails with PNJK_EINVAL and names the bytes it needs. All
+ * no-allocation builds on one GPU share that array: calls that overlap in time on one GPU
+ * (different streams, threads or engines) race; calls issued on one stream are ordered. */
+int pnjk_jk_build_noalloc(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, const double* H,
+                          int64_t ldh, int precision, uint32_t outputs, double* J, double* K, double* J_MO,
+                          double* K_MO, double* H_core, void* stream);
+
 /* Orbital-gradient MO integrals of the RI path: replaces JKH_MO_tmp with
  * p.RI (integrals.py:665-682), i.e. Integrals_MO_RI_CPU / _GPU
  * (integrals.py:703-716), which computeLagrange2 (utils.py:262-265) calls for

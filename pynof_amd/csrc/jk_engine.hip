@@ -13,6 +13,16 @@
 #include "jk_kernels.h"
 #include "pynof_amd.h"
 
+// Workspace of pnjk_jk_build_noalloc: one array per GPU, held by the process from the
+// library's first use of that GPU to its end. Every no-allocation build on a GPU works in
+// this one array, so such builds must not overlap in time on one GPU (they are ordered when
+// issued on one stream); its contents persist between calls and are never assumed zero.
+// Host code must not pass `scratch` itself to a kernel or a copy (on the host the name is a
+// stand-in, not the device address): scratch_ptr() gives each GPU's address, taken once per
+// GPU with hipGetSymbolAddress in pnjk_create.
+static constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace pnjk {
 
 static thread_local std::string g_last_error;
@@ -25,6 +35,30 @@ int set_error(int code, const std::string& msg) {
 namespace {
 
 inline int64_t roundup(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+// Device address of `scratch` per GPU, taken once per GPU (the first pnjk_create on it).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch[kMaxDevices] = {};
+
+// With `device` current: takes the address of this GPU's scratch array if not done yet.
+int scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return set_error(PNJK_EINVAL, "scratch: device id out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return PNJK_OK;
+  void* p = nullptr;
+  const hipError_t he = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (he != hipSuccess || !p)
+    return set_error(PNJK_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(he));
+  g_scratch[device] = static_cast<char*>(p);
+  return PNJK_OK;
+}
+
+char* scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch[device];
+}
 
 // Grow-only device buffer.
 struct DevBuf {
@@ -170,6 +204,8 @@ int pnjk_create(int device, pnjk_engine** out) {
   if (device < 0 || device >= n)
     return set_error(PNJK_EINVAL, "pnjk_create: device " + std::to_string(device) + " of " + std::to_string(n));
   PNJK_HIP_CHECK(hipSetDevice(device));
+  int rc = scratch_init(device);
+  if (rc) return rc;
   pnjk_engine* e = new pnjk_engine();
   e->device = device;
   hipError_t he = hipStreamCreateWithFlags(&e->own, hipStreamNonBlocking);
@@ -467,11 +503,32 @@ struct BuildState {
   bool c_dev = false, h_dev = false;  // C / H are device memory
   double *dJ, *dK, *dJMO, *dKMO, *dH;
   int nt = 0;  // P-tiles of the quarter transform
+  // Device workspace of this build: the engine's grow-only buffers, or pieces of the
+  // library's static scratch array (pnjk_jk_build_noalloc).
+  double *Cp = nullptr, *Cw = nullptr, *X = nullptr, *d = nullptr, *kpart = nullptr, *Hp = nullptr, *T = nullptr;
+  uint16_t *ct16h = nullptr, *ct16l = nullptr, *x16h = nullptr, *x16l = nullptr, *d16h = nullptr, *d16l = nullptr;
 };
 
+// The workspace pieces of one build, in bytes (0: not needed), in a fixed order.
+enum WsPiece { W_CP = 0, W_CW, W_X, W_D, W_KPART, W_HP, W_T, W_CT16H, W_CT16L, W_X16H, W_X16L, W_D16H, W_D16L,
+               W_STAGE, W_COUNT };
+struct WsPlan {
+  size_t bytes[W_COUNT] = {};
+};
+
+// A stage refuses to enqueue work on a workspace pointer its build did not prepare (a
+// kernel would otherwise write through a null address).
+#define PNJK_NEED(ptr)                                                                               \
+  do {                                                                                               \
+    if (!(ptr)) return set_error(PNJK_ESTATE, "internal: workspace " #ptr " was not prepared"); \
+  } while (0)
+
 // Validates the arguments, sizes the workspace, resolves output staging and
-// tiles. Enqueues nothing.
-int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st) {
+// tiles. Enqueues nothing. With `scratch` (scratch_bytes long) the workspace and the
+// staging of unrequested J_MO/K_MO are carved from it and nothing is allocated: every
+// output must then be device memory (host outputs would need staging buffers).
+int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scratch = nullptr,
+                  size_t scratch_bytes = 0) {
   if (e->nbf == 0) return set_error(PNJK_ESTATE, "pnjk_jk_build: no b uploaded");
   if (!a.C) return set_error(PNJK_EINVAL, "pnjk_jk_build: C is NULL");
   if (a.nbf5 < 1 || a.nbf5 > e->nbf)
@@ -508,15 +565,6 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st) {
   const int precision = a.precision;
 
   int rc;
-  const int64_t cp_elems = (int64_t)(g.nbf_p + 8) * g.ldcp + 1024;
-  if ((rc = e->Cp.ensure((size_t)cp_elems * 8))) return rc;
-  // X slack: a 128-column quarter-transform tile reads up to 64 columns past naux_p.
-  if (precision != PNJK_SPLIT && (rc = e->X.ensure((size_t)(n5 * nbf * g.naux_p + 64) * 8))) return rc;
-  if ((rc = e->d.ensure((size_t)(roundup(n5, 160) * g.naux_p) * 8))) return rc;
-  if ((rc = e->kpart.ensure((size_t)(n5 * n5 * (g.naux_p / 64)) * 8))) return rc;
-  if (precision != PNJK_SPLIT && (rc = e->Cw.ensure((size_t)(2 * g.nbf_p * (int64_t)qt_window_cols(g.nbf5)) * 8)))
-    return rc;
-
   st.want_j = a.outputs & PNJK_OUT_J;
   st.want_k = a.outputs & PNJK_OUT_K;
   st.want_jmo = a.outputs & PNJK_OUT_JMO;
@@ -533,6 +581,8 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st) {
   st.hJMO = !st.want_jmo || !is_device_ptr(a.J_MO);  // J_MO and K_MO are always formed
   st.hKMO = !st.want_kmo || !is_device_ptr(a.K_MO);
   st.hH = st.want_h && !is_device_ptr(a.H_core);
+  if (scratch && (st.hJ || st.hK || (st.want_jmo && st.hJMO) || (st.want_kmo && st.hKMO) || st.hH))
+    return set_error(PNJK_EINVAL, "pnjk_jk_build_noalloc: every output must be device memory");
   int64_t stage_elems = 0;
   const int64_t off_J = stage_elems;
   if (st.hJ) stage_elems += jk_elems;
@@ -544,26 +594,68 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st) {
   if (st.hKMO) stage_elems += n5 * n5;
   const int64_t off_H = stage_elems;
   if (st.hH) stage_elems += n5;
-  if ((rc = e->outs.ensure((size_t)std::max<int64_t>(stage_elems, 1) * 8))) return rc;
-  double* ob = e->outs.as<double>();
+
+  WsPlan w;
+  w.bytes[W_CP] = (size_t)((int64_t)(g.nbf_p + 8) * g.ldcp + 1024) * 8;
+  if (precision != PNJK_SPLIT) {
+    // X slack: a 128-column quarter-transform tile reads up to 64 columns past naux_p.
+    w.bytes[W_X] = (size_t)(n5 * nbf * g.naux_p + 64) * 8;
+    w.bytes[W_CW] = (size_t)(2 * g.nbf_p * (int64_t)qt_window_cols(g.nbf5)) * 8;
+  }
+  w.bytes[W_D] = (size_t)(roundup(n5, 160) * g.naux_p) * 8;
+  w.bytes[W_KPART] = (size_t)(n5 * n5 * (g.naux_p / 64)) * 8;
+  if (st.want_h) {
+    w.bytes[W_HP] = (size_t)(nbf * g.nbf_p) * 8;
+    w.bytes[W_T] = (size_t)(nbf * g.ldcp) * 8;
+  }
+  if (precision != PNJK_FP64) {
+    // X16 slack: a 128-column quarter-transform tile reads up to 64 columns past naux_p.
+    w.bytes[W_CT16H] = w.bytes[W_CT16L] = (size_t)(n5 * st.sg.ldk) * 2;
+    w.bytes[W_X16H] = w.bytes[W_X16L] = (size_t)(n5 * nbf * g.naux_p + 64) * 2;
+    w.bytes[W_D16H] = w.bytes[W_D16L] = (size_t)(n5 * g.naux_p) * 2;
+  }
+  w.bytes[W_STAGE] = (size_t)std::max<int64_t>(stage_elems, 1) * 8;
+  void* ptr[W_COUNT] = {};
+  if (scratch) {
+    size_t off = 0;
+    for (int i = 0; i < W_COUNT; ++i) {
+      if (!w.bytes[i]) continue;
+      ptr[i] = scratch + off;
+      off += (size_t)roundup((int64_t)w.bytes[i], 256);
+    }
+    if (off > scratch_bytes)
+      return set_error(PNJK_EINVAL, "pnjk_jk_build_noalloc: this build needs " + std::to_string(off) +
+                                        " bytes of workspace; the library's scratch array holds " +
+                                        std::to_string(scratch_bytes));
+  } else {
+    DevBuf* bufs[W_COUNT] = {&e->Cp, &e->Cw, &e->X, &e->d, &e->kpart, &e->Hp, &e->T, &e->ct16h, &e->ct16l,
+                             &e->x16h, &e->x16l, &e->d16h, &e->d16l, &e->outs};
+    for (int i = 0; i < W_COUNT; ++i) {
+      if (!w.bytes[i]) continue;
+      if ((rc = bufs[i]->ensure(w.bytes[i]))) return rc;
+      ptr[i] = bufs[i]->p;
+    }
+  }
+  st.Cp = static_cast<double*>(ptr[W_CP]);
+  st.Cw = static_cast<double*>(ptr[W_CW]);
+  st.X = static_cast<double*>(ptr[W_X]);
+  st.d = static_cast<double*>(ptr[W_D]);
+  st.kpart = static_cast<double*>(ptr[W_KPART]);
+  st.Hp = static_cast<double*>(ptr[W_HP]);
+  st.T = static_cast<double*>(ptr[W_T]);
+  st.ct16h = static_cast<uint16_t*>(ptr[W_CT16H]);
+  st.ct16l = static_cast<uint16_t*>(ptr[W_CT16L]);
+  st.x16h = static_cast<uint16_t*>(ptr[W_X16H]);
+  st.x16l = static_cast<uint16_t*>(ptr[W_X16L]);
+  st.d16h = static_cast<uint16_t*>(ptr[W_D16H]);
+  st.d16l = static_cast<uint16_t*>(ptr[W_D16L]);
+  double* ob = static_cast<double*>(ptr[W_STAGE]);
   if (st.hJ) st.dJ = ob + off_J;
   if (st.hK) st.dK = ob + off_K;
   if (st.hJMO) st.dJMO = ob + off_JMO;
   if (st.hKMO) st.dKMO = ob + off_KMO;
   if (st.hH) st.dH = ob + off_H;
 
-  if (precision != PNJK_FP64) {
-    // X16 slack: a 128-column quarter-transform tile reads up to 64 columns past naux_p.
-    const size_t ct_bytes = (size_t)(n5 * st.sg.ldk) * 2, x_bytes = (size_t)(n5 * nbf * g.naux_p + 64) * 2,
-                 d_bytes = (size_t)(n5 * g.naux_p) * 2;
-    if ((rc = e->ct16h.ensure(ct_bytes)) || (rc = e->ct16l.ensure(ct_bytes))) return rc;
-    if ((rc = e->x16h.ensure(x_bytes)) || (rc = e->x16l.ensure(x_bytes))) return rc;
-    if ((rc = e->d16h.ensure(d_bytes)) || (rc = e->d16l.ensure(d_bytes))) return rc;
-  }
-  if (st.want_h) {
-    if ((rc = e->Hp.ensure((size_t)(nbf * g.nbf_p) * 8))) return rc;
-    if ((rc = e->T.ensure((size_t)(nbf * g.ldcp) * 8))) return rc;
-  }
   // FP64 kernels run HT/QT in the fp64 and split_refine forms, J^n/K^n in fp64 only.
   const int a_mo = precision == PNJK_SPLIT ? 1 : 0, a_jk = precision == PNJK_FP64 ? 0 : 1;
   auto chosen = [&](int arith, int stg, int kind) {
@@ -578,13 +670,14 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st) {
 
 // C (and H) into the padded device buffers; split planes of C^T.
 int stage_in(pnjk_engine* e, BuildState& st, hipStream_t s) {
+  PNJK_NEED(st.Cp);
   if (st.a.precision != PNJK_FP64) {
     int rc = ensure_split_planes(e, s);
     if (rc) return rc;
   }
   const BuildGeom& g = st.g;
   const int64_t nbf = g.nbf, n5 = g.nbf5;
-  double* Cp = e->Cp.as<double>();
+  double* Cp = st.Cp;
   PNJK_HIP_CHECK(fill(Cp, (int64_t)(g.nbf_p + 8) * g.ldcp + 1024, 0.0, s));
   if (st.c_dev) {
     const int64_t n = nbf * n5;
@@ -595,54 +688,75 @@ int stage_in(pnjk_engine* e, BuildState& st, hipStream_t s) {
     PNJK_HIP_CHECK(hipMemcpy2DAsync(Cp, (size_t)g.ldcp * 8, st.a.C, (size_t)st.a.ldc * 8, (size_t)n5 * 8,
                                     (size_t)nbf, hipMemcpyHostToDevice, s));
   }
-  if (st.a.precision == PNJK_SPLIT)
-    PNJK_HIP_CHECK(launch_split_ct(Cp, g.ldcp, g.nbf, g.nbf5, g.nbf5, st.sg.ldk, e->ct16h.as<uint16_t>(),
-                                   e->ct16l.as<uint16_t>(), s));
-  else
-    PNJK_HIP_CHECK(launch_window_c(Cp, g.ldcp, g.nbf_p, g.nbf5, qt_window_cols(g.nbf5), e->Cw.as<double>(), s));
+  if (st.a.precision == PNJK_SPLIT) {
+    PNJK_NEED(st.ct16h);
+    PNJK_NEED(st.ct16l);
+    PNJK_HIP_CHECK(launch_split_ct(Cp, g.ldcp, g.nbf, g.nbf5, g.nbf5, st.sg.ldk, st.ct16h, st.ct16l, s));
+  } else {
+    PNJK_NEED(st.Cw);
+    PNJK_HIP_CHECK(launch_window_c(Cp, g.ldcp, g.nbf_p, g.nbf5, qt_window_cols(g.nbf5), st.Cw, s));
+  }
   return PNJK_OK;
 }
 
 int stage_ht(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col) {
-  const int prec = st.a.precision;
-  if (prec == PNJK_SPLIT) {
-    PNJK_HIP_CHECK(launch_ht_x3(st.sg, st.ht, e->ct16h.as<uint16_t>(), e->ct16l.as<uint16_t>(),
-                                e->bht16h.as<uint16_t>(), e->bht16l.as<uint16_t>(), e->x16h.as<uint16_t>(),
-                                e->x16l.as<uint16_t>(), s, col));
+  if (st.a.precision == PNJK_SPLIT) {
+    PNJK_NEED(st.ct16h);
+    PNJK_NEED(st.ct16l);
+    PNJK_NEED(st.x16h);
+    PNJK_NEED(st.x16l);
+    PNJK_HIP_CHECK(launch_ht_x3(st.sg, st.ht, st.ct16h, st.ct16l, e->bht16h.as<uint16_t>(), e->bht16l.as<uint16_t>(),
+                                st.x16h, st.x16l, s, col));
   } else {
-    PNJK_HIP_CHECK(launch_ht_f64(st.g, st.ht, e->Cp.as<double>(), e->btri.as<double>(), e->X.as<double>(), s,
-                                 nullptr, nullptr, col));
+    PNJK_NEED(st.Cp);
+    PNJK_NEED(st.X);
+    PNJK_HIP_CHECK(launch_ht_f64(st.g, st.ht, st.Cp, e->btri.as<double>(), st.X, s, nullptr, nullptr, col));
   }
   return PNJK_OK;
 }
 
 int stage_qt(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col) {
+  PNJK_NEED(st.kpart);
+  PNJK_NEED(st.d);
   if (st.a.precision == PNJK_SPLIT) {
-    PNJK_HIP_CHECK(launch_qt_x3(st.sg, st.qt, e->ct16h.as<uint16_t>(), e->ct16l.as<uint16_t>(),
-                                e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), e->kpart.as<double>(),
-                                e->d.as<double>(), e->d16h.as<uint16_t>(), e->d16l.as<uint16_t>(), &st.nt, s, col));
+    PNJK_NEED(st.ct16h);
+    PNJK_NEED(st.ct16l);
+    PNJK_NEED(st.x16h);
+    PNJK_NEED(st.x16l);
+    PNJK_NEED(st.d16h);
+    PNJK_NEED(st.d16l);
+    PNJK_HIP_CHECK(launch_qt_x3(st.sg, st.qt, st.ct16h, st.ct16l, st.x16h, st.x16l, st.kpart, st.d, st.d16h, st.d16l,
+                                &st.nt, s, col));
   } else {
-    PNJK_HIP_CHECK(launch_qt_f64(st.g, st.qt, e->Cw.as<double>(), qt_window_cols(st.g.nbf5), e->X.as<double>(),
-                                 e->kpart.as<double>(), e->d.as<double>(), &st.nt, s, col));
+    PNJK_NEED(st.Cw);
+    PNJK_NEED(st.X);
+    PNJK_HIP_CHECK(launch_qt_f64(st.g, st.qt, st.Cw, qt_window_cols(st.g.nbf5), st.X, st.kpart, st.d, &st.nt, s,
+                                 col));
   }
   return PNJK_OK;
 }
 
 int stage_kmo(pnjk_engine* e, BuildState& st, hipStream_t s) {
-  PNJK_HIP_CHECK(launch_kmo_reduce(st.g.nbf5, st.nt, e->kpart.as<double>(), st.dKMO, s));
+  PNJK_NEED(st.kpart);
+  PNJK_NEED(st.dKMO);
+  PNJK_HIP_CHECK(launch_kmo_reduce(st.g.nbf5, st.nt, st.kpart, st.dKMO, s));
   return PNJK_OK;
 }
 
 int stage_jmo(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col) {
-  PNJK_HIP_CHECK(launch_syrk_f64(1, st.g.nbf5, st.g.naux_p, e->d.as<double>(), st.g.naux_p, 0, st.dJMO, st.g.nbf5, 0,
+  PNJK_NEED(st.d);
+  PNJK_NEED(st.dJMO);
+  PNJK_HIP_CHECK(launch_syrk_f64(1, st.g.nbf5, st.g.naux_p, st.d, st.g.naux_p, 0, st.dJMO, st.g.nbf5, 0,
                                  TileF64{2, 2}, s, col));
   return PNJK_OK;
 }
 
 // split_refine: bf16 planes of the exact X, for K^n in split form.
 int stage_convert_x(pnjk_engine* e, BuildState& st, hipStream_t s) {
-  PNJK_HIP_CHECK(launch_split_btri(e->X.as<double>(), (int64_t)st.g.nbf5 * st.g.nbf * st.g.naux_p,
-                                   e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), s));
+  PNJK_NEED(st.X);
+  PNJK_NEED(st.x16h);
+  PNJK_NEED(st.x16l);
+  PNJK_HIP_CHECK(launch_split_btri(st.X, (int64_t)st.g.nbf5 * st.g.nbf * st.g.naux_p, st.x16h, st.x16l, s));
   return PNJK_OK;
 }
 
@@ -650,9 +764,12 @@ int stage_convert_x(pnjk_engine* e, BuildState& st, hipStream_t s) {
 // caller made them already).
 int stage_jmo_post(pnjk_engine* e, BuildState& st, hipStream_t s, bool convert_x = true) {
   if (st.a.precision == PNJK_SPLIT_REFINE) {
-    if (st.want_j)
-      PNJK_HIP_CHECK(launch_split_btri(e->d.as<double>(), (int64_t)st.g.nbf5 * st.g.naux_p, e->d16h.as<uint16_t>(),
-                                       e->d16l.as<uint16_t>(), s));
+    if (st.want_j) {
+      PNJK_NEED(st.d);
+      PNJK_NEED(st.d16h);
+      PNJK_NEED(st.d16l);
+      PNJK_HIP_CHECK(launch_split_btri(st.d, (int64_t)st.g.nbf5 * st.g.naux_p, st.d16h, st.d16l, s));
+    }
     if (st.want_k && convert_x) {
       int rc = stage_convert_x(e, st, s);
       if (rc) return rc;
@@ -663,22 +780,32 @@ int stage_jmo_post(pnjk_engine* e, BuildState& st, hipStream_t s, bool convert_x
 
 int stage_jn(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col) {
   if (!st.want_j) return PNJK_OK;
-  if (st.a.precision == PNJK_FP64)
-    PNJK_HIP_CHECK(launch_jn_f64(st.g, st.jt, e->d.as<double>(), e->btri.as<double>(), st.dJ, s, col));
-  else
-    PNJK_HIP_CHECK(launch_jn_x3(st.sg, st.jt, e->d16h.as<uint16_t>(), e->d16l.as<uint16_t>(),
-                                e->bt16h.as<uint16_t>(), e->bt16l.as<uint16_t>(), st.dJ, s, col));
+  PNJK_NEED(st.dJ);
+  if (st.a.precision == PNJK_FP64) {
+    PNJK_NEED(st.d);
+    PNJK_HIP_CHECK(launch_jn_f64(st.g, st.jt, st.d, e->btri.as<double>(), st.dJ, s, col));
+  } else {
+    PNJK_NEED(st.d16h);
+    PNJK_NEED(st.d16l);
+    PNJK_HIP_CHECK(launch_jn_x3(st.sg, st.jt, st.d16h, st.d16l, e->bt16h.as<uint16_t>(), e->bt16l.as<uint16_t>(),
+                                st.dJ, s, col));
+  }
   return PNJK_OK;
 }
 
 int stage_k(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col) {
   if (!st.want_k) return PNJK_OK;
+  PNJK_NEED(st.dK);
   const int64_t nbf = st.g.nbf;
-  if (st.a.precision == PNJK_FP64)
-    PNJK_HIP_CHECK(launch_syrk_f64(st.g.nbf5, st.g.nbf, st.g.naux_p, e->X.as<double>(), st.g.naux_p,
-                                   nbf * st.g.naux_p, st.dK, nbf, nbf * nbf, st.kt, s, col));
-  else
-    PNJK_HIP_CHECK(launch_kn_x3(st.sg, st.kt, e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), st.dK, s, col));
+  if (st.a.precision == PNJK_FP64) {
+    PNJK_NEED(st.X);
+    PNJK_HIP_CHECK(launch_syrk_f64(st.g.nbf5, st.g.nbf, st.g.naux_p, st.X, st.g.naux_p, nbf * st.g.naux_p, st.dK,
+                                   nbf, nbf * nbf, st.kt, s, col));
+  } else {
+    PNJK_NEED(st.x16h);
+    PNJK_NEED(st.x16l);
+    PNJK_HIP_CHECK(launch_kn_x3(st.sg, st.kt, st.x16h, st.x16l, st.dK, s, col));
+  }
   return PNJK_OK;
 }
 
@@ -690,9 +817,13 @@ int stage_sym(BuildState& st, double* O, hipStream_t s) {
 
 int stage_hcore(pnjk_engine* e, BuildState& st, hipStream_t s) {
   if (!st.want_h) return PNJK_OK;
+  PNJK_NEED(st.Hp);
+  PNJK_NEED(st.T);
+  PNJK_NEED(st.Cp);
+  PNJK_NEED(st.dH);
   const BuildGeom& g = st.g;
   const int64_t nbf = g.nbf;
-  double* Hp = e->Hp.as<double>();
+  double* Hp = st.Hp;
   PNJK_HIP_CHECK(fill(Hp, nbf * g.nbf_p, 0.0, s));
   if (st.h_dev) {
     const int64_t n = nbf * nbf;
@@ -703,7 +834,7 @@ int stage_hcore(pnjk_engine* e, BuildState& st, hipStream_t s) {
     PNJK_HIP_CHECK(hipMemcpy2DAsync(Hp, (size_t)g.nbf_p * 8, st.a.H, (size_t)st.a.ldh * 8, (size_t)nbf * 8,
                                     (size_t)nbf, hipMemcpyHostToDevice, s));
   }
-  PNJK_HIP_CHECK(launch_hcore_f64(g, Hp, g.nbf_p, e->Cp.as<double>(), e->T.as<double>(), st.dH, s));
+  PNJK_HIP_CHECK(launch_hcore_f64(g, Hp, g.nbf_p, st.Cp, st.T, st.dH, s));
   return PNJK_OK;
 }
 
@@ -723,22 +854,15 @@ int stage_out(BuildState& st, hipStream_t s, bool* sync) {
 
 }  // namespace
 
-extern "C" {
-
-int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, const double* H, int64_t ldh,
-                  int precision, uint32_t outputs, double* J, double* K, double* J_MO, double* K_MO, double* H_core,
-                  void* stream) {
-  if (!e) return set_error(PNJK_EINVAL, "pnjk_jk_build: engine is NULL");
-  std::lock_guard<std::mutex> lk(e->mu);
-  PNJK_HIP_CHECK(hipSetDevice(e->device));
-  hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
+// One build on stream s with the engine's mutex held and its device current; the workspace
+// comes from the engine's buffers, or from `scratch` when given (nothing allocated).
+static int jk_build_locked(pnjk_engine* e, const BuildArgs& a, hipStream_t s, char* scratch, size_t scratch_bytes) {
   auto mark = [&](int st) {
     if (e->prof) (void)hipEventRecord(e->ev[st], s);
   };
-  BuildArgs a{C, ldc, nbf5, H, ldh, precision, outputs, J, K, J_MO, K_MO, H_core};
   BuildState st;
   int rc;
-  if ((rc = build_prepare(e, a, st))) return rc;
+  if ((rc = build_prepare(e, a, st, scratch, scratch_bytes))) return rc;
   mark(ST_IN);
   if ((rc = stage_in(e, st, s))) return rc;
   mark(ST_HT);
@@ -800,6 +924,38 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
   return PNJK_OK;
 }
 
+
+extern "C" {
+
+int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, const double* H, int64_t ldh,
+                  int precision, uint32_t outputs, double* J, double* K, double* J_MO, double* K_MO, double* H_core,
+                  void* stream) {
+  if (!e) return set_error(PNJK_EINVAL, "pnjk_jk_build: engine is NULL");
+  std::lock_guard<std::mutex> lk(e->mu);
+  PNJK_HIP_CHECK(hipSetDevice(e->device));
+  const BuildArgs a{C, ldc, nbf5, H, ldh, precision, outputs, J, K, J_MO, K_MO, H_core};
+  return jk_build_locked(e, a, static_cast<hipStream_t>(stream), nullptr, 0);  // NULL stream: the null stream
+}
+
+int pnjk_jk_build_noalloc(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, const double* H, int64_t ldh,
+                          int precision, uint32_t outputs, double* J, double* K, double* J_MO, double* K_MO,
+                          double* H_core, void* stream) {
+  if (!e) return set_error(PNJK_EINVAL, "pnjk_jk_build_noalloc: engine is NULL");
+  std::lock_guard<std::mutex> lk(e->mu);
+  PNJK_HIP_CHECK(hipSetDevice(e->device));
+  char* ws = scratch_ptr(e->device);
+  if (!ws) return set_error(PNJK_ESTATE, "pnjk_jk_build_noalloc: no scratch address for device " +
+                                             std::to_string(e->device) + " (pnjk_create initialises it)");
+  if (e->nbf == 0) return set_error(PNJK_ESTATE, "pnjk_jk_build_noalloc: no b uploaded");
+  if (precision != PNJK_FP64 && !e->split_ready)
+    return set_error(PNJK_ESTATE, "pnjk_jk_build_noalloc: the split planes of b are built by the first split build "
+                                  "through pnjk_jk_build (they are part of the resident b, not of the workspace)");
+  if (!C || !is_device_ptr(C) || ((outputs & PNJK_OUT_HCORE) && (!H || !is_device_ptr(H))))
+    return set_error(PNJK_EINVAL, "pnjk_jk_build_noalloc: C (and H for H_core) must be device memory");
+  const BuildArgs a{C, ldc, nbf5, H, ldh, precision, outputs, J, K, J_MO, K_MO, H_core};
+  return jk_build_locked(e, a, static_cast<hipStream_t>(stream), ws, kScratchBytes);
+}
+
 }  // extern "C"
 
 // Full-width C, k-major: Cf[nu][p] for nu < nbf, p < nbf (from C, row stride ldc, host or
@@ -842,6 +998,8 @@ static int mo_compute(pnjk_engine* e, BuildState& st, double* dB, int64_t ldrow,
       return rc;
     st.dH = e->hc.as<double>();
     st.hH = false;
+    st.Hp = e->Hp.as<double>();
+    st.T = e->T.as<double>();
   }
   if ((rc = stage_in(e, st, s))) return rc;
   if ((rc = load_cf(e, st.a.C, st.a.ldc, nbf, g.nbf_p, s, &ldcf))) return rc;
@@ -855,16 +1013,16 @@ static int mo_compute(pnjk_engine* e, BuildState& st, double* dB, int64_t ldrow,
       PNJK_HIP_CHECK(launch_split_ct(Cf, ldcf, (int)nbf, (int)nbf, (int)nbf, st.sg.ldk, e->ctf16h.as<uint16_t>(),
                                      e->ctf16l.as<uint16_t>(), s));
       PNJK_HIP_CHECK(launch_mo_x3(st.sg, g.naux, big ? TileF64{6, 4} : TileF64{2, 2}, e->ctf16h.as<uint16_t>(),
-                                  e->ctf16l.as<uint16_t>(), e->x16h.as<uint16_t>(), e->x16l.as<uint16_t>(), dB, ldrow,
+                                  e->ctf16l.as<uint16_t>(), st.x16h, st.x16l, dB, ldrow,
                                   s, bh, bl));
     } else {  // 128 x 64 tiles (160 x 64: the FP64 Lagrangian 3.5 ms slower at nbf = 1000)
       PNJK_HIP_CHECK(
-          launch_mo_f64(g, big ? TileF64{4, 2} : TileF64{2, 2}, Cf, ldcf, e->X.as<double>(), dB, ldrow, s));
+          launch_mo_f64(g, big ? TileF64{4, 2} : TileF64{2, 2}, Cf, ldcf, st.X, dB, ldrow, s));
     }
   }
   if (dHm) {
     if ((rc = stage_hcore(e, st, s))) return rc;
-    PNJK_HIP_CHECK(launch_hmat_f64(g, Cf, ldcf, e->T.as<double>(), dHm, s));
+    PNJK_HIP_CHECK(launch_hmat_f64(g, Cf, ldcf, st.T, dHm, s));
   }
   return PNJK_OK;
 }

@@ -37,6 +37,8 @@ def test_null_engine_errors():
     assert b"engine" in L.pnjk_last_error()
     with pytest.raises(_lib.PnjkError):
         _lib.check(rc)
+    rc = L.pnjk_jk_build_noalloc(None, None, 0, 0, None, 0, 0, 0, None, None, None, None, None, None)
+    assert rc == _lib.EINVAL and b"engine" in L.pnjk_last_error()
     assert L.pnjk_upload_b(None, None, 1, 1, 0.0, None, None) == _lib.EINVAL
     assert L.pnjk_create(0, None) == _lib.EINVAL
     assert L.pnjk_destroy(None) == _lib.OK
