@@ -90,6 +90,15 @@ __host__ __device__ inline bool qt_owns(int p, int r, int n) { return !((n & 1) 
 // n plus whole tiles of at most 160 rows over the window.
 __host__ __device__ inline int qt_window_cols(int n) { return (n + qt_window(n) + 160 + 31) / 32 * 32; }
 
+// Lower-triangle tile pair number rem -> (mt, nt), nt <= mt, rem = mt*(mt+1)/2 + nt.
+__host__ __device__ inline void tri_pair(int rem, int& mt, int& nt) {
+  int m = (int)((sqrt(8.0 * rem + 1.0) - 1.0) * 0.5);
+  while ((m + 1) * (m + 2) / 2 <= rem) ++m;
+  while (m * (m + 1) / 2 > rem) --m;
+  mt = m;
+  nt = rem - m * (m + 1) / 2;
+}
+
 // Row index of the packed lower triangle: t = mu*(mu+1)/2 + nu for mu >= nu.
 __device__ __forceinline__ int64_t tri_index(int a, int b) {
   const int64_t hi = a > b ? a : b, lo = a > b ? b : a;

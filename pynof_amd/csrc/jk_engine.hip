@@ -617,16 +617,12 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
   w.bytes[W_STAGE] = (size_t)std::max<int64_t>(stage_elems, 1) * 8;
   void* ptr[W_COUNT] = {};
   if (scratch) {
-    size_t off = 0;
-    for (int i = 0; i < W_COUNT; ++i) {
-      if (!w.bytes[i]) continue;
-      ptr[i] = scratch + off;
-      off += (size_t)roundup((int64_t)w.bytes[i], 256);
-    }
+    const size_t off = ws_carve(w.bytes, W_COUNT, nullptr, ptr);
     if (off > scratch_bytes)
       return set_error(PNJK_EINVAL, "pnjk_jk_build_noalloc: this build needs " + std::to_string(off) +
                                         " bytes of workspace; the library's scratch array holds " +
                                         std::to_string(scratch_bytes));
+    ws_carve(w.bytes, W_COUNT, scratch, ptr);
   } else {
     DevBuf* bufs[W_COUNT] = {&e->Cp, &e->Cw, &e->X, &e->d, &e->kpart, &e->Hp, &e->T, &e->ct16h, &e->ct16l,
                              &e->x16h, &e->x16l, &e->d16h, &e->d16l, &e->outs};

@@ -79,6 +79,18 @@ struct GroupCollector {
   }
 };
 
+// Lays out n pieces of bytes[i] bytes (0: piece not needed, ptr[i] = null) one after the
+// other from base, each at a multiple of 256 bytes; returns the bytes used. With base null
+// only the size is computed (ptr untouched).
+inline size_t ws_carve(const size_t* bytes, int n, char* base, void** ptr) {
+  size_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    if (base) ptr[i] = bytes[i] ? base + off : nullptr;
+    if (bytes[i]) off += (bytes[i] + 255) / 256 * 256;
+  }
+  return off;
+}
+
 // Geometry of one build. Row strides are in elements.
 struct BuildGeom {
   int nbf;        // basis functions

@@ -621,11 +621,7 @@ struct ProbSYRK {
   __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.batch = id / tiles_per_batch;
     const int rem = id - t.batch * tiles_per_batch;
-    int mt = (int)((sqrt(8.0 * rem + 1.0) - 1.0) * 0.5);
-    while ((mt + 1) * (mt + 2) / 2 <= rem) ++mt;
-    while (mt * (mt + 1) / 2 > rem) --mt;
-    t.mt = mt;
-    t.nt = rem - mt * (mt + 1) / 2;
+    tri_pair(rem, t.mt, t.nt);
     t.m0 = t.mt * BM;
     t.n0 = t.nt * BN;
     return true;
