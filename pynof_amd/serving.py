@@ -78,6 +78,7 @@ class Request:
     energies: list = dataclasses.field(default_factory=list)
     error: str | None = None
     footprint: int = 0
+    lane: int = 0                  # size class (Server lanes)
 
     def form_for_build(self, i: int) -> str:
         if i == self.n_builds - 1 and self.tol < split_error_bound(self.nbf):
@@ -110,11 +111,42 @@ class Executor:
     def release(self, req: Request) -> None:
         raise NotImplementedError
 
+    # Optional asynchronous steps (``async_steps = True``), used by a Server with lanes: a
+    # lane's step is enqueued and returns a handle at once, so steps of different lanes run
+    # on the device together. Such a Server sets ``lanes_active`` before it admits anything.
+    async_steps = False
+    lanes_active = False
+
+    def step_async(self, reqs: list[Request], lane: int):
+        """Enqueues build number req.builds_done of every request in `reqs` (all of one lane);
+        returns a handle. Energies are appended to req.energies (they may be read once the
+        step is done)."""
+        raise NotImplementedError
+
+    def step_done(self, handle) -> bool:
+        raise NotImplementedError
+
+    def wait_any(self, handles, timeout: float) -> None:
+        """Returns when one of the handles' steps is done or after `timeout` seconds."""
+        raise NotImplementedError
+
 
 class Server:
     def __init__(self, executor: Executor, hbm_budget: float = 200e9, max_active: int = 64, clock=time.perf_counter,
-                 sleep=time.sleep):
+                 sleep=time.sleep, lanes=None):
+        """lanes: ascending nbf bounds of the size classes, e.g. (128, 256) for nbf < 128,
+        128 <= nbf < 256 and nbf >= 256. With lanes (and an executor with asynchronous steps)
+        each size class steps on its own, as soon as its previous step is done, so the builds
+        of small requests run beside those of large ones instead of waiting for them; without,
+        every step advances every active request together."""
         self.ex = executor
+        self.lanes = tuple(int(b) for b in lanes) if lanes else ()
+        if self.lanes and not getattr(executor, "async_steps", False):
+            raise ValueError("lanes need an executor with asynchronous steps")
+        if list(self.lanes) != sorted(set(self.lanes)):
+            raise ValueError("lane bounds must be strictly ascending")
+        if self.lanes:
+            executor.lanes_active = True
         self.budget = float(hbm_budget)
         self.max_active = int(max_active)
         self.clock = clock
@@ -133,6 +165,7 @@ class Server:
         if req.builds_done or req.energies or req.done is not None:
             raise ValueError(f"request {req.rid} was already served; submit a fresh Request")
         req.footprint = int(self.ex.footprint(req))
+        req.lane = sum(req.nbf >= b for b in self.lanes)
         if req.footprint > self.budget:
             req.error = f"footprint {req.footprint} B exceeds the device budget {int(self.budget)} B"
             self.rejected.append(req)
@@ -194,7 +227,15 @@ class Server:
     def _abandon(self) -> None:
         """After the loop stopped on an error (a time-out, a failed step): releases the
         requests it admitted but did not finish, so their device memory is freed, and lists
-        them with the rejected ones (done stays None); a normal end leaves nothing to release."""
+        them with the rejected ones (done stays None); a normal end leaves nothing to release.
+        Lane steps still in flight are waited for first (their requests' buffers are in use)."""
+        for _, h in getattr(self, "_inflight", {}).values():
+            try:
+                while not self.ex.step_done(h):
+                    self.ex.wait_any([h], 0.01)
+            except Exception:  # best effort: the step's failure is not this request's to report
+                pass
+        self._inflight = {}
         left = []
         for r, fut in self.admitting:  # the pool has shut down: every admission has ended
             exc = fut.exception()
@@ -214,7 +255,24 @@ class Server:
             self.in_use -= r.footprint
             self.rejected.append(r)
 
+    def _finish_step(self, batch: list[Request]) -> None:
+        self.steps += 1
+        self.step_sizes.append(len(batch))
+        t = self.clock()
+        for r in batch:
+            r.builds_done += 1
+            if r.builds_done >= r.n_builds:
+                r.done = t
+                self.active.remove(r)
+                try:
+                    self.ex.release(r)
+                finally:  # the request has left the device's books even if release failed
+                    self.in_use -= r.footprint
+                    self.finished.append(r)
+
     def _loop(self, end: float) -> None:
+        if self.lanes:
+            return self._loop_lanes(end)
         while self.pending or self.active or self.admitting:
             now = self.clock()
             if now > end:
@@ -228,19 +286,39 @@ class Server:
                 continue
             batch = list(self.active)
             self.ex.step(batch)
-            self.steps += 1
-            self.step_sizes.append(len(batch))
-            t = self.clock()
-            for r in batch:
-                r.builds_done += 1
-                if r.builds_done >= r.n_builds:
-                    r.done = t
-                    self.active.remove(r)
-                    try:
-                        self.ex.release(r)
-                    finally:  # the request has left the device's books even if release failed
-                        self.in_use -= r.footprint
-                        self.finished.append(r)
+            self._finish_step(batch)
+
+    def _loop_lanes(self, end: float) -> None:
+        inflight: dict[int, tuple] = {}  # lane -> (batch, handle)
+        self._inflight = inflight
+        while self.pending or self.active or self.admitting or inflight:
+            now = self.clock()
+            if now > end:
+                raise TimeoutError("serving loop exceeded its time limit")
+            self._admit(now)
+            launched = False
+            for lane in range(len(self.lanes) + 1):
+                if lane in inflight:
+                    continue
+                batch = [r for r in self.active if r.lane == lane]
+                if batch:
+                    inflight[lane] = (batch, self.ex.step_async(batch, lane))
+                    launched = True
+            finished = False
+            for lane in list(inflight):
+                batch, h = inflight[lane]
+                if self.ex.step_done(h):
+                    del inflight[lane]
+                    self._finish_step(batch)
+                    finished = True
+            if launched or finished:
+                continue
+            if inflight:
+                self.ex.wait_any([h for _, h in inflight.values()], 0.002)
+            elif self.admitting:
+                wait([self.admitting[0][1]], timeout=0.05, return_when=FIRST_COMPLETED)
+            elif self.pending:
+                self.sleep(max(0.0, min(self.pending[0].arrival - now, 0.05)))
 
     def stats(self) -> dict:
         lat = np.array([r.latency for r in self.finished]) if self.finished else np.zeros(0)
@@ -321,6 +399,7 @@ class DeviceExecutor(Executor):
         self.dev = torch.device("cuda", device)
         self.keep_outs = keep_outs
         self._build_stream = torch.cuda.current_stream(self.dev)  # the stream step() runs on
+        self._lane_streams: dict[int, object] = {}  # lane -> stream of its asynchronous steps
         self.device = device
         self.with_jk = with_jk
         self.keep_final_C = keep_final_C
@@ -349,9 +428,10 @@ class DeviceExecutor(Executor):
         with torch.cuda.device(self.dev), torch.cuda.stream(self._admit_stream):
             st = self._admit_on_stream(req)
             self._admit_stream.synchronize()
+        used_on = self._stream_of(req.lane) if self.lanes_active else self._build_stream
         for t in [st["C"], st["H"], st["n"], st.get("gamma"), *st["outs"].values()]:
             if t is not None:
-                t.record_stream(self._build_stream)
+                t.record_stream(used_on)
         with self._state_lock:
             self.state[req.rid] = st
 
@@ -398,7 +478,41 @@ class DeviceExecutor(Executor):
             self._groups[key] = grp
         return grp
 
+    async_steps = True
+
+    def _stream_of(self, lane: int):
+        with self._state_lock:
+            st = self._lane_streams.get(lane)
+            if st is None:
+                st = self._lane_streams[lane] = self.torch.cuda.Stream(self.dev)
+            return st
+
     def step(self, reqs):
+        self._enqueue(reqs)
+        self.torch.cuda.current_stream(self.dev).synchronize()
+
+    def step_async(self, reqs, lane: int):
+        torch = self.torch
+        s = self._stream_of(lane)
+        with torch.cuda.stream(s):
+            self._enqueue(reqs)
+            ev = torch.cuda.Event()
+            ev.record(s)
+        return ev
+
+    def step_done(self, handle) -> bool:
+        return handle.query()
+
+    def wait_any(self, handles, timeout: float) -> None:
+        t_end = time.perf_counter() + timeout
+        while time.perf_counter() < t_end:
+            if any(h.query() for h in handles):
+                return
+            time.sleep(2e-4)
+
+    def _enqueue(self, reqs):
+        """Enqueues one build of every request in `reqs` on the current stream, then each
+        request's energy and the orbital rotation for its next build."""
         torch = self.torch
         by_form: dict[str, list] = {}
         for r in reqs:
@@ -427,7 +541,6 @@ class DeviceExecutor(Executor):
                 k = a - a.T
                 k *= 1e-2 / torch.linalg.norm(k)
                 s["C"] = s["C"] @ _expm_small(k)
-        torch.cuda.current_stream(self.dev).synchronize()
 
     def release(self, req: Request) -> None:
         with self._state_lock:

@@ -99,7 +99,8 @@ def test_poisson_mix_tolerances_and_alone(gpu_required):
     rotated by exp(kappa) between builds, minimization.py:200-253, utils.py:550-565),
     capped at its first 24 requests, arriving at 200 req/s so that they overlap. Every request's
     final energy is within its own tolerance (Eh, absolute) of the FP64 form's energy at
-    its final orbitals, and every request's energies are the same in the mix as alone."""
+    its final orbitals, and every request's energies are the same in the mix as alone, with
+    lockstep steps and with size-class lanes."""
     import torch
 
     from pynof_amd.serving import poisson_requests, split_error_bound
@@ -133,6 +134,17 @@ def test_poisson_mix_tolerances_and_alone(gpu_required):
         print(f"rid {r.rid} nbf {r.nbf} tol {r.tol:g} refined {refined} final |E - E_fp64| = {err:.3e} Eh", flush=True)
         assert err <= r.tol, (r.rid, r.nbf, r.tol, err)
         eng.close()
+    # the same mix with size-class lanes (each class steps on its own stream): same results
+    srv2 = Server(DeviceExecutor(0, keep_final_C=True, keep_outs=keep), hbm_budget=200e9, lanes=(128, 256))
+    for r in poisson_requests(24, rate=200.0, seed=5000):
+        srv2.submit(r)
+    srv2.run(timeout=600)
+    assert not srv2.rejected and len(srv2.finished) == 24
+    assert {r.lane for r in srv2.finished} == {0, 1, 2}
+    for r in srv2.finished:
+        assert r.energies == mix[r.rid].energies, r.rid
+        for k in keep:
+            assert np.array_equal(r.final_outs[k], mix[r.rid].final_outs[k]), (r.rid, k)
     for r0 in poisson_requests(24, rate=200.0, seed=5000):
         r0.arrival = 0.0
         s1 = Server(DeviceExecutor(0, keep_final_C=True, keep_outs=keep), hbm_budget=200e9)

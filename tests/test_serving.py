@@ -294,3 +294,66 @@ def test_failed_release_keeps_the_books():
     with pytest.raises(RuntimeError):
         srv.run(t0=0.0)
     assert [r.rid for r in srv.finished] == [0] and srv.in_use == 0 and not srv.active
+
+
+class LaneExecutor(FakeExecutor):
+    """Asynchronous steps on a simulated clock: a lane's step takes cost_per_build times the
+    largest nbf of its batch / 100, and lanes run at the same time."""
+
+    async_steps = True
+
+    def __init__(self, clock):
+        super().__init__(clock)
+        self.launches = []  # (time, lane, rids)
+
+    def step_async(self, reqs, lane):
+        assert all(r.rid in self.live and r.lane == lane for r in reqs)
+        for r in reqs:
+            r.energies.append(float(r.rid))
+        self.launches.append((self.clock.t, lane, tuple(r.rid for r in reqs)))
+        return self.clock.t + self.cost * max(r.nbf for r in reqs) / 100.0
+
+    def step_done(self, handle):
+        return self.clock.t >= handle
+
+    def wait_any(self, handles, timeout):
+        self.clock.t = max(self.clock.t, min(min(handles), self.clock.t + timeout))
+
+
+def test_lanes_step_independently():
+    clk = Clock()
+    ex = LaneExecutor(clk)
+    srv = Server(ex, hbm_budget=1e12, clock=clk, sleep=clk.sleep, lanes=(128, 256))
+    big, small = make(0, 800, n=4), make(1, 40, arrival=0.001, n=4)
+    mid = make(2, 200, arrival=0.002, n=4)
+    for r in (big, small, mid):
+        srv.submit(r)
+    assert (big.lane, small.lane, mid.lane) == (2, 0, 1)
+    srv.run(t0=0.0)
+    assert sorted(r.rid for r in srv.finished) == [0, 1, 2] and srv.in_use == 0
+    assert all(len(r.energies) == r.n_builds for r in srv.finished)
+    # the small request steps while the big one's step (0.08 per build) is in flight: it is
+    # done long before the big request, not after four of its steps
+    assert small.done < 0.04 < big.done
+    assert {lane for _, lane, _ in ex.launches} == {0, 1, 2}
+    for _, lane, rids in ex.launches:  # a lane's batch holds only its own size class
+        assert all({0: 2, 1: 0, 2: 1}[rid] == lane for rid in rids)
+
+
+def test_lanes_need_async_executor():
+    clk = Clock()
+    with pytest.raises(ValueError):
+        Server(FakeExecutor(clk), lanes=(128,))
+    with pytest.raises(ValueError):
+        Server(LaneExecutor(clk), lanes=(256, 128))
+
+
+def test_lanes_timeout_waits_for_inflight_steps():
+    clk = Clock()
+    ex = LaneExecutor(clk)
+    srv = Server(ex, hbm_budget=1e12, clock=clk, sleep=clk.sleep, lanes=(128,))
+    srv.submit(make(0, 800, n=1000))
+    with pytest.raises(TimeoutError):
+        srv.run(t0=0.0, timeout=0.5)
+    assert srv.in_use == 0 and not srv.active and [r.rid for r in srv.rejected] == [0]
+    assert ex.live == set()
