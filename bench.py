@@ -151,6 +151,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="large", choices=["water", "mid", "large", "batch64", "serving", "gemmref"])
     ap.add_argument("--requests", type=int, default=120, help="serving: requests per run")
+    ap.add_argument("--batch-range", default="40,400",
+                    help="batch64: nbf range of the 64 instances (the batch workload is 40,400)")
     ap.add_argument("--lanes", default="128,256",
                     help="serving: ascending nbf bounds of the size-class lanes, or 'none' for lockstep steps")
     ap.add_argument("--form-steps", type=int, default=5, help="builds timed per single form")
@@ -513,7 +515,8 @@ def bench_batch(args):
     from pynof_amd import JKEngine, synth, workcount
     from pynof_amd.batch import JKGroup
 
-    ws = synth.batch_workloads(64)
+    lo, hi = (int(x) for x in args.batch_range.split(","))
+    ws = synth.batch_workloads(64, lo=lo, hi=hi)
     dev = torch.device("cuda", 0)
     log(f"batch64: {len(ws)} instances, nbf {min(w.nbf for w in ws)}..{max(w.nbf for w in ws)}")
     engs, Cs, Hs = [], [], []
@@ -528,7 +531,7 @@ def bench_batch(args):
     torch.cuda.empty_cache()
     n5s = [w.nbf5 for w in ws]
     flops = sum(workcount.build_flops(w.nbf, w.naux, w.nbf5) for w in ws)
-    grp = JKGroup(engs)
+    grp = JKGroup(engs, nbf5s=n5s)  # the tuned tiles, for the sequential passes too
     outs = grp.alloc_outputs(n5s)
     stream = torch.cuda.current_stream(dev).cuda_stream
     res = {}
@@ -561,8 +564,8 @@ def bench_batch(args):
         "n_gpus": 1,
         "higher_is_better": True,
         "dtype": "bf16x3 (fp32 accumulate)",
-        "data": "synthetic: device-drawn b, seeded C/H; nbf ~ U[40,400] (seed 4000)",
-        "config": {"workload": "batch64 nbf~U[40,400] naux=ceil8(3.6 nbf) nbf5=nbf",
+        "data": f"synthetic: device-drawn b, seeded C/H; nbf ~ U[{lo},{hi}] (seed 4000)",
+        "config": {"workload": f"batch64 nbf~U[{lo},{hi}] naux=ceil8(3.6 nbf) nbf5=nbf",
                    "sizes": [w.nbf for w in ws], "useful_flops_per_pass": flops},
         "results": res,
         "grouped_graph_speedup_vs_sequential": {

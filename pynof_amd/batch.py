@@ -26,12 +26,18 @@ def _ptr_array(ptrs):
 class JKGroup:
     """A fixed set of engines built together."""
 
-    def __init__(self, engines: list[JKEngine], tuned: dict | None = None, precision_hint: str = "split"):
+    def __init__(self, engines: list[JKEngine], tuned: dict | None = None, precision_hint: str = "split",
+                 nbf5s=None):
+        """nbf5s: the orbital counts the group will build with, when known: their tiles are
+        applied to the engines now (otherwise at the first build), so single builds on the
+        same engines run the same tiles."""
         if not engines:
             raise ValueError("a group needs at least one engine")
         self.engines = list(engines)
         self._table = autotune.load() if tuned is None else tuned
         self._tiled_for = None  # (nbf, naux, nbf5) per engine of the tiles last applied
+        if nbf5s is not None:
+            self._apply_tiles(nbf5s)
         L = _lib.lib()
         handles = _ptr_array([e._h.value for e in self.engines])
         h = ctypes.c_void_p()
