@@ -121,22 +121,43 @@ bool is_device_ptr(const void* p) {
   return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
+// Quarter-transform tile for the window of h = nbf5 / 2 + 1 rows (rules read off the tuner's
+// timings at nbf5 = 160 ... 1000 on MI355X). FP64: the row tile (160, 128 or 64) that pads
+// the window least, ties to 128, then 64 rows. BF16x3 (copy-bound, so larger tiles pay): the
+// largest row tile whose rows are at least 75% live.
+TileF64 qt_tile(int arith, int nbf5) {
+  const int h = nbf5 / 2 + 1;
+  auto padded = [&](int bm) { return (h + bm - 1) / bm * bm; };
+  if (arith == 0) {
+    TileF64 best{4, 2};
+    int rows = padded(128);
+    if (padded(64) < rows) best = TileF64{2, 4}, rows = padded(64);
+    if (padded(160) < rows) best = TileF64{5, 2};
+    return best;
+  }
+  for (int fm : {5, 4}) {
+    if (4 * h >= 3 * padded(32 * fm)) return TileF64{fm, 4};
+  }
+  return TileF64{2, 4};
+}
+
 // Built-in tile shapes (from autotuning at nbf=1000, naux=3600, nbf5=300 on MI355X;
 // small problems keep 64 x 64 tiles so the grid fills the chip).
 TileF64 default_tile(int arith, int stage_kind, int nbf, int nbf5) {
   // stage_kind: 0 HT, 1 QT, 2 JN, 3 K
   const bool big = nbf5 > 128 && nbf >= 256;
+  if (stage_kind == 1 && nbf5 > 128) return qt_tile(arith, nbf5);
   if (arith == 0) {
     switch (stage_kind) {
       case 0: return big ? TileF64{5, 4} : TileF64{2, 2};
-      case 1: return big ? TileF64{5, 2} : TileF64{2, 2};
+      case 1: return TileF64{2, 2};
       case 2: return big ? TileF64{5, 4} : TileF64{2, 2};
       default: return big ? TileF64{4, 4} : TileF64{2, 2};
     }
   }
   switch (stage_kind) {
     case 0: return big ? TileF64{5, 5} : TileF64{2, 2};
-    case 1: return big ? TileF64{5, 4} : TileF64{2, 2};
+    case 1: return TileF64{2, 2};
     case 2: return big ? TileF64{5, 4} : TileF64{2, 2};
     default: return nbf >= 256 ? TileF64{4, 4} : TileF64{2, 2};
   }
