@@ -287,11 +287,11 @@ uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int preci
   if (precision != PNJK_SPLIT) s += (uint64_t)(2 * nbf_p * qt_window_cols((int)nbf5)) * 8;  // Cw
   if (precision != PNJK_FP64) {
     const int64_t ntri = nbf * (nbf + 1) / 2;
-    s += (uint64_t)(ntri * naux_p) * 4;                      // btri16 hi/lo
+    s += (uint64_t)(roundup(ntri, 16) * naux_p) * 4;         // btri16 hi/lo (whole 16-row blocks)
     s += (uint64_t)(nbf * naux_p * nbf_p) * 4;               // bht16 hi/lo
-    s += (uint64_t)(nbf5 * nbf_p) * 4;                       // Ct16 hi/lo
+    s += (uint64_t)(roundup(nbf5, 16) * nbf_p) * 4;          // Ct16 hi/lo (whole 16-row blocks)
     s += (uint64_t)(nbf5 * nbf * naux_p + 64) * 4;           // X16 hi/lo (+ slack)
-    s += (uint64_t)(nbf5 * naux_p) * 4;                      // d16 hi/lo
+    s += (uint64_t)(roundup(nbf5, 16) * naux_p) * 4;         // d16 hi/lo (whole 16-row blocks)
   }
   return s;
 }
@@ -301,11 +301,12 @@ static int ensure_split_planes(pnjk_engine* e, hipStream_t s) {
   if (e->split_ready) return PNJK_OK;
   const int64_t ntri = e->ntri, naux_p = e->naux_p, nbf = e->nbf, ldk = roundup(nbf, 32);
   int rc;
-  const size_t tri_bytes = (size_t)(ntri * naux_p) * 2, ht_bytes = (size_t)(nbf * naux_p * ldk) * 2;
+  // bt16 in whole 16-row blocks (blk_index)
+  const size_t tri_bytes = (size_t)(roundup(ntri, 16) * naux_p) * 2, ht_bytes = (size_t)(nbf * naux_p * ldk) * 2;
   if ((rc = e->bt16h.ensure(tri_bytes)) || (rc = e->bt16l.ensure(tri_bytes))) return rc;
   if ((rc = e->bht16h.ensure(ht_bytes)) || (rc = e->bht16l.ensure(ht_bytes))) return rc;
-  PNJK_HIP_CHECK(launch_split_btri(e->btri.as<double>(), ntri * naux_p, e->bt16h.as<uint16_t>(),
-                                   e->bt16l.as<uint16_t>(), s));
+  PNJK_HIP_CHECK(launch_split_blocked(e->btri.as<double>(), ntri, naux_p, e->bt16h.as<uint16_t>(),
+                                      e->bt16l.as<uint16_t>(), s));
   PNJK_HIP_CHECK(launch_build_bht(e->btri.as<double>(), (int)nbf, (int)naux_p, (int)ldk, e->bht16h.as<uint16_t>(),
                                   e->bht16l.as<uint16_t>(), s));
   e->split_ready = true;
@@ -631,9 +632,9 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
   }
   if (precision != PNJK_FP64) {
     // X16 slack: a 128-column quarter-transform tile reads up to 64 columns past naux_p.
-    w.bytes[W_CT16H] = w.bytes[W_CT16L] = (size_t)(n5 * st.sg.ldk) * 2;
+    w.bytes[W_CT16H] = w.bytes[W_CT16L] = (size_t)(roundup(n5, 16) * st.sg.ldk) * 2;  // whole 16-row blocks
     w.bytes[W_X16H] = w.bytes[W_X16L] = (size_t)(n5 * nbf * g.naux_p + 64) * 2;
-    w.bytes[W_D16H] = w.bytes[W_D16L] = (size_t)(n5 * g.naux_p) * 2;
+    w.bytes[W_D16H] = w.bytes[W_D16L] = (size_t)(roundup(n5, 16) * g.naux_p) * 2;  // whole 16-row blocks
   }
   w.bytes[W_STAGE] = (size_t)std::max<int64_t>(stage_elems, 1) * 8;
   void* ptr[W_COUNT] = {};
@@ -785,7 +786,7 @@ int stage_jmo_post(pnjk_engine* e, BuildState& st, hipStream_t s, bool convert_x
       PNJK_NEED(st.d);
       PNJK_NEED(st.d16h);
       PNJK_NEED(st.d16l);
-      PNJK_HIP_CHECK(launch_split_btri(st.d, (int64_t)st.g.nbf5 * st.g.naux_p, st.d16h, st.d16l, s));
+      PNJK_HIP_CHECK(launch_split_blocked(st.d, st.g.nbf5, st.g.naux_p, st.d16h, st.d16l, s));
     }
     if (st.want_k && convert_x) {
       int rc = stage_convert_x(e, st, s);
@@ -1025,7 +1026,7 @@ static int mo_compute(pnjk_engine* e, BuildState& st, double* dB, int64_t ldrow,
     if ((rc = stage_ht(e, st, s, nullptr))) return rc;
     const bool big = nbf >= 256;
     if (precision == PNJK_SPLIT) {
-      const size_t ct_bytes = (size_t)(nbf * st.sg.ldk) * 2;
+      const size_t ct_bytes = (size_t)(roundup(nbf, 16) * st.sg.ldk) * 2;  // whole 16-row blocks
       if ((rc = e->ctf16h.ensure(ct_bytes)) || (rc = e->ctf16l.ensure(ct_bytes))) return rc;
       PNJK_HIP_CHECK(launch_split_ct(Cf, ldcf, (int)nbf, (int)nbf, (int)nbf, st.sg.ldk, e->ctf16h.as<uint16_t>(),
                                      e->ctf16l.as<uint16_t>(), s));

@@ -204,10 +204,14 @@ struct SplitGeom {
 
 // hi/lo bf16 planes of n packed FP64 values.
 hipError_t launch_split_btri(const double* btri, int64_t n, uint16_t* hi, uint16_t* lo, hipStream_t s);
+// Split planes of a row-major rows x cols matrix (cols a multiple of 32) in 16-row x 32-column
+// blocks (blk_index in jk_split.hip); rows rounded up to 16 with zeros.
+hipError_t launch_split_blocked(const double* src, int64_t rows, int64_t cols, uint16_t* hi, uint16_t* lo,
+                                hipStream_t s);
 // bht[nu][P][mu] planes (mu contiguous, stride ldk) from btri.
 hipError_t launch_build_bht(const double* btri, int nbf, int naux_p, int ldk, uint16_t* hi, uint16_t* lo,
                             hipStream_t s);
-// Ct[q][mu] planes (rows q < rows, stride ldk) from Cp[mu][q].
+// Ct[q][mu] planes (rows q < rows rounded up to 16, in 16 x 32 blocks, blk_index) from Cp[mu][q].
 hipError_t launch_split_ct(const double* Cp, int ldcp, int nbf, int nbf5, int rows, int ldk, uint16_t* hi,
                            uint16_t* lo, hipStream_t s);
 // X16[q][nu][P] = split(sum_mu C[mu][q] b[mu][nu][P]).

@@ -6,9 +6,10 @@
 // stages both planes of both operands through double-buffered LDS in a
 // row-major, k-contiguous layout (one 16-byte fragment per lane per MFMA), so
 // every operand needs its contraction index contiguous. Layouts chosen for that:
-//   Ct16[q][mu]        C^T, per build                 (A of HT, QT)
-//   bht16[nu][P][mu]   b with mu contiguous, at upload (B of HT)
-//   btri16[t][P]       packed lower triangle, at upload (B of JN)
+//   Ct16[q][mu]        C^T in 16-row x 32-mu blocks, per build (A of HT, QT)
+//   bht16[nu][P][mu]   b with mu contiguous, in 16-row x 32-mu blocks (blk_index),
+//                      at upload (B of HT)
+//   btri16[t][P]       packed lower triangle in 16-row x 32-P blocks, at upload (B of JN)
 //   X16[q][nu][P]      half-transformed b, per build  (A/B of K^n; B of QT with
 //                      P contiguous: copied to LDS as is and read with the
 //                      LDS transpose read)
@@ -51,7 +52,7 @@ struct Planes {
 enum LdKind { RK = 0, KN = 2 };
 
 // Direct HBM -> LDS copy of both planes of a ROWS x 32 tile whose contraction
-// index k is contiguous in memory (global element (r, k) at off(r) + k), with
+// index k is contiguous in memory (the 32 elements from k0 of row r at at(r, k0)), with
 // global_load_lds_dwordx4: lane L of one wave instruction writes 16 bytes at the
 // LDS base + 16 L, so one instruction fills 16 tile rows (1 KB). Lane L fetches
 // the global chunk that belongs in swizzled slot L & 3 of row L >> 2. The
@@ -71,7 +72,7 @@ struct Dma {
       if (kInst % NW == 0 || i < kInst) {
         const int r = 16 * i + rr;
         const int c = slot ^ (((r >> 1) ^ (r >> 2)) & 3);
-        const int64_t o = s.off(r0 + r) + k0 + 8 * c;
+        const int64_t o = s.at(r0 + r, k0) + 8 * c;
         __builtin_amdgcn_global_load_lds((const PNJK_GLOBAL void*)(s.p.hi + o),
                                          (__attribute__((address_space(3))) void*)(sh + 16 * i * kLDK), 16, 0, 0);
         __builtin_amdgcn_global_load_lds((const PNJK_GLOBAL void*)(s.p.lo + o),
@@ -285,6 +286,21 @@ struct SrcRows {  // element (r, k) at r_clamped*ld + k
   int64_t base, ld;
   int rmax;
   __device__ __forceinline__ int64_t off(int r) const { return base + (int64_t)(r < rmax ? r : rmax - 1) * ld; }
+  __device__ __forceinline__ int64_t at(int r, int k0) const { return off(r) + k0; }
+};
+
+// Blocked rows: 16-row x 32-k blocks of 512 elements (1 KB per plane), block (r / 16, k / 32)
+// at ((r / 16) * kblocks + k / 32) * 512, row r % 16 of it at (r % 16) * 32. One DMA instruction
+// (16 rows x 32 k) then reads one contiguous 1 KB, whole cache lines, where row-major rows give
+// it 16 half lines: the split half-transform measured 5-8% faster with blocked operands.
+__host__ __device__ inline int64_t blk_index(int64_t r, int64_t k, int64_t kblocks) {
+  return ((r >> 4) * kblocks + (k >> 5)) * 512 + (r & 15) * 32 + (k & 31);
+}
+struct SrcBlocked {
+  Planes p;
+  int64_t kblocks;
+  int64_t rmax;
+  __device__ __forceinline__ int64_t at(int64_t r, int k0) const { return blk_index(r < rmax ? r : rmax - 1, k0, kblocks); }
 };
 
 template <int FM, int FN>
@@ -313,8 +329,8 @@ struct ProbHT {
   }
   __host__ __device__ int ksteps() const { return ldk / kBK; }
   __device__ int row_limit(const Tile&) const { return nbf5; }
-  __device__ SrcRows a_src(const Tile&) const { return {ct, 0, ldk, nbf5}; }
-  __device__ SrcRows b_src(const Tile&) const { return {bht, 0, ldk, (int)N}; }
+  __device__ SrcBlocked a_src(const Tile&) const { return {ct, ldk / kBK, nbf5}; }
+  __device__ SrcBlocked b_src(const Tile&) const { return {bht, ldk / kBK, N}; }
   template <int FM, int FN>
   __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
 #pragma unroll
@@ -352,11 +368,13 @@ struct SrcXT {  // element (k = nu, n = P) at p*nbf*naux_p + min(nu, nbf-1)*naux
   __device__ __forceinline__ int64_t off(int k) const { return base + (int64_t)(k < kmax ? k : kmax - 1) * ld; }
 };
 
-struct SrcWindow {  // window row r of batch b: C^T row qt_row(b, min(r, h - 1), n), k contiguous
+struct SrcWindow {  // window row r of batch b: C^T row qt_row(b, min(r, h - 1), n) of the blocked C^T planes
   Planes p;
-  int64_t ld;
+  int64_t kblocks;
   int b, h, n;
-  __device__ __forceinline__ int64_t off(int r) const { return (int64_t)qt_row(b, r < h ? r : h - 1, n) * ld; }
+  __device__ __forceinline__ int64_t at(int r, int k0) const {
+    return blk_index(qt_row(b, r < h ? r : h - 1, n), k0, kblocks);
+  }
 };
 
 struct ProbQT {
@@ -380,7 +398,7 @@ struct ProbQT {
   }
   __host__ __device__ int ksteps() const { return ldk / kBK; }
   __device__ int row_limit(const Tile&) const { return h; }
-  __device__ SrcWindow a_src(const Tile& t) const { return {ct, ldk, t.batch, h, nbf5}; }
+  __device__ SrcWindow a_src(const Tile& t) const { return {ct, ldk / kBK, t.batch, h, nbf5}; }
   __device__ SrcXT b_src(const Tile& t) const { return {x, (int64_t)t.batch * nbf * naux_p, naux_p, nbf}; }
   // kpart slot of window row w of batch p, or -1 when the row is past the window or its pair
   // belongs to the other batch.
@@ -402,12 +420,12 @@ struct ProbQT {
           int dm, dn;
           coords<FM, FN>(lane, wm, wn, 0, j, r, dm, dn);
           if (dm == 0 && t.n0 + dn < naux_p) {
-            const int64_t o = (int64_t)p * naux_p + t.n0 + dn;
+            const int64_t o = (int64_t)p * naux_p + t.n0 + dn, ob = blk_index(p, t.n0 + dn, naux_p / 32);
             const float v = acc[0][j][r];
             gmem(d)[o] = v;
             const uint16_t hb = bf16_rne(v);
-            gmem(dhi)[o] = hb;
-            gmem(dlo)[o] = bf16_rne(v - bf16_to_f(hb));
+            gmem(dhi)[ob] = hb;  // the split planes of d are blocked (J^n's A operand)
+            gmem(dlo)[ob] = bf16_rne(v - bf16_to_f(hb));
           }
         }
     }
@@ -491,7 +509,7 @@ struct ProbMOT {
   }
   __host__ __device__ int ksteps() const { return ldk / kBK; }
   __device__ int row_limit(const Tile&) const { return nbf; }
-  __device__ SrcRows a_src(const Tile&) const { return {ct, 0, ldk, nbf}; }
+  __device__ SrcBlocked a_src(const Tile&) const { return {ct, ldk / kBK, nbf}; }
   __device__ SrcXT b_src(const Tile& t) const { return {x, (int64_t)t.batch * nbf * naux_p, naux_p, nbf}; }
   template <int FM, int FN>
   __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
@@ -579,8 +597,8 @@ struct ProbJN {
   }
   __host__ __device__ int ksteps() const { return naux_p / kBK; }
   __device__ int row_limit(const Tile&) const { return nbf5; }
-  __device__ SrcRows a_src(const Tile&) const { return {dd, 0, naux_p, nbf5}; }
-  __device__ SrcRows b_src(const Tile&) const { return {bt, 0, naux_p, (int)ntri}; }
+  __device__ SrcBlocked a_src(const Tile&) const { return {dd, naux_p / kBK, nbf5}; }  // d16, bt16 are blocked
+  __device__ SrcBlocked b_src(const Tile&) const { return {bt, naux_p / kBK, ntri}; }
   template <int FM, int FN>
   __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
     const int64_t nn = (int64_t)nbf * nbf;
@@ -662,6 +680,20 @@ __global__ void k_split_packed(const double* __restrict__ src, int64_t n, uint16
   lo[i] = l;
 }
 
+// Split planes of a row-major rows x cols FP64 matrix in 16-row x 32-column blocks
+// (blk_index, cols a multiple of 32), rows padded with zeros to rows_p (a multiple of 16).
+__global__ void k_split_blocked(const double* __restrict__ src, int64_t rows, int64_t cols, int64_t rows_p,
+                                uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows_p * cols) return;
+  const int64_t r = i / cols, c = i - r * cols;
+  uint16_t h, l;
+  split2(r < rows ? src[i] : 0.0, h, l);
+  const int64_t o = blk_index(r, c, cols / 32);
+  hi[o] = h;
+  lo[o] = l;
+}
+
 // bht[nu][P][mu] (mu < ldk, zero for mu >= nbf) from btri[tri(mu,nu)][P]:
 // a 32 (mu) x 64 (P) tile through LDS per workgroup; grid (P tiles, mu tiles, nu).
 __global__ void k_build_bht(const double* __restrict__ btri, int nbf, int naux_p, int ldk, uint16_t* __restrict__ hi,
@@ -682,7 +714,7 @@ __global__ void k_build_bht(const double* __restrict__ btri, int nbf, int naux_p
     if (p < naux_p && mu < ldk) {
       uint16_t h, l;
       split2(tileb[m][pp], h, l);
-      const int64_t o = ((int64_t)nu * naux_p + p) * ldk + mu;
+      const int64_t o = blk_index((int64_t)nu * naux_p + p, mu, ldk / 32);
       hi[o] = h;
       lo[o] = l;
     }
@@ -690,16 +722,17 @@ __global__ void k_build_bht(const double* __restrict__ btri, int nbf, int naux_p
 }
 
 // Ct[q][mu] (q < rows, mu < ldk) from Cp[mu][q] (zero outside nbf x nbf5).
-__global__ void k_split_ct(const double* __restrict__ Cp, int ldcp, int nbf, int nbf5, int rows, int ldk,
+__global__ void k_split_ct(const double* __restrict__ Cp, int ldcp, int nbf, int nbf5, int rows_p, int ldk,
                            uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)rows * ldk) return;
+  if (i >= (int64_t)rows_p * ldk) return;
   const int q = (int)(i / ldk), mu = (int)(i % ldk);
   const double v = (q < nbf5 && mu < nbf) ? Cp[(int64_t)mu * ldcp + q] : 0.0;
   uint16_t h, l;
   split2(v, h, l);
-  hi[i] = h;
-  lo[i] = l;
+  const int64_t o = blk_index(q, mu, ldk / 32);
+  hi[o] = h;
+  lo[o] = l;
 }
 
 template <int FM, int FN, int WGM, int WGN, class Prob>
@@ -751,6 +784,14 @@ inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
 }  // namespace
 
+hipError_t launch_split_blocked(const double* src, int64_t rows, int64_t cols, uint16_t* hi, uint16_t* lo,
+                                hipStream_t s) {
+  if (cols % 32 != 0) return hipErrorInvalidValue;
+  const int64_t rows_p = (rows + 15) / 16 * 16, n = rows_p * cols;
+  hipLaunchKernelGGL(k_split_blocked, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, src, rows, cols, rows_p, hi, lo);
+  return hipGetLastError();
+}
+
 hipError_t launch_split_btri(const double* btri, int64_t n, uint16_t* hi, uint16_t* lo, hipStream_t s) {
   hipLaunchKernelGGL(k_split_packed, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, btri, n, hi, lo);
   return hipGetLastError();
@@ -801,8 +842,9 @@ hipError_t launch_build_bht(const double* btri, int nbf, int naux_p, int ldk, ui
 
 hipError_t launch_split_ct(const double* Cp, int ldcp, int nbf, int nbf5, int rows, int ldk, uint16_t* hi,
                            uint16_t* lo, hipStream_t s) {
-  const int64_t n = (int64_t)rows * ldk;
-  hipLaunchKernelGGL(k_split_ct, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, Cp, ldcp, nbf, nbf5, rows, ldk, hi,
+  const int rows_p = (rows + 15) / 16 * 16;  // whole 16-row blocks
+  const int64_t n = (int64_t)rows_p * ldk;
+  hipLaunchKernelGGL(k_split_ct, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, Cp, ldcp, nbf, nbf5, rows_p, ldk, hi,
                      lo);
   return hipGetLastError();
 }
