@@ -40,7 +40,7 @@ OUTS = ("J", "K", "J_MO", "K_MO", "H_core")
 PROFILE = os.path.join(ROOT, "profiles", "jk_build_large_kernels.md")
 SPLIT_KERNEL = {"half_transform": "k_gemm_x3<5, 5, 2, 2, ProbHT>", "quarter_transform": "k_gemm_x3<5, 4, 2, 2, ProbQT>",
                 "coulomb_jn": "k_gemm_x3<5, 4, 2, 2, ProbJN>", "exchange_kn": "k_gemm_x3<4, 4, 2, 2, ProbSYRK>"}
-FP64_KERNEL = {"half_transform": "k_gemm_f64<5, 4, 8, ProbHT<false> >", "quarter_transform": "k_gemm_f64<2, 2, 16, ProbQT>",
+FP64_KERNEL = {"half_transform": "k_gemm_f64<5, 4, 8, ProbHT>", "quarter_transform": "k_gemm_f64<5, 2, 16, ProbQT>",
                "coulomb_jn": "k_gemm_f64<5, 4, 8, ProbJN>", "exchange_kn": "k_gemm_f64<4, 4, 8, ProbSYRK>"}
 
 

@@ -288,9 +288,9 @@ uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int preci
   if (precision != PNJK_FP64) {
     const int64_t ntri = nbf * (nbf + 1) / 2;
     s += (uint64_t)(roundup(ntri, 16) * naux_p) * 4;         // btri16 hi/lo (whole 16-row blocks)
-    s += (uint64_t)(nbf * naux_p * nbf_p) * 4;               // bht16 hi/lo
+    s += (uint64_t)(roundup(nbf, 16) * naux_p * nbf_p) * 4;  // bht16 hi/lo (rows nu < nbf rounded up to 16)
     s += (uint64_t)(roundup(nbf5, 16) * nbf_p) * 4;          // Ct16 hi/lo (whole 16-row blocks)
-    s += (uint64_t)(nbf5 * nbf * naux_p + 64) * 4;           // X16 hi/lo (+ slack)
+    s += (uint64_t)(nbf5 * x16_stride((int)nbf, (int)naux_p) + kX16Slack) * 4;  // X16 hi/lo (+ slack)
     s += (uint64_t)(roundup(nbf5, 16) * naux_p) * 4;         // d16 hi/lo (whole 16-row blocks)
   }
   return s;
@@ -302,7 +302,7 @@ static int ensure_split_planes(pnjk_engine* e, hipStream_t s) {
   const int64_t ntri = e->ntri, naux_p = e->naux_p, nbf = e->nbf, ldk = roundup(nbf, 32);
   int rc;
   // bt16 in whole 16-row blocks (blk_index)
-  const size_t tri_bytes = (size_t)(roundup(ntri, 16) * naux_p) * 2, ht_bytes = (size_t)(nbf * naux_p * ldk) * 2;
+  const size_t tri_bytes = (size_t)(roundup(ntri, 16) * naux_p) * 2, ht_bytes = (size_t)(roundup(nbf, 16) * naux_p * ldk) * 2;
   if ((rc = e->bt16h.ensure(tri_bytes)) || (rc = e->bt16l.ensure(tri_bytes))) return rc;
   if ((rc = e->bht16h.ensure(ht_bytes)) || (rc = e->bht16l.ensure(ht_bytes))) return rc;
   PNJK_HIP_CHECK(launch_split_blocked(e->btri.as<double>(), ntri, naux_p, e->bt16h.as<uint16_t>(),
@@ -631,9 +631,8 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
     w.bytes[W_T] = (size_t)(nbf * g.ldcp) * 8;
   }
   if (precision != PNJK_FP64) {
-    // X16 slack: a 128-column quarter-transform tile reads up to 64 columns past naux_p.
     w.bytes[W_CT16H] = w.bytes[W_CT16L] = (size_t)(roundup(n5, 16) * st.sg.ldk) * 2;  // whole 16-row blocks
-    w.bytes[W_X16H] = w.bytes[W_X16L] = (size_t)(n5 * nbf * g.naux_p + 64) * 2;
+    w.bytes[W_X16H] = w.bytes[W_X16L] = (size_t)(n5 * x16_stride(nbf, g.naux_p) + kX16Slack) * 2;
     w.bytes[W_D16H] = w.bytes[W_D16L] = (size_t)(roundup(n5, 16) * g.naux_p) * 2;  // whole 16-row blocks
   }
   w.bytes[W_STAGE] = (size_t)std::max<int64_t>(stage_elems, 1) * 8;
@@ -728,7 +727,7 @@ int stage_ht(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col)
   } else {
     PNJK_NEED(st.Cp);
     PNJK_NEED(st.X);
-    PNJK_HIP_CHECK(launch_ht_f64(st.g, st.ht, st.Cp, e->btri.as<double>(), st.X, s, nullptr, nullptr, col));
+    PNJK_HIP_CHECK(launch_ht_f64(st.g, st.ht, st.Cp, e->btri.as<double>(), st.X, s, col));
   }
   return PNJK_OK;
 }
@@ -774,7 +773,7 @@ int stage_convert_x(pnjk_engine* e, BuildState& st, hipStream_t s) {
   PNJK_NEED(st.X);
   PNJK_NEED(st.x16h);
   PNJK_NEED(st.x16l);
-  PNJK_HIP_CHECK(launch_split_btri(st.X, (int64_t)st.g.nbf5 * st.g.nbf * st.g.naux_p, st.x16h, st.x16l, s));
+  PNJK_HIP_CHECK(launch_split_x16(st.X, st.sg, st.x16h, st.x16l, s));
   return PNJK_OK;
 }
 

@@ -256,16 +256,11 @@ __device__ __forceinline__ void elem_coords(int lane, int wm, int wn, int i, int
   dn = wn + 16 * j + (lane & 15);
 }
 
-// kSplit: the epilogue also writes the bf16 split planes of X (same layout);
-// a separate instantiation, so the plain FP64 kernel keeps its register budget.
-template <bool kSplit>
 struct ProbHT {
   static constexpr int kA = KM, kB = KM;
   const double* Cp;
   const double* btri;
   double* X;
-  uint16_t* xhi;
-  uint16_t* xlo;
   int nbf, naux_p, nbf5, ldcp, nbf_p, mtiles;
   int64_t N;
   __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
@@ -292,15 +287,7 @@ struct ProbHT {
           elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int q = t.m0 + dm;
           const int64_t n = (int64_t)t.n0 + dn;
-          if (q < nbf5 && n < N) {
-            gmem(X)[(int64_t)q * N + n] = acc[i][j][r];
-            if (kSplit) {
-              uint16_t h, l;
-              split2(acc[i][j][r], h, l);
-              gmem(xhi)[(int64_t)q * N + n] = h;
-              gmem(xlo)[(int64_t)q * N + n] = l;
-            }
-          }
+          if (q < nbf5 && n < N) gmem(X)[(int64_t)q * N + n] = acc[i][j][r];
         }
   }
 };
@@ -934,15 +921,12 @@ inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
 }  // namespace
 
-template <bool kSplit>
-hipError_t launch_ht_f64_t(const BuildGeom& g, TileF64 tc, const double* Cp, const double* btri, double* X,
-                           hipStream_t s, uint16_t* x_hi, uint16_t* x_lo, GroupCollector* col) {
-  ProbHT<kSplit> pr;
+hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* btri, double* X,
+                         hipStream_t s, GroupCollector* col) {
+  ProbHT pr;
   pr.Cp = Cp;
   pr.btri = btri;
   pr.X = X;
-  pr.xhi = x_hi;
-  pr.xlo = x_lo;
   pr.nbf = g.nbf;
   pr.naux_p = g.naux_p;
   pr.nbf5 = g.nbf5;
@@ -954,11 +938,6 @@ hipError_t launch_ht_f64_t(const BuildGeom& g, TileF64 tc, const double* Cp, con
   return launch_prob(pr, tc, pr.mtiles * cdiv(pr.N, BN), s, col);
 }
 
-hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* btri, double* X,
-                         hipStream_t s, uint16_t* x_hi, uint16_t* x_lo, GroupCollector* col) {
-  if (x_hi && x_lo) return launch_ht_f64_t<true>(g, tc, Cp, btri, X, s, x_hi, x_lo, col);
-  return launch_ht_f64_t<false>(g, tc, Cp, btri, X, s, nullptr, nullptr, col);
-}
 
 hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cw, int ldw, const double* X, double* kpart,
                          double* d, int* ntiles, hipStream_t s, GroupCollector* col) {

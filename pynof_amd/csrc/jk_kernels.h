@@ -104,10 +104,8 @@ struct BuildGeom {
 };
 
 // X[q][nu*naux_p + P] = sum_mu Cp[mu][q] b_tri[tri(mu,nu)][P]   (half-transform)
-// When x_hi/x_lo are given, also writes the bf16 split planes of X (same layout).
 hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* btri, double* X,
-                         hipStream_t s, uint16_t* x_hi = nullptr, uint16_t* x_lo = nullptr,
-                         GroupCollector* col = nullptr);
+                         hipStream_t s, GroupCollector* col = nullptr);
 
 // Y_p[q][P] = sum_nu C[nu][q] X[p][nu][P] over the cyclic window of rows q (qt_window);
 // writes the row sums of Y_p[q]^2 per 64 aux columns of the pairs batch p owns into
@@ -202,19 +200,32 @@ struct SplitGeom {
   int ldk;     // contraction stride over mu/nu of Ct16 and bht16 (nbf rounded up to 32, zero tail)
 };
 
+// X16[q] holds the (nu, P) matrix of orbital q in 16-row x 32-column blocks (blk_index in
+// jk_split.hip) over nbf rounded up to 16 rows: this many elements per orbital.
+__host__ __device__ inline int64_t x16_stride(int nbf, int naux_p) { return (int64_t)((nbf + 15) / 16 * 16) * naux_p; }
+// Elements past the last orbital of X16 that a 128-column quarter-transform tile may read (64
+// columns past naux_p of the last row block land up to two 512-element blocks further on).
+constexpr int64_t kX16Slack = 1024;
+
 // hi/lo bf16 planes of n packed FP64 values.
 hipError_t launch_split_btri(const double* btri, int64_t n, uint16_t* hi, uint16_t* lo, hipStream_t s);
 // Split planes of a row-major rows x cols matrix (cols a multiple of 32) in 16-row x 32-column
 // blocks (blk_index in jk_split.hip); rows rounded up to 16 with zeros.
 hipError_t launch_split_blocked(const double* src, int64_t rows, int64_t cols, uint16_t* hi, uint16_t* lo,
                                 hipStream_t s);
-// bht[nu][P][mu] planes (mu contiguous, stride ldk) from btri.
+// X16 planes (x16_stride layout, zero pad rows) of FP64 X[q][nu][P] (row stride naux_p, nbf rows
+// per orbital) for nbf5 orbitals.
+hipError_t launch_split_x16(const double* X, const SplitGeom& g, uint16_t* hi, uint16_t* lo, hipStream_t s);
+// bht planes: row n = blk_index(nu, P, naux_p / 32) of the (nu, P) matrix (nu < nbf rounded up to
+// 16, zero rows past nbf) holds b[mu][nu][P] over mu < ldk (zero for mu >= nbf); the rows are
+// stored in 16 x 32 blocks over (n, mu). The half-transform's output row X16[q][n] is then in
+// the x16_stride layout.
 hipError_t launch_build_bht(const double* btri, int nbf, int naux_p, int ldk, uint16_t* hi, uint16_t* lo,
                             hipStream_t s);
 // Ct[q][mu] planes (rows q < rows rounded up to 16, in 16 x 32 blocks, blk_index) from Cp[mu][q].
 hipError_t launch_split_ct(const double* Cp, int ldcp, int nbf, int nbf5, int rows, int ldk, uint16_t* hi,
                            uint16_t* lo, hipStream_t s);
-// X16[q][nu][P] = split(sum_mu C[mu][q] b[mu][nu][P]).
+// X16[q] = split(sum_mu C[mu][q] b[mu][nu][P]) in the x16_stride layout.
 hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
                         const uint16_t* bht_hi, const uint16_t* bht_lo, uint16_t* x_hi, uint16_t* x_lo,
                         hipStream_t s, GroupCollector* col = nullptr);
@@ -234,7 +245,7 @@ hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
 hipError_t launch_jn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* d_hi, const uint16_t* d_lo,
                         const uint16_t* bt_hi, const uint16_t* bt_lo, double* J, hipStream_t s,
                         GroupCollector* col = nullptr);
-// K^n from split X16 (FP64 output, lower triangle of tiles only).
+// K^n from split X16 (x16_stride layout; FP64 output, lower triangle of tiles only).
 hipError_t launch_kn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* x_hi, const uint16_t* x_lo, double* K,
                         hipStream_t s, GroupCollector* col = nullptr);
 

@@ -93,8 +93,8 @@ __device__ __forceinline__ int qswz(int k) {
   else return ((k >> 1) & 1) | ((k >> 2) & 2);  // rows of 128 bytes: odd rows sit 32 banks on
 }
 
-// Direct HBM -> LDS copy of both planes of a 32 (k) x COLS tile whose columns are
-// contiguous in memory (global element (k, n) at off(k) + n). LDS keeps the
+// Direct HBM -> LDS copy of both planes of a 32 (k) x COLS tile; the source maps (k, n) to its
+// element offset, eight consecutive n from a multiple of 8 being contiguous. LDS keeps the
 // orientation: row k holds the COLS elements, quads swizzled by qswz. Lane L of
 // instruction i writes the 16 bytes at LDS chunk 64 i + L, i.e. row k =
 // (64 i + L) / (COLS / 8); it fetches the global chunk that belongs there.
@@ -111,7 +111,7 @@ struct DmaKN {
       if (kInst % NW == 0 || i < kInst) {
         const int o = 64 * i + lane;
         const int k = o / kRowChunks, pc = o % kRowChunks;
-        const int64_t g = s.off(k0 + k) + n0 + 16 * ((pc >> 1) ^ qswz<COLS>(k)) + 8 * (pc & 1);
+        const int64_t g = s.at(k0 + k, n0 + 16 * ((pc >> 1) ^ qswz<COLS>(k)) + 8 * (pc & 1));
         __builtin_amdgcn_global_load_lds((const PNJK_GLOBAL void*)(s.p.hi + g),
                                          (__attribute__((address_space(3))) void*)(sh + 512 * i), 16, 0, 0);
         __builtin_amdgcn_global_load_lds((const PNJK_GLOBAL void*)(s.p.lo + g),
@@ -309,7 +309,8 @@ __device__ __forceinline__ void coords(int lane, int wm, int wn, int i, int j, i
   dn = wn + 16 * j + (lane & 15);
 }
 
-// X16[q][n] = split(sum_mu Ct[q][mu] bht[n][mu]),  n = nu*naux_p + P
+// X16[q][n] = split(sum_mu Ct[q][mu] bht[n][mu]), n = blk_index(nu, P, naux_p / 32): row q of
+// X16 is the (nu, P) matrix of orbital q in 16 x 32 blocks, written contiguously.
 struct ProbHT {
   static constexpr int kA = RK, kB = RK;
   static constexpr bool kRowSkip = false;
@@ -361,11 +362,12 @@ struct ProbHT {
 // (max(p, q), min(p, q)). Tiles are 64 or 128 aux columns wide (FN = 2 or 4); both give the
 // same kpart bits: a 64-column partial is always S(first 32 columns) + S(last 32 columns),
 // S being one wave's sum over two 16-column blocks reduced across 16 lanes.
-struct SrcXT {  // element (k = nu, n = P) at p*nbf*naux_p + min(nu, nbf-1)*naux_p + P
+// Element (k = nu, n = P) of X16[p]: base + blk_index(min(nu, nbf - 1), P, naux_p / 32).
+struct SrcXT {
   Planes p;
-  int64_t base, ld;
+  int64_t base, kblocks;
   int kmax;
-  __device__ __forceinline__ int64_t off(int k) const { return base + (int64_t)(k < kmax ? k : kmax - 1) * ld; }
+  __device__ __forceinline__ int64_t at(int k, int n) const { return base + blk_index(k < kmax ? k : kmax - 1, n, kblocks); }
 };
 
 struct SrcWindow {  // window row r of batch b: C^T row qt_row(b, min(r, h - 1), n) of the blocked C^T planes
@@ -399,7 +401,7 @@ struct ProbQT {
   __host__ __device__ int ksteps() const { return ldk / kBK; }
   __device__ int row_limit(const Tile&) const { return h; }
   __device__ SrcWindow a_src(const Tile& t) const { return {ct, ldk / kBK, t.batch, h, nbf5}; }
-  __device__ SrcXT b_src(const Tile& t) const { return {x, (int64_t)t.batch * nbf * naux_p, naux_p, nbf}; }
+  __device__ SrcXT b_src(const Tile& t) const { return {x, t.batch * x16_stride(nbf, naux_p), naux_p / 32, nbf}; }
   // kpart slot of window row w of batch p, or -1 when the row is past the window or its pair
   // belongs to the other batch.
   __device__ __forceinline__ int64_t kslot(int p, int w) const {
@@ -510,7 +512,7 @@ struct ProbMOT {
   __host__ __device__ int ksteps() const { return ldk / kBK; }
   __device__ int row_limit(const Tile&) const { return nbf; }
   __device__ SrcBlocked a_src(const Tile&) const { return {ct, ldk / kBK, nbf}; }
-  __device__ SrcXT b_src(const Tile& t) const { return {x, (int64_t)t.batch * nbf * naux_p, naux_p, nbf}; }
+  __device__ SrcXT b_src(const Tile& t) const { return {x, t.batch * x16_stride(nbf, naux_p), naux_p / 32, nbf}; }
   template <int FM, int FN>
   __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
 #pragma unroll
@@ -634,7 +636,7 @@ struct ProbSYRK {
   static constexpr bool kLower = true;  // rows above the diagonal are never stored
   Planes x;
   double* O;
-  int64_t lda, sa, ldo, so;
+  int64_t sa, ldo, so;  // sa: X16 elements per batch (blocked rows of K columns)
   int M, K, tiles_per_batch;
   __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.batch = id / tiles_per_batch;
@@ -646,8 +648,8 @@ struct ProbSYRK {
   }
   __host__ __device__ int ksteps() const { return K / kBK; }
   __device__ int row_limit(const Tile&) const { return M; }
-  __device__ SrcRows a_src(const Tile& t) const { return {x, (int64_t)t.batch * sa, lda, M}; }
-  __device__ SrcRows b_src(const Tile& t) const { return {x, (int64_t)t.batch * sa, lda, M}; }
+  __device__ SrcBlocked a_src(const Tile& t) const { return {{x.hi + t.batch * sa, x.lo + t.batch * sa}, K / kBK, M}; }
+  __device__ SrcBlocked b_src(const Tile& t) const { return a_src(t); }
   template <int FM, int FN>
   __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
     double* o = O + (int64_t)t.batch * so;
@@ -682,20 +684,41 @@ __global__ void k_split_packed(const double* __restrict__ src, int64_t n, uint16
 
 // Split planes of a row-major rows x cols FP64 matrix in 16-row x 32-column blocks
 // (blk_index, cols a multiple of 32), rows padded with zeros to rows_p (a multiple of 16).
-__global__ void k_split_blocked(const double* __restrict__ src, int64_t rows, int64_t cols, int64_t rows_p,
+// nb such matrices one after the other: source stride rows x cols, destination rows_p x cols.
+// The destination is written in its own order: thread g fills elements [8 g, 8 g + 8) of
+// both planes (one 16-byte store each) from 8 consecutive source values of one row.
+__global__ void k_split_blocked(const double* __restrict__ src, int nb, int64_t rows, int64_t cols, int64_t rows_p,
                                 uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows_p * cols) return;
-  const int64_t r = i / cols, c = i - r * cols;
-  uint16_t h, l;
-  split2(r < rows ? src[i] : 0.0, h, l);
-  const int64_t o = blk_index(r, c, cols / 32);
-  hi[o] = h;
-  lo[o] = l;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int kb = (int)(cols / 32), rb = (int)(rows_p / 16);
+  const int64_t blk = g >> 6;  // 16 x 32 block: 64 chunks of 8
+  if (blk >= (int64_t)nb * rb * kb) return;
+  const int in = (int)(g & 63);
+  const int64_t b = blk / ((int64_t)rb * kb);
+  const int64_t rest = blk - b * rb * kb;
+  const int64_t r = (rest / kb) * 16 + (in >> 2), c = (rest % kb) * 32 + 8 * (in & 3);
+  u16x8 h, l;
+  if (r < rows) {
+    const double* p = src + (b * rows + r) * cols + c;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      uint16_t x, y;
+      split2(p[e], x, y);
+      h[e] = x;
+      l[e] = y;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) h[e] = l[e] = 0;
+  }
+  *reinterpret_cast<u16x8*>(hi + 8 * g) = h;
+  *reinterpret_cast<u16x8*>(lo + 8 * g) = l;
 }
 
-// bht[nu][P][mu] (mu < ldk, zero for mu >= nbf) from btri[tri(mu,nu)][P]:
-// a 32 (mu) x 64 (P) tile through LDS per workgroup; grid (P tiles, mu tiles, nu).
+// bht row n = blk_index(nu, P, naux_p / 32) holds b[mu][nu][P] over mu < ldk (zero for mu >= nbf
+// and for nu >= nbf), rows in 16 x 32 blocks over (n, mu); from btri[tri(mu,nu)][P]:
+// a 32 (mu) x 64 (P) tile through LDS per workgroup; grid (P tiles, mu tiles, nu < nbf
+// rounded up to 16).
 __global__ void k_build_bht(const double* __restrict__ btri, int nbf, int naux_p, int ldk, uint16_t* __restrict__ hi,
                             uint16_t* __restrict__ lo) {
   __shared__ double tileb[32][65];
@@ -704,7 +727,7 @@ __global__ void k_build_bht(const double* __restrict__ btri, int nbf, int naux_p
     const int m = e / 64, pp = e % 64;
     const int mu = mu0 + m, p = p0 + pp;
     double v = 0.0;
-    if (mu < nbf && p < naux_p) v = btri[tri_index(mu, nu) * naux_p + p];
+    if (mu < nbf && nu < nbf && p < naux_p) v = btri[tri_index(mu, nu) * naux_p + p];
     tileb[m][pp] = v;
   }
   __syncthreads();
@@ -714,7 +737,7 @@ __global__ void k_build_bht(const double* __restrict__ btri, int nbf, int naux_p
     if (p < naux_p && mu < ldk) {
       uint16_t h, l;
       split2(tileb[m][pp], h, l);
-      const int64_t o = blk_index((int64_t)nu * naux_p + p, mu, ldk / 32);
+      const int64_t o = blk_index(blk_index(nu, p, naux_p / 32), mu, ldk / 32);
       hi[o] = h;
       lo[o] = l;
     }
@@ -787,8 +810,17 @@ inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 hipError_t launch_split_blocked(const double* src, int64_t rows, int64_t cols, uint16_t* hi, uint16_t* lo,
                                 hipStream_t s) {
   if (cols % 32 != 0) return hipErrorInvalidValue;
-  const int64_t rows_p = (rows + 15) / 16 * 16, n = rows_p * cols;
-  hipLaunchKernelGGL(k_split_blocked, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, src, rows, cols, rows_p, hi, lo);
+  const int64_t rows_p = (rows + 15) / 16 * 16, n = rows_p * cols / 8;
+  hipLaunchKernelGGL(k_split_blocked, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, src, 1, rows, cols, rows_p, hi,
+                     lo);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_x16(const double* X, const SplitGeom& g, uint16_t* hi, uint16_t* lo, hipStream_t s) {
+  if (g.naux_p % 32 != 0) return hipErrorInvalidValue;
+  const int64_t rows_p = (g.nbf + 15) / 16 * 16, n = (int64_t)g.nbf5 * rows_p * g.naux_p / 8;
+  hipLaunchKernelGGL(k_split_blocked, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, X, g.nbf5, (int64_t)g.nbf,
+                     (int64_t)g.naux_p, rows_p, hi, lo);
   return hipGetLastError();
 }
 
@@ -835,7 +867,7 @@ hipError_t launch_lag_x3(int nbf, int nbf5, int naux, const double* B, int64_t l
 
 hipError_t launch_build_bht(const double* btri, int nbf, int naux_p, int ldk, uint16_t* hi, uint16_t* lo,
                             hipStream_t s) {
-  const dim3 grid(cdiv(naux_p, 64), cdiv(ldk, 32), nbf);
+  const dim3 grid(cdiv(naux_p, 64), cdiv(ldk, 32), (nbf + 15) / 16 * 16);
   hipLaunchKernelGGL(k_build_bht, grid, dim3(256), 0, s, btri, nbf, naux_p, ldk, hi, lo);
   return hipGetLastError();
 }
@@ -859,7 +891,7 @@ hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
   pr.xlo = x_lo;
   pr.nbf5 = g.nbf5;
   pr.ldk = g.ldk;
-  pr.N = (int64_t)g.nbf * g.naux_p;
+  pr.N = x16_stride(g.nbf, g.naux_p);
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
   pr.m_base = 0;
   pr.mtiles = cdiv(g.nbf5, BM);
@@ -966,8 +998,7 @@ hipError_t launch_kn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* x_hi, co
   ProbSYRK pr;
   pr.x = {x_hi, x_lo};
   pr.O = K;
-  pr.lda = g.naux_p;
-  pr.sa = (int64_t)g.nbf * g.naux_p;
+  pr.sa = x16_stride(g.nbf, g.naux_p);
   pr.ldo = g.nbf;
   pr.so = (int64_t)g.nbf * g.nbf;
   pr.M = g.nbf;
