@@ -176,6 +176,20 @@ int pnjk_fock_rc(pnjk_engine* e, const double* J, const double* K, int64_t nbf, 
                  const double* H, int64_t ldh, const double* cj12, const double* ck12, int64_t no1, int64_t nbeta,
                  int64_t nalpha, double* F, void* stream);
 
+/* Generalised Fock matrices, restricted open shell: replaces computeF_RO_driver /
+ * computeF_RO_CPU / computeF_RO_GPU (utils.py:87-93, :96-171, :174-246), which
+ * ENERGY1r calls when p.no1 > 0 and p.MSpin != 0 (utils.py:422-425). With
+ * D = [0, nbeta), S = [nbeta, nalpha) (singly occupied), W = [nalpha, nbf5):
+ *   i in D or W: F[i] = n[i] H + w_i n[i] J[i] + sum_{j in D, W} (cj12'[i][j] J[j]
+ *                - ck12'[i][j] K[j]) + n[i] sum_{s in S} (J[s] - K[s] / 2),
+ *   i in S:      F[i] = H / 2 + sum_{s in S, s != i} (J[s] - K[s]) / 2
+ *                + sum_{j in D, W} n[j] (J[j] - K[j] / 2),
+ * with w_i, ini and the zero diagonals as for pnjk_fock_rc. Same arguments, memory
+ * rules and limits as pnjk_fock_rc; the same GEMM with these coefficients. */
+int pnjk_fock_ro(pnjk_engine* e, const double* J, const double* K, int64_t nbf, int64_t nbf5, const double* n,
+                 const double* H, int64_t ldh, const double* cj12, const double* ck12, int64_t no1, int64_t nbeta,
+                 int64_t nalpha, double* F, void* stream);
+
 /* Lagrange multipliers from the Fock matrices: replaces computeLagrange
  * (utils.py:249-259):
  *   G[m][i]    = sum_n F[i][m][n] C[n][i]                 i < nbf5

@@ -271,6 +271,71 @@ def fock_rc(J, K, n, H, cj12, ck12, nbeta, nalpha, no1=0):
     return F
 
 
+def fock_ro(J, K, n, H, cj12, ck12, nbeta, nalpha, no1=0):
+    """Generalised Fock matrices of the restricted open-shell case, computeF_RO_CPU
+    (utils.py:96-171), with D = [0, nbeta), S = [nbeta, nalpha), W = [nalpha, nbf5):
+    n_i H on D and W, H / 2 on S (:104-106); n_i J[i] on [ini, nbeta) and W (:109-114);
+    C^J', C^K' (diagonal of the [ini:, ini:] block zeroed, :121, :138) over the D/W x D/W
+    blocks only (:122-152); SUMij (:155-161): n_i (J[s] - K[s] / 2) summed over s in S on
+    D and W rows, and on S rows (J[s] - K[s]) / 2 over s in S less the row's own term;
+    PRODWROij (:164-169): sum_j n_j (J[j] - K[j] / 2) over j in D and W on S rows. The
+    caller's arrays are left alone."""
+    nbf5 = J.shape[0]
+    ini = no1 if no1 > 1 else 0
+    cj = np.array(cj12, dtype=np.float64, copy=True)
+    ck = np.array(ck12, dtype=np.float64, copy=True)
+    np.fill_diagonal(cj[ini:, ini:], 0)
+    np.fill_diagonal(ck[ini:, ini:], 0)
+    b, a = nbeta, nalpha
+    F = np.zeros((nbf5,) + H.shape)
+    F[:b] += np.einsum("i,mn->imn", n[:b], H, optimize=True)
+    F[b:a] += 0.5 * H
+    F[a:] += np.einsum("i,mn->imn", n[a:], H, optimize=True)
+    F[ini:b] += np.einsum("i,imn->imn", n[ini:b], J[ini:b], optimize=True)
+    F[a:] += np.einsum("i,imn->imn", n[a:], J[a:], optimize=True)
+    for r in (slice(0, b), slice(a, nbf5)):
+        for c in (slice(0, b), slice(a, nbf5)):
+            F[r] += np.einsum("ij,jmn->imn", cj[r, c], J[c], optimize=True)
+            F[r] -= np.einsum("ij,jmn->imn", ck[r, c], K[c], optimize=True)
+    js = J[b:a] - 0.5 * K[b:a]
+    F[:b] += np.einsum("i,jmn->imn", n[:b], js, optimize=True)
+    F[a:] += np.einsum("i,jmn->imn", n[a:], js, optimize=True)
+    F[b:a] += 0.5 * np.sum(J[b:a] - K[b:a], axis=0) - 0.5 * (J[b:a] - K[b:a])
+    dw = np.r_[0:b, a:nbf5]
+    F[b:a] += np.einsum("j,jmn->mn", n[dw], J[dw] - 0.5 * K[dw], optimize=True)
+    return F
+
+
+def fock_ro_loops(J, K, n, H, cj12, ck12, nbeta, nalpha, no1=0):
+    """fock_ro by explicit loops over i, j (tiny inputs only), one coefficient of H, J[j] and
+    K[j] per (i, j) from the orbital classes of i and j."""
+    nbf5 = J.shape[0]
+    ini = no1 if no1 > 1 else 0
+
+    def single(k):
+        return nbeta <= k < nalpha
+
+    F = np.zeros((nbf5,) + H.shape)
+    for i in range(nbf5):
+        F[i] = (0.5 if single(i) else n[i]) * H
+        for j in range(nbf5):
+            a = c = 0.0  # coefficients of J[j] and K[j]
+            if single(i) and single(j):
+                if j != i:
+                    a, c = 0.5, -0.5
+            elif single(i):
+                a, c = n[j], -0.5 * n[j]
+            elif single(j):
+                a, c = n[i], -0.5 * n[i]
+            else:
+                if not (i == j and i >= ini):
+                    a, c = cj12[i, j], -ck12[i, j]
+                if i == j and i >= ini:
+                    a += n[i]
+            F[i] += a * J[j] + c * K[j]
+    return F
+
+
 def lagrange_fock(F, C, noptorb):
     """Lagrange multipliers from the Fock matrices, computeLagrange (utils.py:249-259):
     G[m, i] = sum_n F[i, m, n] C[n, i]; elag[i, j] = sum_m C[m, i] G[m, j] for i < noptorb,

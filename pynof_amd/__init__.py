@@ -13,6 +13,7 @@ from .integrals import (  # noqa: F401
     JKH_MO_tmp,
     computeLagrangeConvergency,
     computeF_RC_driver,
+    computeF_RO_driver,
     computeJKH_MO,
     computeJKj,
     computeLagrange,

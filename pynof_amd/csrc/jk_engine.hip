@@ -1170,22 +1170,25 @@ static int to_device_2d(const double* src, int64_t lds, double* dst, int64_t ldd
   return PNJK_OK;
 }
 
-int pnjk_fock_rc(pnjk_engine* e, const double* J, const double* K, int64_t nbf, int64_t nbf5, const double* n,
-                 const double* H, int64_t ldh, const double* cj12, const double* ck12, int64_t no1, int64_t nbeta,
-                 int64_t nalpha, double* F, void* stream) {
-  if (!e) return set_error(PNJK_EINVAL, "pnjk_fock_rc: engine is NULL");
+// pnjk_fock_rc and pnjk_fock_ro: one coefficient kernel and one FP64 GEMM, the shell chosen by
+// open_shell; `fn` names the entry point in error messages.
+static int fock_common(pnjk_engine* e, const char* fn, bool open_shell, const double* J, const double* K,
+                       int64_t nbf, int64_t nbf5, const double* n, const double* H, int64_t ldh,
+                       const double* cj12, const double* ck12, int64_t no1, int64_t nbeta, int64_t nalpha, double* F,
+                       void* stream) {
+  if (!e) return set_error(PNJK_EINVAL, std::string(fn) + ": engine is NULL");
   std::lock_guard<std::mutex> lk(e->mu);
   PNJK_HIP_CHECK(hipSetDevice(e->device));
   hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
   if (!J || !K || !n || !H || !cj12 || !ck12 || !F)
-    return set_error(PNJK_EINVAL, "pnjk_fock_rc: J, K, n, H, cj12, ck12 and F are required");
+    return set_error(PNJK_EINVAL, std::string(fn) + ": J, K, n, H, cj12, ck12 and F are required");
   if (nbf < 1 || nbf5 < 1 || nbf5 > nbf || nbf * nbf > (int64_t)1 << 30)
-    return set_error(PNJK_EINVAL, "pnjk_fock_rc: need 1 <= nbf5 <= nbf and nbf^2 <= 2^30");
-  if (ldh < nbf) return set_error(PNJK_EINVAL, "pnjk_fock_rc: ldh < nbf");
+    return set_error(PNJK_EINVAL, std::string(fn) + ": need 1 <= nbf5 <= nbf and nbf^2 <= 2^30");
+  if (ldh < nbf) return set_error(PNJK_EINVAL, std::string(fn) + ": ldh < nbf");
   if (no1 < 0 || nbeta < 0 || nalpha < nbeta || nalpha > nbf5)
-    return set_error(PNJK_EINVAL, "pnjk_fock_rc: need no1 >= 0 and 0 <= nbeta <= nalpha <= nbf5");
+    return set_error(PNJK_EINVAL, std::string(fn) + ": need no1 >= 0 and 0 <= nbeta <= nalpha <= nbf5");
   const int64_t n5 = nbf5, nn = nbf * nbf;
-  const int ini = no1 > 1 ? (int)no1 : 0;  // utils.py:31-33
+  const int ini = no1 > 1 ? (int)no1 : 0;  // utils.py:31-33, :101-103
   // J and K feed the GEMM in place when they are device rows with 16-byte aligned pairs;
   // otherwise they are staged with an even row stride.
   const bool direct = is_device_ptr(J) && is_device_ptr(K) && nn % 2 == 0 && (uintptr_t)J % 16 == 0 &&
@@ -1194,7 +1197,7 @@ int pnjk_fock_rc(pnjk_engine* e, const double* J, const double* K, int64_t nbf, 
   const int64_t kp = roundup(2 * n5, 16);
   int rc;
   if ((!direct && (rc = e->fjk.ensure((size_t)(2 * n5 * ld) * 8))) || (rc = e->fh.ensure((size_t)nn * 8)) ||
-      (rc = e->fco.ensure((size_t)(n5 * kp + 2 * n5 * n5 + n5) * 8)))
+      (rc = e->fco.ensure((size_t)(n5 * kp + 2 * n5 * n5 + 2 * n5) * 8)))
     return rc;
   const bool host_f = !is_device_ptr(F);
   if (host_f && (rc = e->ff.ensure((size_t)(n5 * nn) * 8))) return rc;
@@ -1211,12 +1214,13 @@ int pnjk_fock_rc(pnjk_engine* e, const double* J, const double* K, int64_t nbf, 
   double* cjd = A + n5 * kp;
   double* ckd = cjd + n5 * n5;
   double* nd = ckd + n5 * n5;
+  double* hcd = nd + n5;
   if ((rc = to_device_2d(H, ldh, e->fh.as<double>(), nbf, nbf, nbf, s)) || (rc = to_device(cj12, cjd, n5 * n5, s)) ||
       (rc = to_device(ck12, ckd, n5 * n5, s)) || (rc = to_device(n, nd, n5, s)))
     return rc;
   double* dF = host_f ? e->ff.as<double>() : F;
   PNJK_HIP_CHECK(launch_fock_f64((int)nbf, (int)n5, dJ, dK, ld, e->fh.as<double>(), nd, cjd, ckd, ini, (int)nbeta,
-                                 (int)nalpha, A, dF, s));
+                                 (int)nalpha, open_shell, A, hcd, dF, s));
   if (host_f) {
     PNJK_HIP_CHECK(hipMemcpyAsync(F, dF, (size_t)(n5 * nn) * 8, hipMemcpyDeviceToHost, s));
     PNJK_HIP_CHECK(hipStreamSynchronize(s));
@@ -1226,6 +1230,18 @@ int pnjk_fock_rc(pnjk_engine* e, const double* J, const double* K, int64_t nbf, 
     PNJK_HIP_CHECK(hipStreamSynchronize(s));
   }
   return PNJK_OK;
+}
+
+int pnjk_fock_rc(pnjk_engine* e, const double* J, const double* K, int64_t nbf, int64_t nbf5, const double* n,
+                 const double* H, int64_t ldh, const double* cj12, const double* ck12, int64_t no1, int64_t nbeta,
+                 int64_t nalpha, double* F, void* stream) {
+  return fock_common(e, "pnjk_fock_rc", false, J, K, nbf, nbf5, n, H, ldh, cj12, ck12, no1, nbeta, nalpha, F, stream);
+}
+
+int pnjk_fock_ro(pnjk_engine* e, const double* J, const double* K, int64_t nbf, int64_t nbf5, const double* n,
+                 const double* H, int64_t ldh, const double* cj12, const double* ck12, int64_t no1, int64_t nbeta,
+                 int64_t nalpha, double* F, void* stream) {
+  return fock_common(e, "pnjk_fock_ro", true, J, K, nbf, nbf5, n, H, ldh, cj12, ck12, no1, nbeta, nalpha, F, stream);
 }
 
 int pnjk_lagrange_fock(pnjk_engine* e, const double* F, int64_t nbf, int64_t nbf5, const double* C, int64_t ldc,

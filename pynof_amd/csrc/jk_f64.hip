@@ -656,8 +656,8 @@ struct ProbFock {
   const double* A;
   const double* J;
   const double* K;
-  const double* H;  // nbf x nbf, row stride nbf
-  const double* nocc;
+  const double* H;   // nbf x nbf, row stride nbf
+  const double* hc;  // coefficient of H in row i
   double* F;
   int64_t ld, nn;
   int nbf, nbf5, kp, mtiles;
@@ -685,7 +685,7 @@ struct ProbFock {
           elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int m = t.m0 + dm;
           const int64_t n = (int64_t)t.n0 + dn;
-          if (m < nbf5 && n < nn) gmem(F)[(int64_t)m * nn + n] = acc[i][j][r] + nocc[m] * H[n];
+          if (m < nbf5 && n < nn) gmem(F)[(int64_t)m * nn + n] = acc[i][j][r] + hc[m] * H[n];
         }
   }
 };
@@ -1190,22 +1190,37 @@ hipError_t launch_fill_cols(double* p, int rows, int64_t ld, int64_t c0, int64_t
   return hipGetLastError();
 }
 
-// A[i][k] of the Fock GEMM (utils.py:34-52): k < nbf5: C^J[i][k] (zero on the diagonal for
-// i >= ini) + n_i for k == i when ini <= i < nbeta or i >= nalpha; nbf5 <= k < 2 nbf5: -C^K[i][k - nbf5]
-// (same diagonal rule); zero up to kp.
+// Row i of the Fock GEMM: F[i] = hc[i] H + sum_j A[i][j] J[j] + sum_j A[i][nbf5 + j] K[j] (A zero
+// from 2 nbf5 up to kp). D = [0, nbeta), S = [nbeta, nalpha), W = [nalpha, nbf5); C^J', C^K' are
+// the coefficients with a zero diagonal for i >= ini (ini = no1 if no1 > 1 else 0).
+// Closed shell (utils.py:25-54): hc = n_i; A[i][j] = C^J'[i][j] + n_i [j == i and i in [ini, nbeta)
+//   or W]; A[i][nbf5 + j] = -C^K'[i][j].
+// Open shell (utils.py:96-171): rows in D and W as above but with C^J', C^K' over columns in D and W
+//   only (:123-152), plus n_i J[s] - n_i K[s] / 2 for every s in S (SUMij, :155-161); rows in S:
+//   hc = 1/2 (:104), (J[s] - K[s]) / 2 for s in S other than i (SUMij less its diagonal,
+//   :158-159) and n_j J[j] - n_j K[j] / 2 for j in D and W (PRODWROij, :164-169).
 __global__ void k_fock_coef(const double* __restrict__ cj, const double* __restrict__ ck,
                             const double* __restrict__ nocc, int n5, int ini, int nbeta, int nalpha, int kp,
-                            double* __restrict__ A) {
+                            int open, double* __restrict__ A, double* __restrict__ hc) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n5 * kp) return;
   const int i = idx / kp, k = idx % kp;
+  const bool iS = i >= nbeta && i < nalpha;
+  if (k == 0) hc[i] = (open && iS) ? 0.5 : nocc[i];
   double v = 0.0;
-  if (k < n5) {
-    v = (k == i && i >= ini) ? 0.0 : cj[(int64_t)i * n5 + k];
-    if (k == i && ((i >= ini && i < nbeta) || i >= nalpha)) v += nocc[i];
-  } else if (k < 2 * n5) {
-    const int j = k - n5;
-    v = (j == i && i >= ini) ? 0.0 : -ck[(int64_t)i * n5 + j];
+  if (k < 2 * n5) {
+    const bool kj = k < n5;  // J part, else K part
+    const int j = kj ? k : k - n5;
+    const bool jS = j >= nbeta && j < nalpha;
+    if (!open || (!iS && !jS)) {
+      if (!(j == i && i >= ini)) v = kj ? cj[(int64_t)i * n5 + j] : -ck[(int64_t)i * n5 + j];
+    }
+    if (kj && j == i && ((i >= ini && i < nbeta) || i >= nalpha)) v += nocc[i];
+    if (open) {
+      if (!iS && jS) v += kj ? nocc[i] : -0.5 * nocc[i];
+      if (iS && jS && j != i) v += kj ? 0.5 : -0.5;
+      if (iS && !jS) v += kj ? nocc[j] : -0.5 * nocc[j];
+    }
   }
   A[idx] = v;
 }
@@ -1260,12 +1275,12 @@ __global__ void k_elag_scatter(const double* __restrict__ src, int nbf, int nbf5
 
 hipError_t launch_fock_f64(int nbf, int n5, const double* J, const double* K, int64_t ld, const double* Hd,
                            const double* nocc, const double* cj, const double* ck, int ini, int nbeta, int nalpha,
-                           double* A, double* F, hipStream_t s) {
+                           bool open_shell, double* A, double* hc, double* F, hipStream_t s) {
   const int64_t nn = (int64_t)nbf * nbf;
   if (ld % 2 != 0 || ld < nn || ld < 2) return hipErrorInvalidValue;
   const int kp = (2 * n5 + 15) / 16 * 16;
   hipLaunchKernelGGL(k_fock_coef, dim3(cdiv((int64_t)n5 * kp, 256)), dim3(256), 0, s, cj, ck, nocc, n5, ini, nbeta,
-                     nalpha, kp, A);
+                     nalpha, kp, (int)open_shell, A, hc);
   const bool big = n5 > 64 && nn >= (1 << 16);
   // 64 x 128 tiles: at nbf = 1000, nbf5 = 300 the GEMM takes 6.8 ms against 8.5 (160 x 128),
   // 7.4 (160 x 64, 128 x 64) and 9.7 (128 x 128): the K extent (2 nbf5) is short.
@@ -1275,7 +1290,7 @@ hipError_t launch_fock_f64(int nbf, int n5, const double* J, const double* K, in
   pr.J = J;
   pr.K = K;
   pr.H = Hd;
-  pr.nocc = nocc;
+  pr.hc = hc;
   pr.F = F;
   pr.ld = ld;
   pr.nn = nn;

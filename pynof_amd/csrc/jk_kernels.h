@@ -135,12 +135,12 @@ hipError_t launch_lag_x3(int nbf, int nbf5, int naux, const double* B, int64_t l
                          const double* ck, uint16_t* bh, uint16_t* bl, uint16_t* wh, uint16_t* wl, double* part,
                          bool planes_ready, hipStream_t s);
 hipError_t launch_lag_coef(const double* src, double* dst, int n, hipStream_t s);
-// Closed-shell generalised Fock matrices F (nbf5 x nbf x nbf) from J^n, K^n rows of stride
-// ld (even, >= nbf^2), device H (nbf x nbf), occupations and coefficients; A (nbf5 x
-// roundup(2 nbf5, 16)) is workspace.
+// Generalised Fock matrices F (nbf5 x nbf x nbf), closed shell or (open_shell) restricted open
+// shell, from J^n, K^n rows of stride ld (even, >= nbf^2), device H (nbf x nbf), occupations and
+// coefficients; A (nbf5 x roundup(2 nbf5, 16)) and hc (nbf5) are workspace.
 hipError_t launch_fock_f64(int nbf, int n5, const double* J, const double* K, int64_t ld, const double* Hd,
                            const double* nocc, const double* cj, const double* ck, int ini, int nbeta, int nalpha,
-                           double* A, double* F, hipStream_t s);
+                           bool open_shell, double* A, double* hc, double* F, hipStream_t s);
 // Lagrange multipliers from F (nbf5 x nbf x nbf): G (nbf x ldcp, + 64 slack, columns past
 // nbf5 finite) and tmp (nbf x nbf5) workspace; Cf the full-width C of ldcf columns (as for
 // the MO integrals); elag nbf x nbf.

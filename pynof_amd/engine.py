@@ -294,6 +294,16 @@ class JKEngine:
         cj12, ck12 (nbf5, nbf5), whose [ini:, ini:] diagonals (ini = no1 if no1 > 1 else 0)
         are taken as zero; inputs are not modified. Needs no b. Device (torch) J gives a
         device F; host inputs give a host F."""
+        return self._fock("pnjk_fock_rc", J, K, n, H, cj12, ck12, no1, nbeta, nalpha, out, stream)
+
+    def fock_ro(self, J, K, n, H, cj12, ck12, no1: int, nbeta: int, nalpha: int, out=None, stream=None):
+        """Restricted open-shell generalised Fock matrices (computeF_RO_CPU/_GPU,
+        utils.py:96-246): as fock_rc, with the singly occupied orbitals [nbeta, nalpha)
+        entering as the reference's SUMij and PRODWROij terms (utils.py:154-169) and the
+        C^J/C^K sums restricted to the other orbitals."""
+        return self._fock("pnjk_fock_ro", J, K, n, H, cj12, ck12, no1, nbeta, nalpha, out, stream)
+
+    def _fock(self, fn, J, K, n, H, cj12, ck12, no1, nbeta, nalpha, out, stream):
         Ji, Ki, Hi = _as_input(J, "J"), _as_input(K, "K"), _as_input(H, "H")
         if Ji.ndim != 3 or Ji.shape[1] != Ji.shape[2] or tuple(Ki.shape) != tuple(Ji.shape):
             raise ValueError(f"J and K must have one shape (nbf5, nbf, nbf); got {tuple(Ji.shape)}, {tuple(Ki.shape)}")
@@ -312,7 +322,7 @@ class JKEngine:
             F = np.empty((nbf5, nbf, nbf), dtype=np.float64)
         if stream is None and on_device:
             stream = _torch().cuda.current_stream(Ji.device).cuda_stream
-        _lib.check(_lib.lib().pnjk_fock_rc(
+        _lib.check(getattr(_lib.lib(), fn)(
             self._h, _ptr(Ji), _ptr(Ki), nbf, nbf5, _ptr(ni), _ptr(Hi), nbf, _ptr(cji), _ptr(cki), int(no1),
             int(nbeta), int(nalpha), _ptr(F), stream))
         return F

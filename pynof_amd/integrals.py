@@ -2,9 +2,10 @@
 
 ``computeJKj(C, I, b_mnl, p)``, ``computeJKH_MO(C, H, I, b_mnl, p)``,
 ``JKH_MO_tmp(C, H, I, b_mnl, p)``, ``computeLagrange2(n, cj12, ck12, C, H, I,
-b_mnl, p)``, ``computeF_RC_driver(J, K, n, H, cj12, ck12, p)`` and
-``computeLagrange(F, C, p)`` keep the reference's signatures (src/pynof/integrals.py:172,
-:307, :665; utils.py:262, :15, :249), argument meaning and return values:
+b_mnl, p)``, ``computeF_RC_driver(J, K, n, H, cj12, ck12, p)``,
+``computeF_RO_driver(J, K, n, H, cj12, ck12, p)`` and ``computeLagrange(F, C, p)`` keep
+the reference's signatures (src/pynof/integrals.py:172, :307, :665; utils.py:262, :15,
+:87, :249), argument meaning and return values:
 host FP64 arrays J, K of shape (nbf5, nbf, nbf); J_MO, K_MO (nbf5, nbf5) with
 H_core (nbf5,); the orbital-gradient pair H_mat (nbf, nbf5), b_MO
 (nbf, nbf5, naux); the Lagrangian elag (nbf, nbf) with Hmat; the generalised Fock
@@ -118,12 +119,9 @@ def _plain_engine(p) -> JKEngine:
         return eng
 
 
-def computeF_RC_driver(J, K, n, H, cj12, ck12, p):
-    """Closed-shell generalised Fock matrices F (nbf5, nbf, nbf), host FP64
-    (computeF_RC_driver, utils.py:15-54; the GPU form :58-84 returns the same host
-    array). As in the reference, the diagonals of the caller's cj12[ini:, ini:] and
-    ck12[ini:, ini:] are set to zero in place (:45, :50), ini = p.no1 if p.no1 > 1
-    else 0. Device (torch) J and K give a device F."""
+def _zero_diagonals(cj12, ck12, p):
+    """The reference's np.fill_diagonal(c[ini:, ini:], 0) on the caller's arrays (utils.py:45,
+    :50, :121, :138), ini = p.no1 if p.no1 > 1 else 0; torch tensors too."""
     import numpy as np
 
     ini = p.no1 if p.no1 > 1 else 0
@@ -132,8 +130,28 @@ def computeF_RC_driver(J, K, n, H, cj12, ck12, p):
             np.fill_diagonal(c[ini:, ini:], 0)
         else:
             c[ini:, ini:].fill_diagonal_(0)
+
+
+def computeF_RC_driver(J, K, n, H, cj12, ck12, p):
+    """Closed-shell generalised Fock matrices F (nbf5, nbf, nbf), host FP64
+    (computeF_RC_driver, utils.py:15-54; the GPU form :58-84 returns the same host
+    array). As in the reference, the diagonals of the caller's cj12[ini:, ini:] and
+    ck12[ini:, ini:] are set to zero in place (:45, :50), ini = p.no1 if p.no1 > 1
+    else 0. Device (torch) J and K give a device F."""
+    _zero_diagonals(cj12, ck12, p)
     nbf5 = int(p.nbf5)
     return _plain_engine(p).fock_rc(J, K, n[:nbf5], H, cj12, ck12, p.no1, p.nbeta, p.nalpha)
+
+
+def computeF_RO_driver(J, K, n, H, cj12, ck12, p):
+    """Restricted open-shell generalised Fock matrices F (nbf5, nbf, nbf), host FP64
+    (computeF_RO_driver, utils.py:87-93; CPU form :96-171, GPU form :174-246, both
+    returning a host array). The caller's cj12[ini:, ini:] and ck12[ini:, ini:] diagonals
+    are zeroed in place as the reference does (:121, :138). Device (torch) J and K give a
+    device F."""
+    _zero_diagonals(cj12, ck12, p)
+    nbf5 = int(p.nbf5)
+    return _plain_engine(p).fock_ro(J, K, n[:nbf5], H, cj12, ck12, p.no1, p.nbeta, p.nalpha)
 
 
 def computeLagrange(F, C, p):
@@ -161,10 +179,11 @@ class _OnDevice:
 def ENERGY1r(C, n, H, I, b_mnl, cj12, ck12, p):
     """(E, elag, sumdiff, maxdiff) of one orbital step (utils.py:415-432), host outputs as the
     reference returns them. no1 == 0: computeLagrange2 on the device (b_MO never leaves it),
-    E = tr elag[:nbf5, :nbf5] + sum n_i Hmat[i, i]. no1 > 0 (MSpin = 0): J^n, K^n, the Fock
-    matrices and elag stay on the device (the reference moves J^n, K^n and F across the bus),
-    E as computeE_elec (utils.py:385-402) from the build's H_core = diag(C^T H C). As in the
-    reference, the diagonals of the caller's cj12 and ck12 are zeroed in place."""
+    E = tr elag[:nbf5, :nbf5] + sum n_i Hmat[i, i]. no1 > 0: J^n, K^n, the Fock matrices
+    (closed shell for MSpin == 0, restricted open shell otherwise, utils.py:422-425) and elag
+    stay on the device (the reference moves J^n, K^n and F across the bus), E as computeE_elec
+    (utils.py:385-402) from the build's H_core = diag(C^T H C). As in the reference, the
+    diagonals of the caller's cj12 and ck12 are zeroed in place."""
     import numpy as np
     import torch
 
@@ -176,18 +195,15 @@ def ENERGY1r(C, n, H, I, b_mnl, cj12, ck12, p):
         E = float(np.trace(elag[:n5, :n5]) + np.dot(n[:n5], np.diagonal(Hmat[:n5, :n5])))
         sumdiff, maxdiff = _engine_for(b_mnl, p).lagrange_convergence(elag)
         return E, elag, sumdiff, maxdiff
-    if getattr(p, "MSpin", 0) != 0:
-        raise NotImplementedError("the open-shell Fock matrices (computeF_RO, utils.py:87-246) are not implemented")
-    ini = p.no1 if p.no1 > 1 else 0
-    for c in (cj12, ck12):
-        np.fill_diagonal(c[ini:, ini:], 0)
+    _zero_diagonals(cj12, ck12, p)
     eng = _engine_for(b_mnl, p)
     dev = torch.device("cuda", int(getattr(p, "device", 0)))
     td = lambda x: torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(dev)  # noqa: E731
     Cd, Hd = td(C), td(H)
     prec = getattr(p, "precision", "fp64")
     out = eng.build(Cd, n5, H=Hd, precision=prec, outputs=("J", "K", "H_core"))
-    F = eng.fock_rc(out["J"], out["K"], td(n[:n5]), Hd, td(cj12), td(ck12), p.no1, p.nbeta, p.nalpha)
+    fock = eng.fock_rc if getattr(p, "MSpin", 0) == 0 else eng.fock_ro
+    F = fock(out["J"], out["K"], td(n[:n5]), Hd, td(cj12), td(ck12), p.no1, p.nbeta, p.nalpha)
     elag_d = eng.lagrange_fock(F, Cd, p.noptorb)
     sumdiff, maxdiff = eng.lagrange_convergence(elag_d)
     hc = out["H_core"].cpu().numpy()

@@ -61,9 +61,32 @@ def test_energy1r_no1_positive(gpu_required, high_spin):
     assert np.all(np.diag(cj_in)[no1:] == 0) and np.all(np.diag(ck_in)[no1:] == 0)
     sd0, md0 = conv(el)
     assert abs(sd - sd0) <= 1e-9 * sd0 and abs(md - md0) <= 1e-9 * md0
-    p.MSpin = 1
-    with pytest.raises(NotImplementedError):
-        ENERGY1r(C, n, H, [], b, cj.copy(), ck.copy(), p)
+    release_cached_engines()
+
+
+@pytest.mark.parametrize("high_spin", [False, True])
+def test_energy1r_open_shell(gpu_required, high_spin):
+    """no1 > 0 and MSpin != 0: the restricted open-shell Fock route (utils.py:422-425)."""
+    nbf, naux, nbf5, no1, nsoc = 34, 120, 27, 2, 3
+    print(f"case: ENERGY1r open shell no1={no1} nsoc={nsoc} high_spin={high_spin}", flush=True)
+    b, C, H, n, cj, ck = inputs(nbf, naux, nbf5, 1115)
+    p = param(nbf=nbf, nbf5=nbf5, nbfaux=naux, ndoc=6, RI=True, no1=no1, nsoc=nsoc, noptorb=nbf - 1,
+              HighSpin=high_spin)
+    p.MSpin = nsoc  # set after construction, as the reference's callers do (parameters.py:107)
+    bt, al = p.nbeta, p.nalpha
+    n[bt:al] = 1.0 if high_spin else 0.5
+    J, K, _, _, hc = O.jk_build(C, H, b, nbf5)
+    F = O.fock_ro(J, K, n, H, cj, ck, bt, al, no1)
+    el = O.lagrange_fock(F, C, p.noptorb)
+    E_ref = np.trace(el[:nbf5, :nbf5]) + np.dot(n[:bt], hc[:bt]) + np.dot(n[al:], hc[al:])
+    E_ref += 0.5 * hc[bt:al].sum() if high_spin else np.dot(n[bt:al], hc[bt:al])  # utils.py:391-396
+    cj_in, ck_in = cj.copy(), ck.copy()
+    E, elag, sd, md = ENERGY1r(C, n, H, [], b, cj_in, ck_in, p)
+    assert abs(E - E_ref) <= 1e-10 * abs(E_ref)
+    assert np.linalg.norm(elag - el) <= 1e-10 * np.linalg.norm(el)
+    assert np.all(np.diag(cj_in)[no1:] == 0) and np.all(np.diag(ck_in)[no1:] == 0)
+    sd0, md0 = conv(el)
+    assert abs(sd - sd0) <= 1e-9 * sd0 and abs(md - md0) <= 1e-9 * md0
     release_cached_engines()
 
 

@@ -1,12 +1,13 @@
-"""Generalised Fock matrices and the Lagrangian from them (GPU only): pnjk_fock_rc and
-pnjk_lagrange_fock against the FP64 oracle (computeF_RC_CPU, utils.py:25-54, and
-computeLagrange, utils.py:249-259, restated), through the C ABI."""
+"""Generalised Fock matrices and the Lagrangian from them (GPU only): pnjk_fock_rc,
+pnjk_fock_ro and pnjk_lagrange_fock against the FP64 oracle (computeF_RC_CPU,
+utils.py:25-54, computeF_RO_CPU, :96-171, and computeLagrange, :249-259, restated), through
+the C ABI."""
 import numpy as np
 import pytest
 
 from oracle import jk_oracle as O
 from pynof_amd import JKEngine, _lib, param, synth
-from pynof_amd.integrals import computeF_RC_driver, computeLagrange, release_cached_engines
+from pynof_amd.integrals import computeF_RC_driver, computeF_RO_driver, computeLagrange, release_cached_engines
 
 pytestmark = pytest.mark.gpu
 FP64_TOL = 1e-10
@@ -57,6 +58,43 @@ def test_fock_vs_oracle(eng, nbf, nbf5, no1, nbeta, nalpha):
         el = eng.lagrange_fock(F, C, noptorb)
         assert el.shape == (nbf, nbf) and rel(el, el_ref) <= FP64_TOL
         assert np.all(el[noptorb:] == 0.0) and np.all(el[:, nbf5:] == 0.0)
+
+
+# (nbf, nbf5, no1, nbeta, nalpha) with singly occupied orbitals [nbeta, nalpha); nsoc = 0 too.
+RO_CASES = [(1, 1, 0, 0, 1), (5, 3, 0, 1, 2), (24, 24, 2, 8, 10), (37, 20, 3, 6, 9), (64, 50, 0, 10, 13),
+            (130, 100, 1, 40, 45), (260, 90, 4, 30, 34), (40, 30, 2, 12, 12)]
+
+
+@pytest.mark.parametrize("nbf,nbf5,no1,nbeta,nalpha", RO_CASES)
+def test_fock_ro_vs_oracle(eng, nbf, nbf5, no1, nbeta, nalpha):
+    print(f"case: fock_ro nbf={nbf} nbf5={nbf5} no1={no1} nbeta={nbeta} nalpha={nalpha}", flush=True)
+    J, K, H, n, cj, ck = fock_inputs(nbf, nbf5, 800 + nbf)
+    n[nbeta:nalpha] = 0.5
+    F_ref = O.fock_ro(J, K, n, H, cj, ck, nbeta, nalpha, no1)
+    cj0, ck0 = cj.copy(), ck.copy()
+    F = eng.fock_ro(J, K, n, H, cj, ck, no1, nbeta, nalpha)
+    assert np.array_equal(cj, cj0) and np.array_equal(ck, ck0)
+    assert F.shape == (nbf5, nbf, nbf) and rel(F, F_ref) <= FP64_TOL
+    if nbeta == nalpha:  # no singly occupied orbitals: the closed-shell matrices
+        assert rel(F, eng.fock_rc(J, K, n, H, cj, ck, no1, nbeta, nalpha)) <= 1e-14
+    C = synth.make_C(nbf, 810 + nbf)
+    el = eng.lagrange_fock(F, C, nbf)
+    assert rel(el, O.lagrange_fock(F_ref, C, nbf)) <= FP64_TOL
+
+
+def test_fock_ro_device_io(eng):
+    import torch
+
+    nbf, nbf5, no1, nbeta, nalpha = 48, 40, 2, 12, 15
+    print(f"case: fock_ro device io nbf={nbf} nbf5={nbf5}", flush=True)
+    J, K, H, n, cj, ck = fock_inputs(nbf, nbf5, 820)
+    F_ref = O.fock_ro(J, K, n, H, cj, ck, nbeta, nalpha, no1)
+    dev = [torch.from_numpy(x).cuda() for x in (J, K, n, H, cj, ck)]
+    F = eng.fock_ro(*dev, no1, nbeta, nalpha)
+    torch.cuda.synchronize()
+    assert F.is_cuda and rel(F.cpu().numpy(), F_ref) <= FP64_TOL
+    F2 = eng.fock_ro(J, K, n, H, cj, ck, no1, nbeta, nalpha)  # host in, host out
+    assert np.array_equal(F2, F.cpu().numpy())
 
 
 def test_fock_device_io_staging_and_determinism(eng):
@@ -136,6 +174,21 @@ def test_dropin_fock_and_lagrange(gpu_required):
     release_cached_engines()
 
 
+def test_dropin_fock_ro(gpu_required):
+    nbf, nbf5, no1 = 30, 22, 3
+    print("case: computeF_RO_driver drop-in", flush=True)
+    J, K, H, n, cj, ck = fock_inputs(nbf, nbf5, 830)
+    p = param(nbf=nbf, nbf5=nbf5, ndoc=7, nsoc=2, RI=True, no1=no1, noptorb=nbf - 2)
+    F_ref = O.fock_ro(J, K, n, H, cj, ck, p.nbeta, p.nalpha, no1)
+    cj_in, ck_in = cj.copy(), ck.copy()
+    F = computeF_RO_driver(J, K, n, H, cj_in, ck_in, p)
+    assert isinstance(F, np.ndarray) and rel(F, F_ref) <= FP64_TOL
+    # the reference zeroes the caller's [ini:, ini:] diagonals in place (utils.py:121, :138)
+    assert np.all(np.diag(cj_in)[no1:] == 0.0) and np.array_equal(np.diag(cj_in)[:no1], np.diag(cj)[:no1])
+    assert np.all(np.diag(ck_in)[no1:] == 0.0) and np.array_equal(np.diag(ck_in)[:no1], np.diag(ck)[:no1])
+    release_cached_engines()
+
+
 def test_fock_errors(eng):
     nbf, nbf5 = 12, 8
     J, K, H, n, cj, ck = fock_inputs(nbf, nbf5, 795)
@@ -161,3 +214,14 @@ def test_fock_errors(eng):
                                 None) == _lib.EINVAL  # noptorb > nbf
     with pytest.raises(ValueError):
         eng.fock_rc(J, K[:-1], n, H, cj, ck, 0, 3, 3)
+    ro = lambda nbeta=3, nalpha=5, Fp=F.ctypes.data: L.pnjk_fock_ro(  # noqa: E731
+        eng._h, pj, pk, nbf, nbf5, pn, ph, nbf, pcj, pck, 0, nbeta, nalpha, Fp, None)
+    assert ro() == _lib.OK
+    assert ro(nalpha=nbf5 + 1) == _lib.EINVAL
+    assert ro(nbeta=6) == _lib.EINVAL
+    assert ro(Fp=None) == _lib.EINVAL
+    assert L.pnjk_fock_ro(None, pj, pk, nbf, nbf5, pn, ph, nbf, pcj, pck, 0, 3, 5, F.ctypes.data,
+                          None) == _lib.EINVAL
+    assert "pnjk_fock_ro" in L.pnjk_last_error().decode()
+    with pytest.raises(ValueError):
+        eng.fock_ro(J, K, n[:-1], H, cj, ck, 0, 3, 3)

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from oracle import jk_oracle as O
+from oracle import pnof_oracle as P
 from pynof_amd import synth
 
 
@@ -328,6 +329,106 @@ def test_fock_planted_mistake_fails():
     # C^K applied to J instead of K
     wrong = F + np.einsum("ij,jmn->imn", ck - np.diag(np.diag(ck)), K - J)
     assert rel(O.lagrange_fock(wrong, C, 6), el2) > 1e-3
+
+
+# ---- restricted open-shell Fock (computeF_RO_CPU, utils.py:96-171)
+
+def _ro_energy(C, H, b, n, cj, ck, p):
+    """The open-shell energy as a function of C: calce (pnof.py:924-988) with its singly
+    occupied block replaced by the textbook ROHF one. For parallel-spin singles of n = 1/2,
+    every pair contributes J_is - K_is once: sum_{i < s in S} (J_is - K_is). pnof.py:976-988
+    instead carries half of that pair sum and a self-exchange -n_i K_ii; the Fock route of
+    utils.py:155-169 follows the textbook block."""
+    JM, KM, Hc = O.jkh_mo_ri(C, H, b, p.nbf5)
+    e = P.calce(n, cj, ck, JM, KM, Hc, p)
+    S = slice(p.nbeta, p.nalpha)
+    ns, js, ks = n[S], JM[S, S], KM[S, S]
+    ref_block = 0.5 * (ns @ js.sum(axis=0) - ns @ np.diagonal(js)) - 0.5 * (ns @ ks.sum(axis=0) + ns @ np.diagonal(ks))
+    m = js - ks
+    np.fill_diagonal(m, 0.0)
+    return e - ref_block + 0.5 * m.sum()
+
+
+def _ro_check(F, C, H, b, n, cj, ck, p, eps=1e-5):
+    """max |dE/dC - 4 F_i C_i| / max |dE/dC| over the first nbf5 columns (central differences)."""
+    nbf, n5 = C.shape[0], p.nbf5
+    G = np.einsum("imn,ni->mi", F, C[:, :n5])
+    dE = np.zeros((nbf, n5))
+    for m in range(nbf):
+        for i in range(n5):
+            Cp, Cm = C.copy(), C.copy()
+            Cp[m, i] += eps
+            Cm[m, i] -= eps
+            dE[m, i] = (_ro_energy(Cp, H, b, n, cj, ck, p) - _ro_energy(Cm, H, b, n, cj, ck, p)) / (2 * eps)
+    return np.max(np.abs(dE - 4.0 * G)) / np.max(np.abs(dE))
+
+
+def _ro_inputs(p, seed, pnof_coefficients):
+    b, C, H, _ = tiny(10, 11, p.nbf5, seed=seed)
+    rng = np.random.default_rng(seed + 5)
+    if pnof_coefficients:  # the functional's own coefficients (core orbitals at n = 1)
+        n, _ = P.occupation(rng.standard_normal(p.nv), p)
+        cj, ck = P.coefficients(n, p)
+    else:  # any symmetric coefficients, singles at n = 1/2
+        n = rng.uniform(0.05, 0.95, p.nbf5)
+        n[p.nbeta:p.nalpha] = 0.5
+        cj, ck = rng.standard_normal((2, p.nbf5, p.nbf5))
+        cj, ck = cj + cj.T, ck + ck.T
+    return b, C, H, n, cj, ck
+
+
+RO_LAYOUTS = [  # (no1, ndoc, nsoc, ncwo, pnof coefficients)
+    (0, 2, 2, 1, False), (0, 2, 1, 2, False), (1, 1, 3, 1, False), (0, 3, 2, 1, True),
+    (2, 2, 1, 2, True), (3, 1, 2, 2, True)]
+
+
+@pytest.mark.parametrize("no1,ndoc,nsoc,ncwo,pc", RO_LAYOUTS)
+def test_fock_ro_is_the_energy_derivative(no1, ndoc, nsoc, ncwo, pc):
+    """G = F_i C_i (utils.py:251) is a quarter of the C-gradient of the open-shell energy
+    above: every coefficient of every row (D, S and W) is pinned by central differences."""
+    p = P.Pnof(no1=no1, ndoc=ndoc, nsoc=nsoc, ncwo=ncwo, ipnof=8, MSpin=nsoc, occ_method="Trigonometric")
+    b, C, H, n, cj, ck = _ro_inputs(p, 120 + no1 + 7 * nsoc, pc)
+    J, K = O.jkj_ri(C, b, p.nbf5)
+    F = O.fock_ro(J, K, n, H, cj, ck, p.nbeta, p.nalpha, no1)
+    assert _ro_check(F, C, H, b, n, cj, ck, p) < 1e-7
+
+
+@pytest.mark.parametrize("no1,ndoc,nsoc,ncwo,pc", RO_LAYOUTS)
+def test_fock_ro_loops(no1, ndoc, nsoc, ncwo, pc):
+    """The coefficient of J[j], K[j] per (i, j): this also pins each single's own SUMij term,
+    which removes (J[s] - K[s]) / 2 from F[s] (utils.py:159) and which the gradient cannot
+    see, since (J[s] - K[s]) C_s = 0."""
+    p = P.Pnof(no1=no1, ndoc=ndoc, nsoc=nsoc, ncwo=ncwo, ipnof=8, MSpin=nsoc, occ_method="Trigonometric")
+    b, C, H, n, cj, ck = _ro_inputs(p, 140 + no1, pc)
+    J, K = O.jkj_ri(C, b, p.nbf5)
+    F = O.fock_ro(J, K, n, H, cj, ck, p.nbeta, p.nalpha, no1)
+    assert rel(F, O.fock_ro_loops(J, K, n, H, cj, ck, p.nbeta, p.nalpha, no1)) < 1e-13
+
+
+def test_fock_ro_without_singles_is_fock_rc():
+    b, C, H, n, cj, ck = lag_inputs(seed=93)
+    J, K = O.jkj_ri(C, b, 4)
+    for no1 in (0, 2):
+        assert rel(O.fock_ro(J, K, n, H, cj, ck, 3, 3, no1), O.fock_rc(J, K, n, H, cj, ck, 3, 3, no1)) < 1e-14
+
+
+def test_fock_ro_planted_mistakes_fail():
+    p = P.Pnof(no1=0, ndoc=2, nsoc=2, ncwo=1, ipnof=8, MSpin=2, occ_method="Trigonometric")
+    b, C, H, n, cj, ck = _ro_inputs(p, 130, False)
+    J, K = O.jkj_ri(C, b, p.nbf5)
+    F = O.fock_ro(J, K, n, H, cj, ck, p.nbeta, p.nalpha)
+    S = slice(p.nbeta, p.nalpha)
+    # PRODWROij with K at full weight
+    dw = np.r_[0:p.nbeta, p.nalpha:p.nbf5]
+    w1 = F.copy()
+    w1[S] -= 0.5 * np.einsum("j,jmn->mn", n[dw], K[dw])
+    # SUMij on the doubly occupied rows with K at full weight
+    w2 = F.copy()
+    w2[:p.nbeta] -= 0.5 * np.einsum("i,jmn->imn", n[:p.nbeta], K[S])
+    # the closed-shell form (C^J/C^K over the singles too, no SUMij/PRODWROij)
+    w3 = O.fock_rc(J, K, n, H, cj, ck, p.nbeta, p.nalpha)
+    for w in (w1, w2, w3):
+        assert _ro_check(w, C, H, b, n, cj, ck, p) > 1e-3
 
 
 # ---------------------------------------------------------------- b producer (integrals.py:35-49)
