@@ -634,6 +634,7 @@ def bench_serving(args):
         t0 = time.perf_counter()
         srv.run(timeout=600)
         wall = time.perf_counter() - t0
+        srv.ex.close()
         st = srv.stats()
         st["wall_s"] = round(wall, 3)
         st["requests_per_s"] = round(st["finished"] / wall, 3)

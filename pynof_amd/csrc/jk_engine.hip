@@ -1487,7 +1487,7 @@ struct pnjk_group {
   std::vector<pnjk_engine*> eng;
   int device = 0;
   std::mutex mu;
-  hipStream_t cap = nullptr;            // capture stream
+  hipStream_t cap = nullptr;            // capture stream (made on the first graph-mode build)
   std::vector<BuildState> st;
   std::vector<char> sig;                // signature of the cached plan
   std::vector<char> in_sig;             // C and H pointers the captured graph reads
@@ -1500,13 +1500,25 @@ struct pnjk_group {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool graph_valid = false;
+  hipEvent_t done = nullptr;  // recorded after each build's last launch
   ~pnjk_group() {
+    if (done) (void)hipEventDestroy(done);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (cap) (void)hipStreamDestroy(cap);
     plan_dev.release();
   }
 };
+
+// Plan buffers of destroyed groups, per device, handed to new groups: freeing device memory
+// synchronises with the whole device, and a serving loop destroys a group whenever its set of
+// requests changes. A buffer enters the pool only after its group's last build has run (the
+// event recorded behind it), so no launch still reads it when a new group overwrites it.
+namespace {
+constexpr size_t kPlanPoolMax = 16;
+std::mutex g_plan_mu;
+std::vector<DevBuf> g_plan_pool[64];
+}  // namespace
 
 namespace {
 
@@ -1571,8 +1583,17 @@ int pnjk_group_create(pnjk_engine* const* engines, int count, pnjk_group** out) 
   g->eng.assign(engines, engines + count);
   g->device = engines[0]->device;
   g->plan.resize(GS_COUNT);
-  hipError_t he = hipSetDevice(g->device);
-  if (he == hipSuccess) he = hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking);
+  if (g->device >= 0 && g->device < 64) {
+    std::lock_guard<std::mutex> pl(g_plan_mu);
+    std::vector<DevBuf>& pool = g_plan_pool[g->device];
+    if (!pool.empty()) {
+      g->plan_dev = pool.back();
+      pool.pop_back();
+    }
+  }
+  // The capture stream is made on the first graph-mode build: creating a stream synchronises
+  // with the device, and groups that never capture (a serving loop's) need none.
+  const hipError_t he = hipSetDevice(g->device);
   if (he != hipSuccess) {
     delete g;
     return set_error(PNJK_EHIP, std::string("pnjk_group_create: ") + hipGetErrorString(he));
@@ -1584,7 +1605,15 @@ int pnjk_group_create(pnjk_engine* const* engines, int count, pnjk_group** out) 
 int pnjk_group_destroy(pnjk_group* g) {
   if (!g) return PNJK_OK;
   (void)hipSetDevice(g->device);
-  (void)hipStreamSynchronize(g->cap);
+  if (g->cap) (void)hipStreamSynchronize(g->cap);
+  if (g->plan_dev.p && g->device >= 0 && g->device < 64 && (!g->done || hipEventSynchronize(g->done) == hipSuccess)) {
+    std::lock_guard<std::mutex> pl(g_plan_mu);
+    std::vector<DevBuf>& pool = g_plan_pool[g->device];
+    if (pool.size() < kPlanPoolMax) {
+      pool.push_back(g->plan_dev);
+      g->plan_dev = DevBuf{};
+    }
+  }
   delete g;
   return PNJK_OK;
 }
@@ -1693,7 +1722,9 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
         b.bounds_off = bytes;
         bytes = roundup((int64_t)(bytes + (kXcds + 1) * sizeof(int)), 256);
       }
-    PNJK_HIP_CHECK(hipStreamSynchronize(s));  // the previous plan may still be in use
+    // This group's earlier builds may still read the plan buffer (a pooled buffer's previous
+    // group had finished when it entered the pool); other work on the stream is no concern.
+    if (g->done) PNJK_HIP_CHECK(hipEventSynchronize(g->done));
     if ((rc = g->plan_dev.ensure(std::max<size_t>(bytes, 256)))) return rc;
     char* base = static_cast<char*>(g->plan_dev.p);
     if (!g->sym.empty())
@@ -1720,13 +1751,23 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
     g->in_sig = in_sig;
     g->graph_valid = false;
   }
-  if (!use_graph) return group_enqueue(g, s);
+  // The event behind the build's launches tells pnjk_group_destroy when the plan buffer is free.
+  auto mark = [&]() -> int {
+    if (!g->done) PNJK_HIP_CHECK(hipEventCreateWithFlags(&g->done, hipEventDisableTiming));
+    PNJK_HIP_CHECK(hipEventRecord(g->done, s));
+    return PNJK_OK;
+  };
+  if (!use_graph) {
+    if ((rc = group_enqueue(g, s))) return rc;
+    return mark();
+  }
   if (!g->graph_valid) {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     g->exec = nullptr;
     g->graph = nullptr;
     PNJK_HIP_CHECK(hipStreamSynchronize(s));
+    if (!g->cap) PNJK_HIP_CHECK(hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking));
     PNJK_HIP_CHECK(hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal));
     rc = group_enqueue(g, g->cap);
     hipGraph_t graph = nullptr;
@@ -1741,7 +1782,7 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
     g->graph_valid = true;
   }
   PNJK_HIP_CHECK(hipGraphLaunch(g->exec, s));
-  return PNJK_OK;
+  return mark();
 }
 
 int pnjk_set_profiling(pnjk_engine* e, int on) {

@@ -352,18 +352,19 @@ def poisson_requests(count: int, rate: float, seed: int = 5000, median_nbf: int 
     return out
 
 
-def _expm_small(k):
+def _expm_small(k, eye=None):
     """exp(k) for a matrix of norm ||k||_F <= 1e-2, as its Taylor series to degree 8 in
     Horner form: the first omitted term is below 1e-2^9 / 9! < 1e-23, far under FP64
     rounding. Unlike torch.linalg.matrix_exp, which picks its degree from the norm on the
     host, it never waits for the device, so the loop queues every request's rotation
-    without a synchronisation."""
+    without a synchronisation. Each Horner step is one addmm (eye + (k p) / j)."""
     import torch
 
-    eye = torch.eye(k.shape[0], dtype=k.dtype, device=k.device)
-    p = eye + k / 8.0
+    if eye is None:
+        eye = torch.eye(k.shape[0], dtype=k.dtype, device=k.device)
+    p = torch.add(eye, k, alpha=1.0 / 8.0)
     for j in (7.0, 6.0, 5.0, 4.0, 3.0, 2.0, 1.0):
-        p = eye + (k @ p) / j
+        p = torch.addmm(eye, k, p, alpha=1.0 / j)
     return p
 
 
@@ -387,12 +388,23 @@ class DeviceExecutor(Executor):
     run. The per-request state table is guarded by a lock.
 
     ``keep_final_C`` keeps each request's final C and the outputs named in ``keep_outs``
-    (default: all) on the host at release."""
+    (default: all) on the host at release.
+
+    Engines are pooled: a released request's engine keeps its device buffers (they are
+    grow-only) and serves a later request, so admission creates no streams and allocates
+    nothing when an idle engine fits, and release frees nothing (device frees and stream
+    creation synchronise with the whole GPU, so they stalled every lane's launches). An
+    idle engine is reused for a request whose engine bytes it holds with at most 20% to
+    spare, or, failing that, the largest idle engine smaller than the request grows to it;
+    at most ``pool_engines`` engines and ``pool_bytes`` of device memory stay idle (beyond
+    that a released engine is closed). So engine memory beyond the admitted requests' own
+    footprints is at most 20% of the budget plus ``pool_bytes``. ``close()`` frees the pool."""
 
     concurrent_admission = True
 
     def __init__(self, device: int = 0, with_jk: bool = True, keep_final_C: bool = False,
-                 occupation: bool = False, occ_lr: float = 0.05, keep_outs=None):
+                 occupation: bool = False, occ_lr: float = 0.05, keep_outs=None, pool_engines: int = 32,
+                 pool_bytes: float = 32e9):
         import torch
 
         self.torch = torch
@@ -409,6 +421,47 @@ class DeviceExecutor(Executor):
         self._state_lock = threading.Lock()
         self._admit_stream = None
         self._groups: dict = {}
+        self._idle: list = []  # (device bytes, engine) of pooled engines
+        self._idle_lock = threading.Lock()
+        self.pool_engines = int(pool_engines)
+        self.pool_bytes = float(pool_bytes)
+
+    def _engine_bytes(self, req: Request) -> int:
+        """Device bytes a request's engine holds: packed b and the split_refine workspace
+        with the split planes."""
+        from . import _lib
+
+        L = _lib.lib()
+        b_bytes = req.nbf * (req.nbf + 1) // 2 * L.pnjk_packed_ld(req.nbf, req.naux) * 8
+        return int(b_bytes + L.pnjk_workspace_bytes(req.nbf, req.naux, req.nbf5, 2))
+
+    def _take_engine(self, req: Request):
+        from .engine import JKEngine
+
+        need = self._engine_bytes(req)
+        with self._idle_lock:
+            fits = [x for x in self._idle if need <= x[0] <= 1.2 * need]
+            grows = [x for x in self._idle if x[0] < need]
+            pick = min(fits, key=lambda x: x[0]) if fits else (max(grows, key=lambda x: x[0]) if grows else None)
+            if pick is not None:
+                self._idle.remove(pick)
+                return pick[1]
+        return JKEngine(self.device)
+
+    def _put_engine(self, eng) -> None:
+        fp = eng.footprint
+        with self._idle_lock:
+            if len(self._idle) < self.pool_engines and sum(x[0] for x in self._idle) + fp <= self.pool_bytes:
+                self._idle.append((fp, eng))
+                return
+        eng.close()
+
+    def close(self) -> None:
+        """Closes the pooled engines (their device memory is freed)."""
+        with self._idle_lock:
+            idle, self._idle = self._idle, []
+        for _, e in idle:
+            e.close()
 
     def footprint(self, req: Request) -> int:
         from . import _lib
@@ -440,7 +493,7 @@ class DeviceExecutor(Executor):
         from .synth import device_b_packed, make_H, make_n
 
         torch = self.torch
-        eng = JKEngine(self.device)
+        eng = self._take_engine(req)
         btri = device_b_packed(req.nbf, req.naux, req.seed, 0.05, JKEngine.packed_ld(req.nbf, req.naux), self.dev)
         eng.upload_b_packed(btri, req.nbf, req.naux)
         del btri
@@ -520,6 +573,10 @@ class DeviceExecutor(Executor):
         outputs = ("J", "K", "J_MO", "K_MO", "H_core") if self.with_jk else ("J_MO", "K_MO", "H_core")
         for form, rs in by_form.items():
             st = [self.state[r.rid] for r in rs]
+            if len(rs) == 1:  # one request (every final refinement, most steps at low load): no group plan
+                st[0]["eng"].build(st[0]["C"], rs[0].nbf5, H=st[0]["H"], precision=form, outputs=outputs,
+                                   out=st[0]["outs"])
+                continue
             self._group(rs).build([s["C"] for s in st], [r.nbf5 for r in rs], [s["H"] for s in st],
                                   precision=form, outputs=outputs, outs=[s["outs"] for s in st])
         for r in reqs:  # energies (FP64, on the device), then the orbital rotation for the next build
@@ -540,7 +597,9 @@ class DeviceExecutor(Executor):
                 a = torch.randn((r.nbf, r.nbf), dtype=torch.float64, device=self.dev, generator=s["gen"])
                 k = a - a.T
                 k *= 1e-2 / torch.linalg.norm(k)
-                s["C"] = s["C"] @ _expm_small(k)
+                if "eye" not in s:
+                    s["eye"] = torch.eye(r.nbf, dtype=torch.float64, device=self.dev)
+                s["C"] = s["C"] @ _expm_small(k, s["eye"])
 
     def release(self, req: Request) -> None:
         with self._state_lock:
@@ -556,6 +615,4 @@ class DeviceExecutor(Executor):
         for k in list(self._groups):
             if req.rid in k:
                 self._groups.pop(k).close()
-        # In line: freeing the engine on a background thread measured no faster (its frees
-        # slowed the loop's launches and group set-up by as much as they saved).
-        s["eng"].close()
+        self._put_engine(s["eng"])

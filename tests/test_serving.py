@@ -357,3 +357,62 @@ def test_lanes_timeout_waits_for_inflight_steps():
         srv.run(t0=0.0, timeout=0.5)
     assert srv.in_use == 0 and not srv.active and [r.rid for r in srv.rejected] == [0]
     assert ex.live == set()
+
+
+class _FakeEngine:
+    """A stand-in for JKEngine in the pool policy test: its device bytes and whether it
+    was closed."""
+
+    def __init__(self, nbytes):
+        self.nbytes = nbytes
+        self.closed = False
+
+    @property
+    def footprint(self):
+        return self.nbytes
+
+    def close(self):
+        self.closed = True
+
+
+def test_engine_pool_policy(monkeypatch):
+    """DeviceExecutor's engine pool (no device needed): an idle engine serves a request whose
+    engine bytes it holds with at most 20% to spare, else the largest smaller one grows, else
+    a new engine is made; released engines stay idle within the count and byte caps and are
+    closed beyond them; close() empties the pool."""
+    import threading
+
+    from pynof_amd import serving as S
+
+    ex = object.__new__(S.DeviceExecutor)
+    ex._idle, ex._idle_lock, ex.pool_engines, ex.pool_bytes, ex.device = [], threading.Lock(), 2, 300.0, 0
+    made = []
+
+    class NewEngine(_FakeEngine):
+        def __init__(self, device):
+            super().__init__(0)
+            made.append(self)
+
+    monkeypatch.setattr("pynof_amd.engine.JKEngine", NewEngine)
+    need = {}
+    monkeypatch.setattr(S.DeviceExecutor, "_engine_bytes", lambda self, r: need[r])
+    a, b = _FakeEngine(100), _FakeEngine(150)
+    ex._put_engine(a)
+    ex._put_engine(b)
+    c = _FakeEngine(60)
+    ex._put_engine(c)  # a third idle engine is over the count cap: closed
+    assert c.closed and [e for _, e in ex._idle] == [a, b]
+    need["r1"] = 90  # 100 holds 90 with 11% to spare
+    assert ex._take_engine("r1") is a
+    need["r2"] = 100  # 150 is 50% over 100: the largest smaller engine grows (none), so a new one
+    e2 = ex._take_engine("r2")
+    assert e2 is made[-1] and [e for _, e in ex._idle] == [b]
+    need["r3"] = 200  # nothing holds 200: the largest smaller engine (150) grows to it
+    assert ex._take_engine("r3") is b and ex._idle == []
+    ex._put_engine(_FakeEngine(200))
+    big = _FakeEngine(120)
+    ex._put_engine(big)  # 200 + 120 bytes exceed the 300-byte cap: closed
+    assert big.closed and len(ex._idle) == 1
+    idle = ex._idle[0][1]
+    ex.close()
+    assert idle.closed and ex._idle == []
