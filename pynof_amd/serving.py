@@ -103,6 +103,11 @@ class Executor:
     def admit(self, req: Request) -> None:
         raise NotImplementedError
 
+    def admitted_footprint(self, req: Request) -> int:
+        """Device bytes the admitted request holds (at least its estimated footprint): an
+        executor that reuses buffers may hold more than the estimate."""
+        return req.footprint
+
     def step(self, reqs: list[Request]) -> None:
         """Runs build number req.builds_done of every request in `reqs` and appends
         each request's energy to req.energies."""
@@ -199,6 +204,10 @@ class Server:
                 self._activate(r)
 
     def _activate(self, r: Request) -> None:
+        held = int(self.ex.admitted_footprint(r))  # the books take what admission really holds
+        if held > r.footprint:
+            self.in_use += held - r.footprint
+            r.footprint = held
         r.admitted = self.clock()
         self.active.append(r)
 
@@ -394,16 +403,19 @@ class DeviceExecutor(Executor):
     grow-only) and serves a later request, so admission creates no streams and allocates
     nothing when an idle engine fits, and release frees nothing (device frees and stream
     creation synchronise with the whole GPU, so they stalled every lane's launches). An
-    idle engine is reused for a request whose engine bytes it holds with at most 20% to
-    spare, or, failing that, the largest idle engine smaller than the request grows to it;
-    at most ``pool_engines`` engines and ``pool_bytes`` of device memory stay idle (beyond
-    that a released engine is closed). So engine memory beyond the admitted requests' own
-    footprints is at most 20% of the budget plus ``pool_bytes``. ``close()`` frees the pool."""
+    idle engine is reused for a request whose engine bytes it holds with at most 50% to
+    spare (the smallest such); otherwise the request gets a new engine (an idle engine is
+    never grown: growing frees its buffers). A reused engine's spare bytes are charged to the
+    request (``admitted_footprint``), so the server's budget covers every active engine's real
+    size. At most ``pool_engines`` engines (default: the server's default active count) and
+    ``pool_bytes`` of device memory stay idle (beyond that a released engine is closed), so
+    the device holds at most the budget plus ``pool_bytes`` (plus one admission's spare,
+    charged as it completes). ``close()`` frees the pool."""
 
     concurrent_admission = True
 
     def __init__(self, device: int = 0, with_jk: bool = True, keep_final_C: bool = False,
-                 occupation: bool = False, occ_lr: float = 0.05, keep_outs=None, pool_engines: int = 32,
+                 occupation: bool = False, occ_lr: float = 0.05, keep_outs=None, pool_engines: int = 64,
                  pool_bytes: float = 32e9):
         import torch
 
@@ -425,6 +437,7 @@ class DeviceExecutor(Executor):
         self._idle_lock = threading.Lock()
         self.pool_engines = int(pool_engines)
         self.pool_bytes = float(pool_bytes)
+        self.pool_spare = 1.5
 
     def _engine_bytes(self, req: Request) -> int:
         """Device bytes a request's engine holds: packed b and the split_refine workspace
@@ -440,10 +453,9 @@ class DeviceExecutor(Executor):
 
         need = self._engine_bytes(req)
         with self._idle_lock:
-            fits = [x for x in self._idle if need <= x[0] <= 1.2 * need]
-            grows = [x for x in self._idle if x[0] < need]
-            pick = min(fits, key=lambda x: x[0]) if fits else (max(grows, key=lambda x: x[0]) if grows else None)
-            if pick is not None:
+            fits = [x for x in self._idle if need <= x[0] <= self.pool_spare * need]
+            if fits:
+                pick = min(fits, key=lambda x: x[0])
                 self._idle.remove(pick)
                 return pick[1]
         return JKEngine(self.device)
@@ -487,6 +499,13 @@ class DeviceExecutor(Executor):
                 t.record_stream(used_on)
         with self._state_lock:
             self.state[req.rid] = st
+
+    def admitted_footprint(self, req: Request) -> int:
+        """The estimate plus what the request's engine holds beyond its own engine bytes (a
+        pooled engine may be up to half as large again)."""
+        with self._state_lock:
+            eng = self.state[req.rid]["eng"]
+        return req.footprint + max(0, eng.footprint - self._engine_bytes(req))
 
     def _admit_on_stream(self, req: Request) -> dict:
         from .engine import JKEngine

@@ -377,15 +377,16 @@ class _FakeEngine:
 
 def test_engine_pool_policy(monkeypatch):
     """DeviceExecutor's engine pool (no device needed): an idle engine serves a request whose
-    engine bytes it holds with at most 20% to spare, else the largest smaller one grows, else
-    a new engine is made; released engines stay idle within the count and byte caps and are
-    closed beyond them; close() empties the pool."""
+    engine bytes it holds with at most 50% to spare (the smallest such), otherwise a new
+    engine is made (an idle engine is never grown); released engines stay idle within the
+    count and byte caps and are closed beyond them; close() empties the pool."""
     import threading
 
     from pynof_amd import serving as S
 
     ex = object.__new__(S.DeviceExecutor)
-    ex._idle, ex._idle_lock, ex.pool_engines, ex.pool_bytes, ex.device = [], threading.Lock(), 2, 300.0, 0
+    ex._idle, ex._idle_lock, ex.device = [], threading.Lock(), 0
+    ex.pool_engines, ex.pool_bytes, ex.pool_spare = 2, 300.0, 1.5
     made = []
 
     class NewEngine(_FakeEngine):
@@ -396,23 +397,47 @@ def test_engine_pool_policy(monkeypatch):
     monkeypatch.setattr("pynof_amd.engine.JKEngine", NewEngine)
     need = {}
     monkeypatch.setattr(S.DeviceExecutor, "_engine_bytes", lambda self, r: need[r])
-    a, b = _FakeEngine(100), _FakeEngine(150)
+    a, b = _FakeEngine(100), _FakeEngine(140)
     ex._put_engine(a)
     ex._put_engine(b)
     c = _FakeEngine(60)
     ex._put_engine(c)  # a third idle engine is over the count cap: closed
     assert c.closed and [e for _, e in ex._idle] == [a, b]
-    need["r1"] = 90  # 100 holds 90 with 11% to spare
+    need["r1"] = 90  # both hold 90 within 50%: the smaller one
     assert ex._take_engine("r1") is a
-    need["r2"] = 100  # 150 is 50% over 100: the largest smaller engine grows (none), so a new one
-    e2 = ex._take_engine("r2")
-    assert e2 is made[-1] and [e for _, e in ex._idle] == [b]
-    need["r3"] = 200  # nothing holds 200: the largest smaller engine (150) grows to it
-    assert ex._take_engine("r3") is b and ex._idle == []
-    ex._put_engine(_FakeEngine(200))
-    big = _FakeEngine(120)
-    ex._put_engine(big)  # 200 + 120 bytes exceed the 300-byte cap: closed
-    assert big.closed and len(ex._idle) == 1
-    idle = ex._idle[0][1]
+    need["r2"] = 150  # nothing holds 150: a new engine (140 is not grown)
+    assert ex._take_engine("r2") is made[-1] and [e for _, e in ex._idle] == [b]
+    need["r3"] = 90  # 140 is more than 50% over 90: a new engine
+    assert ex._take_engine("r3") is made[-1] and len(made) == 2
+    ex._put_engine(_FakeEngine(150))  # 140 + 150 bytes: within both caps
+    third = _FakeEngine(5)
+    ex._put_engine(third)  # a third idle engine: over the count cap
+    ex.pool_engines = 3
+    heavy = _FakeEngine(20)
+    ex._put_engine(heavy)  # 140 + 150 + 20 bytes exceed the byte cap
+    assert third.closed and heavy.closed and len(ex._idle) == 2
+    idle = [e for _, e in ex._idle]
     ex.close()
-    assert idle.closed and ex._idle == []
+    assert all(e.closed for e in idle) and ex._idle == []
+
+
+def test_admitted_footprint_charges_the_budget():
+    """An executor whose admission holds more than the estimate (a reused, larger engine)
+    has the difference charged to the budget when the request activates, and released."""
+
+    class Holder(FakeExecutor):
+        def admitted_footprint(self, req):
+            return req.footprint + 30
+
+    clk = Clock()
+    ex = Holder(clk)
+    srv = Server(ex, hbm_budget=100e6, clock=clk, sleep=clk.sleep)
+    reqs = [make(i, 10, n=2) for i in range(3)]
+    for r in reqs:
+        srv.submit(r)
+    est = reqs[0].footprint
+    srv._admit(clk())
+    assert len(srv.active) == 3 and all(r.footprint == est + 30 for r in srv.active)
+    assert srv.in_use == 3 * (est + 30)
+    srv.run(timeout=60)
+    assert len(srv.finished) == 3 and srv.in_use == 0
