@@ -44,6 +44,33 @@ def test_mix_equals_alone(gpu_required):
             assert np.array_equal(a.final_outs[k], m.final_outs[k]), (r0.rid, k)
 
 
+def test_pooled_engine_reuse_equals_fresh(gpu_required):
+    """A request served on a pooled engine (left by a larger request, whose b, split planes
+    and refine workspace it held) gives the bits of the same request on a fresh engine."""
+    print("case: serving pooled engine reuse vs fresh", flush=True)
+    big = Request(rid=0, nbf=104, naux=synth.ceil8(3.6 * 104), nbf5=104, ndoc=synth.default_ndoc(104), tol=1e-9,
+                  n_builds=3, seed=8100)
+    small = Request(rid=1, nbf=96, naux=synth.ceil8(3.6 * 96), nbf5=96, ndoc=synth.default_ndoc(96), tol=1e-9,
+                    n_builds=3, seed=8200)
+    ex = DeviceExecutor(0, keep_final_C=True)
+    srv = Server(ex, hbm_budget=200e9)
+    srv.submit(big)
+    srv.run(timeout=300)
+    assert len(ex._idle) == 1  # the larger request's engine is idle in the pool
+    pooled = ex._idle[0][1]
+    srv = Server(ex, hbm_budget=200e9)
+    srv.submit(small)
+    srv.run(timeout=300)
+    assert ex._idle[0][1] is pooled  # the small request ran on it and returned it
+    fresh, _ = serve([copy.deepcopy(Request(rid=1, nbf=96, naux=synth.ceil8(3.6 * 96), nbf5=96,
+                                            ndoc=synth.default_ndoc(96), tol=1e-9, n_builds=3, seed=8200))])
+    f = fresh[1]
+    assert small.energies == f.energies
+    for k in ("J_MO", "K_MO", "H_core", "J", "K"):
+        assert np.array_equal(small.final_outs[k], f.final_outs[k]), k
+    ex.close()
+
+
 def test_tight_request_final_energy_is_fp64(gpu_required):
     import torch
 
