@@ -452,13 +452,19 @@ class DeviceExecutor(Executor):
         from .engine import JKEngine
 
         need = self._engine_bytes(req)
+        pick = None
         with self._idle_lock:
             fits = [x for x in self._idle if need <= x[0] <= self.pool_spare * need]
             if fits:
                 pick = min(fits, key=lambda x: x[0])
                 self._idle.remove(pick)
-                return pick[1]
-        return JKEngine(self.device)
+        if pick is None:
+            return JKEngine(self.device)
+        from . import autotune
+
+        for arith in ("fp64", "split"):  # the last request's tile shapes do not carry over
+            autotune.apply(pick[1], arith, None)
+        return pick[1]
 
     def _put_engine(self, eng) -> None:
         fp = eng.footprint

@@ -395,6 +395,8 @@ def test_engine_pool_policy(monkeypatch):
             made.append(self)
 
     monkeypatch.setattr("pynof_amd.engine.JKEngine", NewEngine)
+    reset = []  # a reused engine gets the built-in tile shapes back
+    monkeypatch.setattr("pynof_amd.autotune.apply", lambda eng, arith, entry: reset.append((eng, arith, entry)))
     need = {}
     monkeypatch.setattr(S.DeviceExecutor, "_engine_bytes", lambda self, r: need[r])
     a, b = _FakeEngine(100), _FakeEngine(140)
@@ -404,7 +406,7 @@ def test_engine_pool_policy(monkeypatch):
     ex._put_engine(c)  # a third idle engine is over the count cap: closed
     assert c.closed and [e for _, e in ex._idle] == [a, b]
     need["r1"] = 90  # both hold 90 within 50%: the smaller one
-    assert ex._take_engine("r1") is a
+    assert ex._take_engine("r1") is a and reset == [(a, "fp64", None), (a, "split", None)]
     need["r2"] = 150  # nothing holds 150: a new engine (140 is not grown)
     assert ex._take_engine("r2") is made[-1] and [e for _, e in ex._idle] == [b]
     need["r3"] = 90  # 140 is more than 50% over 90: a new engine
