@@ -188,6 +188,8 @@ struct pnjk_engine {
   int64_t ntri = 0;
   DevBuf btri;
   DevBuf bt16h, bt16l, bht16h, bht16l;  // split planes of b, built on the first split build
+  DevBuf bdh;                            // b's diagonal rows halved, then one zero row (with the planes)
+  DevBuf Wmo;                            // split_refine: W = C^T L C (nbf5^2 rows of naux_p)
   DevBuf Cp, Hp, X, d, kpart, T, outs, stage, scal;
   DevBuf Cw;                                     // window copies of C for the FP64 quarter transform
   DevBuf ct16h, ct16l, x16h, x16l, d16h, d16l;  // split workspace
@@ -201,12 +203,13 @@ struct pnjk_engine {
   // Tile override per arithmetic (0: FP64 kernels, 1: BF16x3 kernels) and stage;
   // {0, 0} selects the built-in choice.
   TileF64 tiles[2][ST_COUNT] = {};
-  DevBuf* all[42];
+  DevBuf* all[44];
   int nall = 0;
   pnjk_engine() {
     for (DevBuf* b : {&btri, &bt16h, &bt16l, &bht16h, &bht16l, &Cp, &Hp, &X, &d, &kpart, &T, &outs, &stage, &scal,
                       &ct16h, &ct16l, &x16h, &x16l, &d16h, &d16l, &Cf, &ctf16h, &ctf16l, &hc,
-                      &bmo, &hm, &lagw, &lagg, &lagp, &lagc, &bm16h, &bm16l, &w16h, &w16l, &fjk, &fh, &fco, &ff, &occ, &Cw})
+                      &bmo, &hm, &lagw, &lagg, &lagp, &lagc, &bm16h, &bm16l, &w16h, &w16l, &fjk, &fh, &fco, &ff, &occ, &Cw,
+                      &bdh, &Wmo})
       all[nall++] = b;
   }
 };
@@ -282,9 +285,10 @@ uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int preci
   s += (uint64_t)(nbf * nbf_p) * 8;                          // Hp
   s += (uint64_t)(nbf5 * nbf * naux_p + 64) * 8;             // X (+ slack)
   s += (uint64_t)(roundup(nbf5, 160) * naux_p) * 8;          // d
-  s += (uint64_t)(nbf5 * nbf5 * (naux_p / 64)) * 8;          // kpart
+  if (precision != PNJK_SPLIT_REFINE) s += (uint64_t)(nbf5 * nbf5 * (naux_p / 64)) * 8;  // kpart
   s += (uint64_t)(nbf * ldcp) * 8;                           // T
-  if (precision != PNJK_SPLIT) s += (uint64_t)(2 * nbf_p * qt_window_cols((int)nbf5)) * 8;  // Cw
+  if (precision == PNJK_FP64) s += (uint64_t)(2 * nbf_p * qt_window_cols((int)nbf5)) * 8;  // Cw
+  if (precision == PNJK_SPLIT_REFINE) s += (uint64_t)(nbf5 * nbf5 * naux_p) * 8;          // W
   if (precision != PNJK_FP64) {
     const int64_t ntri = nbf * (nbf + 1) / 2;
     s += (uint64_t)(roundup(ntri, 16) * naux_p) * 4;         // btri16 hi/lo (whole 16-row blocks)
@@ -292,11 +296,13 @@ uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int preci
     s += (uint64_t)(roundup(nbf5, 16) * nbf_p) * 4;          // Ct16 hi/lo (whole 16-row blocks)
     s += (uint64_t)(nbf5 * x16_stride((int)nbf, (int)naux_p) + kX16Slack) * 4;  // X16 hi/lo (+ slack)
     s += (uint64_t)(roundup(nbf5, 16) * naux_p) * 4;         // d16 hi/lo (whole 16-row blocks)
+    s += (uint64_t)((nbf + 1) * naux_p) * 8;                 // halved diagonal rows of b and a zero row
   }
   return s;
 }
 
-// Builds the bf16 planes of b the split kernels read (once per uploaded b).
+// Builds the bf16 planes of b the split kernels read, and the halved diagonal rows the
+// refinement's triangular half-transform reads (once per uploaded b).
 static int ensure_split_planes(pnjk_engine* e, hipStream_t s) {
   if (e->split_ready) return PNJK_OK;
   const int64_t ntri = e->ntri, naux_p = e->naux_p, nbf = e->nbf, ldk = roundup(nbf, 32);
@@ -305,8 +311,12 @@ static int ensure_split_planes(pnjk_engine* e, hipStream_t s) {
   const size_t tri_bytes = (size_t)(roundup(ntri, 16) * naux_p) * 2, ht_bytes = (size_t)(roundup(nbf, 16) * naux_p * ldk) * 2;
   if ((rc = e->bt16h.ensure(tri_bytes)) || (rc = e->bt16l.ensure(tri_bytes))) return rc;
   if ((rc = e->bht16h.ensure(ht_bytes)) || (rc = e->bht16l.ensure(ht_bytes))) return rc;
+  if ((rc = e->bdh.ensure((size_t)((nbf + 1) * naux_p) * 8))) return rc;
   PNJK_HIP_CHECK(launch_split_blocked(e->btri.as<double>(), ntri, naux_p, e->bt16h.as<uint16_t>(),
                                       e->bt16l.as<uint16_t>(), s));
+  // split_refine's triangular half-transform reads b's diagonal halved and a zero row
+  PNJK_HIP_CHECK(launch_half_diag(e->btri.as<double>(), (int)nbf, (int)naux_p, e->bdh.as<double>(),
+                                  e->bdh.as<double>() + nbf * naux_p, s));
   PNJK_HIP_CHECK(launch_build_bht(e->btri.as<double>(), (int)nbf, (int)naux_p, (int)ldk, e->bht16h.as<uint16_t>(),
                                   e->bht16l.as<uint16_t>(), s));
   e->split_ready = true;
@@ -520,6 +530,7 @@ struct BuildState {
   BuildGeom g;
   SplitGeom sg;
   TileF64 ht, qt, jt, kt;
+  TileF64 ht16;  // split_refine: the BF16x3 half-transform (X16 for K^n) beside the FP64 MO path
   bool want_j, want_k, want_jmo, want_kmo, want_h;
   bool hJ, hK, hJMO, hKMO, hH;  // output goes to host memory (staged)
   bool c_dev = false, h_dev = false;  // C / H are device memory
@@ -529,11 +540,12 @@ struct BuildState {
   // library's static scratch array (pnjk_jk_build_noalloc).
   double *Cp = nullptr, *Cw = nullptr, *X = nullptr, *d = nullptr, *kpart = nullptr, *Hp = nullptr, *T = nullptr;
   uint16_t *ct16h = nullptr, *ct16l = nullptr, *x16h = nullptr, *x16l = nullptr, *d16h = nullptr, *d16l = nullptr;
+  double* W = nullptr;  // split_refine: C^T L C
 };
 
 // The workspace pieces of one build, in bytes (0: not needed), in a fixed order.
 enum WsPiece { W_CP = 0, W_CW, W_X, W_D, W_KPART, W_HP, W_T, W_CT16H, W_CT16L, W_X16H, W_X16L, W_D16H, W_D16L,
-               W_STAGE, W_COUNT };
+               W_STAGE, W_W, W_COUNT };
 struct WsPlan {
   size_t bytes[W_COUNT] = {};
 };
@@ -621,11 +633,13 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
   w.bytes[W_CP] = (size_t)((int64_t)(g.nbf_p + 8) * g.ldcp + 1024) * 8;
   if (precision != PNJK_SPLIT) {
     // X slack: a 128-column quarter-transform tile reads up to 64 columns past naux_p.
+    // (split_refine keeps its triangular half-transform T here.)
     w.bytes[W_X] = (size_t)(n5 * nbf * g.naux_p + 64) * 8;
-    w.bytes[W_CW] = (size_t)(2 * g.nbf_p * (int64_t)qt_window_cols(g.nbf5)) * 8;
   }
+  if (precision == PNJK_FP64) w.bytes[W_CW] = (size_t)(2 * g.nbf_p * (int64_t)qt_window_cols(g.nbf5)) * 8;
+  if (precision == PNJK_SPLIT_REFINE) w.bytes[W_W] = (size_t)(n5 * n5 * g.naux_p) * 8;
   w.bytes[W_D] = (size_t)(roundup(n5, 160) * g.naux_p) * 8;
-  w.bytes[W_KPART] = (size_t)(n5 * n5 * (g.naux_p / 64)) * 8;
+  if (precision != PNJK_SPLIT_REFINE) w.bytes[W_KPART] = (size_t)(n5 * n5 * (g.naux_p / 64)) * 8;
   if (st.want_h) {
     w.bytes[W_HP] = (size_t)(nbf * g.nbf_p) * 8;
     w.bytes[W_T] = (size_t)(nbf * g.ldcp) * 8;
@@ -646,7 +660,7 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
     ws_carve(w.bytes, W_COUNT, scratch, ptr);
   } else {
     DevBuf* bufs[W_COUNT] = {&e->Cp, &e->Cw, &e->X, &e->d, &e->kpart, &e->Hp, &e->T, &e->ct16h, &e->ct16l,
-                             &e->x16h, &e->x16l, &e->d16h, &e->d16l, &e->outs};
+                             &e->x16h, &e->x16l, &e->d16h, &e->d16l, &e->outs, &e->Wmo};
     for (int i = 0; i < W_COUNT; ++i) {
       if (!w.bytes[i]) continue;
       if ((rc = bufs[i]->ensure(w.bytes[i]))) return rc;
@@ -666,6 +680,7 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
   st.x16l = static_cast<uint16_t*>(ptr[W_X16L]);
   st.d16h = static_cast<uint16_t*>(ptr[W_D16H]);
   st.d16l = static_cast<uint16_t*>(ptr[W_D16L]);
+  st.W = static_cast<double*>(ptr[W_W]);
   double* ob = static_cast<double*>(ptr[W_STAGE]);
   if (st.hJ) st.dJ = ob + off_J;
   if (st.hK) st.dK = ob + off_K;
@@ -682,6 +697,7 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
   st.qt = chosen(a_mo, ST_QT, 1);
   st.jt = chosen(a_jk, ST_JN, 2);
   st.kt = chosen(a_jk, ST_K, 3);
+  st.ht16 = chosen(1, ST_HT, 0);
   return PNJK_OK;
 }
 
@@ -705,7 +721,7 @@ int stage_in(pnjk_engine* e, BuildState& st, hipStream_t s) {
     PNJK_HIP_CHECK(hipMemcpy2DAsync(Cp, (size_t)g.ldcp * 8, st.a.C, (size_t)st.a.ldc * 8, (size_t)n5 * 8,
                                     (size_t)nbf, hipMemcpyHostToDevice, s));
   }
-  if (st.a.precision == PNJK_SPLIT) {
+  if (st.a.precision != PNJK_FP64) {  // split: C^T planes; split_refine: for its split half-transform
     PNJK_NEED(st.ct16h);
     PNJK_NEED(st.ct16l);
     PNJK_HIP_CHECK(launch_split_ct(Cp, g.ldcp, g.nbf, g.nbf5, g.nbf5, st.sg.ldk, st.ct16h, st.ct16l, s));
@@ -765,33 +781,6 @@ int stage_jmo(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col
   PNJK_NEED(st.dJMO);
   PNJK_HIP_CHECK(launch_syrk_f64(1, st.g.nbf5, st.g.naux_p, st.d, st.g.naux_p, 0, st.dJMO, st.g.nbf5, 0,
                                  TileF64{2, 2}, s, col));
-  return PNJK_OK;
-}
-
-// split_refine: bf16 planes of the exact X, for K^n in split form.
-int stage_convert_x(pnjk_engine* e, BuildState& st, hipStream_t s) {
-  PNJK_NEED(st.X);
-  PNJK_NEED(st.x16h);
-  PNJK_NEED(st.x16l);
-  PNJK_HIP_CHECK(launch_split_x16(st.X, st.sg, st.x16h, st.x16l, s));
-  return PNJK_OK;
-}
-
-// split_refine: J^n, K^n in split form from the exact d and X (X's planes here unless the
-// caller made them already).
-int stage_jmo_post(pnjk_engine* e, BuildState& st, hipStream_t s, bool convert_x = true) {
-  if (st.a.precision == PNJK_SPLIT_REFINE) {
-    if (st.want_j) {
-      PNJK_NEED(st.d);
-      PNJK_NEED(st.d16h);
-      PNJK_NEED(st.d16l);
-      PNJK_HIP_CHECK(launch_split_blocked(st.d, st.g.nbf5, st.g.naux_p, st.d16h, st.d16l, s));
-    }
-    if (st.want_k && convert_x) {
-      int rc = stage_convert_x(e, st, s);
-      if (rc) return rc;
-    }
-  }
   return PNJK_OK;
 }
 
@@ -882,16 +871,11 @@ static int jk_build_locked(pnjk_engine* e, const BuildArgs& a, hipStream_t s, ch
   if ((rc = build_prepare(e, a, st, scratch, scratch_bytes))) return rc;
   mark(ST_IN);
   if ((rc = stage_in(e, st, s))) return rc;
-  mark(ST_HT);
-  if ((rc = stage_ht(e, st, s, nullptr))) return rc;
-  // split_refine: the split K^n needs only X, so its planes and K^n run on the side stream
-  // beside the FP64 quarter transform (which keeps the MFMA pipes 40% busy). The other forms
-  // run J_MO and H_core on the side stream beside J^n (about 0.3 ms hidden per build at
-  // nbf=1000); J^n's mirror stays in line: beside K^n it slowed K^n by more than it saved.
-  // (In split_refine's stage profile, K^n's time therefore shows inside the quarter
-  // transform's.)
   const bool refine = st.a.precision == PNJK_SPLIT_REFINE;
-  const bool k_side = refine && st.want_k, side = !refine;
+  // The forms other than split_refine run J_MO and H_core on the side stream beside J^n
+  // (about 0.3 ms hidden per build at nbf=1000); J^n's mirror stays in line: beside K^n it
+  // slowed K^n by more than it saved.
+  const bool side = !refine, k_side = refine && st.want_k;
   hipStream_t sa = (side || k_side) ? e->aux : s;
   bool forked = false;
   auto fork = [&]() -> int {
@@ -904,28 +888,69 @@ static int jk_build_locked(pnjk_engine* e, const BuildArgs& a, hipStream_t s, ch
   // below, so no work of this build is left running past the call on the side stream.
   auto body = [&]() -> int {
     int r;
-    if (k_side) {
-      if ((r = fork()) || (r = stage_convert_x(e, st, sa)) || (r = stage_k(e, st, sa, nullptr))) return r;
-    }
+    mark(ST_HT);
+    if ((r = stage_ht(e, st, s, nullptr))) return r;
     mark(ST_QT);
     if ((r = stage_qt(e, st, s, nullptr))) return r;
     mark(ST_KMO);
     if ((r = stage_kmo(e, st, s))) return r;
     mark(ST_JMO);
-    if (side && (r = fork())) return r;
-    hipStream_t sj = side ? sa : s;  // J_MO (and in split_refine the planes of d) stream
-    if ((r = stage_jmo(e, st, sj, nullptr)) || (r = stage_jmo_post(e, st, sj, !k_side))) return r;
-    if (side && (r = stage_hcore(e, st, sa))) return r;
+    if ((r = fork())) return r;
+    if ((r = stage_jmo(e, st, sa, nullptr)) || (r = stage_hcore(e, st, sa))) return r;
     mark(ST_JN);
     if ((r = stage_jn(e, st, s, nullptr))) return r;
     if (st.want_j && (r = stage_sym(st, st.dJ, s))) return r;
     mark(ST_K);
-    if (!k_side && (r = stage_k(e, st, s, nullptr))) return r;  // K^n comes out symmetric
+    if ((r = stage_k(e, st, s, nullptr))) return r;  // K^n comes out symmetric
     mark(ST_HC);
-    if (!side && (r = stage_hcore(e, st, s))) return r;
     return PNJK_OK;
   };
-  rc = body();
+  // split_refine: FP64 MO integrals from b = L + L^T (L below the diagonal, b / 2 on it):
+  // T = L C (the triangular half-transform, half the half-transform's work), W = C^T T and
+  // Y_pq = W_pq + W_qp, so K_MO and d come from W; the split half-transform of C^T b and K^n
+  // run on the side stream beside them, and J^n (split, from the FP64 d) after W.
+  auto refine_body = [&]() -> int {
+    int r;
+    const BuildGeom& g = st.g;
+    PNJK_NEED(st.X);
+    PNJK_NEED(st.W);
+    PNJK_NEED(st.d);
+    PNJK_NEED(st.dKMO);
+    if (k_side) {
+      PNJK_NEED(st.ct16h);
+      PNJK_NEED(st.ct16l);
+      PNJK_NEED(st.x16h);
+      PNJK_NEED(st.x16l);
+      if ((r = fork())) return r;
+      PNJK_HIP_CHECK(launch_ht_x3(st.sg, st.ht16, st.ct16h, st.ct16l, e->bht16h.as<uint16_t>(),
+                                  e->bht16l.as<uint16_t>(), st.x16h, st.x16l, sa, nullptr));
+      if ((r = stage_k(e, st, sa, nullptr))) return r;
+    }
+    mark(ST_HT);
+    TileF64 tt = st.ht;  // a tile spans at most two row slots: at most naux_p columns
+    if (32 * tt.fn > g.naux_p) tt.fn = 2;
+    const double* dh = e->bdh.as<double>();
+    PNJK_HIP_CHECK(
+        launch_htl_f64(g, tt, st.Cp, e->btri.as<double>(), dh, dh + (int64_t)g.nbf * g.naux_p, st.X, s));
+    mark(ST_QT);
+    PNJK_HIP_CHECK(launch_w_f64(g, g.nbf5 > 64 ? TileF64{5, 2} : TileF64{2, 2}, st.Cp, st.X, st.W, s));
+    mark(ST_KMO);
+    PNJK_HIP_CHECK(launch_ymo_f64(g, st.W, st.dKMO, st.d, s));
+    mark(ST_JMO);
+    if ((r = stage_jmo(e, st, s, nullptr))) return r;
+    if (st.want_j) {
+      PNJK_NEED(st.d16h);
+      PNJK_NEED(st.d16l);
+      PNJK_HIP_CHECK(launch_split_blocked(st.d, g.nbf5, g.naux_p, st.d16h, st.d16l, s));
+    }
+    mark(ST_JN);
+    if ((r = stage_jn(e, st, s, nullptr))) return r;
+    if (st.want_j && (r = stage_sym(st, st.dJ, s))) return r;
+    mark(ST_K);  // K^n ran on the side stream
+    mark(ST_HC);
+    return stage_hcore(e, st, s);
+  };
+  rc = refine ? refine_body() : body();
   if (forked) {  // join the side stream
     const hipError_t j1 = hipEventRecord(e->join_ev, sa);
     const hipError_t j2 = j1 == hipSuccess ? hipStreamWaitEvent(s, e->join_ev, 0) : j1;
@@ -1546,8 +1571,6 @@ int group_enqueue(pnjk_group* g, hipStream_t s) {
   for (size_t i = 0; i < n; ++i)
     if ((rc = stage_kmo(g->eng[i], g->st[i], s))) return rc;
   if ((rc = launch_plan(g, GS_JMO, s))) return rc;
-  for (size_t i = 0; i < n; ++i)
-    if ((rc = stage_jmo_post(g->eng[i], g->st[i], s))) return rc;
   const char* base = static_cast<const char*>(g->plan_dev.p);
   if ((rc = launch_plan(g, GS_JN, s))) return rc;
   PNJK_HIP_CHECK(launch_symmetrize_grouped(reinterpret_cast<const SymJob*>(base + g->sym_off),
@@ -1632,6 +1655,20 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
   hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
   auto at = [](double* const* a, size_t i) { return a ? a[i] : nullptr; };
   int rc;
+  if (precision == PNJK_SPLIT_REFINE) {
+    // The refinement's FP64 MO path (a triangular half-transform, W and its reduction) is
+    // not grouped: each instance is built on its own, in order, on this stream (a captured
+    // graph is not used for it).
+    for (size_t i = 0; i < n; ++i) {
+      const BuildArgs a{C[i], ldc[i], nbf5[i], H ? H[i] : nullptr, ldh ? ldh[i] : 0, precision, outputs,
+                        at(J, i), at(K, i), at(J_MO, i), at(K_MO, i), at(H_core, i)};
+      for (const double* o : {a.J, a.K, a.J_MO, a.K_MO, a.H_core})
+        if (o && !is_device_ptr(o))
+          return set_error(PNJK_EINVAL, "pnjk_group_build: outputs of a grouped build must be device memory");
+      if ((rc = jk_build_locked(g->eng[i], a, s, nullptr, 0))) return rc;
+    }
+    return PNJK_OK;
+  }
   g->st.resize(n);
   // The plan (problem structs, tile tables) refers to the engines' buffers and the outputs;
   // C and H are only read by the per-call staging, so new C or H tensors (an optimiser's

@@ -103,6 +103,18 @@ struct Loader<MK, ROWS, BK> : LoaderMK<MK, ROWS, BK> {};
 template <int ROWS, int BK>
 struct Loader<MKLANE, ROWS, BK> : LoaderMK<MKLANE, ROWS, BK> {};
 
+// k-steps of one tile: a problem whose k extent depends on the tile (a triangular
+// iteration space) defines ksteps_tile(t, BK); the others have one extent for all tiles.
+template <class Prob>
+__device__ __forceinline__ auto tile_ksteps(const Prob& p, const Tile& t, int BK, int)
+    -> decltype(p.ksteps_tile(t, BK)) {
+  return p.ksteps_tile(t, BK);
+}
+template <class Prob>
+__device__ __forceinline__ int tile_ksteps(const Prob& p, const Tile&, int BK, long) {
+  return p.ksteps(BK);
+}
+
 // The GEMM core. Accumulator element (fm, fn, r) of a lane is output element
 //   m = m0 + wm + 16*fm + lane/16 + 4*r,  n = n0 + wn + 16*fn + lane%16
 // (the v_mfma_f64_16x16x4f64 result layout).
@@ -112,7 +124,7 @@ __device__ __forceinline__ void gemm_tile_f64(const Prob& prob, const Tile& t, d
   constexpr int SA = BK * (BM + kPad), SB = BK * (BN + kPad);
   const auto as = prob.a_src(t);
   const auto bs = prob.b_src(t);
-  const int nk = prob.ksteps(BK);
+  const int nk = tile_ksteps(prob, t, BK, 0);
 
   // (The wave index is left as a vector value here: making it scalar with
   // readfirstlane turns the MFMA block predicates into scalar branches, which
@@ -288,6 +300,88 @@ struct ProbHT {
           const int q = t.m0 + dm;
           const int64_t n = (int64_t)t.n0 + dn;
           if (q < nbf5 && n < N) gmem(X)[(int64_t)q * N + n] = acc[i][j][r];
+        }
+  }
+};
+
+// Orbital mu of output row slot m of the triangular half-transform: even slots take the
+// rows from the end of the triangle, odd ones from its start (nbf - 1, 0, nbf - 2, 1, ...), so
+// that any contiguous run of slots holds long and short rows alike.
+__host__ __device__ __forceinline__ int tri_row_of_slot(int m, int nbf) {
+  return (m & 1) ? (m - 1) / 2 : nbf - 1 - m / 2;
+}
+
+// b as the K x N operand of the triangular half-transform: element (nu, c), c = m naux_p + P
+// over row slots m (tri_row_of_slot), is L[mu][nu][P] with L = b below the diagonal, b / 2 on
+// it (dhalf, the diagonal rows halved) and 0 above (zeros, one zero row), so b = L + L^T.
+struct SrcKMBtriL {
+  const double* btri;
+  const double* dhalf;
+  const double* zeros;
+  int nbf, naux_p;
+  __device__ __forceinline__ const double* at(int k, int c) const {
+    int m = c / naux_p;
+    const int p = c - m * naux_p;
+    m = m < nbf ? m : nbf - 1;
+    const int mu = tri_row_of_slot(m, nbf);
+    if (k < mu) return btri + tri_index(mu, k) * naux_p + p;
+    if (k == mu) return dhalf + (int64_t)mu * naux_p + p;
+    return zeros + p;
+  }
+};
+
+// T[q][mu][P] = sum_{nu <= mu} C[nu][q] L[mu][nu][P] (L above): half the half-transform's
+// work, since b = L + L^T gives Y_p[q] = W[p][q] + W[q][p] with W[p][q][P] = sum_mu C[mu][q]
+// T[p][mu][P]. Columns run over row slots (c = m naux_p + P) and each tile's k-loop stops at
+// the largest mu among its columns, so tiles carry from 1 to nbf / BK k-steps.
+struct ProbHTL {
+  static constexpr int kA = KM, kB = KM;
+  const double* Cp;
+  const double* btri;
+  const double* dhalf;
+  const double* zeros;
+  double* T;
+  int nbf, naux_p, nbf5, ldcp, nbf_p, mtiles, bn;  // bn: tile columns (<= naux_p)
+  int64_t N;
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.batch = 0;
+    t.mt = id % mtiles;
+    t.nt = id / mtiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf5; }
+  __host__ __device__ int ksteps(int BK) const { return nbf_p / BK; }
+  __device__ int ksteps_tile(const Tile& t, int BK) const {
+    // the tile's columns span row slots m_lo .. m_hi (two at most: bn <= naux_p)
+    const int64_t c_hi = (int64_t)t.n0 + bn - 1;
+    int m_lo = (int)(t.n0 / naux_p), m_hi = (int)((c_hi < N ? c_hi : N - 1) / naux_p);
+    m_lo = m_lo < nbf ? m_lo : nbf - 1;
+    m_hi = m_hi < nbf ? m_hi : nbf - 1;
+    int mu = tri_row_of_slot(m_lo, nbf);
+    for (int m = m_lo + 1; m <= m_hi; ++m) mu = max(mu, tri_row_of_slot(m, nbf));
+    return (mu + BK) / BK;  // k = nu in [0, mu]
+  }
+  __device__ SrcKMDense a_src(const Tile&) const { return {Cp, ldcp}; }
+  __device__ SrcKMBtriL b_src(const Tile&) const { return {btri, dhalf, zeros, nbf, naux_p}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4d (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int q = t.m0 + dm;
+          const int64_t c = (int64_t)t.n0 + dn;
+          if (q < nbf5 && c < N) {
+            const int m = (int)(c / naux_p);
+            const int64_t p = c - (int64_t)m * naux_p;
+            gmem(T)[(int64_t)q * N + (int64_t)tri_row_of_slot(m, nbf) * naux_p + p] = acc[i][j][r];
+          }
         }
   }
 };
@@ -529,6 +623,7 @@ struct ProbMO {
   double* out;
   int64_t ldrow;  // output row stride (nbf5 * naux for the reference layout)
   int nbf, naux, naux_p, nbf5, ldcf, nbf_p, mtiles, ntiles;
+  int rows;  // output rows (columns of Cf): nbf for b_MO, nbf5 for the refinement's W
   __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.mt = id % mtiles;
     const int rest = id / mtiles;
@@ -538,7 +633,7 @@ struct ProbMO {
     t.n0 = t.nt * BN;
     return true;
   }
-  __device__ bool block_active(const Tile&, int r0, int) const { return r0 < nbf; }
+  __device__ bool block_active(const Tile&, int r0, int) const { return r0 < rows; }
   __host__ __device__ int ksteps(int BK) const { return nbf_p / BK; }
   __device__ SrcKMDense a_src(const Tile&) const { return {Cf, ldcf}; }
   __device__ SrcKMRows b_src(const Tile& t) const { return {X + (int64_t)t.batch * nbf * naux_p, naux_p, nbf}; }
@@ -553,7 +648,7 @@ struct ProbMO {
           int dm, dn;
           elem_coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
           const int m = t.m0 + dm, n = t.n0 + dn;
-          if (m < nbf && n < naux) gmem(out)[(int64_t)m * ldrow + (int64_t)t.batch * naux + n] = acc[i][j][r];
+          if (m < rows && n < naux) gmem(out)[(int64_t)m * ldrow + (int64_t)t.batch * naux + n] = acc[i][j][r];
         }
   }
 };
@@ -1125,7 +1220,52 @@ hipError_t launch_mo_f64(const BuildGeom& g, TileF64 tc, const double* Cf, int l
   pr.nbf5 = g.nbf5;
   pr.ldcf = ldcf;
   pr.nbf_p = g.nbf_p;
+  pr.rows = g.nbf;
   pr.mtiles = cdiv(g.nbf, BM);
+  pr.ntiles = g.naux_p / BN;
+  return launch_prob(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s);
+}
+
+hipError_t launch_htl_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* btri, const double* dhalf,
+                          const double* zeros, double* T, hipStream_t s) {
+  if (32 * tc.fn > g.naux_p) return hipErrorInvalidValue;  // a tile spans at most two row slots
+  ProbHTL pr;
+  pr.Cp = Cp;
+  pr.btri = btri;
+  pr.dhalf = dhalf;
+  pr.zeros = zeros;
+  pr.T = T;
+  pr.nbf = g.nbf;
+  pr.naux_p = g.naux_p;
+  pr.nbf5 = g.nbf5;
+  pr.ldcp = g.ldcp;
+  pr.nbf_p = g.nbf_p;
+  pr.N = (int64_t)g.nbf * g.naux_p;
+  const int BM = 32 * tc.fm, BN = 32 * tc.fn;
+  pr.bn = BN;
+  pr.mtiles = cdiv(g.nbf5, BM);
+  return launch_prob(pr, tc, pr.mtiles * cdiv(pr.N, BN), s);
+}
+
+hipError_t launch_w_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* T, double* W,
+                        hipStream_t s) {
+  const int BM = 32 * tc.fm, BN = 32 * tc.fn;
+  // A tile reads columns up to mtiles * BM of each C row: past ldcp they run into the next
+  // row (Cp carries slack rows), and only output rows below nbf5 are written.
+  if (g.naux_p % BN != 0) return hipErrorInvalidValue;
+  ProbMO pr;
+  pr.Cf = Cp;
+  pr.X = T;
+  pr.out = W;
+  pr.ldrow = (int64_t)g.nbf5 * g.naux_p;
+  pr.nbf = g.nbf;
+  pr.naux = g.naux_p;  // W[q][p][P] over the padded aux range (T is zero there)
+  pr.naux_p = g.naux_p;
+  pr.nbf5 = g.nbf5;
+  pr.ldcf = g.ldcp;
+  pr.nbf_p = g.nbf_p;
+  pr.rows = g.nbf5;
+  pr.mtiles = cdiv(g.nbf5, BM);
   pr.ntiles = g.naux_p / BN;
   return launch_prob(pr, tc, pr.mtiles * pr.ntiles * g.nbf5, s);
 }
@@ -1188,6 +1328,54 @@ hipError_t launch_fill_cols(double* p, int rows, int64_t ld, int64_t c0, int64_t
   const int64_t n = (int64_t)rows * (c1 - c0);
   hipLaunchKernelGGL(k_fill_cols, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, p, rows, ld, c0, c1);
   return hipGetLastError();
+}
+
+// Refinement MO integrals from W (row (x, y) of naux_p values holds W_xy = (C^T L C)_xy):
+// Y_pq = W_pq + W_qp. One workgroup per pair p >= q (pair = p (p + 1) / 2 + q):
+// K_MO[p][q] = K_MO[q][p] = sum_{P < naux} Y_pq[P]^2, each thread summing P = tid + 256 j in
+// increasing j, then a fixed tree over the workgroup; on the diagonal also
+// d[p][P] = Y_pp[P] = 2 W_pp[P] for P < naux_p (zero past naux, as W is).
+__global__ void __launch_bounds__(256) k_ymo(const double* __restrict__ W, int n5, int naux, int naux_p,
+                                             double* __restrict__ kmo, double* __restrict__ d) {
+  __shared__ double red[256];
+  const int pair = blockIdx.x;
+  int p = (int)((sqrt(8.0 * pair + 1.0) - 1.0) * 0.5);
+  while ((int64_t)(p + 1) * (p + 2) / 2 <= pair) ++p;
+  while ((int64_t)p * (p + 1) / 2 > pair) --p;
+  const int q = pair - p * (p + 1) / 2;
+  const double* a = W + ((int64_t)p * n5 + q) * naux_p;
+  const double* b = W + ((int64_t)q * n5 + p) * naux_p;
+  double acc = 0.0;
+  for (int P = threadIdx.x; P < naux; P += 256) {
+    const double y = a[P] + b[P];
+    acc = fma(y, y, acc);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    kmo[(int64_t)p * n5 + q] = red[0];
+    kmo[(int64_t)q * n5 + p] = red[0];
+  }
+  if (p == q)
+    for (int P = threadIdx.x; P < naux_p; P += 256) d[(int64_t)p * naux_p + P] = 2.0 * a[P];
+}
+
+// dhalf[mu][P] = b[mu][mu][P] / 2 (the diagonal rows of the packed b, halved) and one zero
+// row of naux_p values: the diagonal and upper parts of L in the triangular half-transform.
+__global__ void k_half_diag(const double* __restrict__ btri, int nbf, int naux_p, double* __restrict__ dhalf,
+                            double* __restrict__ zeros) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)nbf * naux_p;
+  if (i < n) {
+    const int64_t mu = i / naux_p, p = i - mu * naux_p;
+    dhalf[i] = 0.5 * btri[tri_index(mu, mu) * naux_p + p];
+  } else if (i < n + naux_p) {
+    zeros[i - n] = 0.0;
+  }
 }
 
 // Row i of the Fock GEMM: F[i] = hc[i] H + sum_j A[i][j] J[j] + sum_j A[i][nbf5 + j] K[j] (A zero
@@ -1301,6 +1489,18 @@ hipError_t launch_fock_f64(int nbf, int n5, const double* J, const double* K, in
   const int64_t tiles = (int64_t)pr.mtiles * cdiv(nn, 32 * tc.fn);
   if (tiles > 2147483647LL) return hipErrorInvalidValue;
   return launch_prob(pr, tc, (int)tiles, s);
+}
+
+hipError_t launch_ymo_f64(const BuildGeom& g, const double* W, double* kmo, double* d, hipStream_t s) {
+  const int64_t pairs = (int64_t)g.nbf5 * (g.nbf5 + 1) / 2;
+  hipLaunchKernelGGL(k_ymo, dim3((unsigned)pairs), dim3(256), 0, s, W, g.nbf5, g.naux, g.naux_p, kmo, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_half_diag(const double* btri, int nbf, int naux_p, double* dhalf, double* zeros, hipStream_t s) {
+  const int64_t n = (int64_t)nbf * naux_p + naux_p;
+  hipLaunchKernelGGL(k_half_diag, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, btri, nbf, naux_p, dhalf, zeros);
+  return hipGetLastError();
 }
 
 hipError_t launch_lagrange_fock_f64(const BuildGeom& g, const double* F, const double* Cf, int ldcf, double* G,

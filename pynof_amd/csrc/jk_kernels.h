@@ -114,6 +114,17 @@ hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const
 // 64-column partials in *ntiles.
 hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cw, int ldw, const double* X, double* kpart,
                          double* d, int* ntiles, hipStream_t s, GroupCollector* col = nullptr);
+// The refinement's FP64 MO integrals from the symmetric b = L + L^T (L: b below the diagonal,
+// b / 2 on it). T[q][mu][P] = sum_{nu <= mu} C[nu][q] L[mu][nu][P] (T as X: rows of naux_p,
+// nbf rows per orbital); dhalf (nbf x naux_p) and zeros (naux_p) from launch_half_diag.
+hipError_t launch_htl_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* btri, const double* dhalf,
+                          const double* zeros, double* T, hipStream_t s);
+hipError_t launch_half_diag(const double* btri, int nbf, int naux_p, double* dhalf, double* zeros, hipStream_t s);
+// W (nbf5 x nbf5 rows of naux_p): row (x, y) = W_xy = sum_mu C[mu][x] T[y][mu][P], i.e. C^T L C.
+hipError_t launch_w_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* T, double* W,
+                        hipStream_t s);
+// From W: K_MO (both triangles) = sum_{P < naux} (W_pq + W_qp)^2 and d[p][P] = 2 W_pp[P].
+hipError_t launch_ymo_f64(const BuildGeom& g, const double* W, double* kmo, double* d, hipStream_t s);
 // Cw (2 x rows x ldw): Cw0[nu][j] = Cp[nu][j mod n], Cw1[nu][j] = Cp[nu][(j + 1) mod n].
 hipError_t launch_window_c(const double* Cp, int ldcp, int rows, int n, int ldw, double* Cw, hipStream_t s);
 

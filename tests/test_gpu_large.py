@@ -4,10 +4,11 @@
    contraction of the build runs over the full mu, nu ranges, so this checks the
    half-transform outside span(C_q) too, which the full-size identities below cannot.
    FP64 form within 1e-10 (normwise relative) of the oracle, split within 3e-5,
-   split_refine's MO outputs bitwise the FP64 form's.
+   split_refine's MO outputs within 1e-12 of the FP64 form's (H_core bitwise).
 2. Full size (naux=3600, b drawn on the device): every form's errors against the
-   FP64 form, energies in Eh. split_refine's J_MO, K_MO, H_core are bitwise the FP64
-   form's (its energy is the FP64 energy, error 0 Eh); the split energy is within
+   FP64 form, energies in Eh. split_refine's J_MO, K_MO are within 1e-12 of the FP64
+   form's (they come from the triangular route b = L + L^T), H_core is bitwise, and its
+   energy is within 1e-9 Eh of the FP64 energy; the split energy is within
    SPLIT_E_EH_LARGE (measured 5e-4 to 2.3e-3 Eh for two C; |E| ~ 5.7e4 Eh).
 """
 import numpy as np
@@ -51,9 +52,11 @@ def test_large_shape_aux_slice_vs_oracle(gpu_required):
     for k in ("J", "K", "J_MO", "K_MO"):
         assert res["split"][1][k] <= SPLIT_TOL, k
         assert res["split_refine"][1][k] <= (1e-10 if k in ("J_MO", "K_MO") else SPLIT_TOL), k
-    for k in ("J_MO", "K_MO", "H_core"):
-        assert np.array_equal(res["split_refine"][0][k], res["fp64"][0][k]), k
+    for k in ("J_MO", "K_MO"):
+        assert rel(res["split_refine"][0][k], res["fp64"][0][k]) <= 1e-12, k
+    assert np.array_equal(res["split_refine"][0]["H_core"], res["fp64"][0]["H_core"])
     assert abs(res["fp64"][2] - e_ref) <= 1e-9  # Eh
+    assert abs(res["split_refine"][2] - res["fp64"][2]) <= 1e-9  # Eh
     eng.close()
 
 
@@ -93,9 +96,10 @@ def test_large_full_forms(gpu_required):
         for k in ("J", "K", "J_MO", "K_MO"):
             assert errs[k] <= SPLIT_TOL, (prec, k)
         if prec == "split_refine":
-            for k in ("J_MO", "K_MO", "H_core"):
-                assert np.array_equal(r[k].cpu().numpy(), mo[k]), k
-            assert e == e64
+            for k in ("J_MO", "K_MO"):
+                assert errs[k] <= 1e-12, k
+            assert np.array_equal(r["H_core"].cpu().numpy(), mo["H_core"])
+            assert abs(e - e64) <= 1e-9  # Eh
         else:
             assert abs(e - e64) <= SPLIT_E_EH_LARGE
     del res, ref

@@ -1,7 +1,7 @@
 """Serving loop on the GPU: every request's energies and final MO integrals are
 the same alone and in the mix (bitwise: grouped builds equal single builds and
 each request's rotations are seeded per request), a 1e-9 Eh request's final
-energy equals the FP64 form's energy at its final orbitals, and on the Poisson
+energy is within 1e-9 Eh of the FP64 form's energy at its final orbitals, and on the Poisson
 request mix every request's final energy is within its own tolerance (Eh) of
 the FP64 form's."""
 import copy
@@ -84,10 +84,13 @@ def test_tight_request_final_energy_is_fp64(gpu_required):
     n = synth.make_n(r.nbf5, r.ndoc, r.seed + 2)
     o = eng.build(r.final_C, r.nbf5, H=H, precision="fp64", outputs=("J_MO", "K_MO", "H_core"))
     torch.cuda.synchronize()
-    for k in ("J_MO", "K_MO", "H_core"):
-        assert np.array_equal(o[k], r.final_outs[k]), k
+    # the refined build's MO integrals come from the triangular FP64 route: within 1e-12 of
+    # the FP64 form's (H_core bitwise), and the request's energy within 1e-9 Eh
+    for k in ("J_MO", "K_MO"):
+        assert np.linalg.norm(o[k] - r.final_outs[k]) <= 1e-12 * np.linalg.norm(o[k]), k
+    assert np.array_equal(o["H_core"], r.final_outs["H_core"])
     e1, e2 = synth.energy_terms(n, o["J_MO"], o["K_MO"], o["H_core"])
-    assert abs((e1 + e2) - r.energies[-1]) <= 1e-10 * abs(e1 + e2)
+    assert abs((e1 + e2) - r.energies[-1]) <= 1e-9
     eng.close()
 
 

@@ -34,7 +34,7 @@ def test_noalloc_equals_build_two_calls(gpu_required):
     import torch
 
     L = _lib.lib()
-    cases = [(96, 352, 96, 21), (40, 144, 30, 22)]  # second call: a smaller instance
+    cases = [(88, 320, 80, 21), (40, 144, 30, 22)]  # second call: a smaller instance
     engs = []
     for nbf, naux, n5, seed in cases:
         e = JKEngine(0)
