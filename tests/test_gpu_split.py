@@ -3,9 +3,9 @@
 split:        every contraction in BF16x3 (hi*hi + hi*lo + lo*hi, FP32 accumulation).
               Bar: normwise relative error <= 3e-5 for J, K, J_MO, K_MO (a BF16x3
               product carries ~2^-16 relative error); H_core is FP64 (1e-10).
-split_refine: J_MO, K_MO, H_core through the FP64 path (bar 1e-10 against the oracle,
-              bitwise equal to the FP64 form's, so the energy is the FP64 energy);
-              J, K in BF16x3 from the exact half-transform (3e-5).
+split_refine: J_MO, K_MO in FP64 by the triangular route (bar 1e-10 against the oracle,
+              1e-12 against the FP64 form's), H_core bitwise the FP64 form's, so the energy
+              is the FP64 energy to 1e-9 Eh; J, K in BF16x3 (3e-5).
 Energies are compared in Eh (FP64_E_EH, SPLIT_E_EH_MID below).
 """
 import numpy as np
