@@ -99,6 +99,32 @@ __host__ __device__ inline void tri_pair(int rem, int& mt, int& nt) {
   nt = rem - m * (m + 1) / 2;
 }
 
+// Tile number rem of a symmetric product's lower triangle of tiles whose first npair
+// numbers are diagonal pair tiles: rem < npair is pair s = rem, covering tiles (2s, 2s),
+// (2s + 1, 2s) and (2s + 1, 2s + 1) (returned as mt = 2s, nt = -1); the other numbers are
+// the tiles (mt, nt), nt <= mt, that no pair covers, in row-major order (rows 2s and
+// 2s + 1 keep their 2s tiles left of the pair, rows from 2 npair on are whole). With
+// npair = 0 this is tri_pair. A triangle of T tile rows has T (T + 1) / 2 - 2 npair numbers.
+__host__ __device__ inline void syrk_tile(int rem, int npair, int& mt, int& nt) {
+  if (rem < npair) {
+    mt = 2 * rem;
+    nt = -1;
+    return;
+  }
+  rem -= npair;
+  const int paired = 2 * npair * (npair - 1);  // tiles of rows 0 .. 2 npair - 1
+  if (rem < paired) {
+    int s = (int)((1.0 + sqrt(1.0 + 2.0 * rem)) * 0.5);
+    while (2 * s * (s - 1) > rem) --s;
+    while (2 * (s + 1) * s <= rem) ++s;
+    const int r = rem - 2 * s * (s - 1);
+    mt = r < 2 * s ? 2 * s : 2 * s + 1;
+    nt = r < 2 * s ? r : r - 2 * s;
+  } else {
+    tri_pair(rem - paired + npair * (2 * npair + 1), mt, nt);
+  }
+}
+
 // Row index of the packed lower triangle: t = mu*(mu+1)/2 + nu for mu >= nu.
 __device__ __forceinline__ int64_t tri_index(int a, int b) {
   const int64_t hi = a > b ? a : b, lo = a > b ? b : a;

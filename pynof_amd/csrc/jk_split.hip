@@ -245,6 +245,137 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
   prob.template epilogue<FM, FN>(t, acc, wm, wn, lane, reinterpret_cast<double*>(sA0));
 }
 
+// Problems with diagonal pair tiles (Prob::kDiagPairs): a symmetric product A A^T whose
+// lower triangle of BM x BM tiles would have half-empty diagonal tiles.
+template <class P, class = void>
+struct HasDiagPairs : std::false_type {};
+template <class P>
+struct HasDiagPairs<P, std::void_t<decltype(P::kDiagPairs)>> : std::bool_constant<P::kDiagPairs> {};
+template <int FM, int FN, int WGM, int WGN>
+constexpr bool kDiagPairShape = FM == FN && WGM == 2 && WGN == 2 && FM % 2 == 0;
+
+// Diagonal pair tile of a symmetric product O = A A^T (t.nt < 0): the two diagonal tiles
+// of panels P0 = rows [m0, m0 + BM) and P1 = [m0 + BM, m0 + 2 BM) and the tile between
+// them, i.e. the lower triangle of one 2BM x 2BM block, from one copy of its 2BM rows per
+// k-step (P0 into the A buffers, P1 into the B buffers, both as the regular tile copies
+// them). In 16-row strips g = 0..S-1 (S = 2BM / 16) the triangle holds blocks (g, c),
+// c <= g. Folded row f < S/2 pairs strip S-1-f (columns 0..S-1-f) with strip f (columns
+// 0..f): S + 1 blocks for every f, accumulator j = c for the long strip and j = S - c for
+// the short one, so a block's accumulator index is fixed by its column and each B
+// fragment, read once per k-step, serves every strip of the wave that needs it. Wave w
+// takes folded rows w, w + 4, ...: every wave issues the same MFMA count. Each output
+// (m, n), n <= m, comes from the same fragments in the same order as in the regular
+// tiles (A = strip of m, B = strip of n), so the results are those of the regular tiles
+// bit for bit; the copy round trip that sets a k-step's time is paid once for three
+// tiles' work (K^n at nbf = 1000 on 128 x 128 tiles: 3.50 -> 3.18 ms, 28 instead of 36
+// workgroups per orbital).
+template <int FM, int WGM, int WGN, class Prob>
+__device__ __forceinline__ void diag_pair_x3(const Prob& prob, const Tile& t, uint16_t* sA0, uint16_t* sA1,
+                                             uint16_t* sB0, uint16_t* sB1) {
+  constexpr int BM = 16 * FM * WGM, NW = WGM * WGN;
+  constexpr int S = 2 * BM / 16, H = S / 2, R = H / NW;  // strips, strips per panel, folded rows per wave
+  static_assert(H % NW == 0, "folded rows are dealt evenly over the waves");
+  const auto as = prob.a_src(t);
+  const int nk = prob.ksteps();
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  auto fetch = [&](int k0, uint16_t* a, uint16_t* b) {
+    Dma<BM, NW>::issue(as, t.m0, k0, a, a + BM * kLDK, wave, lane);
+    Dma<BM, NW>::issue(as, t.m0 + BM, k0, b, b + BM * kLDK, wave, lane);
+  };
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  fetch(0, sA0, sB0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+
+  v4f acc[R][S + 1];
+#pragma unroll
+  for (int k = 0; k < R; ++k)
+#pragma unroll
+    for (int j = 0; j <= S; ++j) acc[k][j] = v4f{0.f, 0.f, 0.f, 0.f};
+  // Folded row k of this wave is f = wave + NW k: short strip f, long strip S - 1 - f. The
+  // block tests compare the wave index with constants, so most fold at compile time.
+  // Strips past the last row (in the last pair tile) compute on clamped rows like the others
+  // and are not stored.
+  __builtin_assume(wave >= 0 && wave < NW);
+  const int frow = lane & 15, fk = 8 * (lane >> 4);
+  auto step = [&](const int kt, auto stage) {
+    constexpr int cur = decltype(stage)::value;
+    uint16_t* const p0 = cur ? sA1 : sA0;
+    uint16_t* const p1 = cur ? sB1 : sB0;
+    if (kt + 1 < nk) fetch((kt + 1) * kBK, cur ? sA0 : sA1, cur ? sB0 : sB1);  // wave-uniform
+    auto strip = [&](int g, bf16x8& h, bf16x8& l) {
+      const uint16_t* b = g < H ? p0 : p1;
+      const int o = swz(16 * (g < H ? g : g - H) + frow, fk >> 3);
+      h = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(b + o));
+      l = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(b + BM * kLDK + o));
+    };
+    bf16x8 lh[R], ll[R], sh[R], sl[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      strip(S - 1 - (wave + NW * k), lh[k], ll[k]);
+      strip(wave + NW * k, sh[k], sl[k]);
+    }
+    // B fragments one column ahead, read whether or not the wave needs the column.
+    bf16x8 bh, bl;
+    strip(0, bh, bl);
+    __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+    for (int c = 0; c < S; ++c) {
+      bf16x8 nh, nl;
+      if (c + 1 < S) strip(c + 1, nh, nl);
+      if (c + wave <= S - 1) {  // column c is in the wave's longest strip
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          if (c + wave + NW * k <= S - 1) {  // block (long strip, c)
+            acc[k][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ll[k], bh, acc[k][c], 0, 0, 0);
+            acc[k][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lh[k], bl, acc[k][c], 0, 0, 0);
+            acc[k][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lh[k], bh, acc[k][c], 0, 0, 0);
+          }
+          if (c <= wave + NW * k) {  // block (short strip, c)
+            acc[k][S - c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sl[k], bh, acc[k][S - c], 0, 0, 0);
+            acc[k][S - c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[k], bl, acc[k][S - c], 0, 0, 0);
+            acc[k][S - c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[k], bh, acc[k][S - c], 0, 0, 0);
+          }
+        }
+      }
+      if (c + 1 < S) {
+        bh = nh;
+        bl = nl;
+      }
+    }
+    __builtin_amdgcn_s_setprio(0);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies into the next buffers have landed
+    __syncthreads();
+  };
+  for (int kt = 0; kt < nk; kt += 2) {
+    step(kt, S0{});
+    if (kt + 1 < nk) step(kt + 1, S1{});
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k)
+#pragma unroll
+    for (int j = 0; j <= S; ++j) {
+      const int f = wave + NW * k;
+      const bool lng = j + f <= S - 1;
+      prob.store_block(t, acc[k][j], lng ? S - 1 - f : f, lng ? j : S - j, lane);
+    }
+}
+
+// A tile of the problem: a diagonal pair tile where the problem has them (t.nt < 0, only
+// for the shapes that take them), else the regular tile.
+template <int FM, int FN, int WGM, int WGN, class Prob>
+__device__ __forceinline__ void run_tile_x3(const Prob& prob, const Tile& t, uint16_t* sA0, uint16_t* sA1,
+                                            uint16_t* sB0, uint16_t* sB1) {
+  if constexpr (HasDiagPairs<Prob>::value && kDiagPairShape<FM, FN, WGM, WGN>) {
+    if (t.nt < 0) {
+      diag_pair_x3<FM, WGM, WGN>(prob, t, sA0, sA1, sB0, sB1);
+      return;
+    }
+  }
+  gemm_tile_x3<FM, FN, WGM, WGN>(prob, t, sA0, sA1, sB0, sB1);
+}
+
 // One output tile per workgroup. Two waves per SIMD: four-wave workgroups run
 // two per CU, eight-wave workgroups one.
 template <int FM, int FN, int WGM, int WGN, class Prob>
@@ -254,7 +385,7 @@ __global__ void __launch_bounds__(64 * WGM * WGN, 2) k_gemm_x3(const Prob prob) 
   __shared__ __align__(16) uint16_t sB0[2 * BN * kLDK], sB1[2 * BN * kLDK];
   Tile t;
   if (!prob.tile(xcd_remap(blockIdx.x, gridDim.x), BM, BN, t)) return;
-  gemm_tile_x3<FM, FN, WGM, WGN>(prob, t, sA0, sA1, sB0, sB1);
+  run_tile_x3<FM, FN, WGM, WGN>(prob, t, sA0, sA1, sB0, sB1);
 }
 
 // Grouped form over a (slot, local tile) table (see jk_kernels.h): one workgroup
@@ -277,7 +408,7 @@ __global__ void __launch_bounds__(64 * WGM * WGN, 2)
   const int slot = __builtin_amdgcn_readfirstlane(e.x), local = __builtin_amdgcn_readfirstlane(e.y);
   const Prob prob = probs[slot];
   Tile t;
-  if (prob.tile(local, BM, BN, t)) gemm_tile_x3<FM, FN, WGM, WGN>(prob, t, sA0, sA1, sB0, sB1);
+  if (prob.tile(local, BM, BN, t)) run_tile_x3<FM, FN, WGM, WGN>(prob, t, sA0, sA1, sB0, sB1);
 }
 
 // ---------------------------------------------------------------- sources
@@ -630,26 +761,41 @@ struct ProbJN {
 // output). Each value with i >= j goes to (i, j) and, for i > j, also to (j, i): the
 // output is exactly symmetric without a separate pass (the upper half of a diagonal
 // tile, whose own products pair the planes the other way round, is not stored).
+// With npair > 0 (the tile shapes of kDiagPairShape) the first npair tiles of a batch are
+// diagonal pair tiles (diag_pair_x3: panels 2s and 2s + 1 in one tile, nt = -1), and the
+// others are the lower-triangle tiles they do not cover (syrk_tile).
 struct ProbSYRK {
   static constexpr int kA = RK, kB = RK;
   static constexpr bool kRowSkip = true;
   static constexpr bool kLower = true;  // rows above the diagonal are never stored
+  static constexpr bool kDiagPairs = true;
   Planes x;
   double* O;
   int64_t sa, ldo, so;  // sa: X16 elements per batch (blocked rows of K columns)
-  int M, K, tiles_per_batch;
+  int M, K, tiles_per_batch, npair;
   __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
     t.batch = id / tiles_per_batch;
-    const int rem = id - t.batch * tiles_per_batch;
-    tri_pair(rem, t.mt, t.nt);
+    syrk_tile(id - t.batch * tiles_per_batch, npair, t.mt, t.nt);
     t.m0 = t.mt * BM;
-    t.n0 = t.nt * BN;
+    t.n0 = t.nt < 0 ? t.m0 + BM : t.nt * BN;
     return true;
   }
   __host__ __device__ int ksteps() const { return K / kBK; }
   __device__ int row_limit(const Tile&) const { return M; }
   __device__ SrcBlocked a_src(const Tile& t) const { return {{x.hi + t.batch * sa, x.lo + t.batch * sa}, K / kBK, M}; }
   __device__ SrcBlocked b_src(const Tile& t) const { return a_src(t); }
+  // The 16 x 16 block of rows m0 + 16 g and columns m0 + 16 c of a diagonal pair tile.
+  __device__ void store_block(const Tile& t, const v4f& v, int g, int c, int lane) const {
+    double* o = O + (int64_t)t.batch * so;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = t.m0 + 16 * g + out_row(lane, r), n = t.m0 + 16 * c + (lane & 15);
+      if (m < M && n <= m) {
+        gmem(o)[(int64_t)m * ldo + n] = v[r];
+        if (n < m) gmem(o)[(int64_t)n * ldo + m] = v[r];
+      }
+    }
+  }
   template <int FM, int FN>
   __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
     double* o = O + (int64_t)t.batch * so;
@@ -1004,7 +1150,10 @@ hipError_t launch_kn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* x_hi, co
   pr.M = g.nbf;
   pr.K = g.naux_p;
   const int side = cdiv(g.nbf, BM);
-  pr.tiles_per_batch = side * (side + 1) / 2;
+  // Diagonal pair tiles for the shapes whose kernels take them: 128 x 128 (4, 4), the
+  // built-in choice, and 64 x 64 (2, 2).
+  pr.npair = (tc.fm == 2 || tc.fm == 4) ? side / 2 : 0;
+  pr.tiles_per_batch = side * (side + 1) / 2 - 2 * pr.npair;
   return launch_x3(pr, tc, pr.tiles_per_batch * g.nbf5, s, col);
 }
 

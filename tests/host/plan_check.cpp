@@ -4,7 +4,8 @@
 //     unordered orbital pair {p, q}, diagonal included, is owned by exactly one batch;
 //   * the FP64 quarter transform's window copy of C: every column a tile reads is inside
 //     qt_window_cols(n) and holds the orbital the window row needs;
-//   * tri_pair, the lower-triangle tile-pair decode of the SYRK problems;
+//   * tri_pair, the lower-triangle tile-pair decode of the SYRK problems, and syrk_tile,
+//     the same with diagonal pair tiles: every lower-triangle tile covered exactly once;
 //   * GroupBatch::xcd_bounds, the per-XCD cut of a grouped launch's tile table;
 //   * GroupCollector::add, the valid-tile enumeration of a grouped launch;
 //   * ws_carve, the workspace layout of a build (also the no-allocation build's scratch).
@@ -76,6 +77,36 @@ static void check_tri_pair() {
     tri_pair(rem, mt, nt);
     CHECK(nt >= 0 && nt <= mt && mt * (mt + 1) / 2 + nt == rem, "rem=%d -> (%d,%d)", rem, mt, nt);
   }
+}
+
+static void check_syrk_tile() {
+  for (int T = 1; T <= 64; ++T)
+    for (int npair = 0; npair <= T / 2; ++npair) {
+      const int count = T * (T + 1) / 2 - 2 * npair;
+      std::vector<int> cover((size_t)T * T, 0);
+      int last = -1;  // row-major position of the previous regular tile
+      for (int rem = 0; rem < count; ++rem) {
+        int mt = -9, nt = -9;
+        syrk_tile(rem, npair, mt, nt);
+        if (rem < npair) {
+          CHECK(nt == -1 && mt == 2 * rem && mt + 1 < T, "T=%d npair=%d rem=%d -> (%d,%d)", T, npair, rem, mt, nt);
+          if (nt != -1 || mt < 0 || mt + 1 >= T) continue;
+          ++cover[(size_t)mt * T + mt];
+          ++cover[(size_t)(mt + 1) * T + mt];
+          ++cover[(size_t)(mt + 1) * T + mt + 1];
+        } else {
+          CHECK(nt >= 0 && nt <= mt && mt < T, "T=%d npair=%d rem=%d -> (%d,%d)", T, npair, rem, mt, nt);
+          if (nt < 0 || nt > mt || mt >= T) continue;
+          CHECK(mt * T + nt > last, "T=%d npair=%d rem=%d: not in row-major order", T, npair, rem);
+          last = mt * T + nt;
+          ++cover[(size_t)mt * T + nt];
+        }
+      }
+      for (int a = 0; a < T; ++a)
+        for (int b = 0; b <= a; ++b)
+          CHECK(cover[(size_t)a * T + b] == 1, "T=%d npair=%d: tile (%d,%d) covered %d times", T, npair, a, b,
+                cover[(size_t)a * T + b]);
+    }
 }
 
 static void check_xcd_bounds(std::mt19937& rng) {
@@ -170,6 +201,7 @@ int main() {
   check_window();
   check_window_copy();
   check_tri_pair();
+  check_syrk_tile();
   check_xcd_bounds(rng);
   check_collector();
   check_carve(rng);
