@@ -20,6 +20,8 @@
 // symmetric in mu,nu, integrals.py:35-49), halving its HBM footprint; the
 // symmetric outputs are computed on one triangle of tiles; a tiled transpose
 // (k_symmetrize) then fills the other triangle with coalesced reads and writes.
+#include <type_traits>
+
 #include "jk_common.h"
 #include "jk_kernels.h"
 
@@ -115,6 +117,69 @@ __device__ __forceinline__ int tile_ksteps(const Prob& p, const Tile&, int BK, l
   return p.ksteps(BK);
 }
 
+// Problems whose diagonal tiles (mt == nt) hold only their lower triangle (Prob::kFoldDiag:
+// the symmetric products A A^T), for the 128 x 128 tile.
+template <class P, class = void>
+struct HasFoldDiag : std::false_type {};
+template <class P>
+struct HasFoldDiag<P, std::void_t<decltype(P::kFoldDiag)>> : std::bool_constant<P::kFoldDiag> {};
+
+// A diagonal tile of a symmetric product: its S = BM / 16 row strips hold the blocks
+// (g, c), c <= g. Wave w takes strips S - 1 - w (columns 0..S-1-w) and w (columns 0..w),
+// S + 1 blocks for every wave (accumulator c for the long strip, S - c for the short one),
+// where the regular 2 x 2 wave split gives one wave the full 64 x 64 block below the
+// diagonal and another nothing, so the tile took as long as a full one. Every block reads
+// the same LDS fragments (A = its strip, B = its column) in the same order as in the
+// regular split, so the values are the same bit for bit. The tile's A and B rows are the
+// same, so one operand copy per k-step serves both.
+template <int FM, int BK, class Prob>
+__device__ __forceinline__ void gemm_diag_f64(const Prob& prob, const Tile& t, double* sA) {
+  constexpr int BM = 32 * FM, S = BM / 16, NW = kThreads / 64;
+  static_assert(S == 2 * NW, "two strips per wave");
+  constexpr int SA = BK * (BM + kPad);
+  const auto as = prob.a_src(t);  // A and B are the same rows here: one copy serves both
+  const int nk = tile_ksteps(prob, t, BK, 0);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int gl = S - 1 - wave, gs = wave;
+
+  Loader<Prob::kA, BM, BK> la;
+  la.load(as, t.m0, 0);
+  la.store(sA);
+  __syncthreads();
+
+  v4d acc[S + 1];
+#pragma unroll
+  for (int j = 0; j <= S; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
+  int kt = 0;
+  do {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) la.load(as, t.m0, (kt + 1) * BK);
+    const double* a_l = sA + cur * SA + (lane & 15);
+    const double* b_l = a_l;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      const double al = a_l[kr * (BM + kPad) + 16 * gl], as_ = a_l[kr * (BM + kPad) + 16 * gs];
+      double b[S];
+#pragma unroll
+      for (int c = 0; c < S; ++c) b[c] = b_l[kr * (BM + kPad) + 16 * c];
+#pragma unroll
+      for (int c = 0; c < S; ++c) {
+        if (c <= gl) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(al, b[c], acc[c], 0, 0, 0);
+        if (c <= gs) acc[S - c] = __builtin_amdgcn_mfma_f64_16x16x4f64(as_, b[c], acc[S - c], 0, 0, 0);
+      }
+    }
+    if (more) la.store(sA + (cur ^ 1) * SA);
+    __syncthreads();
+  } while (++kt < nk);
+#pragma unroll
+  for (int j = 0; j <= S; ++j) {
+    const bool lng = j <= gl;
+    prob.store_block(t, acc[j], lng ? gl : gs, lng ? j : S - j, lane);
+  }
+}
+
 // The GEMM core. Accumulator element (fm, fn, r) of a lane is output element
 //   m = m0 + wm + 16*fm + lane/16 + 4*r,  n = n0 + wn + 16*fn + lane%16
 // (the v_mfma_f64_16x16x4f64 result layout).
@@ -124,6 +189,12 @@ __device__ __forceinline__ void gemm_tile_f64(const Prob& prob, const Tile& t, d
   constexpr int SA = BK * (BM + kPad), SB = BK * (BN + kPad);
   const auto as = prob.a_src(t);
   const auto bs = prob.b_src(t);
+  if constexpr (HasFoldDiag<Prob>::value && FM == 4 && FN == 4) {
+    if (t.mt == t.nt) {
+      gemm_diag_f64<FM, BK>(prob, t, sA);
+      return;
+    }
+  }
   const int nk = tile_ksteps(prob, t, BK, 0);
 
   // (The wave index is left as a vector value here: making it scalar with
@@ -496,6 +567,7 @@ struct ProbQT {
 
 struct ProbSYRK {
   static constexpr int kA = MK, kB = MK;
+  static constexpr bool kFoldDiag = true;  // diagonal 128 x 128 tiles: gemm_diag_f64
   const double* A;
   double* O;
   int64_t lda, sa, ldo, so;
@@ -515,6 +587,18 @@ struct ProbSYRK {
   __host__ __device__ int ksteps(int BK) const { return K / BK; }
   __device__ SrcMKDense a_src(const Tile& t) const { return {A + (int64_t)t.batch * sa, lda, M}; }
   __device__ SrcMKDense b_src(const Tile& t) const { return {A + (int64_t)t.batch * sa, lda, M}; }
+  // The 16 x 16 block of rows m0 + 16 g and columns n0 + 16 c of a diagonal tile.
+  __device__ void store_block(const Tile& t, const v4d& v, int g, int c, int lane) const {
+    double* o = O + (int64_t)t.batch * so;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = t.m0 + 16 * g + (lane >> 4) + 4 * r, n = t.n0 + 16 * c + (lane & 15);
+      if (m < M && n <= m) {
+        gmem(o)[(int64_t)m * ldo + n] = v[r];
+        if (n < m) gmem(o)[(int64_t)n * ldo + m] = v[r];
+      }
+    }
+  }
   template <int FM, int FN>
   __device__ void epilogue(const Tile& t, v4d (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
     double* o = O + (int64_t)t.batch * so;
