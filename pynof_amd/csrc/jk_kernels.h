@@ -218,8 +218,6 @@ __host__ __device__ inline int64_t x16_stride(int nbf, int naux_p) { return (int
 // columns past naux_p of the last row block land up to two 512-element blocks further on).
 constexpr int64_t kX16Slack = 1024;
 
-// hi/lo bf16 planes of n packed FP64 values.
-hipError_t launch_split_btri(const double* btri, int64_t n, uint16_t* hi, uint16_t* lo, hipStream_t s);
 // Split planes of a row-major rows x cols matrix (cols a multiple of 32) in 16-row x 32-column
 // blocks (blk_index in jk_split.hip); rows rounded up to 16 with zeros.
 hipError_t launch_split_blocked(const double* src, int64_t rows, int64_t cols, uint16_t* hi, uint16_t* lo,

@@ -970,11 +970,6 @@ hipError_t launch_split_x16(const double* X, const SplitGeom& g, uint16_t* hi, u
   return hipGetLastError();
 }
 
-hipError_t launch_split_btri(const double* btri, int64_t n, uint16_t* hi, uint16_t* lo, hipStream_t s) {
-  hipLaunchKernelGGL(k_split_packed, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, btri, n, hi, lo);
-  return hipGetLastError();
-}
-
 // W16[b][k'] = split(ck[b][q] B[b][k']) for k' = q naux + P < nbf5 naux, zero up to ldrow.
 __global__ void k_lag_w16(const double* __restrict__ B, int64_t ldrow, int naux, int nbf5,
                           const double* __restrict__ ck, uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
