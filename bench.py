@@ -303,6 +303,24 @@ def main():
                      "traffic": tr64["bytes"] if tr64 else None}
     roofline_fp64["frac"] = round(roofline_fp64["achieved"] / PEAK_FP64, 4)
 
+    # ---- the drop-in boundary's host path: host C in, host J and K out (4.8 GB over PCIe into
+    # the page-locked arrays the engine allocates for outputs this large); not `value`
+    log("host outputs (drop-in path)")
+    C_host = Cs[0]
+    eng.build(C_host, nbf5, H=None, precision="split", outputs=("J", "K"))
+    host_ms = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        hres = eng.build(C_host, nbf5, H=None, precision="split", outputs=("J", "K"))
+        host_ms.append(1e3 * (time.perf_counter() - t0))
+    d2h_bytes = hres["J"].nbytes + hres["K"].nbytes
+    pinned = bool(torch.from_numpy(hres["J"]).is_pinned())
+    del hres
+    split_ms = forms["split"]["ms_per_build"]
+    host_path = {"form": "split", "ms_per_build": round(float(np.median(host_ms)), 2),
+                 "ms_per_build_device_outputs": split_ms, "d2h_bytes": d2h_bytes, "page_locked": pinned,
+                 "d2h_GBps": round(d2h_bytes / ((float(np.median(host_ms)) - split_ms) * 1e-3) / 1e9, 1)}
+
     # ---- orbital-gradient MO integrals (JKH_MO_tmp): HT + full-row quarter transform + H_mat
     mo = {}
     mo_flops = 2.0 * nbf5 * nbf * nbf * naux + 2.0 * nbf * nbf5 * nbf * naux + 2.0 * nbf * nbf * nbf5 * 2
@@ -451,6 +469,7 @@ def main():
         "fp64_build_tflops": round(total_flops * forms["fp64"]["builds_per_s"] / 1e12, 3),
         "roofline": roofline,
         "roofline_fp64": roofline_fp64,
+        "host_outputs": host_path,
         "mo_integrals": mo,
         "lagrangian": lag,
         "fock": fock,

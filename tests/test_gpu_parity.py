@@ -205,3 +205,39 @@ def test_device_outputs_ordered_on_current_stream(eng):
         out = eng.build(Cd, nbf5, outputs=("J", "K", "J_MO", "K_MO"))
         s = float((2.0 * out["J_MO"] - out["K_MO"]).sum())  # no synchronize before this
         assert s == want
+
+
+def test_dropin_host_outputs_page_locked(gpu_required):
+    """Host outputs of 64 MiB and more come back in page-locked memory (device-to-host at
+    full PCIe rate) as plain numpy arrays, with the bits of a build into device outputs;
+    the arrays own their memory for as long as they live."""
+    import gc
+
+    import torch
+
+    nbf, naux, nbf5 = 220, 100, 180  # J, K 69.7 MB each
+    print(f"case: drop-in host outputs pinned nbf={nbf} naux={naux} nbf5={nbf5}", flush=True)
+    b = synth.make_b(nbf, naux, 41, 0.05)
+    C = synth.make_C(nbf, 42)
+    p = param(nbf, nbf5, nbfaux=naux, ndoc=1)
+    J, K = computeJKj(C, [], b, p)
+    for a in (J, K):
+        assert isinstance(a, np.ndarray) and a.dtype == np.float64 and a.shape == (nbf5, nbf, nbf)
+        assert a.flags.c_contiguous and a.flags.writeable
+        assert torch.from_numpy(a).is_pinned()
+    e = JKEngine(0)
+    e.upload_b(b)
+    d = e.build(torch.from_numpy(C).cuda(), nbf5, outputs=("J", "K"))
+    torch.cuda.synchronize()
+    assert np.array_equal(J, d["J"].cpu().numpy()) and np.array_equal(K, d["K"].cpu().numpy())
+    assert rel(J, O.jkj_ri(C, b, nbf5)[0]) <= TOL
+    # small outputs stay in ordinary numpy memory
+    JM, KM, Hc = computeJKH_MO(C, synth.make_H(nbf, 43), [], b, p)
+    assert not torch.from_numpy(JM).is_pinned()
+    # an output that outlives the call keeps its contents after later builds reuse the pool
+    Jc = J.copy()
+    del K
+    gc.collect()
+    J2, K2 = computeJKj(C @ synth.rotation(nbf, 6001), [], b, p)
+    assert np.array_equal(J, Jc) and not np.shares_memory(J, J2)
+    e.close()
