@@ -321,6 +321,25 @@ def main():
                  "ms_per_build_device_outputs": split_ms, "d2h_bytes": d2h_bytes, "page_locked": pinned,
                  "d2h_GBps": round(d2h_bytes / ((float(np.median(host_ms)) - split_ms) * 1e-3) / 1e9, 1)}
 
+    # ---- computeJKH_MO's build (J_MO, K_MO, H_core only; the orbital optimiser's energy call,
+    # pnof.py:1107): the FP64 form takes the triangular route there (T = L C, W = C^T T)
+    log("MO-only builds (computeJKH_MO)")
+    mo_outs = ("J_MO", "K_MO", "H_core")
+    mo_only = {}
+    for form in ("fp64", "split"):
+        eng.build(Cd[0], nbf5, H=Hd, precision=form, outputs=mo_outs, out=out, stream=stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = max(1, args.form_steps)
+        e0.record()
+        for _ in range(reps):
+            eng.build(Cd[0], nbf5, H=Hd, precision=form, outputs=mo_outs, out=out, stream=stream)
+        e1.record()
+        torch.cuda.synchronize()
+        mo_only[form] = {"ms": round(e0.elapsed_time(e1) / reps, 3)}
+    mo_only["fp64"]["route"] = "triangular (T = L C, W = C^T T)" if nbf5 < nbf else "half + quarter transform"
+    fst = forms["fp64"]["stage_ms"]
+    mo_only["fp64"]["full_build_ht_plus_qt_ms"] = round(fst["half_transform"] + fst["quarter_transform"], 3)
+
     # ---- orbital-gradient MO integrals (JKH_MO_tmp): HT + full-row quarter transform + H_mat
     mo = {}
     mo_flops = 2.0 * nbf5 * nbf * nbf * naux + 2.0 * nbf * nbf5 * nbf * naux + 2.0 * nbf * nbf * nbf5 * 2
@@ -470,6 +489,7 @@ def main():
         "roofline": roofline,
         "roofline_fp64": roofline_fp64,
         "host_outputs": host_path,
+        "mo_only_builds": mo_only,
         "mo_integrals": mo,
         "lagrangian": lag,
         "fock": fock,

@@ -200,6 +200,7 @@ struct pnjk_engine {
   DevBuf occ;                                    // occupation step: every array of the step
   rocblas_handle blas = nullptr;                 // for the metric eigensolver (b from 3-centre integrals)
   bool split_ready = false;
+  bool hd_ready = false;  // bdh holds the halved diagonal rows of the resident b
   // Tile override per arithmetic (0: FP64 kernels, 1: BF16x3 kernels) and stage;
   // {0, 0} selects the built-in choice.
   TileF64 tiles[2][ST_COUNT] = {};
@@ -288,7 +289,11 @@ uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int preci
   if (precision != PNJK_SPLIT_REFINE) s += (uint64_t)(nbf5 * nbf5 * (naux_p / 64)) * 8;  // kpart
   s += (uint64_t)(nbf * ldcp) * 8;                           // T
   if (precision == PNJK_FP64) s += (uint64_t)(2 * nbf_p * qt_window_cols((int)nbf5)) * 8;  // Cw
-  if (precision == PNJK_SPLIT_REFINE) s += (uint64_t)(nbf5 * nbf5 * naux_p) * 8;          // W
+  // W: split_refine, and FP64 builds without J and K when nbf5 < nbf (the triangular route;
+  // this total bounds both routes of an FP64 build)
+  const bool tri_fp64 = precision == PNJK_FP64 && nbf5 < nbf;
+  if (precision == PNJK_SPLIT_REFINE || tri_fp64) s += (uint64_t)(nbf5 * nbf5 * naux_p) * 8;
+  if (tri_fp64) s += (uint64_t)((nbf + 1) * naux_p) * 8;   // halved diagonal rows of b and a zero row
   if (precision != PNJK_FP64) {
     const int64_t ntri = nbf * (nbf + 1) / 2;
     s += (uint64_t)(roundup(ntri, 16) * naux_p) * 4;         // btri16 hi/lo (whole 16-row blocks)
@@ -301,6 +306,19 @@ uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int preci
   return s;
 }
 
+// The halved diagonal rows of b and a zero row, which the triangular half-transform reads
+// (once per uploaded b).
+static int ensure_half_diag(pnjk_engine* e, hipStream_t s) {
+  if (e->hd_ready) return PNJK_OK;
+  const int64_t naux_p = e->naux_p, nbf = e->nbf;
+  int rc;
+  if ((rc = e->bdh.ensure((size_t)((nbf + 1) * naux_p) * 8))) return rc;
+  PNJK_HIP_CHECK(launch_half_diag(e->btri.as<double>(), (int)nbf, (int)naux_p, e->bdh.as<double>(),
+                                  e->bdh.as<double>() + nbf * naux_p, s));
+  e->hd_ready = true;
+  return PNJK_OK;
+}
+
 // Builds the bf16 planes of b the split kernels read, and the halved diagonal rows the
 // refinement's triangular half-transform reads (once per uploaded b).
 static int ensure_split_planes(pnjk_engine* e, hipStream_t s) {
@@ -311,12 +329,9 @@ static int ensure_split_planes(pnjk_engine* e, hipStream_t s) {
   const size_t tri_bytes = (size_t)(roundup(ntri, 16) * naux_p) * 2, ht_bytes = (size_t)(roundup(nbf, 16) * naux_p * ldk) * 2;
   if ((rc = e->bt16h.ensure(tri_bytes)) || (rc = e->bt16l.ensure(tri_bytes))) return rc;
   if ((rc = e->bht16h.ensure(ht_bytes)) || (rc = e->bht16l.ensure(ht_bytes))) return rc;
-  if ((rc = e->bdh.ensure((size_t)((nbf + 1) * naux_p) * 8))) return rc;
   PNJK_HIP_CHECK(launch_split_blocked(e->btri.as<double>(), ntri, naux_p, e->bt16h.as<uint16_t>(),
                                       e->bt16l.as<uint16_t>(), s));
-  // split_refine's triangular half-transform reads b's diagonal halved and a zero row
-  PNJK_HIP_CHECK(launch_half_diag(e->btri.as<double>(), (int)nbf, (int)naux_p, e->bdh.as<double>(),
-                                  e->bdh.as<double>() + nbf * naux_p, s));
+  if ((rc = ensure_half_diag(e, s))) return rc;
   PNJK_HIP_CHECK(launch_build_bht(e->btri.as<double>(), (int)nbf, (int)naux_p, (int)ldk, e->bht16h.as<uint16_t>(),
                                   e->bht16l.as<uint16_t>(), s));
   e->split_ready = true;
@@ -325,6 +340,7 @@ static int ensure_split_planes(pnjk_engine* e, hipStream_t s) {
 
 static int upload_common(pnjk_engine* e, int64_t nbf, int64_t naux) {
   e->split_ready = false;
+  e->hd_ready = false;
   if (nbf < 1 || naux < 1) return set_error(PNJK_EINVAL, "upload_b: nbf and naux must be >= 1");
   if (nbf > 46000) return set_error(PNJK_EINVAL, "upload_b: nbf too large");
   e->nbf = (int)nbf;
@@ -487,6 +503,7 @@ int pnjk_upload_b_3c(pnjk_engine* e, const double* mnk, int64_t nbf, int64_t nau
   std::swap(e->btri.p, out.p);  // `out` now holds the 3-centre integrals and is freed on return
   std::swap(e->btri.bytes, out.bytes);
   e->split_ready = false;
+  e->hd_ready = false;
   e->nbf = nbf_ready;
   if (footprint) *footprint = pnjk_footprint(e);
   return PNJK_OK;
@@ -540,7 +557,10 @@ struct BuildState {
   // library's static scratch array (pnjk_jk_build_noalloc).
   double *Cp = nullptr, *Cw = nullptr, *X = nullptr, *d = nullptr, *kpart = nullptr, *Hp = nullptr, *T = nullptr;
   uint16_t *ct16h = nullptr, *ct16l = nullptr, *x16h = nullptr, *x16l = nullptr, *d16h = nullptr, *d16l = nullptr;
-  double* W = nullptr;  // split_refine: C^T L C
+  double* W = nullptr;  // triangular route: C^T L C
+  // J_MO, K_MO by the triangular route (T = L C, W = C^T T, Y_pq = W_pq + W_qp; the
+  // refinement's FP64 MO path) instead of the half- and quarter transforms.
+  bool tri = false;
 };
 
 // The workspace pieces of one build, in bytes (0: not needed), in a fixed order.
@@ -562,7 +582,7 @@ struct WsPlan {
 // staging of unrequested J_MO/K_MO are carved from it and nothing is allocated: every
 // output must then be device memory (host outputs would need staging buffers).
 int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scratch = nullptr,
-                  size_t scratch_bytes = 0) {
+                  size_t scratch_bytes = 0, bool fp64_tri = false) {
   if (e->nbf == 0) return set_error(PNJK_ESTATE, "pnjk_jk_build: no b uploaded");
   if (!a.C) return set_error(PNJK_EINVAL, "pnjk_jk_build: C is NULL");
   if (a.nbf5 < 1 || a.nbf5 > e->nbf)
@@ -604,6 +624,12 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
   st.want_jmo = a.outputs & PNJK_OUT_JMO;
   st.want_kmo = a.outputs & PNJK_OUT_KMO;
   st.want_h = a.outputs & PNJK_OUT_HCORE;
+  // The triangular route forms the MO integrals in nbf^2 nbf5 + 2 nbf nbf5^2 flops against
+  // 2 nbf5 nbf^2 + nbf5 (nbf5 + 1) nbf for the half- and quarter transforms: fewer whenever
+  // nbf5 < nbf. split_refine always takes it; an FP64 build takes it when its caller allows
+  // it and asks for neither J nor K, which need the whole half-transformed X.
+  st.tri = precision == PNJK_SPLIT_REFINE ||
+           (fp64_tri && precision == PNJK_FP64 && !st.want_j && !st.want_k && n5 < nbf);
   const int64_t jk_elems = n5 * nbf * nbf;
   st.dJ = a.J;
   st.dK = a.K;
@@ -636,10 +662,10 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
     // (split_refine keeps its triangular half-transform T here.)
     w.bytes[W_X] = (size_t)(n5 * nbf * g.naux_p + 64) * 8;
   }
-  if (precision == PNJK_FP64) w.bytes[W_CW] = (size_t)(2 * g.nbf_p * (int64_t)qt_window_cols(g.nbf5)) * 8;
-  if (precision == PNJK_SPLIT_REFINE) w.bytes[W_W] = (size_t)(n5 * n5 * g.naux_p) * 8;
+  if (precision == PNJK_FP64 && !st.tri) w.bytes[W_CW] = (size_t)(2 * g.nbf_p * (int64_t)qt_window_cols(g.nbf5)) * 8;
+  if (st.tri) w.bytes[W_W] = (size_t)(n5 * n5 * g.naux_p) * 8;
   w.bytes[W_D] = (size_t)(roundup(n5, 160) * g.naux_p) * 8;
-  if (precision != PNJK_SPLIT_REFINE) w.bytes[W_KPART] = (size_t)(n5 * n5 * (g.naux_p / 64)) * 8;
+  if (!st.tri) w.bytes[W_KPART] = (size_t)(n5 * n5 * (g.naux_p / 64)) * 8;
   if (st.want_h) {
     w.bytes[W_HP] = (size_t)(nbf * g.nbf_p) * 8;
     w.bytes[W_T] = (size_t)(nbf * g.ldcp) * 8;
@@ -707,6 +733,9 @@ int stage_in(pnjk_engine* e, BuildState& st, hipStream_t s) {
   if (st.a.precision != PNJK_FP64) {
     int rc = ensure_split_planes(e, s);
     if (rc) return rc;
+  } else if (st.tri) {
+    int rc = ensure_half_diag(e, s);
+    if (rc) return rc;
   }
   const BuildGeom& g = st.g;
   const int64_t nbf = g.nbf, n5 = g.nbf5;
@@ -725,7 +754,7 @@ int stage_in(pnjk_engine* e, BuildState& st, hipStream_t s) {
     PNJK_NEED(st.ct16h);
     PNJK_NEED(st.ct16l);
     PNJK_HIP_CHECK(launch_split_ct(Cp, g.ldcp, g.nbf, g.nbf5, g.nbf5, st.sg.ldk, st.ct16h, st.ct16l, s));
-  } else {
+  } else if (!st.tri) {  // the FP64 quarter transform's window copies of C
     PNJK_NEED(st.Cw);
     PNJK_HIP_CHECK(launch_window_c(Cp, g.ldcp, g.nbf_p, g.nbf5, qt_window_cols(g.nbf5), st.Cw, s));
   }
@@ -868,14 +897,16 @@ static int jk_build_locked(pnjk_engine* e, const BuildArgs& a, hipStream_t s, ch
   };
   BuildState st;
   int rc;
-  if ((rc = build_prepare(e, a, st, scratch, scratch_bytes))) return rc;
+  // (the no-allocation build keeps the FP64 route of the whole half-transform: the halved
+  // diagonal rows of the triangular route are an engine buffer that it would have to allocate)
+  if ((rc = build_prepare(e, a, st, scratch, scratch_bytes, scratch == nullptr))) return rc;
   mark(ST_IN);
   if ((rc = stage_in(e, st, s))) return rc;
   const bool refine = st.a.precision == PNJK_SPLIT_REFINE;
   // The forms other than split_refine run J_MO and H_core on the side stream beside J^n
   // (about 0.3 ms hidden per build at nbf=1000); J^n's mirror stays in line: beside K^n it
   // slowed K^n by more than it saved.
-  const bool side = !refine, k_side = refine && st.want_k;
+  const bool side = !st.tri, k_side = refine && st.want_k;
   hipStream_t sa = (side || k_side) ? e->aux : s;
   bool forked = false;
   auto fork = [&]() -> int {
@@ -905,10 +936,11 @@ static int jk_build_locked(pnjk_engine* e, const BuildArgs& a, hipStream_t s, ch
     mark(ST_HC);
     return PNJK_OK;
   };
-  // split_refine: FP64 MO integrals from b = L + L^T (L below the diagonal, b / 2 on it):
-  // T = L C (the triangular half-transform, half the half-transform's work), W = C^T T and
-  // Y_pq = W_pq + W_qp, so K_MO and d come from W; the split half-transform of C^T b and K^n
-  // run on the side stream beside them, and J^n (split, from the FP64 d) after W.
+  // The triangular route (split_refine; FP64 builds without J and K): FP64 MO integrals from
+  // b = L + L^T (L below the diagonal, b / 2 on it): T = L C (the triangular half-transform,
+  // half the half-transform's work), W = C^T T and Y_pq = W_pq + W_qp, so K_MO and d come
+  // from W. In split_refine the split half-transform of C^T b and K^n run on the side stream
+  // beside them, and J^n (split, from the FP64 d) after W.
   auto refine_body = [&]() -> int {
     int r;
     const BuildGeom& g = st.g;
@@ -950,7 +982,7 @@ static int jk_build_locked(pnjk_engine* e, const BuildArgs& a, hipStream_t s, ch
     mark(ST_HC);
     return stage_hcore(e, st, s);
   };
-  rc = refine ? refine_body() : body();
+  rc = st.tri ? refine_body() : body();
   if (forked) {  // join the side stream
     const hipError_t j1 = hipEventRecord(e->join_ev, sa);
     const hipError_t j2 = j1 == hipSuccess ? hipStreamWaitEvent(s, e->join_ev, 0) : j1;

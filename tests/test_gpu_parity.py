@@ -241,3 +241,38 @@ def test_dropin_host_outputs_page_locked(gpu_required):
     J2, K2 = computeJKj(C @ synth.rotation(nbf, 6001), [], b, p)
     assert np.array_equal(J, Jc) and not np.shares_memory(J, J2)
     e.close()
+
+
+@pytest.mark.parametrize("nbf,naux,nbf5", [(60, 200, 40), (130, 400, 70), (97, 300, 96), (2, 3, 1), (40, 100, 40)])
+def test_fp64_mo_only_builds(eng, nbf, naux, nbf5):
+    """An FP64 build asked for J_MO, K_MO and H_core only forms the MO integrals by the
+    triangular route (T = L C, W = C^T T) when nbf5 < nbf: against the oracle at the FP64
+    bar, within 1e-12 of the full build (another summation order; the two routes agree
+    exactly when nbf5 = nbf, which keeps the full route), H_core bitwise, and the drop-in
+    computeJKH_MO gives the same arrays."""
+    import torch
+
+    print(f"case: fp64 MO-only nbf={nbf} naux={naux} nbf5={nbf5}", flush=True)
+    b = synth.make_b(nbf, naux, 51, 0.05)
+    C = synth.make_C(nbf, 52)
+    H = synth.make_H(nbf, 53)
+    eng.upload_b(b)
+    full = eng.build(C, nbf5, H=H)
+    mo = eng.build(C, nbf5, H=H, outputs=("J_MO", "K_MO", "H_core"))
+    JMr, KMr, Hr = O.jkh_mo_ri(C, H, b, nbf5)
+    for k, r in (("J_MO", JMr), ("K_MO", KMr), ("H_core", Hr)):
+        assert rel(mo[k], r) <= TOL, k
+        assert np.linalg.norm(mo[k] - full[k]) <= 1e-12 * np.linalg.norm(full[k]), k
+    assert np.array_equal(mo["H_core"], full["H_core"])
+    if nbf5 == nbf:
+        for k in ("J_MO", "K_MO"):
+            assert np.array_equal(mo[k], full[k]), k
+    # device inputs and outputs give the same bits as host ones
+    dev = eng.build(torch.from_numpy(C).cuda(), nbf5, H=torch.from_numpy(H).cuda(),
+                    outputs=("J_MO", "K_MO", "H_core"))
+    torch.cuda.synchronize()
+    for k in mo:
+        assert np.array_equal(dev[k].cpu().numpy(), mo[k]), k
+    p = param(nbf, nbf5, nbfaux=naux, ndoc=1)
+    JM, KM, Hc = computeJKH_MO(C, H, [], b, p)
+    assert np.array_equal(JM, mo["J_MO"]) and np.array_equal(KM, mo["K_MO"]) and np.array_equal(Hc, mo["H_core"])
