@@ -1,0 +1,68 @@
+"""FP64 parity at the large workload's full size against the oracle (GPU only, opt-in).
+
+nbf=1000, naux=3600, nbf5=300, b decay 0.02: the FP64 form's J^n, K^n, J_MO, K_MO and
+H_core against the oracle with the reference's call sequence (integrals.py:208-218 and
+:348-362), at the FP64 bar of 1e-10 normwise relative, and the energy within 1e-9 Eh. The
+oracle needs about 10 minutes of host time and 50 GB of host memory here, so the test runs
+only with PYNOF_AMD_FULL_PARITY=1; the default GPU suite covers this shape on an aux slice
+(test_gpu_large.py). J^n and K^n are computed by the oracle in slices of 30 orbitals (each
+orbital's matrices depend on that orbital only), printing one line per slice.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from oracle import jk_oracle as O
+from pynof_amd import JKEngine, synth
+
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skipif(os.environ.get("PYNOF_AMD_FULL_PARITY") != "1",
+                       reason="full-size oracle run (about 10 minutes, 50 GB of host memory): "
+                              "set PYNOF_AMD_FULL_PARITY=1"),
+]
+TOL = 1e-10
+SLICE = 30
+
+
+def test_large_fp64_against_oracle(gpu_required):
+    w = synth.workload(2)
+    nbf, naux, nbf5 = w.nbf, w.naux, w.nbf5
+    print(f"case: full-size FP64 parity nbf={nbf} naux={naux} nbf5={nbf5} lam={w.lam}", flush=True)
+    eng = JKEngine(0)
+    bt = synth.device_b_packed(nbf, naux, w.seed_b, w.lam, JKEngine.packed_ld(nbf, naux))
+    eng.upload_b_packed(bt, nbf, naux)
+    del bt
+    b = eng.download_b()  # the resident b in the reference layout, 28.8 GB
+    C = synth.make_C(nbf, w.seed_c)
+    H = synth.make_H(nbf, w.seed_h)
+    out = eng.build(C, nbf5, H=H, precision="fp64")
+    eng.close()
+    print("  device build done", flush=True)
+
+    # J^n, K^n: the oracle per slice of orbitals; normwise relative error over the whole tensor
+    worst = {}
+    for name in ("J", "K"):
+        worst[name] = [0.0, 0.0]  # max |gpu - oracle|, max |oracle|
+    t0 = time.perf_counter()
+    for q0 in range(0, nbf5, SLICE):
+        q1 = min(nbf5, q0 + SLICE)
+        Jr, Kr = O.jkj_ri(C[:, q0:q1], b, q1 - q0)
+        for name, r in (("J", Jr), ("K", Kr)):
+            worst[name][0] = max(worst[name][0], float(np.max(np.abs(out[name][q0:q1] - r))))
+            worst[name][1] = max(worst[name][1], float(np.max(np.abs(r))))
+        print(f"  oracle J^n, K^n orbitals {q0}..{q1 - 1} done ({time.perf_counter() - t0:.0f} s)", flush=True)
+    JMr, KMr, Hr = O.jkh_mo_ri(C, H, b, nbf5)
+    print(f"  oracle J_MO, K_MO, H_core done ({time.perf_counter() - t0:.0f} s)", flush=True)
+    errs = {name: d / m for name, (d, m) in worst.items()}
+    for name, r in (("J_MO", JMr), ("K_MO", KMr), ("H_core", Hr)):
+        errs[name] = float(np.max(np.abs(out[name] - r)) / np.max(np.abs(r)))
+    n = synth.make_n(nbf5, w.ndoc, w.seed_n)
+    e_gpu = O.energy(n, out["J_MO"], out["K_MO"], out["H_core"])
+    e_ref = O.energy(n, JMr, KMr, Hr)
+    print(f"  normwise relative errors {errs}; E = {e_ref:.10f} Eh, gpu - oracle {e_gpu - e_ref:.3e} Eh", flush=True)
+    for name, e in errs.items():
+        assert e <= TOL, (name, e)
+    assert abs(e_gpu - e_ref) <= 1e-9
