@@ -3,7 +3,7 @@
 nbf=1000, naux=3600, nbf5=300, b decay 0.02: the FP64 form's J^n, K^n, J_MO, K_MO and
 H_core against the oracle with the reference's call sequence (integrals.py:208-218 and
 :348-362), at the FP64 bar of 1e-10 normwise relative, and the energy within 1e-9 Eh. The
-oracle needs about 10 minutes of host time and 50 GB of host memory here, so the test runs
+oracle needs about 4 minutes on 16 host threads and 50 GB of host memory, so the test runs
 only with PYNOF_AMD_FULL_PARITY=1; the default GPU suite covers this shape on an aux slice
 (test_gpu_large.py). J^n and K^n are computed by the oracle in slices of 30 orbitals (each
 orbital's matrices depend on that orbital only), printing one line per slice.
@@ -20,7 +20,7 @@ from pynof_amd import JKEngine, synth
 pytestmark = [
     pytest.mark.gpu,
     pytest.mark.skipif(os.environ.get("PYNOF_AMD_FULL_PARITY") != "1",
-                       reason="full-size oracle run (about 10 minutes, 50 GB of host memory): "
+                       reason="full-size oracle run (about 4 minutes, 50 GB of host memory): "
                               "set PYNOF_AMD_FULL_PARITY=1"),
 ]
 TOL = 1e-10
