@@ -10,7 +10,8 @@ from pynof_amd.integrals import JKH_MO_tmp, release_cached_engines
 pytestmark = pytest.mark.gpu
 FP64_TOL = 1e-10
 SPLIT_TOL = 3e-5
-SHAPES = [(1, 1, 1), (5, 7, 3), (24, 116, 24), (37, 45, 20), (70, 200, 70), (130, 472, 100), (97, 352, 60)]
+SHAPES = [(1, 1, 1), (5, 7, 3), (24, 116, 24), (37, 45, 20), (70, 200, 70), (130, 472, 100), (97, 352, 60),
+          (420, 96, 400)]  # the last: nbf5 > 320
 
 
 def rel(a, b):

@@ -47,8 +47,10 @@ def test_water_golden(eng):
     assert abs(e - meta["E"]) <= TOL * abs(meta["E"])
 
 
+# (the last two: nbf5 > 320, so three 160-row M-tiles in the half-transform and J^n and a
+# quarter-transform window of two M-tiles; aux slices keep the oracle quick)
 SHAPES = [(1, 1, 1), (2, 3, 1), (5, 7, 3), (17, 33, 9), (40, 144, 40), (65, 130, 64), (129, 200, 100),
-          (130, 70, 130), (161, 96, 161), (97, 520, 33)]
+          (130, 70, 130), (161, 96, 161), (97, 520, 33), (420, 96, 400), (352, 64, 337)]
 
 
 @pytest.mark.parametrize("nbf,naux,nbf5", SHAPES)

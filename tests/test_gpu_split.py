@@ -37,7 +37,7 @@ def eng(gpu_required):
 
 
 SHAPES = [(1, 1, 1), (5, 7, 3), (24, 116, 24), (40, 144, 40), (65, 130, 64), (130, 70, 130), (161, 96, 161),
-          (97, 520, 33)]
+          (97, 520, 33), (420, 96, 400), (352, 64, 337)]  # the last two: nbf5 > 320 (several M-tiles)
 
 
 @pytest.mark.parametrize("nbf,naux,nbf5", SHAPES)
