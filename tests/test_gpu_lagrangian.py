@@ -36,7 +36,8 @@ def eng(gpu_required):
 
 @pytest.mark.parametrize("nbf,naux,nbf5,nbeta,nalpha", [(1, 1, 1, 1, 1), (6, 7, 4, 2, 2), (24, 116, 24, 5, 5),
                                                         (37, 45, 20, 6, 9), (70, 200, 70, 0, 0),
-                                                        (130, 472, 100, 40, 40), (97, 352, 60, 20, 25)])
+                                                        (130, 472, 100, 40, 40), (97, 352, 60, 20, 25),
+                                                        (420, 96, 400, 100, 110)])
 def test_lagrangian_vs_oracle(eng, nbf, naux, nbf5, nbeta, nalpha):
     print(f"case: lagrangian nbf={nbf} naux={naux} nbf5={nbf5} nbeta={nbeta} nalpha={nalpha}", flush=True)
     b, C, H, n, cj, ck = inputs(nbf, naux, nbf5, 600 + nbf)
