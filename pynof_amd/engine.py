@@ -217,6 +217,8 @@ class JKEngine:
                 res[name] = _torch().empty(shapes[name], dtype=_torch().float64, device=Ci.device)
             else:
                 res[name] = _host_empty(shapes[name])
+        if nbf5 == 0:  # no orbitals: zero-size outputs, as the reference's slicing C[:, :0] gives
+            return res
         if stream is None and on_device:
             stream = _torch().cuda.current_stream(Ci.device).cuda_stream
         ptrs = [(_ptr(res[n]) if n in res else None) for n in ("J", "K", "J_MO", "K_MO", "H_core")]

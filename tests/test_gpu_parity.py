@@ -278,3 +278,28 @@ def test_fp64_mo_only_builds(eng, nbf, naux, nbf5):
     p = param(nbf, nbf5, nbfaux=naux, ndoc=1)
     JM, KM, Hc = computeJKH_MO(C, H, [], b, p)
     assert np.array_equal(JM, mo["J_MO"]) and np.array_equal(KM, mo["K_MO"]) and np.array_equal(Hc, mo["H_core"])
+
+
+def test_no_orbitals_gives_empty_outputs(gpu_required):
+    """nbf5 = 0: zero-size outputs of the reference's shapes, as its slicing C[:, :0] gives
+    (integrals.py:211, :351-360), for host and device C. (param itself refuses nbf5 = 0, as
+    the reference's parameters do; a plain object with nbf5 and RI reaches the drop-ins.)"""
+    import types
+
+    import torch
+
+    print("case: nbf5 = 0", flush=True)
+    w = synth.workload(0)
+    b, C, H, _ = synth.make_inputs(w)
+    p = types.SimpleNamespace(nbf5=0, RI=True)
+    J, K = computeJKj(C, [], b, p)
+    JM, KM, Hc = computeJKH_MO(C, H, [], b, p)
+    Jr, Kr = O.jkj_ri(C, b, 0)
+    JMr, KMr, Hr = O.jkh_mo_ri(C, H, b, 0)
+    for a, r in ((J, Jr), (K, Kr), (JM, JMr), (KM, KMr), (Hc, Hr)):
+        assert isinstance(a, np.ndarray) and a.shape == r.shape and a.size == 0
+    e = JKEngine(0)
+    e.upload_b(b)
+    d = e.build(torch.from_numpy(C).cuda(), 0, H=torch.from_numpy(H).cuda())
+    assert d["J"].shape == (0, w.nbf, w.nbf) and d["J"].is_cuda and d["H_core"].shape == (0,)
+    e.close()
