@@ -339,6 +339,21 @@ def main():
     mo_only["fp64"]["route"] = "triangular (T = L C, W = C^T T)" if nbf5 < nbf else "half + quarter transform"
     fst = forms["fp64"]["stage_ms"]
     mo_only["fp64"]["full_build_ht_plus_qt_ms"] = round(fst["half_transform"] + fst["quarter_transform"], 3)
+    # ---- computeJKj's build (J and K only; ENERGY1r with no1 > 0, utils.py:421): the quarter
+    # transform forms d alone (window row 0) and J_MO, K_MO are skipped
+    log("J, K-only builds (computeJKj)")
+    jk_only = {}
+    for form in ("fp64", "split"):
+        eng.build(Cd[0], nbf5, precision=form, outputs=("J", "K"), out=out, stream=stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = max(1, args.form_steps)
+        e0.record()
+        for _ in range(reps):
+            eng.build(Cd[0], nbf5, precision=form, outputs=("J", "K"), out=out, stream=stream)
+        e1.record()
+        torch.cuda.synchronize()
+        jk_only[form] = {"ms": round(e0.elapsed_time(e1) / reps, 3),
+                         "full_build_ms": forms[form]["ms_per_build"]}
 
     # ---- orbital-gradient MO integrals (JKH_MO_tmp): HT + full-row quarter transform + H_mat
     mo = {}
@@ -490,6 +505,7 @@ def main():
         "roofline_fp64": roofline_fp64,
         "host_outputs": host_path,
         "mo_only_builds": mo_only,
+        "jk_only_builds": jk_only,
         "mo_integrals": mo,
         "lagrangian": lag,
         "fock": fock,

@@ -561,6 +561,10 @@ struct BuildState {
   // J_MO, K_MO by the triangular route (T = L C, W = C^T T, Y_pq = W_pq + W_qp; the
   // refinement's FP64 MO path) instead of the half- and quarter transforms.
   bool tri = false;
+  // Quarter-transform window rows: 0 for the whole window; 1 when neither J_MO nor K_MO is
+  // requested and only d (window row 0, for J^n) is needed, with the bits of the whole
+  // window's d.
+  int qt_rows = 0;
 };
 
 // The workspace pieces of one build, in bytes (0: not needed), in a fixed order.
@@ -630,6 +634,7 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
   // it and asks for neither J nor K, which need the whole half-transformed X.
   st.tri = precision == PNJK_SPLIT_REFINE ||
            (fp64_tri && precision == PNJK_FP64 && !st.want_j && !st.want_k && n5 < nbf);
+  st.qt_rows = (st.want_jmo || st.want_kmo) ? 0 : 1;
   const int64_t jk_elems = n5 * nbf * nbf;
   st.dJ = a.J;
   st.dK = a.K;
@@ -638,8 +643,8 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
   st.dH = a.H_core;
   st.hJ = st.want_j && !is_device_ptr(a.J);
   st.hK = st.want_k && !is_device_ptr(a.K);
-  st.hJMO = !st.want_jmo || !is_device_ptr(a.J_MO);  // J_MO and K_MO are always formed
-  st.hKMO = !st.want_kmo || !is_device_ptr(a.K_MO);
+  st.hJMO = !st.want_jmo || !is_device_ptr(a.J_MO);  // staged when not requested (the triangular
+  st.hKMO = !st.want_kmo || !is_device_ptr(a.K_MO);  // route's reduction writes K_MO regardless)
   st.hH = st.want_h && !is_device_ptr(a.H_core);
   if (scratch && (st.hJ || st.hK || (st.want_jmo && st.hJMO) || (st.want_kmo && st.hKMO) || st.hH))
     return set_error(PNJK_EINVAL, "pnjk_jk_build_noalloc: every output must be device memory");
@@ -780,6 +785,9 @@ int stage_ht(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col)
 int stage_qt(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col) {
   PNJK_NEED(st.kpart);
   PNJK_NEED(st.d);
+  if (!st.want_j && !st.want_jmo && !st.want_kmo) return PNJK_OK;  // nothing reads d or Y
+  // d alone (window row 0): the smallest row tile, 128 aux columns
+  const TileF64 qt = st.qt_rows == 1 ? TileF64{2, 4} : st.qt;
   if (st.a.precision == PNJK_SPLIT) {
     PNJK_NEED(st.ct16h);
     PNJK_NEED(st.ct16l);
@@ -787,18 +795,19 @@ int stage_qt(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col)
     PNJK_NEED(st.x16l);
     PNJK_NEED(st.d16h);
     PNJK_NEED(st.d16l);
-    PNJK_HIP_CHECK(launch_qt_x3(st.sg, st.qt, st.ct16h, st.ct16l, st.x16h, st.x16l, st.kpart, st.d, st.d16h, st.d16l,
-                                &st.nt, s, col));
+    PNJK_HIP_CHECK(launch_qt_x3(st.sg, qt, st.ct16h, st.ct16l, st.x16h, st.x16l, st.kpart, st.d, st.d16h, st.d16l,
+                                &st.nt, s, col, st.qt_rows));
   } else {
     PNJK_NEED(st.Cw);
     PNJK_NEED(st.X);
-    PNJK_HIP_CHECK(launch_qt_f64(st.g, st.qt, st.Cw, qt_window_cols(st.g.nbf5), st.X, st.kpart, st.d, &st.nt, s,
-                                 col));
+    PNJK_HIP_CHECK(launch_qt_f64(st.g, qt, st.Cw, qt_window_cols(st.g.nbf5), st.X, st.kpart, st.d, &st.nt, s, col,
+                                 st.qt_rows));
   }
   return PNJK_OK;
 }
 
 int stage_kmo(pnjk_engine* e, BuildState& st, hipStream_t s) {
+  if (!st.want_kmo) return PNJK_OK;
   PNJK_NEED(st.kpart);
   PNJK_NEED(st.dKMO);
   PNJK_HIP_CHECK(launch_kmo_reduce(st.g.nbf5, st.nt, st.kpart, st.dKMO, s));
@@ -806,6 +815,7 @@ int stage_kmo(pnjk_engine* e, BuildState& st, hipStream_t s) {
 }
 
 int stage_jmo(pnjk_engine* e, BuildState& st, hipStream_t s, GroupCollector* col) {
+  if (!st.want_jmo) return PNJK_OK;
   PNJK_NEED(st.d);
   PNJK_NEED(st.dJMO);
   PNJK_HIP_CHECK(launch_syrk_f64(1, st.g.nbf5, st.g.naux_p, st.d, st.g.naux_p, 0, st.dJMO, st.g.nbf5, 0,

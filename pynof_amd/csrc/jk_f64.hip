@@ -1119,7 +1119,7 @@ hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const
 
 
 hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cw, int ldw, const double* X, double* kpart,
-                         double* d, int* ntiles, hipStream_t s, GroupCollector* col) {
+                         double* d, int* ntiles, hipStream_t s, GroupCollector* col, int window) {
   // naux_p is a multiple of 64; a 128-column tile may cover 64 columns past it,
   // which the B loader reads from the workspace slack of X and the epilogue drops.
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
@@ -1134,7 +1134,8 @@ hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cw, int l
   pr.nbf5 = g.nbf5;
   pr.ldw = ldw;
   pr.nbf_p = g.nbf_p;
-  pr.h = qt_window(g.nbf5);
+  if (window < 0 || window > qt_window(g.nbf5)) return hipErrorInvalidValue;
+  pr.h = window ? window : qt_window(g.nbf5);
   pr.mtiles = cdiv(pr.h, BM);
   // The window copy must cover every column a tile reads, (p or p - 1) + mtiles * BM - 1 <=
   // nbf5 - 2 + mtiles * BM, at an even row stride.

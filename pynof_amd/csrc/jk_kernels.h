@@ -111,9 +111,11 @@ hipError_t launch_ht_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const
 // writes the row sums of Y_p[q]^2 per 64 aux columns of the pairs batch p owns into
 // kpart[(max(p,q)*nbf5 + min(p,q))*nt + tile] and d[p][P] = Y_p[p][P]. Cw: the two window
 // copies of C (launch_window_c), ldw >= qt_window_cols(nbf5). Returns the number of
-// 64-column partials in *ntiles.
+// 64-column partials in *ntiles. window = 1 computes window row 0 only (d, and the diagonal
+// pair's partials): each output element's MFMA sequence does not depend on the other rows,
+// so d has the bits of the whole window's; 0 is the whole window.
 hipError_t launch_qt_f64(const BuildGeom& g, TileF64 tc, const double* Cw, int ldw, const double* X, double* kpart,
-                         double* d, int* ntiles, hipStream_t s, GroupCollector* col = nullptr);
+                         double* d, int* ntiles, hipStream_t s, GroupCollector* col = nullptr, int window = 0);
 // The refinement's FP64 MO integrals from the symmetric b = L + L^T (L: b below the diagonal,
 // b / 2 on it). T[q][mu][P] = sum_{nu <= mu} C[nu][q] L[mu][nu][P] (T as X: rows of naux_p,
 // nbf rows per orbital); dhalf (nbf x naux_p) and zeros (naux_p) from launch_half_diag.
@@ -247,9 +249,10 @@ hipError_t launch_mo_x3(const SplitGeom& g, int naux, TileF64 tc, const uint16_t
                         uint16_t* bmo_hi = nullptr, uint16_t* bmo_lo = nullptr);
 // Zeroes columns [c0, c1) of two bf16 planes (rows of stride ld).
 hipError_t launch_fill_cols16(uint16_t* h, uint16_t* l, int rows, int64_t ld, int64_t c0, int64_t c1, hipStream_t s);
+// The BF16x3 quarter transform, as launch_qt_f64 (window likewise).
 hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
                         const uint16_t* x_hi, const uint16_t* x_lo, double* kpart, double* d, uint16_t* d_hi,
-                        uint16_t* d_lo, int* ntiles, hipStream_t s, GroupCollector* col = nullptr);
+                        uint16_t* d_lo, int* ntiles, hipStream_t s, GroupCollector* col = nullptr, int window = 0);
 // J^n from split d and btri planes (FP64 output, mu >= nu only).
 hipError_t launch_jn_x3(const SplitGeom& g, TileF64 tc, const uint16_t* d_hi, const uint16_t* d_lo,
                         const uint16_t* bt_hi, const uint16_t* bt_lo, double* J, hipStream_t s,

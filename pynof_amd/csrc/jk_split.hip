@@ -1041,7 +1041,7 @@ hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
 
 hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
                         const uint16_t* x_hi, const uint16_t* x_lo, double* kpart, double* d, uint16_t* d_hi,
-                        uint16_t* d_lo, int* ntiles, hipStream_t s, GroupCollector* col) {
+                        uint16_t* d_lo, int* ntiles, hipStream_t s, GroupCollector* col, int window) {
   // naux_p is a multiple of 64; a 128-column tile may cover 64 columns past it,
   // which the B loader reads from the workspace slack and the epilogue drops.
   const int BM = 32 * tc.fm, BN = 32 * tc.fn;
@@ -1057,7 +1057,8 @@ hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
   pr.naux_p = g.naux_p;
   pr.nbf5 = g.nbf5;
   pr.ldk = g.ldk;
-  pr.h = qt_window(g.nbf5);
+  if (window < 0 || window > qt_window(g.nbf5)) return hipErrorInvalidValue;
+  pr.h = window ? window : qt_window(g.nbf5);
   pr.mtiles = cdiv(pr.h, BM);
   pr.ntiles = cdiv(g.naux_p, BN);
   pr.nt64 = g.naux_p / 64;

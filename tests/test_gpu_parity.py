@@ -303,3 +303,25 @@ def test_no_orbitals_gives_empty_outputs(gpu_required):
     d = e.build(torch.from_numpy(C).cuda(), 0, H=torch.from_numpy(H).cuda())
     assert d["J"].shape == (0, w.nbf, w.nbf) and d["J"].is_cuda and d["H_core"].shape == (0,)
     e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_jk_without_mo_outputs(eng, precision):
+    """computeJKj's build (J and K, no J_MO or K_MO) runs the quarter transform for d alone
+    (window row 0) and skips the K_MO reduction and J_MO: J and K keep the bits of the full
+    build's (d has the same MFMA sequence either way); J alone and K alone as well."""
+    nbf, naux, nbf5 = 150, 400, 120
+    print(f"case: J, K without MO outputs {precision} nbf={nbf} naux={naux} nbf5={nbf5}", flush=True)
+    b = synth.make_b(nbf, naux, 61, 0.05)
+    C = synth.make_C(nbf, 62)
+    H = synth.make_H(nbf, 63)
+    eng.upload_b(b)
+    full = eng.build(C, nbf5, H=H, precision=precision)
+    jk = eng.build(C, nbf5, precision=precision, outputs=("J", "K"))
+    assert np.array_equal(jk["J"], full["J"]) and np.array_equal(jk["K"], full["K"])
+    j = eng.build(C, nbf5, precision=precision, outputs=("J",))
+    k = eng.build(C, nbf5, precision=precision, outputs=("K",))
+    assert np.array_equal(j["J"], full["J"]) and np.array_equal(k["K"], full["K"])
+    if precision == "fp64":
+        Jr, Kr = O.jkj_ri(C, b, nbf5)
+        assert rel(jk["J"], Jr) <= TOL and rel(jk["K"], Kr) <= TOL
