@@ -267,8 +267,10 @@ int pnjk_energy(pnjk_engine* e, int64_t nbf5, int64_t nbeta, int64_t nalpha, int
  * and H. The launch plan (problem structs and tile tables on the device) is
  * kept across calls whose engines, outputs, sizes, precision and tiles are
  * unchanged; new C or H arrays reuse it. Results are bitwise those of
- * pnjk_jk_build on each engine alone. An engine must not be used elsewhere
- * while its group builds. */
+ * pnjk_jk_build on each engine alone. split_refine, and PNJK_FP64 without J and K
+ * when an instance has nbf5 < nbf, take pnjk_jk_build's triangular MO path, which is
+ * not grouped: those calls build the instances one after another on the stream, with
+ * no graph. An engine must not be used elsewhere while its group builds. */
 typedef struct pnjk_group pnjk_group;
 int pnjk_group_create(pnjk_engine* const* engines, int count, pnjk_group** out);
 int pnjk_group_destroy(pnjk_group* g);

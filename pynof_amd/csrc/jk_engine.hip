@@ -1697,10 +1697,15 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
   hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
   auto at = [](double* const* a, size_t i) { return a ? a[i] : nullptr; };
   int rc;
-  if (precision == PNJK_SPLIT_REFINE) {
-    // The refinement's FP64 MO path (a triangular half-transform, W and its reduction) is
-    // not grouped: each instance is built on its own, in order, on this stream (a captured
-    // graph is not used for it).
+  // The triangular FP64 MO path (a triangular half-transform, W and its reduction) is not
+  // grouped: split_refine, and FP64 builds without J and K when an instance has nbf5 < nbf
+  // (where pnjk_jk_build takes that path), build each instance on its own, in order, on
+  // this stream (a captured graph is not used for them), so every instance takes the path
+  // it takes alone.
+  bool per_instance = precision == PNJK_SPLIT_REFINE;
+  if (precision == PNJK_FP64 && !(outputs & (PNJK_OUT_J | PNJK_OUT_K)))
+    for (size_t i = 0; i < n; ++i) per_instance = per_instance || nbf5[i] < g->eng[i]->nbf;
+  if (per_instance) {
     for (size_t i = 0; i < n; ++i) {
       const BuildArgs a{C[i], ldc[i], nbf5[i], H ? H[i] : nullptr, ldh ? ldh[i] : 0, precision, outputs,
                         at(J, i), at(K, i), at(J_MO, i), at(K_MO, i), at(H_core, i)};

@@ -164,3 +164,32 @@ def test_group_replans_after_reupload_with_smaller_naux(gpu_required):
     g.close()
     for e in engs:
         e.close()
+
+
+SUBSETS = [("J", "K"), ("J_MO", "K_MO", "H_core"), ("K",), ("J", "J_MO"), ("K_MO",)]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "split", "split_refine"])
+def test_group_output_subsets_equal_alone(setup, prec):
+    """Grouped builds asking for part of the outputs (the J, K-only and MO-only routes
+    change each instance's quarter transform and stages) equal single builds asking for
+    the same outputs, bitwise; a group re-plans when the subset changes."""
+    import torch
+
+    engs, Cs, Hs, n5s = setup
+    g = JKGroup(engs, tuned={})
+    for sub in SUBSETS:
+        print(f"case: group vs alone {prec} outputs {sub}", flush=True)
+        hs = Hs if "H_core" in sub else None
+        ref = []
+        for e, c, h, n5 in zip(engs, Cs, Hs, n5s):
+            r = e.build(c, n5, H=h if hs is not None else None, precision=prec, outputs=sub)
+            torch.cuda.synchronize()
+            ref.append({k: v.cpu().numpy() for k, v in r.items()})
+        out = g.build(Cs, n5s, hs, precision=prec, outputs=sub)
+        torch.cuda.synchronize()
+        for i, (o, r) in enumerate(zip(out, ref)):
+            assert set(o) == set(sub), (prec, sub, i, sorted(o))
+            for k in sub:
+                assert np.array_equal(o[k].cpu().numpy(), r[k]), (prec, sub, i, k)
+    g.close()
