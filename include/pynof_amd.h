@@ -119,7 +119,8 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
  * partials, ...) lives in the library's static device array of 64 MiB per GPU (held until
  * the process ends). C, H and every output must be device memory; for the split forms the
  * split planes of b must exist already (the first pnjk_jk_build in a split form after an
- * upload makes them). A build whose workspace exceeds 64 MiB (FP64 form, nbf5 = nbf: about
+ * upload makes them), and so must the halved diagonal rows of b for an FP64 build without
+ * J and K at nbf5 < nbf (the first such pnjk_jk_build, or any split one, makes them). A build whose workspace exceeds 64 MiB (FP64 form, nbf5 = nbf: about
  * nbf = 120 with naux = 3.6 nbf) fails with PNJK_EINVAL and names the bytes it needs. All
  * no-allocation builds on one GPU share that array: calls that overlap in time on one GPU
  * (different streams, threads or engines) race; calls issued on one stream are ordered. */

@@ -907,9 +907,9 @@ static int jk_build_locked(pnjk_engine* e, const BuildArgs& a, hipStream_t s, ch
   };
   BuildState st;
   int rc;
-  // (the no-allocation build keeps the FP64 route of the whole half-transform: the halved
-  // diagonal rows of the triangular route are an engine buffer that it would have to allocate)
-  if ((rc = build_prepare(e, a, st, scratch, scratch_bytes, scratch == nullptr))) return rc;
+  // (the no-allocation build takes the same routes: its entry point checks that the halved
+  // diagonal rows the triangular route reads, part of the resident b, exist already)
+  if ((rc = build_prepare(e, a, st, scratch, scratch_bytes, true))) return rc;
   mark(ST_IN);
   if ((rc = stage_in(e, st, s))) return rc;
   const bool refine = st.a.precision == PNJK_SPLIT_REFINE;
@@ -1034,6 +1034,11 @@ int pnjk_jk_build_noalloc(pnjk_engine* e, const double* C, int64_t ldc, int64_t 
   if (precision != PNJK_FP64 && !e->split_ready)
     return set_error(PNJK_ESTATE, "pnjk_jk_build_noalloc: the split planes of b are built by the first split build "
                                   "through pnjk_jk_build (they are part of the resident b, not of the workspace)");
+  if (precision == PNJK_FP64 && !(outputs & (PNJK_OUT_J | PNJK_OUT_K)) && nbf5 >= 1 && nbf5 < e->nbf &&
+      !e->hd_ready)
+    return set_error(PNJK_ESTATE, "pnjk_jk_build_noalloc: an FP64 build without J and K at nbf5 < nbf takes the "
+                                  "triangular MO route, whose halved diagonal rows of b are built by the first such "
+                                  "build (or any split build) through pnjk_jk_build");
   if (!C || !is_device_ptr(C) || ((outputs & PNJK_OUT_HCORE) && (!H || !is_device_ptr(H))))
     return set_error(PNJK_EINVAL, "pnjk_jk_build_noalloc: C (and H for H_core) must be device memory");
   const BuildArgs a{C, ldc, nbf5, H, ldh, precision, outputs, J, K, J_MO, K_MO, H_core};

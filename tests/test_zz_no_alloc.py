@@ -75,3 +75,37 @@ def test_noalloc_rejects_oversized_and_host_outputs(gpu_required):
                                  Jh.ctypes.data, None, None, None, None, ctypes.c_void_p(s))
     assert rc == _lib.EINVAL and b"device memory" in L.pnjk_last_error()
     e.close()
+
+
+def test_noalloc_fp64_mo_only_takes_the_triangular_route(gpu_required):
+    """An FP64 build without J and K at nbf5 < nbf takes the triangular MO route through
+    both entry points: the no-allocation build refuses it until the halved diagonal rows of
+    b exist (the first such pnjk_jk_build makes them), then equals pnjk_jk_build bitwise, on
+    two calls in a row (the second on a smaller instance)."""
+    import torch
+
+    L = _lib.lib()
+    mo = _lib.OUT_JMO | _lib.OUT_KMO | _lib.OUT_HCORE
+    s = torch.cuda.current_stream().cuda_stream
+    for nbf, naux, n5, seed in [(96, 352, 60, 41), (48, 176, 30, 42)]:
+        print(f"case: noalloc fp64 MO-only nbf={nbf} naux={naux} nbf5={n5}", flush=True)
+        e = JKEngine(0)
+        e.upload_b(synth.make_b(nbf, naux, seed, 0.05))
+        C = torch.from_numpy(synth.make_C(nbf, seed + 100)).cuda()
+        H = torch.from_numpy(synth.make_H(nbf, seed + 200)).cuda()
+        ref, got = _outs(torch, nbf, n5), _outs(torch, nbf, n5)
+
+        def call(fn, o):
+            return fn(e._h, C.data_ptr(), C.shape[1], n5, H.data_ptr(), H.shape[1], _lib.FP64, mo, None, None,
+                      o["J_MO"].data_ptr(), o["K_MO"].data_ptr(), o["H_core"].data_ptr(), ctypes.c_void_p(s))
+
+        rc = call(L.pnjk_jk_build_noalloc, got)
+        assert rc == _lib.ESTATE and b"halved diagonal" in L.pnjk_last_error()
+        _lib.check(call(L.pnjk_jk_build, ref))
+        for v in got.values():
+            v.fill_(float("nan"))
+        _lib.check(call(L.pnjk_jk_build_noalloc, got))
+        torch.cuda.synchronize()
+        for k in ("J_MO", "K_MO", "H_core"):
+            assert torch.equal(ref[k], got[k]), (nbf, k)
+        e.close()
