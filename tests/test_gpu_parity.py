@@ -305,12 +305,15 @@ def test_no_orbitals_gives_empty_outputs(gpu_required):
     e.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "split"])
-def test_jk_without_mo_outputs(eng, precision):
+@pytest.mark.parametrize("nbf,naux,nbf5", [(150, 400, 120), (1, 1, 1), (5, 7, 3), (40, 144, 40), (161, 96, 161),
+                                          (352, 64, 337)])
+@pytest.mark.parametrize("precision", ["fp64", "split", "split_refine"])
+def test_jk_without_mo_outputs(eng, precision, nbf, naux, nbf5):
     """computeJKj's build (J and K, no J_MO or K_MO) runs the quarter transform for d alone
     (window row 0) and skips the K_MO reduction and J_MO: J and K keep the bits of the full
-    build's (d has the same MFMA sequence either way); J alone and K alone as well."""
-    nbf, naux, nbf5 = 150, 400, 120
+    build's (d has the same MFMA sequence either way); J alone and K alone as well. Shapes
+    from one orbital to three M-tiles of orbitals and a two-tile window; split_refine keeps
+    its FP64 d."""
     print(f"case: J, K without MO outputs {precision} nbf={nbf} naux={naux} nbf5={nbf5}", flush=True)
     b = synth.make_b(nbf, naux, 61, 0.05)
     C = synth.make_C(nbf, 62)
@@ -322,6 +325,6 @@ def test_jk_without_mo_outputs(eng, precision):
     j = eng.build(C, nbf5, precision=precision, outputs=("J",))
     k = eng.build(C, nbf5, precision=precision, outputs=("K",))
     assert np.array_equal(j["J"], full["J"]) and np.array_equal(k["K"], full["K"])
-    if precision == "fp64":
+    if precision == "fp64" and nbf <= 200:
         Jr, Kr = O.jkj_ri(C, b, nbf5)
         assert rel(jk["J"], Jr) <= TOL and rel(jk["K"], Kr) <= TOL
