@@ -296,7 +296,8 @@ uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int preci
   if (tri_fp64) s += (uint64_t)((nbf + 1) * naux_p) * 8;   // halved diagonal rows of b and a zero row
   if (precision != PNJK_FP64) {
     const int64_t ntri = nbf * (nbf + 1) / 2;
-    s += (uint64_t)(roundup(ntri, 16) * naux_p) * 4;         // btri16 hi/lo (whole 16-row blocks)
+    s += (uint64_t)((roundup(ntri, 16) + roundup(nbf + 1, 16)) * naux_p) * 4;  // bt16 hi/lo + halved diagonal rows
+    if (precision == PNJK_SPLIT && nbf5 < nbf) s += (uint64_t)(nbf5 * nbf5 * naux_p) * 4;  // FP32 W (triangular route)
     s += (uint64_t)(roundup(nbf, 16) * naux_p * nbf_p) * 4;  // bht16 hi/lo (rows nu < nbf rounded up to 16)
     s += (uint64_t)(roundup(nbf5, 16) * nbf_p) * 4;          // Ct16 hi/lo (whole 16-row blocks)
     s += (uint64_t)(nbf5 * x16_stride((int)nbf, (int)naux_p) + kX16Slack) * 4;  // X16 hi/lo (+ slack)
@@ -325,13 +326,18 @@ static int ensure_split_planes(pnjk_engine* e, hipStream_t s) {
   if (e->split_ready) return PNJK_OK;
   const int64_t ntri = e->ntri, naux_p = e->naux_p, nbf = e->nbf, ldk = roundup(nbf, 32);
   int rc;
-  // bt16 in whole 16-row blocks (blk_index)
-  const size_t tri_bytes = (size_t)(roundup(ntri, 16) * naux_p) * 2, ht_bytes = (size_t)(roundup(nbf, 16) * naux_p * ldk) * 2;
+  // bt16 in whole 16-row blocks (blk_index): the packed rows, then from row roundup(ntri, 16)
+  // the halved diagonal rows and a zero row (the split triangular route's L diagonal)
+  const int64_t hd = roundup(ntri, 16);
+  const size_t tri_bytes = (size_t)((hd + roundup(nbf + 1, 16)) * naux_p) * 2,
+               ht_bytes = (size_t)(roundup(nbf, 16) * naux_p * ldk) * 2;
   if ((rc = e->bt16h.ensure(tri_bytes)) || (rc = e->bt16l.ensure(tri_bytes))) return rc;
   if ((rc = e->bht16h.ensure(ht_bytes)) || (rc = e->bht16l.ensure(ht_bytes))) return rc;
   PNJK_HIP_CHECK(launch_split_blocked(e->btri.as<double>(), ntri, naux_p, e->bt16h.as<uint16_t>(),
                                       e->bt16l.as<uint16_t>(), s));
   if ((rc = ensure_half_diag(e, s))) return rc;
+  PNJK_HIP_CHECK(launch_split_blocked(e->bdh.as<double>(), nbf + 1, naux_p, e->bt16h.as<uint16_t>() + hd * naux_p,
+                                      e->bt16l.as<uint16_t>() + hd * naux_p, s));
   PNJK_HIP_CHECK(launch_build_bht(e->btri.as<double>(), (int)nbf, (int)naux_p, (int)ldk, e->bht16h.as<uint16_t>(),
                                   e->bht16l.as<uint16_t>(), s));
   e->split_ready = true;
@@ -561,6 +567,9 @@ struct BuildState {
   // J_MO, K_MO by the triangular route (T = L C, W = C^T T, Y_pq = W_pq + W_qp; the
   // refinement's FP64 MO path) instead of the half- and quarter transforms.
   bool tri = false;
+  // The split form's triangular MO route (builds without J and K at nbf5 < nbf): T16 = L C
+  // and W = C^T T16 in BF16x3, W in FP32, K_MO and d reduced from it.
+  bool tri16 = false;
   // Quarter-transform window rows: 0 for the whole window; 1 when neither J_MO nor K_MO is
   // requested and only d (window row 0, for J^n) is needed, with the bits of the whole
   // window's d.
@@ -586,7 +595,7 @@ struct WsPlan {
 // staging of unrequested J_MO/K_MO are carved from it and nothing is allocated: every
 // output must then be device memory (host outputs would need staging buffers).
 int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scratch = nullptr,
-                  size_t scratch_bytes = 0, bool fp64_tri = false) {
+                  size_t scratch_bytes = 0, bool tri_routes = false) {
   if (e->nbf == 0) return set_error(PNJK_ESTATE, "pnjk_jk_build: no b uploaded");
   if (!a.C) return set_error(PNJK_EINVAL, "pnjk_jk_build: C is NULL");
   if (a.nbf5 < 1 || a.nbf5 > e->nbf)
@@ -630,10 +639,12 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
   st.want_h = a.outputs & PNJK_OUT_HCORE;
   // The triangular route forms the MO integrals in nbf^2 nbf5 + 2 nbf nbf5^2 flops against
   // 2 nbf5 nbf^2 + nbf5 (nbf5 + 1) nbf for the half- and quarter transforms: fewer whenever
-  // nbf5 < nbf. split_refine always takes it; an FP64 build takes it when its caller allows
-  // it and asks for neither J nor K, which need the whole half-transformed X.
-  st.tri = precision == PNJK_SPLIT_REFINE ||
-           (fp64_tri && precision == PNJK_FP64 && !st.want_j && !st.want_k && n5 < nbf);
+  // nbf5 < nbf. split_refine always takes it; an FP64 or split build takes it (in FP64 or
+  // BF16x3) when its caller allows it and asks for neither J nor K, which need the whole
+  // half-transformed X.
+  const bool mo_only = tri_routes && !st.want_j && !st.want_k && n5 < nbf;
+  st.tri = precision == PNJK_SPLIT_REFINE || (mo_only && precision == PNJK_FP64);
+  st.tri16 = mo_only && precision == PNJK_SPLIT;
   st.qt_rows = (st.want_jmo || st.want_kmo) ? 0 : 1;
   const int64_t jk_elems = n5 * nbf * nbf;
   st.dJ = a.J;
@@ -669,8 +680,9 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
   }
   if (precision == PNJK_FP64 && !st.tri) w.bytes[W_CW] = (size_t)(2 * g.nbf_p * (int64_t)qt_window_cols(g.nbf5)) * 8;
   if (st.tri) w.bytes[W_W] = (size_t)(n5 * n5 * g.naux_p) * 8;
+  if (st.tri16) w.bytes[W_W] = (size_t)(n5 * n5 * g.naux_p) * 4;  // FP32 W
   w.bytes[W_D] = (size_t)(roundup(n5, 160) * g.naux_p) * 8;
-  if (!st.tri) w.bytes[W_KPART] = (size_t)(n5 * n5 * (g.naux_p / 64)) * 8;
+  if (!st.tri && !st.tri16) w.bytes[W_KPART] = (size_t)(n5 * n5 * (g.naux_p / 64)) * 8;
   if (st.want_h) {
     w.bytes[W_HP] = (size_t)(nbf * g.nbf_p) * 8;
     w.bytes[W_T] = (size_t)(nbf * g.ldcp) * 8;
@@ -916,7 +928,7 @@ static int jk_build_locked(pnjk_engine* e, const BuildArgs& a, hipStream_t s, ch
   // The forms other than split_refine run J_MO and H_core on the side stream beside J^n
   // (about 0.3 ms hidden per build at nbf=1000); J^n's mirror stays in line: beside K^n it
   // slowed K^n by more than it saved.
-  const bool side = !st.tri, k_side = refine && st.want_k;
+  const bool side = !st.tri && !st.tri16, k_side = refine && st.want_k;
   hipStream_t sa = (side || k_side) ? e->aux : s;
   bool forked = false;
   auto fork = [&]() -> int {
@@ -977,9 +989,8 @@ static int jk_build_locked(pnjk_engine* e, const BuildArgs& a, hipStream_t s, ch
     mark(ST_QT);
     PNJK_HIP_CHECK(launch_w_f64(g, g.nbf5 > 64 ? TileF64{5, 2} : TileF64{2, 2}, st.Cp, st.X, st.W, s));
     mark(ST_KMO);
-    PNJK_HIP_CHECK(launch_ymo_f64(g, st.W, st.dKMO, st.d, s));
-    mark(ST_JMO);
-    if ((r = stage_jmo(e, st, s, nullptr))) return r;
+    PNJK_HIP_CHECK(launch_ymo_f64(g, st.W, st.dKMO, st.d, st.want_jmo ? st.dJMO : nullptr, s));
+    mark(ST_JMO);  // J_MO came with K_MO
     if (st.want_j) {
       PNJK_NEED(st.d16h);
       PNJK_NEED(st.d16l);
@@ -992,7 +1003,33 @@ static int jk_build_locked(pnjk_engine* e, const BuildArgs& a, hipStream_t s, ch
     mark(ST_HC);
     return stage_hcore(e, st, s);
   };
-  rc = st.tri ? refine_body() : body();
+  // The split triangular route: T16 = L C (the HT stage's events), W = C^T T16 (QT), the
+  // K_MO and d reduction (KMO), J_MO and H_core in line.
+  auto tri16_body = [&]() -> int {
+    int r;
+    const BuildGeom& g = st.g;
+    PNJK_NEED(st.ct16h);
+    PNJK_NEED(st.ct16l);
+    PNJK_NEED(st.x16h);
+    PNJK_NEED(st.x16l);
+    PNJK_NEED(st.W);
+    PNJK_NEED(st.d);
+    PNJK_NEED(st.dKMO);
+    mark(ST_HT);
+    PNJK_HIP_CHECK(launch_tl_x3(st.sg, st.ct16h, st.ct16l, e->bt16h.as<uint16_t>(), e->bt16l.as<uint16_t>(),
+                                roundup(g.ntri, 16), st.x16h, st.x16l, s));
+    mark(ST_QT);
+    PNJK_HIP_CHECK(launch_ws_x3(st.sg, st.ct16h, st.ct16l, st.x16h, st.x16l, reinterpret_cast<float*>(st.W), s));
+    mark(ST_KMO);
+    PNJK_HIP_CHECK(
+        launch_ymo_f32(g, reinterpret_cast<const float*>(st.W), st.dKMO, st.d, st.want_jmo ? st.dJMO : nullptr, s));
+    mark(ST_JMO);  // J_MO came with K_MO
+    mark(ST_JN);
+    mark(ST_K);
+    mark(ST_HC);
+    return stage_hcore(e, st, s);
+  };
+  rc = st.tri ? refine_body() : st.tri16 ? tri16_body() : body();
   if (forked) {  // join the side stream
     const hipError_t j1 = hipEventRecord(e->join_ev, sa);
     const hipError_t j2 = j1 == hipSuccess ? hipStreamWaitEvent(s, e->join_ev, 0) : j1;
@@ -1702,13 +1739,13 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
   hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
   auto at = [](double* const* a, size_t i) { return a ? a[i] : nullptr; };
   int rc;
-  // The triangular FP64 MO path (a triangular half-transform, W and its reduction) is not
-  // grouped: split_refine, and FP64 builds without J and K when an instance has nbf5 < nbf
-  // (where pnjk_jk_build takes that path), build each instance on its own, in order, on
-  // this stream (a captured graph is not used for them), so every instance takes the path
-  // it takes alone.
+  // The triangular MO routes (a triangular half-transform, W and its reduction, in FP64 or
+  // BF16x3) are not grouped: split_refine, and FP64 or split builds without J and K when an
+  // instance has nbf5 < nbf (where pnjk_jk_build takes such a route), build each instance on
+  // its own, in order, on this stream (a captured graph is not used for them), so every
+  // instance takes the route it takes alone.
   bool per_instance = precision == PNJK_SPLIT_REFINE;
-  if (precision == PNJK_FP64 && !(outputs & (PNJK_OUT_J | PNJK_OUT_K)))
+  if (precision != PNJK_SPLIT_REFINE && !(outputs & (PNJK_OUT_J | PNJK_OUT_K)))
     for (size_t i = 0; i < n; ++i) per_instance = per_instance || nbf5[i] < g->eng[i]->nbf;
   if (per_instance) {
     for (size_t i = 0; i < n; ++i) {

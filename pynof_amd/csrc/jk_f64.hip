@@ -1419,34 +1419,55 @@ hipError_t launch_fill_cols(double* p, int rows, int64_t ld, int64_t c0, int64_t
 // Y_pq = W_pq + W_qp. One workgroup per pair p >= q (pair = p (p + 1) / 2 + q):
 // K_MO[p][q] = K_MO[q][p] = sum_{P < naux} Y_pq[P]^2, each thread summing P = tid + 256 j in
 // increasing j, then a fixed tree over the workgroup; on the diagonal also
-// d[p][P] = Y_pp[P] = 2 W_pp[P] for P < naux_p (zero past naux, as W is).
-__global__ void __launch_bounds__(256) k_ymo(const double* __restrict__ W, int n5, int naux, int naux_p,
-                                             double* __restrict__ kmo, double* __restrict__ d) {
-  __shared__ double red[256];
+// d[p][P] = Y_pp[P] = 2 W_pp[P] for P < naux_p (zero past naux, as W is). With jmo, also
+// J_MO[p][q] = J_MO[q][p] = sum_P d[p][P] d[q][P] in the same order (the rows W_pp, W_qq are
+// re-read from cache), so that diag(J_MO) = diag(K_MO) bit for bit.
+template <class TW>
+__global__ void __launch_bounds__(256) k_ymo(const TW* __restrict__ W, int n5, int naux, int naux_p,
+                                             double* __restrict__ kmo, double* __restrict__ d,
+                                             double* __restrict__ jmo) {
+  __shared__ double red[256], redj[256];
   const int pair = blockIdx.x;
   int p = (int)((sqrt(8.0 * pair + 1.0) - 1.0) * 0.5);
   while ((int64_t)(p + 1) * (p + 2) / 2 <= pair) ++p;
   while ((int64_t)p * (p + 1) / 2 > pair) --p;
   const int q = pair - p * (p + 1) / 2;
-  const double* a = W + ((int64_t)p * n5 + q) * naux_p;
-  const double* b = W + ((int64_t)q * n5 + p) * naux_p;
+  const TW* a = W + ((int64_t)p * n5 + q) * naux_p;
+  const TW* b = W + ((int64_t)q * n5 + p) * naux_p;
   double acc = 0.0;
   for (int P = threadIdx.x; P < naux; P += 256) {
-    const double y = a[P] + b[P];
+    const double y = (double)a[P] + (double)b[P];
     acc = fma(y, y, acc);
   }
   red[threadIdx.x] = acc;
+  if (jmo) {  // uniform over the grid
+    const TW* wp = W + ((int64_t)p * n5 + p) * naux_p;
+    const TW* wq = W + ((int64_t)q * n5 + q) * naux_p;
+    double accj = 0.0;
+    for (int P = threadIdx.x; P < naux; P += 256) {
+      const double dp = (double)wp[P] + (double)wp[P], dq = (double)wq[P] + (double)wq[P];
+      accj = fma(dp, dq, accj);
+    }
+    redj[threadIdx.x] = accj;
+  }
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    if ((int)threadIdx.x < w) {
+      red[threadIdx.x] += red[threadIdx.x + w];
+      if (jmo) redj[threadIdx.x] += redj[threadIdx.x + w];
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     kmo[(int64_t)p * n5 + q] = red[0];
     kmo[(int64_t)q * n5 + p] = red[0];
+    if (jmo) {
+      jmo[(int64_t)p * n5 + q] = redj[0];
+      jmo[(int64_t)q * n5 + p] = redj[0];
+    }
   }
   if (p == q)
-    for (int P = threadIdx.x; P < naux_p; P += 256) d[(int64_t)p * naux_p + P] = 2.0 * a[P];
+    for (int P = threadIdx.x; P < naux_p; P += 256) d[(int64_t)p * naux_p + P] = 2.0 * (double)a[P];
 }
 
 // dhalf[mu][P] = b[mu][mu][P] / 2 (the diagonal rows of the packed b, halved) and one zero
@@ -1576,9 +1597,17 @@ hipError_t launch_fock_f64(int nbf, int n5, const double* J, const double* K, in
   return launch_prob(pr, tc, (int)tiles, s);
 }
 
-hipError_t launch_ymo_f64(const BuildGeom& g, const double* W, double* kmo, double* d, hipStream_t s) {
+hipError_t launch_ymo_f64(const BuildGeom& g, const double* W, double* kmo, double* d, double* jmo, hipStream_t s) {
   const int64_t pairs = (int64_t)g.nbf5 * (g.nbf5 + 1) / 2;
-  hipLaunchKernelGGL(k_ymo, dim3((unsigned)pairs), dim3(256), 0, s, W, g.nbf5, g.naux, g.naux_p, kmo, d);
+  hipLaunchKernelGGL(k_ymo<double>, dim3((unsigned)pairs), dim3(256), 0, s, W, g.nbf5, g.naux, g.naux_p, kmo, d,
+                     jmo);
+  return hipGetLastError();
+}
+
+hipError_t launch_ymo_f32(const BuildGeom& g, const float* W, double* kmo, double* d, double* jmo, hipStream_t s) {
+  const int64_t pairs = (int64_t)g.nbf5 * (g.nbf5 + 1) / 2;
+  hipLaunchKernelGGL(k_ymo<float>, dim3((unsigned)pairs), dim3(256), 0, s, W, g.nbf5, g.naux, g.naux_p, kmo, d,
+                     jmo);
   return hipGetLastError();
 }
 

@@ -125,8 +125,11 @@ hipError_t launch_half_diag(const double* btri, int nbf, int naux_p, double* dha
 // W (nbf5 x nbf5 rows of naux_p): row (x, y) = W_xy = sum_mu C[mu][x] T[y][mu][P], i.e. C^T L C.
 hipError_t launch_w_f64(const BuildGeom& g, TileF64 tc, const double* Cp, const double* T, double* W,
                         hipStream_t s);
-// From W: K_MO (both triangles) = sum_{P < naux} (W_pq + W_qp)^2 and d[p][P] = 2 W_pp[P].
-hipError_t launch_ymo_f64(const BuildGeom& g, const double* W, double* kmo, double* d, hipStream_t s);
+// From W: K_MO (both triangles) = sum_{P < naux} (W_pq + W_qp)^2 and d[p][P] = 2 W_pp[P];
+// with jmo (else NULL) also J_MO = d d^T, in the same pass.
+hipError_t launch_ymo_f64(const BuildGeom& g, const double* W, double* kmo, double* d, double* jmo, hipStream_t s);
+// The same from the split route's FP32 W (sums in FP64).
+hipError_t launch_ymo_f32(const BuildGeom& g, const float* W, double* kmo, double* d, double* jmo, hipStream_t s);
 // Cw (2 x rows x ldw): Cw0[nu][j] = Cp[nu][j mod n], Cw1[nu][j] = Cp[nu][(j + 1) mod n].
 hipError_t launch_window_c(const double* Cp, int ldcp, int rows, int n, int ldw, double* Cw, hipStream_t s);
 
@@ -240,6 +243,13 @@ hipError_t launch_split_ct(const double* Cp, int ldcp, int nbf, int nbf5, int ro
 hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,
                         const uint16_t* bht_hi, const uint16_t* bht_lo, uint16_t* x_hi, uint16_t* x_lo,
                         hipStream_t s, GroupCollector* col = nullptr);
+// The split triangular MO route: T16[mu][q][P] = split(sum_{nu <= mu} Ct[q][nu] L[mu][nu][P]) from
+// bt16 with the split halved diagonal rows of b (and a zero row) stored from row hd on; then
+// W[p][(q, P)] = sum_mu Ct[p][mu] T16[mu][(q, P)] in FP32 (nbf5 x nbf5 naux_p).
+hipError_t launch_tl_x3(const SplitGeom& g, const uint16_t* ct_hi, const uint16_t* ct_lo, const uint16_t* bt_hi,
+                        const uint16_t* bt_lo, int64_t hd, uint16_t* t_hi, uint16_t* t_lo, hipStream_t s);
+hipError_t launch_ws_x3(const SplitGeom& g, const uint16_t* ct_hi, const uint16_t* ct_lo, const uint16_t* t_hi,
+                        const uint16_t* t_lo, float* W, hipStream_t s);
 // Quarter transform from X16: K_MO row-sum partials, d (FP64 and split planes).
 // BF16x3 orbital-gradient MO integrals from X16 and the split planes of C^T over
 // all nbf columns (rows clamped at nbf); FP64 output b_MO[p][q][P].

@@ -16,6 +16,7 @@
 // Contractions (reference lines as in the FP64 kernels): HT integrals.py:211,
 // QT :354 (-> d :212, K_MO :358), JN :214, K^n :216.
 #include <type_traits>
+#include <utility>
 
 #include "jk_common.h"
 #include "jk_kernels.h"
@@ -152,6 +153,14 @@ struct TrRead {
 // fetched into the other buffer while k-step kt computes; one barrier per k-step.
 // (Three stages in eight-wave workgroups, and in four-wave ones at one workgroup
 // per CU, measured no faster at nbf = 1000.)
+// Problems whose tiles have k-step counts of their own (Prob::ksteps(tile); ksteps() is then
+// the largest, for the grouped launches' cost).
+template <class P, class = void>
+struct HasTileKsteps : std::false_type {};
+template <class P>
+struct HasTileKsteps<P, std::void_t<decltype(std::declval<const P&>().ksteps(std::declval<const Tile&>()))>>
+    : std::true_type {};
+
 template <int FM, int FN, int WGM, int WGN, class Prob>
 __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, uint16_t* sA0, uint16_t* sA1,
                                              uint16_t* sB0, uint16_t* sB1) {
@@ -160,7 +169,11 @@ __device__ __forceinline__ void gemm_tile_x3(const Prob& prob, const Tile& t, ui
   constexpr bool kBkn = Prob::kB == KN;
   const auto as = prob.a_src(t);
   const auto bs = prob.b_src(t);
-  const int nk = prob.ksteps();
+  int nk;
+  if constexpr (HasTileKsteps<Prob>::value)
+    nk = prob.ksteps(t);
+  else
+    nk = prob.ksteps();
 
   // The wave index is uniform over the wave; readfirstlane makes that visible to
   // the compiler, so per-wave offsets stay scalar.
@@ -481,6 +494,124 @@ struct ProbHT {
             gmem(xhi)[(int64_t)q * N + n] = h;
             gmem(xlo)[(int64_t)q * N + n] = bf16_rne(v - bf16_to_f(h));
           }
+        }
+  }
+};
+
+// The split triangular MO route (builds without J and K at nbf5 < nbf), as the FP64 one:
+// b = L + L^T with L[mu][nu] = b[mu][nu] for nu < mu and b[mu][mu] / 2 on the diagonal, so
+// Y_pq = W_pq + W_qp with W = C^T T and T = L C.
+//
+// T16[mu][q][P] = split(sum_{nu <= mu} Ct[q][nu] L[mu][nu][P]): batch mu is a GEMM with
+// K = mu + 1. Its B rows (k = nu, KN: P contiguous) are the packed rows tri(mu, nu), nu < mu,
+// of bt16, then row mu of the halved diagonal rows stored after the packed rows (row
+// hd + mu), then their zero row (hd + nbf) for nu > mu in the last k-step. Columns past
+// naux_p (a partial last tile) read the row's last 8 and are not stored.
+struct SrcTriL {
+  Planes p;
+  int64_t kb, base, hd;  // base = tri(mu, 0)
+  int mu, nbf, naux_p;
+  __device__ __forceinline__ int64_t at(int k, int n) const {
+    const int64_t row = k < mu ? base + k : (k == mu ? hd + mu : hd + nbf);
+    return blk_index(row, n < naux_p ? n : naux_p - 8, kb);
+  }
+};
+
+// Batches are dealt so that every XCD's contiguous range of workgroups holds short and long
+// ones alike: batch b is mu = b / 2 for even b and nbf - 1 - b / 2 for odd b.
+struct ProbTL {
+  static constexpr int kA = RK, kB = KN;
+  static constexpr bool kRowSkip = true;  // rows q >= nbf5
+  static constexpr bool kLower = false;
+  Planes ct, bt;
+  uint16_t* thi;
+  uint16_t* tlo;
+  int nbf, nbf5, ldk, naux_p, mtiles, ntiles;
+  int64_t kb, hd;
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    const int per = mtiles * ntiles, b = id / per, r = id - b * per;
+    t.batch = (b & 1) ? nbf - 1 - (b >> 1) : (b >> 1);
+    t.mt = r % mtiles;
+    t.nt = r / mtiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __host__ __device__ int ksteps() const { return ldk / kBK; }
+  __device__ int ksteps(const Tile& t) const { return t.batch / kBK + 1; }
+  __device__ int row_limit(const Tile&) const { return nbf5; }
+  __device__ SrcBlocked a_src(const Tile&) const { return {ct, ldk / kBK, nbf5}; }
+  __device__ SrcTriL b_src(const Tile& t) const {
+    return {bt, kb, (int64_t)t.batch * (t.batch + 1) / 2, hd, t.batch, nbf, naux_p};
+  }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+    const int64_t row0 = (int64_t)t.batch * nbf5;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int q = t.m0 + dm, P = t.n0 + dn;
+          if (q < nbf5 && P < naux_p) {
+            const float v = acc[i][j][r];
+            const uint16_t h = bf16_rne(v);
+            const int64_t o = (row0 + q) * naux_p + P;
+            gmem(thi)[o] = h;
+            gmem(tlo)[o] = bf16_rne(v - bf16_to_f(h));
+          }
+        }
+  }
+};
+
+// W[p][n] = sum_mu Ct[p][mu] T16[mu][n] over the N = nbf5 naux_p columns n = (q, P) (FP32
+// output). T16 rows mu >= nbf (the zero tail of Ct's k range) read row nbf - 1; columns past N
+// (a partial last tile) read the row's last 8 and are not stored.
+struct SrcT16 {
+  Planes p;
+  int64_t N;
+  int nbf;
+  __device__ __forceinline__ int64_t at(int k, int64_t n) const {
+    return (int64_t)(k < nbf ? k : nbf - 1) * N + (n < N ? n : N - 8);
+  }
+};
+
+struct ProbWS {
+  static constexpr int kA = RK, kB = KN;
+  static constexpr bool kRowSkip = true;  // rows p >= nbf5
+  static constexpr bool kLower = false;
+  Planes ct, tt;
+  float* W;
+  int nbf, nbf5, ldk, mtiles;
+  int64_t N;
+  __host__ __device__ bool tile(int id, int BM, int BN, Tile& t) const {
+    t.batch = 0;
+    t.mt = id % mtiles;
+    t.nt = id / mtiles;
+    t.m0 = t.mt * BM;
+    t.n0 = t.nt * BN;
+    return true;
+  }
+  __host__ __device__ int ksteps() const { return ldk / kBK; }
+  __device__ int row_limit(const Tile&) const { return nbf5; }
+  __device__ SrcBlocked a_src(const Tile&) const { return {ct, ldk / kBK, nbf5}; }
+  __device__ SrcT16 b_src(const Tile&) const { return {tt, N, nbf}; }
+  template <int FM, int FN>
+  __device__ void epilogue(const Tile& t, v4f (&acc)[FM][FN], int wm, int wn, int lane, double*) const {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int dm, dn;
+          coords<FM, FN>(lane, wm, wn, i, j, r, dm, dn);
+          const int p = t.m0 + dm;
+          const int64_t n = (int64_t)t.n0 + dn;
+          if (p < nbf5 && n < N) gmem(W)[(int64_t)p * N + n] = acc[i][j][r];
         }
   }
 };
@@ -1037,6 +1168,47 @@ hipError_t launch_ht_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, c
   pr.m_base = 0;
   pr.mtiles = cdiv(g.nbf5, BM);
   return launch_x3(pr, tc, pr.mtiles * cdiv(pr.N, BN), s, col);
+}
+
+hipError_t launch_tl_x3(const SplitGeom& g, const uint16_t* ct_hi, const uint16_t* ct_lo, const uint16_t* bt_hi,
+                        const uint16_t* bt_lo, int64_t hd, uint16_t* t_hi, uint16_t* t_lo, hipStream_t s) {
+  static_assert(HasTileKsteps<ProbTL>::value, "batch mu runs mu / 32 + 1 k-steps");
+  static_assert(!HasTileKsteps<ProbWS>::value && !HasTileKsteps<ProbHT>::value, "one k-step count per launch");
+  if (g.naux_p % 32 != 0 || g.naux_p < 32) return hipErrorInvalidValue;
+  const TileF64 tc{5, 4};  // 160 x 128 (KN tiles are 64 or 128 columns wide)
+  ProbTL pr;
+  pr.ct = {ct_hi, ct_lo};
+  pr.bt = {bt_hi, bt_lo};
+  pr.thi = t_hi;
+  pr.tlo = t_lo;
+  pr.nbf = g.nbf;
+  pr.nbf5 = g.nbf5;
+  pr.ldk = g.ldk;
+  pr.naux_p = g.naux_p;
+  pr.mtiles = cdiv(g.nbf5, 32 * tc.fm);
+  pr.ntiles = cdiv(g.naux_p, 32 * tc.fn);
+  pr.kb = g.naux_p / 32;
+  pr.hd = hd;
+  const int64_t blocks = (int64_t)g.nbf * pr.mtiles * pr.ntiles;
+  if (blocks > INT32_MAX) return hipErrorInvalidValue;
+  return launch_x3(pr, tc, (int)blocks, s);
+}
+
+hipError_t launch_ws_x3(const SplitGeom& g, const uint16_t* ct_hi, const uint16_t* ct_lo, const uint16_t* t_hi,
+                        const uint16_t* t_lo, float* W, hipStream_t s) {
+  const TileF64 tc{5, 4};
+  ProbWS pr;
+  pr.ct = {ct_hi, ct_lo};
+  pr.tt = {t_hi, t_lo};
+  pr.W = W;
+  pr.nbf = g.nbf;
+  pr.nbf5 = g.nbf5;
+  pr.ldk = g.ldk;
+  pr.N = (int64_t)g.nbf5 * g.naux_p;
+  pr.mtiles = cdiv(g.nbf5, 32 * tc.fm);
+  const int64_t blocks = (int64_t)pr.mtiles * cdiv(pr.N, 32 * tc.fn);
+  if (pr.N + 32 * tc.fn > INT32_MAX || blocks > INT32_MAX) return hipErrorInvalidValue;
+  return launch_x3(pr, tc, (int)blocks, s);
 }
 
 hipError_t launch_qt_x3(const SplitGeom& g, TileF64 tc, const uint16_t* ct_hi, const uint16_t* ct_lo,

@@ -269,6 +269,8 @@ def test_fp64_mo_only_builds(eng, nbf, naux, nbf5):
     if nbf5 == nbf:
         for k in ("J_MO", "K_MO"):
             assert np.array_equal(mo[k], full[k]), k
+    else:  # the route's reduction forms both diagonals in one order: d_p . d_p = sum Y_pp^2
+        assert np.array_equal(np.diag(mo["J_MO"]), np.diag(mo["K_MO"]))
     # device inputs and outputs give the same bits as host ones
     dev = eng.build(torch.from_numpy(C).cuda(), nbf5, H=torch.from_numpy(H).cuda(),
                     outputs=("J_MO", "K_MO", "H_core"))

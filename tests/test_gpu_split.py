@@ -96,3 +96,38 @@ def test_split_is_deterministic(eng):
     c = eng.build(C, 60, precision="split", outputs=("J", "K", "J_MO", "K_MO"))
     for k in a:
         assert np.array_equal(a[k], c[k]), k
+
+
+@pytest.mark.parametrize("nbf,naux,nbf5", [(2, 3, 1), (5, 7, 3), (40, 100, 39), (60, 200, 40), (97, 520, 33),
+                                          (130, 400, 70), (352, 64, 337), (40, 144, 40)])
+def test_split_mo_only_builds(eng, nbf, naux, nbf5):
+    """A split build asked for J_MO, K_MO and H_core only takes the BF16x3 triangular route
+    when nbf5 < nbf (T16 = L C per mu with K = mu + 1, W = C^T T16 in FP32, K_MO and d reduced
+    in FP64): J_MO, K_MO against the oracle at the split bar and within it of the full split
+    build's, H_core bitwise the full build's; nbf5 = nbf keeps the full route (bitwise); host
+    and device inputs give the same bits."""
+    import torch
+
+    print(f"case: split MO-only nbf={nbf} naux={naux} nbf5={nbf5}", flush=True)
+    b = synth.make_b(nbf, naux, 71, 0.05)
+    C = synth.make_C(nbf, 72)
+    H = synth.make_H(nbf, 73)
+    eng.upload_b(b)
+    full = eng.build(C, nbf5, H=H, precision="split")
+    mo = eng.build(C, nbf5, H=H, precision="split", outputs=("J_MO", "K_MO", "H_core"))
+    ref = dict(zip(NAMES, O.jk_build(C, H, b, nbf5)))
+    for k in ("J_MO", "K_MO"):
+        assert rel(mo[k], ref[k]) <= SPLIT_TOL, (k, rel(mo[k], ref[k]))
+        assert rel(mo[k], full[k]) <= SPLIT_TOL, (k, rel(mo[k], full[k]))
+        assert np.array_equal(mo[k], mo[k].T), k
+    assert np.array_equal(mo["H_core"], full["H_core"])
+    if nbf5 == nbf:
+        for k in ("J_MO", "K_MO"):
+            assert np.array_equal(mo[k], full[k]), k
+    else:  # the route's reduction forms both diagonals in one order: d_p . d_p = sum Y_pp^2
+        assert np.array_equal(np.diag(mo["J_MO"]), np.diag(mo["K_MO"]))
+    dev = eng.build(torch.from_numpy(C).cuda(), nbf5, H=torch.from_numpy(H).cuda(), precision="split",
+                    outputs=("J_MO", "K_MO", "H_core"))
+    torch.cuda.synchronize()
+    for k in mo:
+        assert np.array_equal(dev[k].cpu().numpy(), mo[k]), k

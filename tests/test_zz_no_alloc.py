@@ -81,7 +81,8 @@ def test_noalloc_fp64_mo_only_takes_the_triangular_route(gpu_required):
     """An FP64 build without J and K at nbf5 < nbf takes the triangular MO route through
     both entry points: the no-allocation build refuses it until the halved diagonal rows of
     b exist (the first such pnjk_jk_build makes them), then equals pnjk_jk_build bitwise, on
-    two calls in a row (the second on a smaller instance)."""
+    two calls in a row (the second on a smaller instance). The split form's triangular route
+    likewise, once the split planes exist."""
     import torch
 
     L = _lib.lib()
@@ -108,4 +109,17 @@ def test_noalloc_fp64_mo_only_takes_the_triangular_route(gpu_required):
         torch.cuda.synchronize()
         for k in ("J_MO", "K_MO", "H_core"):
             assert torch.equal(ref[k], got[k]), (nbf, k)
+
+        def call16(fn, o):
+            return fn(e._h, C.data_ptr(), C.shape[1], n5, H.data_ptr(), H.shape[1], _lib.SPLIT, mo, None, None,
+                      o["J_MO"].data_ptr(), o["K_MO"].data_ptr(), o["H_core"].data_ptr(), ctypes.c_void_p(s))
+
+        print(f"case: noalloc split MO-only nbf={nbf} naux={naux} nbf5={n5}", flush=True)
+        _lib.check(call16(L.pnjk_jk_build, ref))  # also makes the split planes
+        for v in got.values():
+            v.fill_(float("nan"))
+        _lib.check(call16(L.pnjk_jk_build_noalloc, got))
+        torch.cuda.synchronize()
+        for k in ("J_MO", "K_MO", "H_core"):
+            assert torch.equal(ref[k], got[k]), ("split", nbf, k)
         e.close()
