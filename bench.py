@@ -336,9 +336,11 @@ def main():
         e1.record()
         torch.cuda.synchronize()
         mo_only[form] = {"ms": round(e0.elapsed_time(e1) / reps, 3)}
-    mo_only["fp64"]["route"] = "triangular (T = L C, W = C^T T)" if nbf5 < nbf else "half + quarter transform"
-    fst = forms["fp64"]["stage_ms"]
-    mo_only["fp64"]["full_build_ht_plus_qt_ms"] = round(fst["half_transform"] + fst["quarter_transform"], 3)
+    for form, arith in (("fp64", "FP64"), ("split", "BF16x3")):
+        mo_only[form]["route"] = (f"triangular in {arith} (T = L C, W = C^T T)" if nbf5 < nbf
+                                  else "half + quarter transform")
+        fst = forms[form]["stage_ms"]
+        mo_only[form]["full_build_ht_plus_qt_ms"] = round(fst["half_transform"] + fst["quarter_transform"], 3)
     # ---- computeJKj's build (J and K only; ENERGY1r with no1 > 0, utils.py:421): the quarter
     # transform forms d alone (window row 0) and J_MO, K_MO are skipped
     log("J, K-only builds (computeJKj)")
