@@ -103,6 +103,10 @@ int pnjk_upload_b_packed(pnjk_engine* e, const double* btri_dev, int64_t nbf, in
  *   J_MO     (nbf5, nbf5)                          integrals.py:356
  *   K_MO     (nbf5, nbf5)                          integrals.py:358
  *   H_core   (nbf5)                                integrals.py:360
+ * A build that asks for neither J nor K forms J_MO and K_MO by the triangular route
+ * b = L + L^T (T = L C, W = C^T T, Y_pq = W_pq + W_qp) when that is the faster route:
+ * nbf5 <= 2 nbf / 3 in PNJK_FP64 and nbf5 <= 2 nbf / 5 in PNJK_SPLIT (split_refine always
+ * takes it). Its J_MO and K_MO differ from the full route's in rounding only.
  * Output pointers may be host or device memory. `stream` is a hipStream_t
  * (NULL: the null stream, which is also torch's default stream). The call
  * returns after the outputs are complete when any output is host memory;
@@ -119,8 +123,9 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
  * partials, ...) lives in the library's static device array of 64 MiB per GPU (held until
  * the process ends). C, H and every output must be device memory; for the split forms the
  * split planes of b must exist already (the first pnjk_jk_build in a split form after an
- * upload makes them), and so must the halved diagonal rows of b for an FP64 build without
- * J and K at nbf5 < nbf (the first such pnjk_jk_build, or any split one, makes them). A build whose workspace exceeds 64 MiB (FP64 form, nbf5 = nbf: about
+ * upload makes them), and so must the halved diagonal rows of b for an FP64 build that
+ * takes the triangular MO route (the first such pnjk_jk_build, or any split one, makes
+ * them). A build whose workspace exceeds 64 MiB (FP64 form, nbf5 = nbf: about
  * nbf = 120 with naux = 3.6 nbf) fails with PNJK_EINVAL and names the bytes it needs. All
  * no-allocation builds on one GPU share that array: calls that overlap in time on one GPU
  * (different streams, threads or engines) race; calls issued on one stream are ordered. */
@@ -268,10 +273,9 @@ int pnjk_energy(pnjk_engine* e, int64_t nbf5, int64_t nbeta, int64_t nalpha, int
  * and H. The launch plan (problem structs and tile tables on the device) is
  * kept across calls whose engines, outputs, sizes, precision and tiles are
  * unchanged; new C or H arrays reuse it. Results are bitwise those of
- * pnjk_jk_build on each engine alone. split_refine, and PNJK_FP64 without J and K
- * when an instance has nbf5 < nbf, take pnjk_jk_build's triangular MO path, which is
- * not grouped: those calls build the instances one after another on the stream, with
- * no graph. An engine must not be used elsewhere while its group builds. */
+ * pnjk_jk_build on each engine alone. split_refine, and builds without J and K in which
+ * an instance takes pnjk_jk_build's triangular MO route, are not grouped: those calls
+ * build the instances one after another on the stream, with no graph. An engine must not be used elsewhere while its group builds. */
 typedef struct pnjk_group pnjk_group;
 int pnjk_group_create(pnjk_engine* const* engines, int count, pnjk_group** out);
 int pnjk_group_destroy(pnjk_group* g);

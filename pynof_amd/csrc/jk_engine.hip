@@ -278,6 +278,17 @@ uint64_t pnjk_footprint(const pnjk_engine* e) {
   return s;
 }
 
+// Whether a build without J and K forms its MO integrals by the triangular route (T = L C,
+// W = C^T T). It needs fewer flops whenever nbf5 < nbf (nbf^2 nbf5 + 2 nbf nbf5^2 against
+// 2 nbf5 nbf^2 + nbf5^2 nbf), but its triangular GEMM (batches of K = mu + 1) runs below the
+// half-transform's rate, so it is the faster route only up to about nbf5 / nbf = 2/3 in FP64
+// and 2/5 in BF16x3 (measured over nbf = 200..1000; beyond, up to 1.4x slower).
+static bool mo_tri_route(int precision, int64_t nbf, int64_t nbf5) {
+  if (precision == PNJK_FP64) return 3 * nbf5 <= 2 * nbf;
+  if (precision == PNJK_SPLIT) return 5 * nbf5 <= 2 * nbf;
+  return false;
+}
+
 uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int precision) {
   const int64_t naux_p = pnjk_packed_ld(nbf, naux);
   const int64_t nbf_p = roundup(nbf, 32), ldcp = roundup(nbf5, 32);
@@ -289,15 +300,15 @@ uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int preci
   if (precision != PNJK_SPLIT_REFINE) s += (uint64_t)(nbf5 * nbf5 * (naux_p / 64)) * 8;  // kpart
   s += (uint64_t)(nbf * ldcp) * 8;                           // T
   if (precision == PNJK_FP64) s += (uint64_t)(2 * nbf_p * qt_window_cols((int)nbf5)) * 8;  // Cw
-  // W: split_refine, and FP64 builds without J and K when nbf5 < nbf (the triangular route;
-  // this total bounds both routes of an FP64 build)
-  const bool tri_fp64 = precision == PNJK_FP64 && nbf5 < nbf;
+  // W: split_refine, and FP64 builds without J and K that take the triangular route (this
+  // total bounds both routes of an FP64 build)
+  const bool tri_fp64 = precision == PNJK_FP64 && mo_tri_route(precision, nbf, nbf5);
   if (precision == PNJK_SPLIT_REFINE || tri_fp64) s += (uint64_t)(nbf5 * nbf5 * naux_p) * 8;
   if (tri_fp64) s += (uint64_t)((nbf + 1) * naux_p) * 8;   // halved diagonal rows of b and a zero row
   if (precision != PNJK_FP64) {
     const int64_t ntri = nbf * (nbf + 1) / 2;
     s += (uint64_t)((roundup(ntri, 16) + roundup(nbf + 1, 16)) * naux_p) * 4;  // bt16 hi/lo + halved diagonal rows
-    if (precision == PNJK_SPLIT && nbf5 < nbf) s += (uint64_t)(nbf5 * nbf5 * naux_p) * 4;  // FP32 W (triangular route)
+    if (mo_tri_route(precision, nbf, nbf5)) s += (uint64_t)(nbf5 * nbf5 * naux_p) * 4;  // FP32 W (triangular route)
     s += (uint64_t)(roundup(nbf, 16) * naux_p * nbf_p) * 4;  // bht16 hi/lo (rows nu < nbf rounded up to 16)
     s += (uint64_t)(roundup(nbf5, 16) * nbf_p) * 4;          // Ct16 hi/lo (whole 16-row blocks)
     s += (uint64_t)(nbf5 * x16_stride((int)nbf, (int)naux_p) + kX16Slack) * 4;  // X16 hi/lo (+ slack)
@@ -567,7 +578,7 @@ struct BuildState {
   // J_MO, K_MO by the triangular route (T = L C, W = C^T T, Y_pq = W_pq + W_qp; the
   // refinement's FP64 MO path) instead of the half- and quarter transforms.
   bool tri = false;
-  // The split form's triangular MO route (builds without J and K at nbf5 < nbf): T16 = L C
+  // The split form's triangular MO route (builds without J and K, mo_tri_route): T16 = L C
   // and W = C^T T16 in BF16x3, W in FP32, K_MO and d reduced from it.
   bool tri16 = false;
   // Quarter-transform window rows: 0 for the whole window; 1 when neither J_MO nor K_MO is
@@ -637,12 +648,10 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
   st.want_jmo = a.outputs & PNJK_OUT_JMO;
   st.want_kmo = a.outputs & PNJK_OUT_KMO;
   st.want_h = a.outputs & PNJK_OUT_HCORE;
-  // The triangular route forms the MO integrals in nbf^2 nbf5 + 2 nbf nbf5^2 flops against
-  // 2 nbf5 nbf^2 + nbf5 (nbf5 + 1) nbf for the half- and quarter transforms: fewer whenever
-  // nbf5 < nbf. split_refine always takes it; an FP64 or split build takes it (in FP64 or
-  // BF16x3) when its caller allows it and asks for neither J nor K, which need the whole
-  // half-transformed X.
-  const bool mo_only = tri_routes && !st.want_j && !st.want_k && n5 < nbf;
+  // The triangular route (mo_tri_route): split_refine always takes it; an FP64 or split build
+  // takes it (in FP64 or BF16x3) when its caller allows it, asks for neither J nor K (which
+  // need the whole half-transformed X) and nbf5 / nbf is small enough for it to be faster.
+  const bool mo_only = tri_routes && !st.want_j && !st.want_k && mo_tri_route(precision, nbf, n5);
   st.tri = precision == PNJK_SPLIT_REFINE || (mo_only && precision == PNJK_FP64);
   st.tri16 = mo_only && precision == PNJK_SPLIT;
   st.qt_rows = (st.want_jmo || st.want_kmo) ? 0 : 1;
@@ -1071,11 +1080,11 @@ int pnjk_jk_build_noalloc(pnjk_engine* e, const double* C, int64_t ldc, int64_t 
   if (precision != PNJK_FP64 && !e->split_ready)
     return set_error(PNJK_ESTATE, "pnjk_jk_build_noalloc: the split planes of b are built by the first split build "
                                   "through pnjk_jk_build (they are part of the resident b, not of the workspace)");
-  if (precision == PNJK_FP64 && !(outputs & (PNJK_OUT_J | PNJK_OUT_K)) && nbf5 >= 1 && nbf5 < e->nbf &&
-      !e->hd_ready)
-    return set_error(PNJK_ESTATE, "pnjk_jk_build_noalloc: an FP64 build without J and K at nbf5 < nbf takes the "
-                                  "triangular MO route, whose halved diagonal rows of b are built by the first such "
-                                  "build (or any split build) through pnjk_jk_build");
+  if (precision == PNJK_FP64 && !(outputs & (PNJK_OUT_J | PNJK_OUT_K)) && nbf5 >= 1 &&
+      mo_tri_route(precision, e->nbf, nbf5) && !e->hd_ready)
+    return set_error(PNJK_ESTATE, "pnjk_jk_build_noalloc: an FP64 build without J and K at nbf5 <= 2 nbf / 3 takes "
+                                  "the triangular MO route, whose halved diagonal rows of b are built by the first "
+                                  "such build (or any split build) through pnjk_jk_build");
   if (!C || !is_device_ptr(C) || ((outputs & PNJK_OUT_HCORE) && (!H || !is_device_ptr(H))))
     return set_error(PNJK_EINVAL, "pnjk_jk_build_noalloc: C (and H for H_core) must be device memory");
   const BuildArgs a{C, ldc, nbf5, H, ldh, precision, outputs, J, K, J_MO, K_MO, H_core};
@@ -1741,12 +1750,12 @@ int pnjk_group_build(pnjk_group* g, const double* const* C, const int64_t* ldc, 
   int rc;
   // The triangular MO routes (a triangular half-transform, W and its reduction, in FP64 or
   // BF16x3) are not grouped: split_refine, and FP64 or split builds without J and K when an
-  // instance has nbf5 < nbf (where pnjk_jk_build takes such a route), build each instance on
-  // its own, in order, on this stream (a captured graph is not used for them), so every
-  // instance takes the route it takes alone.
+  // instance takes such a route alone (mo_tri_route), build each instance on its own, in
+  // order, on this stream (a captured graph is not used for them), so every instance takes
+  // the route it takes alone.
   bool per_instance = precision == PNJK_SPLIT_REFINE;
   if (precision != PNJK_SPLIT_REFINE && !(outputs & (PNJK_OUT_J | PNJK_OUT_K)))
-    for (size_t i = 0; i < n; ++i) per_instance = per_instance || nbf5[i] < g->eng[i]->nbf;
+    for (size_t i = 0; i < n; ++i) per_instance = per_instance || mo_tri_route(precision, g->eng[i]->nbf, nbf5[i]);
   if (per_instance) {
     for (size_t i = 0; i < n; ++i) {
       const BuildArgs a{C[i], ldc[i], nbf5[i], H ? H[i] : nullptr, ldh ? ldh[i] : 0, precision, outputs,

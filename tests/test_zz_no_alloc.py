@@ -78,7 +78,7 @@ def test_noalloc_rejects_oversized_and_host_outputs(gpu_required):
 
 
 def test_noalloc_fp64_mo_only_takes_the_triangular_route(gpu_required):
-    """An FP64 build without J and K at nbf5 < nbf takes the triangular MO route through
+    """An FP64 build without J and K at nbf5 <= 2 nbf / 3 takes the triangular MO route through
     both entry points: the no-allocation build refuses it until the halved diagonal rows of
     b exist (the first such pnjk_jk_build makes them), then equals pnjk_jk_build bitwise, on
     two calls in a row (the second on a smaller instance). The split form's triangular route
@@ -88,7 +88,7 @@ def test_noalloc_fp64_mo_only_takes_the_triangular_route(gpu_required):
     L = _lib.lib()
     mo = _lib.OUT_JMO | _lib.OUT_KMO | _lib.OUT_HCORE
     s = torch.cuda.current_stream().cuda_stream
-    for nbf, naux, n5, seed in [(96, 352, 60, 41), (48, 176, 30, 42)]:
+    for nbf, naux, n5, seed in [(96, 352, 36, 41), (48, 176, 18, 42)]:  # nbf5 <= 2 nbf / 5: both forms' route
         print(f"case: noalloc fp64 MO-only nbf={nbf} naux={naux} nbf5={n5}", flush=True)
         e = JKEngine(0)
         e.upload_b(synth.make_b(nbf, naux, seed, 0.05))
