@@ -105,8 +105,11 @@ int pnjk_upload_b_packed(pnjk_engine* e, const double* btri_dev, int64_t nbf, in
  *   H_core   (nbf5)                                integrals.py:360
  * A build that asks for neither J nor K forms J_MO and K_MO by the triangular route
  * b = L + L^T (T = L C, W = C^T T, Y_pq = W_pq + W_qp) when that is the faster route:
- * nbf5 <= 2 nbf / 3 in PNJK_FP64 and nbf5 <= 2 nbf / 5 in PNJK_SPLIT (split_refine always
- * takes it). Its J_MO and K_MO differ from the full route's in rounding only.
+ * nbf5 <= 2 nbf / 3 in PNJK_FP64 and nbf5 <= 2 nbf / 5 in PNJK_SPLIT. Its J_MO and K_MO
+ * differ from the full route's in rounding only. PNJK_SPLIT_REFINE forms its FP64 J_MO and
+ * K_MO by that route up to nbf5 <= 2 nbf / 3 (beside a split half-transform for K^n), and
+ * past it by the FP64 form's half- and quarter transforms (bitwise the FP64 form's J_MO and
+ * K_MO), splitting their FP64 X for K^n.
  * Output pointers may be host or device memory. `stream` is a hipStream_t
  * (NULL: the null stream, which is also torch's default stream). The call
  * returns after the outputs are complete when any output is host memory;

@@ -9,9 +9,10 @@ therefore free to differ between batch mixes without changing results.
 
 Tables are JSON: {"<precision>:<nbf>:<naux>:<nbf5>": {"<stage>": [fm, fn], ...}}.
 The "fp64" entry tunes the FP64 kernels, the "split" entry the BF16x3 kernels
-(the split_refine form uses the FP64 half-transform choice for its triangular
-FP64 half-transform, a fixed tile for its FP64 MO transform, and the split
-choices for its BF16x3 half-transform, J^n and K^n).
+(the split_refine form uses the FP64 half-transform choice for its FP64
+half-transform, triangular or full, a fixed tile for its FP64 MO transform or the
+FP64 quarter-transform choice, and the split choices for its BF16x3
+half-transform, J^n and K^n).
 ``lookup`` returns the entry of the nearest tuned shape (log-distance in nbf,
 naux, nbf5), so instances of untuned sizes get the choice of their nearest
 tuned neighbour.

@@ -297,9 +297,9 @@ uint64_t pnjk_workspace_bytes(int64_t nbf, int64_t naux, int64_t nbf5, int preci
   s += (uint64_t)(nbf * nbf_p) * 8;                          // Hp
   s += (uint64_t)(nbf5 * nbf * naux_p + 64) * 8;             // X (+ slack)
   s += (uint64_t)(roundup(nbf5, 160) * naux_p) * 8;          // d
-  if (precision != PNJK_SPLIT_REFINE) s += (uint64_t)(nbf5 * nbf5 * (naux_p / 64)) * 8;  // kpart
+  s += (uint64_t)(nbf5 * nbf5 * (naux_p / 64)) * 8;          // kpart
   s += (uint64_t)(nbf * ldcp) * 8;                           // T
-  if (precision == PNJK_FP64) s += (uint64_t)(2 * nbf_p * qt_window_cols((int)nbf5)) * 8;  // Cw
+  if (precision != PNJK_SPLIT) s += (uint64_t)(2 * nbf_p * qt_window_cols((int)nbf5)) * 8;  // Cw
   // W: split_refine, and FP64 builds without J and K that take the triangular route (this
   // total bounds both routes of an FP64 build)
   const bool tri_fp64 = precision == PNJK_FP64 && mo_tri_route(precision, nbf, nbf5);
@@ -652,7 +652,10 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
   // takes it (in FP64 or BF16x3) when its caller allows it, asks for neither J nor K (which
   // need the whole half-transformed X) and nbf5 / nbf is small enough for it to be faster.
   const bool mo_only = tri_routes && !st.want_j && !st.want_k && mo_tri_route(precision, nbf, n5);
-  st.tri = precision == PNJK_SPLIT_REFINE || (mo_only && precision == PNJK_FP64);
+  // split_refine's FP64 MO integrals follow the FP64 rule; past it the FP64 half- and quarter
+  // transforms run, and X16 for the split K^n is the split of their FP64 X
+  st.tri = (precision == PNJK_SPLIT_REFINE && mo_tri_route(PNJK_FP64, nbf, n5)) ||
+           (mo_only && precision == PNJK_FP64);
   st.tri16 = mo_only && precision == PNJK_SPLIT;
   st.qt_rows = (st.want_jmo || st.want_kmo) ? 0 : 1;
   const int64_t jk_elems = n5 * nbf * nbf;
@@ -687,7 +690,7 @@ int build_prepare(pnjk_engine* e, const BuildArgs& a, BuildState& st, char* scra
     // (split_refine keeps its triangular half-transform T here.)
     w.bytes[W_X] = (size_t)(n5 * nbf * g.naux_p + 64) * 8;
   }
-  if (precision == PNJK_FP64 && !st.tri) w.bytes[W_CW] = (size_t)(2 * g.nbf_p * (int64_t)qt_window_cols(g.nbf5)) * 8;
+  if (precision != PNJK_SPLIT && !st.tri) w.bytes[W_CW] = (size_t)(2 * g.nbf_p * (int64_t)qt_window_cols(g.nbf5)) * 8;
   if (st.tri) w.bytes[W_W] = (size_t)(n5 * n5 * g.naux_p) * 8;
   if (st.tri16) w.bytes[W_W] = (size_t)(n5 * n5 * g.naux_p) * 4;  // FP32 W
   w.bytes[W_D] = (size_t)(roundup(n5, 160) * g.naux_p) * 8;
@@ -780,7 +783,8 @@ int stage_in(pnjk_engine* e, BuildState& st, hipStream_t s) {
     PNJK_NEED(st.ct16h);
     PNJK_NEED(st.ct16l);
     PNJK_HIP_CHECK(launch_split_ct(Cp, g.ldcp, g.nbf, g.nbf5, g.nbf5, st.sg.ldk, st.ct16h, st.ct16l, s));
-  } else if (!st.tri) {  // the FP64 quarter transform's window copies of C
+  }
+  if (st.a.precision != PNJK_SPLIT && !st.tri) {  // the FP64 quarter transform's window copies of C
     PNJK_NEED(st.Cw);
     PNJK_HIP_CHECK(launch_window_c(Cp, g.ldcp, g.nbf_p, g.nbf5, qt_window_cols(g.nbf5), st.Cw, s));
   }
@@ -959,6 +963,20 @@ static int jk_build_locked(pnjk_engine* e, const BuildArgs& a, hipStream_t s, ch
     mark(ST_JMO);
     if ((r = fork())) return r;
     if ((r = stage_jmo(e, st, sa, nullptr)) || (r = stage_hcore(e, st, sa))) return r;
+    if (refine) {  // the split J^n and K^n of split_refine read the split planes of d and X
+      const BuildGeom& g = st.g;
+      if (st.want_j) {
+        PNJK_NEED(st.d16h);
+        PNJK_NEED(st.d16l);
+        PNJK_HIP_CHECK(launch_split_blocked(st.d, g.nbf5, g.naux_p, st.d16h, st.d16l, s));
+      }
+      if (st.want_k) {
+        PNJK_NEED(st.X);
+        PNJK_NEED(st.x16h);
+        PNJK_NEED(st.x16l);
+        PNJK_HIP_CHECK(launch_split_x16(st.X, st.sg, st.x16h, st.x16l, s));
+      }
+    }
     mark(ST_JN);
     if ((r = stage_jn(e, st, s, nullptr))) return r;
     if (st.want_j && (r = stage_sym(st, st.dJ, s))) return r;

@@ -3,7 +3,8 @@
 split:        every contraction in BF16x3 (hi*hi + hi*lo + lo*hi, FP32 accumulation).
               Bar: normwise relative error <= 3e-5 for J, K, J_MO, K_MO (a BF16x3
               product carries ~2^-16 relative error); H_core is FP64 (1e-10).
-split_refine: J_MO, K_MO in FP64 by the triangular route (bar 1e-10 against the oracle,
+split_refine: J_MO, K_MO in FP64 (the triangular route up to nbf5 = 2 nbf / 3, else the FP64
+              form's own transforms, bitwise its J_MO, K_MO; bar 1e-10 against the oracle,
               1e-12 against the FP64 form's), H_core bitwise the FP64 form's, so the energy
               is the FP64 energy to 1e-9 Eh; J, K in BF16x3 (3e-5).
 Energies are compared in Eh (FP64_E_EH, SPLIT_E_EH_MID below).
@@ -61,7 +62,7 @@ def test_split_shapes(eng, nbf, naux, nbf5):
 
 def test_split_energies_mid(eng):
     """Energies at nbf=280, naux=1100 in Eh: the FP64 form against the oracle within
-    FP64_E_EH; split_refine's MO outputs (FP64, from the triangular route) within 1e-12 of
+    FP64_E_EH; split_refine's MO outputs (FP64; nbf5 = nbf takes the full route) within 1e-12 of
     the FP64 form's, H_core bitwise, its energy within FP64_E_EH of the FP64 energy; the
     split form within SPLIT_E_EH_MID of the FP64 energy."""
     w = synth.workload(1)
@@ -133,3 +134,22 @@ def test_split_mo_only_builds(eng, nbf, naux, nbf5):
     torch.cuda.synchronize()
     for k in mo:
         assert np.array_equal(dev[k].cpu().numpy(), mo[k]), k
+
+
+@pytest.mark.parametrize("nbf,naux,nbf5", [(40, 144, 40), (130, 70, 130), (97, 300, 80), (161, 96, 161), (5, 7, 4)])
+def test_refine_full_route_is_fp64_bitwise(eng, nbf, naux, nbf5):
+    """Past nbf5 = 2 nbf / 3 split_refine forms J_MO and K_MO with the FP64 form's half- and
+    quarter transforms: bitwise the FP64 form's (H_core too); its J^n and K^n, from the split
+    planes of the FP64 d and X, within the split bar of the FP64 form's."""
+    assert 3 * nbf5 > 2 * nbf
+    print(f"case: refine full route nbf={nbf} naux={naux} nbf5={nbf5}", flush=True)
+    b = synth.make_b(nbf, naux, 81, 0.05)
+    C = synth.make_C(nbf, 82)
+    H = synth.make_H(nbf, 83)
+    eng.upload_b(b)
+    f = eng.build(C, nbf5, H=H, precision="fp64")
+    r = eng.build(C, nbf5, H=H, precision="split_refine")
+    for k in ("J_MO", "K_MO", "H_core"):
+        assert np.array_equal(r[k], f[k]), k
+    for k in ("J", "K"):
+        assert rel(r[k], f[k]) <= SPLIT_TOL, (k, rel(r[k], f[k]))
