@@ -129,7 +129,7 @@ int pnjk_jk_build(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, co
  * upload makes them), and so must the halved diagonal rows of b for an FP64 build that
  * takes the triangular MO route (the first such pnjk_jk_build, or any split one, makes
  * them). A build whose workspace exceeds 64 MiB (FP64 form, nbf5 = nbf: about
- * nbf = 120 with naux = 3.6 nbf) fails with PNJK_EINVAL and names the bytes it needs. All
+ * nbf = 125 with naux = 3.6 nbf; split_refine about 109) fails with PNJK_EINVAL and names the bytes it needs. All
  * no-allocation builds on one GPU share that array: calls that overlap in time on one GPU
  * (different streams, threads or engines) race; calls issued on one stream are ordered. */
 int pnjk_jk_build_noalloc(pnjk_engine* e, const double* C, int64_t ldc, int64_t nbf5, const double* H,
